@@ -1,0 +1,67 @@
+/*
+ * mak_mi355x.h -- C ABI of the MI355X (gfx950) QR / polar backend for
+ * MatrixAlgebraKit.jl.
+ *
+ * Conventions (shared by every entry point):
+ *   - All matrix pointers are DEVICE pointers (HIP), column-major with an
+ *     explicit leading dimension, like LAPACK and Julia's Matrix
+ *     (MatrixAlgebraKit src/yalapack.jl:179,316-317).
+ *   - Complex matrices are ComplexF64, interleaved (re, im): the layout of
+ *     Julia's ComplexF64 and numpy's complex128.  They are passed as void*.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *     Calls are asynchronous on that stream; nothing synchronises the device.
+ *   - `work` is a caller-owned device workspace of at least the size returned
+ *     by the matching *_workspace_size query (the library allocates nothing;
+ *     this mirrors LAPACK's lwork query, src/yalapack.jl:180-197).
+ *   - Return value: 0 on success; -i when argument i (1-based) is invalid,
+ *     mirroring LAPACK's info / chkargsok
+ *     (ext/MatrixAlgebraKitAMDGPUExt/yarocsolver.jl:88-89); > 0 for a numerical
+ *     failure or a HIP launch error.
+ *   - Optional outputs (R, P) are skipped when their pointer is NULL: the
+ *     reference's "empty R / empty P" convention
+ *     (src/implementations/qr.jl:149, src/implementations/polar.jl:64,76).
+ */
+#ifndef MAK_MI355X_H
+#define MAK_MI355X_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* element types */
+#define MAK_F64 0 /* Float64 */
+#define MAK_C64 1 /* ComplexF64 */
+
+/* ------------------------------------------------------------------ QR --- */
+
+/* Device workspace bytes for mak_qr_{compact,full}_{d,z} on an m x n input.
+ * full = 0 for qr_compact!, 1 for qr_full!.
+ * Replaces the lwork = -1 query of YALAPACK.geqrt!/gemqrt!
+ * (src/yalapack.jl:303-333, :882-928). */
+int mak_qr_workspace_size(int dtype, int full, int m, int n, size_t* bytes);
+
+/* qr_compact!(A, (Q, R), Householder(; positive)) for Float64 / ComplexF64.
+ * A (m x n) is destroyed (it holds the compact WY factors afterwards),
+ * Q is m x min(m,n), R is min(m,n) x n or NULL (R not computed).
+ * positive != 0 gives R a real non-negative diagonal (gauge fix).
+ * Replaces qr_compact!(A, QR, ::Householder) -> qr_householder!
+ * (src/implementations/qr.jl:110-113, :132-188). */
+int mak_qr_compact_d(void* stream, int m, int n, double* A, int lda, double* Q, int ldq, double* R, int ldr,
+                     int positive, void* work, size_t work_bytes);
+int mak_qr_compact_z(void* stream, int m, int n, void* A, int lda, void* Q, int ldq, void* R, int ldr,
+                     int positive, void* work, size_t work_bytes);
+
+/* qr_full!(A, (Q, R), Householder(; positive)): Q is m x m, R is m x n or NULL.
+ * Replaces qr_full!(A, QR, ::Householder) (src/implementations/qr.jl:106-109). */
+int mak_qr_full_d(void* stream, int m, int n, double* A, int lda, double* Q, int ldq, double* R, int ldr,
+                  int positive, void* work, size_t work_bytes);
+int mak_qr_full_z(void* stream, int m, int n, void* A, int lda, void* Q, int ldq, void* R, int ldr,
+                  int positive, void* work, size_t work_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAK_MI355X_H */

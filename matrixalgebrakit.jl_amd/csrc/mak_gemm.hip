@@ -1,0 +1,387 @@
+// FP64 MFMA GEMM for gfx950 (v_mfma_f64_16x16x4_f64), real and complex.
+//
+// C = alpha op(A) op(B) + beta C, column-major, op in {N, T, C}.  One
+// workgroup (4 waves) computes a BM x BN tile; each wave owns a
+// (BM/WM) x (BN/WN) sub-tile of 16x16 MFMA blocks.  Tiles of op(A) and op(B)
+// are staged through LDS as [k][row] / [k][col] so a fragment read is 16
+// consecutive doubles per k.  Complex products use four real MFMAs on split
+// real/imaginary LDS tiles (there is no complex MFMA).
+//
+// Split-K (ksplit > 1) writes per-slice partial products; the WY update uses
+// it for the tall inner products V^H C whose output is only 32 rows.
+#include <algorithm>
+
+#include "mak_common.hpp"
+#include "mak_kernels.hpp"
+
+namespace mak {
+
+template <int OPA>
+__device__ inline double load_a(const double* A, int64_t lda, int gi, int gk, int unitLower) {
+  // op(A)(gi, gk); stored index (r, c)
+  const int r = (OPA == OP_N) ? gi : gk;
+  const int c = (OPA == OP_N) ? gk : gi;
+  if (unitLower && r <= c) return r == c ? 1.0 : 0.0;
+  return A[r + (int64_t)c * lda];
+}
+template <int OPA>
+__device__ inline zd load_a(const zd* A, int64_t lda, int gi, int gk, int unitLower) {
+  const int r = (OPA == OP_N) ? gi : gk;
+  const int c = (OPA == OP_N) ? gk : gi;
+  if (unitLower && r <= c) return r == c ? mkz(1.0, 0.0) : mkz(0.0, 0.0);
+  zd v = A[r + (int64_t)c * lda];
+  if (OPA == OP_C) v.y = -v.y;
+  return v;
+}
+template <int OPB>
+__device__ inline double load_b(const double* B, int64_t ldb, int gk, int gj) {
+  return (OPB == OP_N) ? B[gk + (int64_t)gj * ldb] : B[gj + (int64_t)gk * ldb];
+}
+template <int OPB>
+__device__ inline zd load_b(const zd* B, int64_t ldb, int gk, int gj) {
+  zd v = (OPB == OP_N) ? B[gk + (int64_t)gj * ldb] : B[gj + (int64_t)gk * ldb];
+  if (OPB == OP_C) v.y = -v.y;
+  return v;
+}
+
+template <int BM, int BN, int BK, int WM, int WN, int OPA, int OPB>
+__global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
+  constexpr int WTM = BM / WM, WTN = BN / WN, TM = WTM / 16, TN = WTN / 16;
+  static_assert(WM * WN == 4, "4 waves");
+  __shared__ double As[BK][BM + 2];
+  __shared__ double Bs[BK][BN + 2];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = tid >> 6;
+  const int wm = w % WM, wn = w / WM;
+  const int bz = blockIdx.z;
+  const int bat = bz / g.ksplit;
+  const int ks = bz % g.ksplit;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int kbeg = ks * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  const double* A = reinterpret_cast<const double*>(g.A) + (int64_t)bat * g.strideA;
+  const double* B = reinterpret_cast<const double*>(g.B) + (int64_t)bat * g.strideB;
+  double* C = reinterpret_cast<double*>(g.C) + (int64_t)bat * g.strideC + (int64_t)ks * g.splitStride;
+
+  v4d acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+#pragma unroll 4
+    for (int e = tid; e < BM * BK; e += 256) {
+      int i, k;
+      if (OPA == OP_N) { i = e % BM; k = e / BM; }
+      else { k = e % BK; i = e / BK; }
+      const int gi = m0 + i, gk = k0 + k;
+      As[k][i] = (gi < g.M && gk < kend) ? load_a<OPA>(A, g.lda, gi, gk, g.unitLowerA) : 0.0;
+    }
+#pragma unroll 4
+    for (int e = tid; e < BN * BK; e += 256) {
+      int j, k;
+      if (OPB == OP_N) { k = e % BK; j = e / BK; }
+      else { j = e % BN; k = e / BN; }
+      const int gj = n0 + j, gk = k0 + k;
+      Bs[k][j] = (gj < g.N && gk < kend) ? load_b<OPB>(B, g.ldb, gk, gj) : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double af[TM], bf[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) af[i] = As[kk + (lane >> 4)][wm * WTM + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bf[j] = Bs[kk + (lane >> 4)][wn * WTN + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = mfma_f64(af[i], bf[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+  const double alpha = g.alpha_re, beta = g.beta_re;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * WTM + i * 16 + 4 * (lane >> 4) + r;
+        const int col = n0 + wn * WTN + j * 16 + (lane & 15);
+        if (row < g.M && col < g.N) {
+          double v = alpha * acc[i][j][r];
+          double* cp = C + row + (int64_t)col * g.ldc;
+          if (beta != 0.0) v = fma(beta, *cp, v);
+          *cp = v;
+        }
+      }
+}
+
+template <int BM, int BN, int BK, int WM, int WN, int OPA, int OPB>
+__global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
+  constexpr int WTM = BM / WM, WTN = BN / WN, TM = WTM / 16, TN = WTN / 16;
+  static_assert(WM * WN == 4, "4 waves");
+  __shared__ double Ar[BK][BM + 2], Ai[BK][BM + 2];
+  __shared__ double Br[BK][BN + 2], Bi[BK][BN + 2];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = tid >> 6;
+  const int wm = w % WM, wn = w / WM;
+  const int bz = blockIdx.z;
+  const int bat = bz / g.ksplit;
+  const int ks = bz % g.ksplit;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int kbeg = ks * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  const zd* A = reinterpret_cast<const zd*>(g.A) + (int64_t)bat * g.strideA;
+  const zd* B = reinterpret_cast<const zd*>(g.B) + (int64_t)bat * g.strideB;
+  zd* C = reinterpret_cast<zd*>(g.C) + (int64_t)bat * g.strideC + (int64_t)ks * g.splitStride;
+
+  v4d cr[TM][TN], ci[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      cr[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+      ci[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
+
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    for (int e = tid; e < BM * BK; e += 256) {
+      int i, k;
+      if (OPA == OP_N) { i = e % BM; k = e / BM; }
+      else { k = e % BK; i = e / BK; }
+      const int gi = m0 + i, gk = k0 + k;
+      zd v = (gi < g.M && gk < kend) ? load_a<OPA>(A, g.lda, gi, gk, g.unitLowerA) : mkz(0.0, 0.0);
+      Ar[k][i] = v.x;
+      Ai[k][i] = v.y;
+    }
+    for (int e = tid; e < BN * BK; e += 256) {
+      int j, k;
+      if (OPB == OP_N) { k = e % BK; j = e / BK; }
+      else { j = e % BN; k = e / BN; }
+      const int gj = n0 + j, gk = k0 + k;
+      zd v = (gj < g.N && gk < kend) ? load_b<OPB>(B, g.ldb, gk, gj) : mkz(0.0, 0.0);
+      Br[k][j] = v.x;
+      Bi[k][j] = v.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double ar[TM], ai[TM], nai[TM], br[TN], bi[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        ar[i] = Ar[kk + (lane >> 4)][wm * WTM + i * 16 + (lane & 15)];
+        ai[i] = Ai[kk + (lane >> 4)][wm * WTM + i * 16 + (lane & 15)];
+        nai[i] = -ai[i];
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        br[j] = Br[kk + (lane >> 4)][wn * WTN + j * 16 + (lane & 15)];
+        bi[j] = Bi[kk + (lane >> 4)][wn * WTN + j * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          cr[i][j] = mfma_f64(ar[i], br[j], cr[i][j]);
+          cr[i][j] = mfma_f64(nai[i], bi[j], cr[i][j]);
+          ci[i][j] = mfma_f64(ar[i], bi[j], ci[i][j]);
+          ci[i][j] = mfma_f64(ai[i], br[j], ci[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+  const zd alpha = mkz(g.alpha_re, g.alpha_im), beta = mkz(g.beta_re, g.beta_im);
+  const bool use_beta = (g.beta_re != 0.0 || g.beta_im != 0.0);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * WTM + i * 16 + 4 * (lane >> 4) + r;
+        const int col = n0 + wn * WTN + j * 16 + (lane & 15);
+        if (row < g.M && col < g.N) {
+          zd v = s_mul(alpha, mkz(cr[i][j][r], ci[i][j][r]));
+          zd* cp = C + row + (int64_t)col * g.ldc;
+          if (use_beta) v = s_fma(beta, *cp, v);
+          *cp = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch.  Tile shapes: a square 128x128 tile for general products, a 32-row
+// tile for the skinny V^H C products (M <= 32) of the WY updates.
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int BK, int WM, int WN, int OPA, int OPB, typename T>
+static void launch_gemm_tile(const GemmArgs& g, hipStream_t s) {
+  dim3 grid(cdiv(g.M, BM), cdiv(g.N, BN), g.batch * g.ksplit);
+  if constexpr (ScalarTraits<T>::is_complex)
+    hipLaunchKernelGGL((gemm_z64_kernel<BM, BN, BK, WM, WN, OPA, OPB>), grid, dim3(256), 0, s, g);
+  else
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, WM, WN, OPA, OPB>), grid, dim3(256), 0, s, g);
+}
+
+template <typename T, int OPA, int OPB>
+static void gemm_shape(const GemmArgs& g, hipStream_t s) {
+  constexpr bool cplx = ScalarTraits<T>::is_complex;
+  if (g.M <= 32) {
+    launch_gemm_tile<32, 64, 16, 2, 2, OPA, OPB, T>(g, s);
+  } else if (cplx) {
+    launch_gemm_tile<64, 64, 16, 2, 2, OPA, OPB, T>(g, s);
+  } else {
+    launch_gemm_tile<128, 128, 16, 2, 2, OPA, OPB, T>(g, s);
+  }
+}
+
+template <typename T> void gemm(int opA, int opB, GemmArgs g, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0) return;
+  if (g.batch <= 0) g.batch = 1;
+  if (g.ksplit <= 0) g.ksplit = 1;
+  if (g.kchunk <= 0) g.kchunk = g.K;
+  if (g.K <= 0) {
+    // C = beta C: run with an empty K range (the kernel writes alpha*0 + beta*C)
+    g.K = 0;
+    g.kchunk = 1;
+  }
+  constexpr bool cplx = ScalarTraits<T>::is_complex;
+  const int b = (opB == OP_N) ? 0 : ((opB == OP_T || !cplx) ? 1 : 2);
+  const int a = (opA == OP_N) ? 0 : ((opA == OP_T || !cplx) ? 1 : 2);
+  switch (a * 3 + b) {
+    case 0: gemm_shape<T, OP_N, OP_N>(g, s); break;
+    case 1: gemm_shape<T, OP_N, OP_T>(g, s); break;
+    case 2: gemm_shape<T, OP_N, OP_C>(g, s); break;
+    case 3: gemm_shape<T, OP_T, OP_N>(g, s); break;
+    case 4: gemm_shape<T, OP_T, OP_T>(g, s); break;
+    case 5: gemm_shape<T, OP_T, OP_C>(g, s); break;
+    case 6: gemm_shape<T, OP_C, OP_N>(g, s); break;
+    case 7: gemm_shape<T, OP_C, OP_T>(g, s); break;
+    default: gemm_shape<T, OP_C, OP_C>(g, s); break;
+  }
+}
+template void gemm<double>(int, int, GemmArgs, hipStream_t);
+template void gemm<zd>(int, int, GemmArgs, hipStream_t);
+
+void splitk_plan(int N, int K, int bn, int& ksplit, int& kchunk) {
+  const int tiles = cdiv(N, bn);
+  int ks = cdiv(512, tiles);
+  ks = std::min(ks, std::max(1, K / 256));
+  ks = std::max(ks, 1);
+  kchunk = cdiv(cdiv(K, ks), 16) * 16;
+  ksplit = cdiv(K, kchunk);
+}
+
+// ---------------------------------------------------------------------------
+// WY reduction: W2 = op(T) * sum_z Wp[z]
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) wy_reduce_kernel(const T* __restrict__ Wp, int ksplit, int64_t slice,
+                                                        const T* __restrict__ Tm, int transT, int nbc,
+                                                        int ncols, T* __restrict__ W2) {
+  __shared__ T S[32][65];
+  __shared__ T Tl[32][33];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * 64;
+  for (int i = tid; i < 32 * 32; i += 256) {
+    const int r = i % 32, c = i / 32;
+    T v = s_zero(T());
+    if (r < nbc && c < nbc) v = transT ? s_conj(Tm[c + r * 32]) : Tm[r + c * 32];
+    Tl[r][c] = v;
+  }
+  for (int i = tid; i < 32 * 64; i += 256) {
+    const int r = i % 32, c = i / 32;
+    T s = s_zero(T());
+    if (r < nbc && c0 + c < ncols) {
+      const T* p = Wp + r + (int64_t)(c0 + c) * nbc;
+      for (int z = 0; z < ksplit; ++z) s = s_add(s, p[(int64_t)z * slice]);
+    }
+    S[r][c] = s;
+  }
+  __syncthreads();
+  for (int i = tid; i < 32 * 64; i += 256) {
+    const int r = i % 32, c = i / 32;
+    if (r < nbc && c0 + c < ncols) {
+      T s = s_zero(T());
+      for (int k = 0; k < nbc; ++k) s = s_fma(Tl[r][k], S[k][c], s);
+      W2[r + (int64_t)(c0 + c) * nbc] = s;
+    }
+  }
+}
+
+template <typename T>
+void wy_reduce(const void* Wp, int ksplit, int64_t slice, const void* Tm, int transT, int nbc, int ncols,
+               void* W2, hipStream_t s) {
+  if (ncols <= 0) return;
+  hipLaunchKernelGGL((wy_reduce_kernel<T>), dim3(cdiv(ncols, 64)), dim3(256), 0, s,
+                     reinterpret_cast<const T*>(Wp), ksplit, slice, reinterpret_cast<const T*>(Tm), transT,
+                     nbc, ncols, reinterpret_cast<T*>(W2));
+}
+template void wy_reduce<double>(const void*, int, int64_t, const void*, int, int, int, void*, hipStream_t);
+template void wy_reduce<zd>(const void*, int, int64_t, const void*, int, int, int, void*, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Elementwise helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void set_identity_kernel(T* Q, int64_t ldq, int m, int n) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)m * n) return;
+  const int r = (int)(idx % m), c = (int)(idx / m);
+  Q[r + c * ldq] = (r == c) ? s_one(T()) : s_zero(T());
+}
+template <typename T> void set_identity(void* Q, int64_t ldq, int m, int n, hipStream_t s) {
+  const int64_t tot = (int64_t)m * n;
+  if (tot <= 0) return;
+  hipLaunchKernelGGL((set_identity_kernel<T>), dim3((unsigned)cdiv64(tot, 256)), dim3(256), 0, s,
+                     reinterpret_cast<T*>(Q), ldq, m, n);
+}
+template void set_identity<double>(void*, int64_t, int, int, hipStream_t);
+template void set_identity<zd>(void*, int64_t, int, int, hipStream_t);
+
+template <typename T>
+__global__ void qr_extract_r_kernel(const T* A, int64_t lda, T* R, int64_t ldr, int rrows, int n, int minmn,
+                                    int positive) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)rrows * n) return;
+  const int r = (int)(idx % rrows), c = (int)(idx / rrows);
+  T v = s_zero(T());
+  if (r < minmn && r <= c) {
+    v = A[r + c * lda];
+    if (positive) v = s_mul(s_conj(sign_safe(A[r + r * lda])), v);
+  }
+  R[r + c * ldr] = v;
+}
+template <typename T>
+void qr_extract_r(const void* A, int64_t lda, void* R, int64_t ldr, int rrows, int n, int minmn, int positive,
+                  hipStream_t s) {
+  const int64_t tot = (int64_t)rrows * n;
+  if (tot <= 0) return;
+  hipLaunchKernelGGL((qr_extract_r_kernel<T>), dim3((unsigned)cdiv64(tot, 256)), dim3(256), 0, s,
+                     reinterpret_cast<const T*>(A), lda, reinterpret_cast<T*>(R), ldr, rrows, n, minmn, positive);
+}
+template void qr_extract_r<double>(const void*, int64_t, void*, int64_t, int, int, int, int, hipStream_t);
+template void qr_extract_r<zd>(const void*, int64_t, void*, int64_t, int, int, int, int, hipStream_t);
+
+template <typename T>
+__global__ void qr_gauge_q_kernel(const T* A, int64_t lda, T* Q, int64_t ldq, int m, int minmn) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)m * minmn) return;
+  const int r = (int)(idx % m), c = (int)(idx / m);
+  Q[r + c * ldq] = s_mul(Q[r + c * ldq], sign_safe(A[c + c * lda]));
+}
+template <typename T>
+void qr_gauge_q(const void* A, int64_t lda, void* Q, int64_t ldq, int m, int minmn, hipStream_t s) {
+  const int64_t tot = (int64_t)m * minmn;
+  if (tot <= 0) return;
+  hipLaunchKernelGGL((qr_gauge_q_kernel<T>), dim3((unsigned)cdiv64(tot, 256)), dim3(256), 0, s,
+                     reinterpret_cast<const T*>(A), lda, reinterpret_cast<T*>(Q), ldq, m, minmn);
+}
+template void qr_gauge_q<double>(const void*, int64_t, void*, int64_t, int, int, hipStream_t);
+template void qr_gauge_q<zd>(const void*, int64_t, void*, int64_t, int, int, hipStream_t);
+
+}  // namespace mak

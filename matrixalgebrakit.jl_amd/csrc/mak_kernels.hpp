@@ -1,0 +1,112 @@
+// Argument structs and host launchers shared by the library's translation
+// units.  Every pointer is a device pointer; matrices are column-major.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mak_common.hpp"
+
+namespace mak {
+
+// ---------------------------------------------------------------- panel / TSQR
+struct Geqr2NodeArgs {
+  void* X;          // stacked input rows of this level (factored in place)
+  int64_t ldx;
+  int H;            // rows of the stacked input
+  int nodes;        // workgroups (tree nodes) of this level
+  int chunk;        // rows per node (> 0), or 0 for a balanced split of H
+  int nbc;          // active columns (<= 32)
+  void* Tout;       // T factor of node i at Tout + i * tstride (32 x 32, ld 32)
+  int64_t tstride;
+  void* Rnext;      // R of node i -> rows [i*nbc, i*nbc + nbc) of the next level
+  int64_t ldr;
+  void* V1out;      // copy of the top 32x32 of V per node (unit lower), or null
+};
+
+constexpr int kMaxTreeLevels = 8;
+
+struct TsqrRootArgs {
+  const void* Xtop;  // factored top node (V strictly below, R on/above)
+  int64_t ldxtop;
+  int htop;
+  int nbc;
+  const void* Ttop;
+  int nchain;        // levels below the top
+  const void* chainV1[kMaxTreeLevels];
+  int64_t chainLd[kMaxTreeLevels];
+  const void* chainT[kMaxTreeLevels];
+  void* Mtop;        // out: T_top V_top1^H (32 x 32)
+  void* Tout;        // out: reconstructed T of the panel (32 x 32)
+  void* Uout;        // out: U of the LU of Q - D (32 x 32)
+  void* Apanel;      // out: top nbc rows of the panel <- [D R ; L1]
+  int64_t lda;
+};
+
+struct TsqrExpandArgs {
+  int H;
+  int nodes;
+  int chunk;
+  int nbc;
+  void* X;                 // V of this level's nodes (leaves: the panel itself)
+  int64_t ldx;
+  const void* Tnodes;      // T of node i at Tnodes + i * 32 * 32
+  const void* V1base;      // top 32x32 of V of node i at V1base + i * v1stride
+  int64_t v1stride;
+  int64_t v1ld;
+  const void* Qparent;     // parent level explicit Q (used when Mtop is null)
+  int64_t ldqp;
+  const void* Mtop;        // parent is the top node: Q block = E - V_top Mtop
+  const void* Xtop;
+  int64_t ldxtop;
+  void* Qout;              // inner levels: explicit Q rows of this level
+  int64_t ldqo;
+  const void* Umat;        // leaves: U of the LU; rows >= nbc become V = Q U^{-1}
+};
+
+template <typename T> int geqr2_node_capacity();
+template <typename T> void launch_geqr2_node(const Geqr2NodeArgs& a, int leaf_rows_max, hipStream_t s);
+template <typename T> void launch_tsqr_root_finish(const TsqrRootArgs& a, hipStream_t s);
+template <typename T> void launch_tsqr_expand(const TsqrExpandArgs& a, int max_node_rows, hipStream_t s);
+
+// ---------------------------------------------------------------- GEMM
+enum Op { OP_N = 0, OP_T = 1, OP_C = 2 };
+
+struct GemmArgs {
+  int M, N, K;
+  const void* A;
+  int64_t lda, strideA;   // strideA: element offset between batch entries
+  const void* B;
+  int64_t ldb, strideB;
+  void* C;
+  int64_t ldc, strideC;
+  double alpha_re, alpha_im, beta_re, beta_im;
+  int ksplit;             // >1: slice z of K writes C + z * splitStride (beta must be 0)
+  int kchunk;
+  int64_t splitStride;
+  int batch;
+  int unitLowerA;         // stored A is unit lower trapezoidal (strict lower part valid)
+};
+
+// C = alpha op(A) op(B) + beta C on FP64 MFMA (real) or 4 real MFMAs (complex).
+template <typename T> void gemm(int opA, int opB, GemmArgs g, hipStream_t s);
+
+// Split-K partial count and chunk for an (M x N x K) product with small M.
+void splitk_plan(int N, int K, int bn, int& ksplit, int& kchunk);
+
+// W2 = op(Tm) * sum_z Wp[z]  (nbc x ncols, ld nbc), op = conj-transpose if transT
+template <typename T>
+void wy_reduce(const void* Wp, int ksplit, int64_t slice, const void* Tm, int transT, int nbc,
+               int ncols, void* W2, hipStream_t s);
+
+// ---------------------------------------------------------------- elementwise
+template <typename T> void set_identity(void* Q, int64_t ldq, int m, int n, hipStream_t s);
+// R <- triu(A[0:minmn, :]) row-scaled by conj(sign_safe(diag)) when positive
+template <typename T>
+void qr_extract_r(const void* A, int64_t lda, void* R, int64_t ldr, int rrows, int n, int minmn,
+                  int positive, hipStream_t s);
+// Q[:, j] *= sign_safe(A[j, j]) for j < minmn
+template <typename T>
+void qr_gauge_q(const void* A, int64_t lda, void* Q, int64_t ldq, int m, int minmn, hipStream_t s);
+
+}  // namespace mak

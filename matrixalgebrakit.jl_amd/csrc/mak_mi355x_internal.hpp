@@ -1,0 +1,21 @@
+// Internal declarations shared by the library's host drivers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "mak_mi355x.h"
+#include "mak_common.hpp"
+
+namespace mak {
+
+template <typename T> size_t qr_workspace_bytes(int m, int n, int full);
+template <typename T> void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s);
+template <typename T>
+void orgqr_blocked(const T* A, int64_t lda, int m, int n, T* Q, int64_t ldq, int qcols, char* ws, hipStream_t s);
+template <typename T>
+int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, int64_t ldq, T* R, int64_t ldr,
+              int positive, void* work, size_t work_bytes);
+
+}  // namespace mak

@@ -1,0 +1,316 @@
+// Blocked Householder QR driver (qr_compact! / qr_full!) and its C-ABI.
+//
+// Mirrors MatrixAlgebraKit's qr_householder! (src/implementations/qr.jl:132-188):
+// factor A in place into compact WY form (V strictly below the diagonal, R on
+// and above, one 32x32 T per panel, like LAPACK geqrt), form Q by applying the
+// block reflectors to the identity backwards (orgqr order), copy triu(A) into
+// R and apply the positive gauge (src/common/gauge.jl:16-25).
+//
+// Per panel of 32 columns:
+//   * rows <= node capacity: one geqr2 node (registers + LDS, one workgroup);
+//   * taller: TSQR tree of geqr2 nodes, root finish (LU of Q - D), expansion;
+//   * trailing update C <- C - V (T^H (V^H C)) as split-K MFMA GEMM, a small
+//     reduction with T^H, and an MFMA rank-32 update.
+#include <algorithm>
+#include <vector>
+
+#include "mak_common.hpp"
+#include "mak_kernels.hpp"
+#include "mak_mi355x_internal.hpp"
+
+namespace mak {
+
+namespace {
+
+constexpr int NB = 32;
+
+struct Level {
+  int H;       // stacked rows
+  int nodes;
+  int chunk;   // 0 = balanced (leaves)
+  size_t offX, offQ, offT;  // workspace offsets (bytes); X/Q unused at level 0
+};
+
+struct TreePlan {
+  std::vector<Level> levels;  // levels[0] = leaves
+  size_t offV1 = 0, offM = 0, offU = 0;
+  size_t bytes = 0;           // bytes used by the tree (relative to its base)
+};
+
+template <typename T> TreePlan plan_tree(int H, int nbc, size_t base) {
+  TreePlan p;
+  const int cap = geqr2_node_capacity<T>();
+  const int fan = cap / NB;
+  size_t off = base;
+  auto take = [&](size_t elems) {
+    size_t o = off;
+    off += ((elems * sizeof(T) + 255) / 256) * 256;
+    return o;
+  };
+  Level l0{H, cdiv(H, cap), 0, 0, 0, 0};
+  l0.offT = take((size_t)l0.nodes * NB * NB);
+  p.levels.push_back(l0);
+  p.offV1 = take((size_t)l0.nodes * NB * NB);
+  while (p.levels.back().nodes > 1) {
+    const Level& prev = p.levels.back();
+    Level l;
+    l.H = prev.nodes * nbc;
+    l.chunk = fan * nbc;
+    l.nodes = cdiv(prev.nodes, fan);
+    l.offX = take((size_t)l.H * nbc);
+    l.offQ = take((size_t)l.H * nbc);
+    l.offT = take((size_t)l.nodes * NB * NB);
+    p.levels.push_back(l);
+  }
+  p.offM = take(NB * NB);
+  p.offU = take(NB * NB);
+  p.bytes = off - base;
+  return p;
+}
+
+template <typename T> struct QrLayout {
+  size_t offTpanels = 0;  // NB*NB per panel
+  size_t offTree = 0;
+  size_t offWp = 0, offW2 = 0;
+  size_t wpElems = 0, w2Elems = 0;
+  size_t bytes = 0;
+};
+
+inline size_t align256(size_t b) { return ((b + 255) / 256) * 256; }
+
+template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
+  QrLayout<T> L;
+  const int minmn = std::min(m, n);
+  const int npan = cdiv(std::max(minmn, 1), NB);
+  size_t off = 0;
+  L.offTpanels = off;
+  off += align256((size_t)npan * NB * NB * sizeof(T));
+  L.offTree = off;
+  if (m > geqr2_node_capacity<T>()) {
+    TreePlan tp = plan_tree<T>(m, NB, 0);
+    off += align256(tp.bytes);
+  }
+  const int ncmax = std::max(n, qcols);
+  int ks, kc;
+  splitk_plan(ncmax, m, 64, ks, kc);
+  L.wpElems = (size_t)NB * ncmax * std::max(ks, 1);
+  // ksplit grows as N shrinks: bound ksplit * N over every N <= ncmax
+  L.wpElems = std::max(L.wpElems, (size_t)NB * (size_t)(512 * 64 + ncmax));
+  L.offWp = off;
+  off += align256(L.wpElems * sizeof(T));
+  L.w2Elems = (size_t)NB * ncmax;
+  L.offW2 = off;
+  off += align256(L.w2Elems * sizeof(T));
+  L.bytes = off;
+  return L;
+}
+
+template <typename T>
+void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrLayout<T>& L, hipStream_t s) {
+  T* Ap = A + j0 + (int64_t)j0 * lda;
+  const int H = m - j0;
+  T* Tpan = reinterpret_cast<T*>(ws + L.offTpanels) + (size_t)(j0 / NB) * NB * NB;
+  const int cap = geqr2_node_capacity<T>();
+  if (H <= cap) {
+    Geqr2NodeArgs g{};
+    g.X = Ap; g.ldx = lda; g.H = H; g.nodes = 1; g.chunk = 0; g.nbc = nbc;
+    g.Tout = Tpan; g.tstride = NB * NB;
+    launch_geqr2_node<T>(g, cap, s);
+    return;
+  }
+  char* tb = ws + L.offTree;
+  TreePlan tp = plan_tree<T>(H, nbc, 0);
+  auto ptr = [&](size_t off) { return reinterpret_cast<T*>(tb + off); };
+  const int top = (int)tp.levels.size() - 1;
+  // factor every level bottom-up
+  for (int lev = 0; lev <= top; ++lev) {
+    const Level& lv = tp.levels[lev];
+    Geqr2NodeArgs g{};
+    if (lev == 0) { g.X = Ap; g.ldx = lda; }
+    else { g.X = ptr(lv.offX); g.ldx = lv.H; }
+    g.H = lv.H; g.nodes = lv.nodes; g.chunk = lv.chunk; g.nbc = nbc;
+    g.Tout = ptr(lv.offT); g.tstride = NB * NB;
+    if (lev < top) { g.Rnext = ptr(tp.levels[lev + 1].offX); g.ldr = tp.levels[lev + 1].H; }
+    if (lev == 0) g.V1out = ptr(tp.offV1);
+    launch_geqr2_node<T>(g, cap, s);
+  }
+  // root finish
+  {
+    const Level& lt = tp.levels[top];
+    TsqrRootArgs r{};
+    r.Xtop = ptr(lt.offX); r.ldxtop = lt.H; r.htop = lt.H; r.nbc = nbc;
+    r.Ttop = ptr(lt.offT);
+    r.nchain = top;
+    for (int lev = 0; lev < top; ++lev) {
+      if (lev == 0) { r.chainV1[0] = ptr(tp.offV1); r.chainLd[0] = NB; }
+      else { r.chainV1[lev] = ptr(tp.levels[lev].offX); r.chainLd[lev] = tp.levels[lev].H; }
+      r.chainT[lev] = ptr(tp.levels[lev].offT);
+    }
+    r.Mtop = ptr(tp.offM); r.Tout = Tpan; r.Uout = ptr(tp.offU); r.Apanel = Ap; r.lda = lda;
+    launch_tsqr_root_finish<T>(r, s);
+  }
+  // expansion top-down
+  for (int lev = top - 1; lev >= 0; --lev) {
+    const Level& lv = tp.levels[lev];
+    TsqrExpandArgs e{};
+    e.H = lv.H; e.nodes = lv.nodes; e.chunk = lv.chunk; e.nbc = nbc;
+    if (lev == 0) { e.X = Ap; e.ldx = lda; e.V1base = ptr(tp.offV1); e.v1stride = NB * NB; e.v1ld = NB; }
+    else { e.X = ptr(lv.offX); e.ldx = lv.H; e.V1base = ptr(lv.offX); e.v1stride = lv.chunk; e.v1ld = lv.H; }
+    e.Tnodes = ptr(lv.offT);
+    if (lev == top - 1) {
+      const Level& lt = tp.levels[top];
+      e.Mtop = ptr(tp.offM); e.Xtop = ptr(lt.offX); e.ldxtop = lt.H;
+    } else {
+      e.Qparent = ptr(tp.levels[lev + 1].offQ); e.ldqp = tp.levels[lev + 1].H;
+    }
+    if (lev == 0) e.Umat = ptr(tp.offU);
+    else { e.Qout = ptr(lv.offQ); e.ldqo = lv.H; }
+    const int maxrows = (lev == 0) ? cdiv(lv.H, lv.nodes) + 1 : lv.chunk;
+    launch_tsqr_expand<T>(e, maxrows, s);
+  }
+}
+
+// C (rows x ncols) <- C - V op(T) (V^H C); V is the unit-lower panel at Vp.
+template <typename T>
+void apply_wy(const T* Vp, int64_t ldv, int rows, int nbc, const T* Tm, int transT, T* C, int64_t ldc,
+              int ncols, char* ws, const QrLayout<T>& L, hipStream_t s) {
+  if (ncols <= 0 || rows <= 0) return;
+  T* Wp = reinterpret_cast<T*>(ws + L.offWp);
+  T* W2 = reinterpret_cast<T*>(ws + L.offW2);
+  int ks, kc;
+  splitk_plan(ncols, rows, 64, ks, kc);
+  const int64_t slice = (int64_t)nbc * ncols;
+  {
+    GemmArgs g{};
+    g.M = nbc; g.N = ncols; g.K = rows;
+    g.A = Vp; g.lda = ldv; g.B = C; g.ldb = ldc; g.C = Wp; g.ldc = nbc;
+    g.alpha_re = 1.0; g.beta_re = 0.0;
+    g.ksplit = ks; g.kchunk = kc; g.splitStride = slice; g.batch = 1; g.unitLowerA = 1;
+    gemm<T>(OP_C, OP_N, g, s);
+  }
+  wy_reduce<T>(Wp, ks, slice, Tm, transT, nbc, ncols, W2, s);
+  {
+    GemmArgs g{};
+    g.M = rows; g.N = ncols; g.K = nbc;
+    g.A = Vp; g.lda = ldv; g.B = W2; g.ldb = nbc; g.C = C; g.ldc = ldc;
+    g.alpha_re = -1.0; g.beta_re = 1.0;
+    g.ksplit = 1; g.kchunk = nbc; g.batch = 1; g.unitLowerA = 1;
+    gemm<T>(OP_N, OP_N, g, s);
+  }
+}
+
+}  // namespace
+
+template <typename T> size_t qr_workspace_bytes(int m, int n, int full) {
+  const int minmn = std::min(m, n);
+  const int qcols = full ? m : minmn;
+  return qr_layout<T>(m, n, qcols).bytes;
+}
+template size_t qr_workspace_bytes<double>(int, int, int);
+template size_t qr_workspace_bytes<zd>(int, int, int);
+
+template <typename T>
+void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s) {
+  const int minmn = std::min(m, n);
+  QrLayout<T> L = qr_layout<T>(m, n, (int)qcols);
+  for (int j0 = 0; j0 < minmn; j0 += NB) {
+    const int nbc = std::min(NB, minmn - j0);
+    factor_panel<T>(A, lda, j0, m, nbc, ws, L, s);
+    const int nc = n - j0 - nbc;
+    if (nc > 0) {
+      const T* Tpan = reinterpret_cast<const T*>(ws + L.offTpanels) + (size_t)(j0 / NB) * NB * NB;
+      apply_wy<T>(A + j0 + (int64_t)j0 * lda, lda, m - j0, nbc, Tpan, 1, A + j0 + (int64_t)(j0 + nbc) * lda,
+                  lda, nc, ws, L, s);
+    }
+  }
+}
+
+template <typename T>
+void orgqr_blocked(const T* A, int64_t lda, int m, int n, T* Q, int64_t ldq, int qcols, char* ws, hipStream_t s) {
+  const int minmn = std::min(m, n);
+  QrLayout<T> L = qr_layout<T>(m, n, qcols);
+  set_identity<T>(Q, ldq, m, qcols, s);
+  const int npan = cdiv(minmn, NB);
+  for (int p = npan - 1; p >= 0; --p) {
+    const int j0 = p * NB;
+    const int nbc = std::min(NB, minmn - j0);
+    const T* Tpan = reinterpret_cast<const T*>(ws + L.offTpanels) + (size_t)p * NB * NB;
+    apply_wy<T>(A + j0 + (int64_t)j0 * lda, lda, m - j0, nbc, Tpan, 0, Q + j0 + (int64_t)j0 * ldq, ldq,
+                qcols - j0, ws, L, s);
+  }
+}
+
+template <typename T>
+int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, int64_t ldq, T* R, int64_t ldr,
+              int positive, void* work, size_t work_bytes) {
+  if (m < 0) return -2;
+  if (n < 0) return -3;
+  if (m > 0 && A == nullptr) return -4;
+  if (lda < std::max(1, m)) return -5;
+  const int minmn = std::min(m, n);
+  const int qcols = full ? m : minmn;
+  const int rrows = full ? m : minmn;
+  if ((int64_t)m * qcols > 0 && Q == nullptr) return -6;
+  if (ldq < std::max(1, m)) return -7;
+  if (R != nullptr && ldr < std::max(1, rrows)) return -9;
+  const size_t need = qr_workspace_bytes<T>(m, n, full);
+  if (work == nullptr || work_bytes < need) return -11;
+  char* ws = reinterpret_cast<char*>(work);
+  if (minmn == 0) {
+    set_identity<T>(Q, ldq, m, qcols, s);
+    if (R) qr_extract_r<T>(A, lda, R, ldr, rrows, n, 0, 0, s);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+  }
+  geqrf_blocked<T>(A, lda, m, n, ws, qcols, s);
+  if (R) qr_extract_r<T>(A, lda, R, ldr, rrows, n, minmn, positive, s);
+  orgqr_blocked<T>(A, lda, m, n, Q, ldq, qcols, ws, s);
+  if (positive) qr_gauge_q<T>(A, lda, Q, ldq, m, minmn, s);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+template void geqrf_blocked<double>(double*, int64_t, int, int, char*, size_t, hipStream_t);
+template void geqrf_blocked<zd>(zd*, int64_t, int, int, char*, size_t, hipStream_t);
+template void orgqr_blocked<double>(const double*, int64_t, int, int, double*, int64_t, int, char*, hipStream_t);
+template void orgqr_blocked<zd>(const zd*, int64_t, int, int, zd*, int64_t, int, char*, hipStream_t);
+template int qr_driver<double>(hipStream_t, int, int, int, double*, int64_t, double*, int64_t, double*, int64_t,
+                               int, void*, size_t);
+template int qr_driver<zd>(hipStream_t, int, int, int, zd*, int64_t, zd*, int64_t, zd*, int64_t, int, void*,
+                           size_t);
+
+}  // namespace mak
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int mak_qr_workspace_size(int dtype, int full, int m, int n, size_t* bytes) {
+  if (bytes == nullptr) return -5;
+  if (m < 0) return -3;
+  if (n < 0) return -4;
+  if (dtype == MAK_F64) *bytes = mak::qr_workspace_bytes<double>(m, n, full);
+  else if (dtype == MAK_C64) *bytes = mak::qr_workspace_bytes<mak::zd>(m, n, full);
+  else return -1;
+  return 0;
+}
+
+int mak_qr_compact_d(void* s, int m, int n, double* A, int lda, double* Q, int ldq, double* R, int ldr,
+                     int positive, void* work, size_t work_bytes) {
+  return mak::qr_driver<double>((hipStream_t)s, 0, m, n, A, lda, Q, ldq, R, ldr, positive, work, work_bytes);
+}
+int mak_qr_compact_z(void* s, int m, int n, void* A, int lda, void* Q, int ldq, void* R, int ldr,
+                     int positive, void* work, size_t work_bytes) {
+  return mak::qr_driver<mak::zd>((hipStream_t)s, 0, m, n, (mak::zd*)A, lda, (mak::zd*)Q, ldq, (mak::zd*)R, ldr, positive,
+                                 work, work_bytes);
+}
+int mak_qr_full_d(void* s, int m, int n, double* A, int lda, double* Q, int ldq, double* R, int ldr,
+                  int positive, void* work, size_t work_bytes) {
+  return mak::qr_driver<double>((hipStream_t)s, 1, m, n, A, lda, Q, ldq, R, ldr, positive, work, work_bytes);
+}
+int mak_qr_full_z(void* s, int m, int n, void* A, int lda, void* Q, int ldq, void* R, int ldr, int positive,
+                  void* work, size_t work_bytes) {
+  return mak::qr_driver<mak::zd>((hipStream_t)s, 1, m, n, (mak::zd*)A, lda, (mak::zd*)Q, ldq, (mak::zd*)R, ldr, positive,
+                                 work, work_bytes);
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+"""ctypes binding of libmak_mi355x.so (the C ABI declared in include/mak_mi355x.h).
+
+The library is REQUIRED: importing the compute entry points without it raises
+immediately; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libmak_mi355x.so"
+HEADER = Path(__file__).resolve().parents[2] / "include" / "mak_mi355x.h"
+
+_vp = C.c_void_p
+_i = C.c_int
+_sz = C.c_size_t
+_d = C.c_double
+
+# name -> (restype, argtypes); kept in the order of include/mak_mi355x.h
+SIGNATURES = {
+    "mak_qr_workspace_size": (_i, [_i, _i, _i, _i, C.POINTER(_sz)]),
+    "mak_qr_compact_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
+    "mak_qr_compact_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
+    "mak_qr_full_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
+    "mak_qr_full_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
+}
+
+
+class LibraryNotBuilt(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load(path: os.PathLike | None = None):
+    """Load (once) and type the shared library; raise if it is missing."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise LibraryNotBuilt(
+            f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(the HIP library is required; there is no CPU fallback)")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def declared_symbols(header: os.PathLike = HEADER):
+    """Function names declared in include/mak_mi355x.h."""
+    text = Path(header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"\bint\s+(mak_\w+)\s*\(", text)

@@ -1,0 +1,146 @@
+"""Parity of the HIP qr_compact! / qr_full! path with the CPU oracle.
+
+Tolerances: with positive = true the QR of a full-rank matrix is unique, so Q
+and R are compared elementwise with the oracle: relative Frobenius error
+<= 1e-12 for the Gaussian inputs used here (condition numbers <= ~10^2).
+The north-star bound (errors within 10x LAPACK's) is checked at full size on
+||A - QR|| / ||A|| and ||Q^H Q - I||.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import mak_mi355x as mak
+from oracle import mak_oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = np.load(Path(__file__).parent / "golden" / "golden_small.npz")
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def to_dev(A, dev):
+    return mak.colmajor(torch.from_numpy(np.asfortranarray(A)).to(dev))
+
+
+def run_qr(A, dev, full=False, positive=True, with_r=True):
+    At = to_dev(A, dev)
+    Q, R = mak.initialize_output_qr(At, full)
+    if not with_r:
+        R = torch.empty((0, 0), dtype=At.dtype, device=dev)
+    alg = mak.MI355XHouseholder(positive=positive)
+    f = mak.qr_full_ if full else mak.qr_compact_
+    Q2, R2 = f(At, (Q, R), alg)
+    assert Q2 is Q and R2 is R
+    torch.cuda.synchronize()
+    return Q.cpu().numpy(), R.cpu().numpy()
+
+
+SIZES = [(54, 37), (54, 54), (54, 63), (1, 1), (5, 1), (1, 5), (33, 33), (100, 32), (64, 100),
+         (1024, 512), (1025, 64), (2100, 64), (3000, 200), (200, 300), (700, 700)]
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+@pytest.mark.parametrize("m,n", SIZES, ids=[f"{m}x{n}" for m, n in SIZES])
+def test_qr_compact_matches_oracle(m, n, dt, dev):
+    A = O.gaussian(m, n, dt, seed=1000 + m + n)
+    Q, R = run_qr(A, dev)
+    Qo, Ro = O.qr_compact(A)
+    assert rel(Q, Qo) < 1e-12, rel(Q, Qo)
+    assert rel(R, Ro) < 1e-12, rel(R, Ro)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+@pytest.mark.parametrize("m,n", [(54, 37), (54, 63), (300, 70), (1500, 40)])
+def test_qr_full_matches_oracle(m, n, dt, dev):
+    A = O.gaussian(m, n, dt, seed=2000 + m + n)
+    Q, R = run_qr(A, dev, full=True)
+    Qo, Ro = O.qr_full(A)
+    k = min(m, n)
+    assert rel(Q[:, :k], Qo[:, :k]) < 1e-12
+    assert rel(R, Ro) < 1e-12
+    # the trailing columns of a full Q are only fixed up to a unitary rotation:
+    # check that Q is unitary and spans the complement
+    assert rel(Q.conj().T @ Q, np.eye(m)) < 1e-12
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+@pytest.mark.parametrize("m,n", [(54, 37), (2100, 64), (40, 90)])
+def test_qr_positive_false_and_no_R(m, n, dt, dev):
+    A = O.gaussian(m, n, dt, seed=3000 + m + n)
+    Q, R = run_qr(A, dev, positive=False)
+    assert rel(Q @ R, A) < 1e-13
+    assert O.isisometric(Q)
+    assert np.allclose(np.tril(R, -1), 0)
+    Qp, Rp = run_qr(A, dev, with_r=False)
+    Qo, _ = O.qr_compact(A)
+    assert rel(Qp, Qo) < 1e-12
+
+
+def test_qr_tree_three_levels(dev):
+    # real nodes hold 1024 rows and fan in 32: > 32768 rows needs 3 tree levels
+    A = O.gaussian(40000, 40, np.float64, seed=5)
+    Q, R = run_qr(A, dev)
+    Qo, Ro = O.qr_compact(A)
+    assert rel(Q, Qo) < 1e-12 and rel(R, Ro) < 1e-12
+
+
+def test_qr_complex_tree_three_levels(dev):
+    # complex nodes hold 256 rows and fan in 8: > 2048 rows needs 3 levels
+    A = O.gaussian(2600, 40, np.complex128, seed=6)
+    Q, R = run_qr(A, dev)
+    Qo, Ro = O.qr_compact(A)
+    assert rel(Q, Qo) < 1e-12 and rel(R, Ro) < 1e-12
+
+
+@pytest.mark.parametrize("key", [k[:-2] for k in GOLD.files if k.startswith("qr_") and k.endswith("_A")])
+def test_qr_golden_fixtures(key, dev):
+    A = GOLD[key + "_A"]
+    Q, R = run_qr(A, dev)
+    assert rel(Q, GOLD[key + "_Q"]) < 1e-12 and rel(R, GOLD[key + "_R"]) < 1e-12
+    Qf, Rf = run_qr(A, dev, full=True)
+    k = min(A.shape)
+    assert rel(Qf[:, :k], GOLD[key + "_Qfull"][:, :k]) < 1e-12
+    assert rel(Rf, GOLD[key + "_Rfull"]) < 1e-12
+
+
+def test_qr_rank_deficient_and_special(dev):
+    # exact zero columns, repeated columns and a diagonal input
+    A = O.gaussian(300, 40, np.float64, seed=9)
+    A[:, 5] = 0
+    A[:, 7] = A[:, 3]
+    Q, R = run_qr(A, dev)
+    assert rel(Q @ R, A) < 1e-13 and O.isisometric(Q)
+    assert np.all(np.diagonal(R) >= 0)
+    d = np.array([3.0, -2.0, 0.5, -0.25])
+    Q, R = run_qr(np.diag(d), dev)
+    assert np.allclose(np.diagonal(R), np.abs(d)) and np.allclose(np.diagonal(Q), np.sign(d))
+
+
+def test_qr_input_untouched_by_allocating_form(dev):
+    A = to_dev(O.gaussian(100, 60, np.float64, seed=3), dev)
+    A0 = A.clone()
+    Q, R = mak.qr_compact(A)
+    torch.cuda.synchronize()
+    assert torch.equal(A, A0)
+    assert torch.allclose(Q @ R, A0, atol=1e-12)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+def test_qr_full_size_within_10x_lapack(dt, dev):
+    m, n = 8192, 1024
+    A = O.gaussian(m, n, dt, seed=42)
+    Q, R = run_qr(A, dev)
+    Qo, Ro = O.qr_compact(A)
+    nA = np.linalg.norm(A)
+    res, res_o = np.linalg.norm(A - Q @ R) / nA, np.linalg.norm(A - Qo @ Ro) / nA
+    orth = np.linalg.norm(Q.conj().T @ Q - np.eye(n))
+    orth_o = np.linalg.norm(Qo.conj().T @ Qo - np.eye(n))
+    print(f"{dt.__name__} 8192x1024: |A-QR|/|A| {res:.2e} (LAPACK {res_o:.2e}), "
+          f"|Q^HQ-I| {orth:.2e} (LAPACK {orth_o:.2e}), |R-Rref|/|Rref| {rel(R, Ro):.2e}", flush=True)
+    assert res <= 10 * res_o and orth <= 10 * orth_o
+    assert rel(R, Ro) < 1e-12 and rel(Q, Qo) < 1e-12
