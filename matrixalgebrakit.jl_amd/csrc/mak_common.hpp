@@ -119,10 +119,13 @@ template <typename T> __device__ inline T wave_sum(T v) {
 __device__ inline int lane_id() { return threadIdx.x & (WAVE - 1); }
 
 // ---------------------------------------------------------------------------
-// MFMA wrappers (gfx950).  Fragment layouts for the 16x16 shapes used here:
+// MFMA wrappers (gfx950).  Fragment layouts for the 16x16 shapes used here
+// (measured on gfx950 with a one-wave probe):
 //   A (16 x K): lane l holds row (l % 16), k-group (l / 16)
 //   B (K x 16): lane l holds col (l % 16), k-group (l / 16)
-//   D (16 x 16): lane l, register r holds row 4 * (l / 16) + r, col l % 16
+//   D (16 x 16), f32 / bf16: lane l, register r holds row 4 * (l / 16) + r
+//   D (16 x 16), f64:        lane l, register r holds row 4 * r + (l / 16)
+//   and column l % 16 in both cases.
 // For f64/f32 16x16x4 each k-group is one k; for bf16 16x16x32 each k-group is
 // 8 consecutive k.
 // ---------------------------------------------------------------------------
@@ -139,6 +142,10 @@ __device__ inline v4f mfma_f32(float a, float b, v4f c) {
 __device__ inline v4f mfma_bf16(v8bf a, v8bf b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
+
+// row of result register r held by lane l
+__device__ inline int mfma_f64_row(int lane, int r) { return 4 * r + (lane >> 4); }
+__device__ inline int mfma_f32_row(int lane, int r) { return 4 * (lane >> 4) + r; }
 
 __host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 __host__ __device__ inline int64_t cdiv64(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * WTM + i * 16 + 4 * (lane >> 4) + r;
+        const int row = m0 + wm * WTM + i * 16 + mfma_f64_row(lane, r);
         const int col = n0 + wn * WTN + j * 16 + (lane & 15);
         if (row < g.M && col < g.N) {
           double v = alpha * acc[i][j][r];
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * WTM + i * 16 + 4 * (lane >> 4) + r;
+        const int row = m0 + wm * WTM + i * 16 + mfma_f64_row(lane, r);
         const int col = n0 + wn * WTN + j * 16 + (lane & 15);
         if (row < g.M && col < g.N) {
           zd v = s_mul(alpha, mkz(cr[i][j][r], ci[i][j][r]));

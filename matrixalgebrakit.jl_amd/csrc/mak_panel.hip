@@ -564,7 +564,7 @@ __global__ void __launch_bounds__(NT) tsqr_expand_kernel(TsqrExpandArgs args) {
   for (int j = 0; j < 2; ++j)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int rr = w * 16 + 4 * (lane >> 4) + r;
+      const int rr = w * 16 + mfma_f64_row(lane, r);
       const int col = j * 16 + (lane & 15);
       const int lrow = tile0 + rr;
       const T base = (lrow < nbc && col < nbc) ? Qb[lrow][col] : s_zero(T());
@@ -576,7 +576,7 @@ __global__ void __launch_bounds__(NT) tsqr_expand_kernel(TsqrExpandArgs args) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int rr = w * 16 + 4 * (lane >> 4) + r;
+        const int rr = w * 16 + mfma_f64_row(lane, r);
         const int col = j * 16 + (lane & 15);
         const int lrow = tile0 + rr;
         if (lrow < hnode && col < nbc) Qo[(r0 + lrow) + col * (int64_t)args.ldqo] = q[j][r];
@@ -589,7 +589,7 @@ __global__ void __launch_bounds__(NT) tsqr_expand_kernel(TsqrExpandArgs args) {
   for (int j = 0; j < 2; ++j)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int rr = w * 16 + 4 * (lane >> 4) + r;
+      const int rr = w * 16 + mfma_f64_row(lane, r);
       const int col = j * 16 + (lane & 15);
       lds_put<T>(L, 0, col, rr, q[j][r]);
     }
@@ -599,7 +599,7 @@ __global__ void __launch_bounds__(NT) tsqr_expand_kernel(TsqrExpandArgs args) {
   for (int j = 0; j < 2; ++j)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int rr = w * 16 + 4 * (lane >> 4) + r;
+      const int rr = w * 16 + mfma_f64_row(lane, r);
       const int col = j * 16 + (lane & 15);
       const int lrow = tile0 + rr;
       const int grow = r0 + lrow;

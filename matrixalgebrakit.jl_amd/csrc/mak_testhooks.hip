@@ -1,0 +1,79 @@
+// Internal entry points used only by the test suite to check kernels in
+// isolation (GEMM, one panel node, the blocked factorisation).  They are not
+// part of the public C ABI in include/mak_mi355x.h.
+#include "mak_common.hpp"
+#include "mak_kernels.hpp"
+#include "mak_mi355x_internal.hpp"
+
+extern "C" {
+
+int maktest_gemm(void* stream, int dtype, int opA, int opB, int M, int N, int K, const void* A, int lda,
+                 const void* B, int ldb, void* C, int ldc, double alpha, double beta, int unitLowerA, int ksplit,
+                 int kchunk) {
+  mak::GemmArgs g{};
+  g.M = M; g.N = N; g.K = K;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.alpha_re = alpha; g.beta_re = beta;
+  g.ksplit = ksplit; g.kchunk = kchunk; g.splitStride = (int64_t)ldc * N; g.batch = 1;
+  g.unitLowerA = unitLowerA;
+  if (dtype == MAK_F64) mak::gemm<double>(opA, opB, g, (hipStream_t)stream);
+  else mak::gemm<mak::zd>(opA, opB, g, (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int maktest_geqr2(void* stream, int dtype, int H, int nbc, void* X, int ldx, void* Tout) {
+  const int cap = dtype == MAK_F64 ? mak::geqr2_node_capacity<double>() : mak::geqr2_node_capacity<mak::zd>();
+  if (H > cap || nbc > 32) return -3;
+  mak::Geqr2NodeArgs g{};
+  g.X = X; g.ldx = ldx; g.H = H; g.nodes = 1; g.chunk = 0; g.nbc = nbc;
+  g.Tout = Tout; g.tstride = 32 * 32;
+  if (dtype == MAK_F64) mak::launch_geqr2_node<double>(g, 0, (hipStream_t)stream);
+  else mak::launch_geqr2_node<mak::zd>(g, 0, (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// Blocked factorisation only: A <- compact WY (V below, R above), one 32x32 T
+// per panel written to Tpanels (32 x 32 x npanels); ws from the QR query.
+int maktest_geqrf(void* stream, int dtype, int m, int n, void* A, int lda, void* Tpanels, void* ws) {
+  const int minmn = m < n ? m : n;
+  const int npan = (minmn + 31) / 32;
+  if (dtype == MAK_F64) {
+    mak::geqrf_blocked<double>((double*)A, lda, m, n, (char*)ws, minmn, (hipStream_t)stream);
+    hipMemcpyAsync(Tpanels, ws, sizeof(double) * 32 * 32 * npan, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+  } else {
+    mak::geqrf_blocked<mak::zd>((mak::zd*)A, lda, m, n, (char*)ws, minmn, (hipStream_t)stream);
+    hipMemcpyAsync(Tpanels, ws, sizeof(mak::zd) * 32 * 32 * npan, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+}  // extern "C"
+
+// One wave runs a single MFMA on per-lane operands and writes the 4 result
+// registers of every lane: d[lane * 4 + r].  Used to check fragment layouts.
+__global__ void maktest_mfma_kernel(int which, const double* a, const double* b, double* d) {
+  const int l = threadIdx.x;
+  if (which == 0) {
+    mak::v4d c = {0.0, 0.0, 0.0, 0.0};
+    c = mak::mfma_f64(a[l], b[l], c);
+    for (int r = 0; r < 4; ++r) d[l * 4 + r] = c[r];
+  } else if (which == 1) {
+    mak::v4f c = {0.f, 0.f, 0.f, 0.f};
+    c = mak::mfma_f32((float)a[l], (float)b[l], c);
+    for (int r = 0; r < 4; ++r) d[l * 4 + r] = c[r];
+  } else {
+    mak::v8bf av, bv;
+    for (int e = 0; e < 8; ++e) {
+      av[e] = (__bf16)(float)a[l * 8 + e];
+      bv[e] = (__bf16)(float)b[l * 8 + e];
+    }
+    mak::v4f c = {0.f, 0.f, 0.f, 0.f};
+    c = mak::mfma_bf16(av, bv, c);
+    for (int r = 0; r < 4; ++r) d[l * 4 + r] = c[r];
+  }
+}
+
+extern "C" int maktest_mfma(void* stream, int which, const double* a, const double* b, double* d) {
+  hipLaunchKernelGGL(maktest_mfma_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, which, a, b, d);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
