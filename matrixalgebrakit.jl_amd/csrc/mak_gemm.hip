@@ -44,25 +44,85 @@ __device__ inline zd load_b(const zd* B, int64_t ldb, int gk, int gj) {
   return v;
 }
 
+// Tile order: linear workgroup ids are dealt round-robin to the 8 XCDs; regroup
+// them so every XCD walks a contiguous range of tiles (shared A / B panels stay
+// in that XCD's L2).
+__device__ inline void tile_coords(int& tm, int& tn) {
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int total = gx * gy;
+  int L = blockIdx.x + blockIdx.y * gx;
+  if ((total & 7) == 0) L = (L & 7) * (total >> 3) + (L >> 3);
+  tm = L % gx;
+  tn = L / gx;
+}
+
+// Real FP64 GEMM.  Each thread owns fixed load coordinates; the next K-tile is
+// fetched into registers while the MFMAs of the current tile run.
 template <int BM, int BN, int BK, int WM, int WN, int OPA, int OPB>
-__global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
+__global__ void __launch_bounds__(256, 2) gemm_f64_kernel(GemmArgs g) {
   constexpr int WTM = BM / WM, WTN = BN / WN, TM = WTM / 16, TN = WTN / 16;
+  constexpr int NA = BM * BK / 256, NB_ = BN * BK / 256;
   static_assert(WM * WN == 4, "4 waves");
+  static_assert(NA * 256 == BM * BK && NB_ * 256 == BN * BK, "tile / thread count");
   __shared__ double As[BK][BM + 2];
   __shared__ double Bs[BK][BN + 2];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = tid >> 6;
   const int wm = w % WM, wn = w / WM;
+  int tmi, tni;
+  tile_coords(tmi, tni);
   const int bz = blockIdx.z;
   const int bat = bz / g.ksplit;
   const int ks = bz % g.ksplit;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int m0 = tmi * BM, n0 = tni * BN;
   const int kbeg = ks * g.kchunk;
   const int kend = min(g.K, kbeg + g.kchunk);
   const double* A = reinterpret_cast<const double*>(g.A) + (int64_t)bat * g.strideA;
   const double* B = reinterpret_cast<const double*>(g.B) + (int64_t)bat * g.strideB;
   double* C = reinterpret_cast<double*>(g.C) + (int64_t)bat * g.strideC + (int64_t)ks * g.splitStride;
+  const int64_t lda = g.lda, ldb = g.ldb;
+
+  // load coordinates: A element u of this thread is (ia[u], ka[u]) of the tile
+  // OP_N: i fixed, k = k_a0 + u * (256 / BM); OP_T/C: k fixed, i = i_a0 + u * (256 / BK)
+  const int ia0 = (OPA == OP_N) ? (tid % BM) : (tid / BK);
+  const int ka0 = (OPA == OP_N) ? (tid / BM) : (tid % BK);
+  constexpr int dA = (OPA == OP_N) ? (256 / BM) : (256 / BK);
+  const int jb0 = (OPB == OP_N) ? (tid / BK) : (tid % BN);
+  const int kb0 = (OPB == OP_N) ? (tid % BK) : (tid / BN);
+  constexpr int dB = (OPB == OP_N) ? (256 / BK) : (256 / BN);
+
+  double pa[NA], pb[NB_];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int i = (OPA == OP_N) ? ia0 : ia0 + u * dA;
+      const int k = (OPA == OP_N) ? ka0 + u * dA : ka0;
+      const int gi = m0 + i, gk = k0 + k;
+      pa[u] = (gi < g.M && gk < kend) ? load_a<OPA>(A, lda, gi, gk, g.unitLowerA) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < NB_; ++u) {
+      const int j = (OPB == OP_N) ? jb0 + u * dB : jb0;
+      const int k = (OPB == OP_N) ? kb0 : kb0 + u * dB;
+      const int gj = n0 + j, gk = k0 + k;
+      pb[u] = (gj < g.N && gk < kend) ? load_b<OPB>(B, ldb, gk, gj) : 0.0;
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int i = (OPA == OP_N) ? ia0 : ia0 + u * dA;
+      const int k = (OPA == OP_N) ? ka0 + u * dA : ka0;
+      As[k][i] = pa[u];
+    }
+#pragma unroll
+    for (int u = 0; u < NB_; ++u) {
+      const int j = (OPB == OP_N) ? jb0 + u * dB : jb0;
+      const int k = (OPB == OP_N) ? kb0 : kb0 + u * dB;
+      Bs[k][j] = pb[u];
+    }
+  };
 
   v4d acc[TM][TN];
 #pragma unroll
@@ -70,24 +130,11 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
+  if (kbeg < kend) fetch(kbeg);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-#pragma unroll 4
-    for (int e = tid; e < BM * BK; e += 256) {
-      int i, k;
-      if (OPA == OP_N) { i = e % BM; k = e / BM; }
-      else { k = e % BK; i = e / BK; }
-      const int gi = m0 + i, gk = k0 + k;
-      As[k][i] = (gi < g.M && gk < kend) ? load_a<OPA>(A, g.lda, gi, gk, g.unitLowerA) : 0.0;
-    }
-#pragma unroll 4
-    for (int e = tid; e < BN * BK; e += 256) {
-      int j, k;
-      if (OPB == OP_N) { k = e % BK; j = e / BK; }
-      else { j = e % BN; k = e / BN; }
-      const int gj = n0 + j, gk = k0 + k;
-      Bs[k][j] = (gj < g.N && gk < kend) ? load_b<OPB>(B, g.ldb, gk, gj) : 0.0;
-    }
+    stash();
     __syncthreads();
+    if (k0 + BK < kend) fetch(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       double af[TM], bf[TN];
@@ -95,29 +142,48 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
       for (int i = 0; i < TM; ++i) af[i] = As[kk + (lane >> 4)][wm * WTM + i * 16 + (lane & 15)];
 #pragma unroll
       for (int j = 0; j < TN; ++j) bf[j] = Bs[kk + (lane >> 4)][wn * WTN + j * 16 + (lane & 15)];
+      // operands swapped: the MFMA computes the transposed block, so lane l
+      // holds rows (l % 16) of C and stores are 128-byte column segments
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j) acc[i][j] = mfma_f64(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < TN; ++j) acc[i][j] = mfma_f64(bf[j], af[i], acc[i][j]);
     }
     __syncthreads();
   }
   const double alpha = g.alpha_re, beta = g.beta_re;
+  const int rowb = m0 + wm * WTM + (lane & 15);
+  const int colb = n0 + wn * WTN;
+  if (beta != 0.0) {
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+    for (int i = 0; i < TM; ++i) {
+      double cv[TN][4];
 #pragma unroll
-    for (int j = 0; j < TN; ++j)
+      for (int j = 0; j < TN; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * WTM + i * 16 + mfma_f64_row(lane, r);
-        const int col = n0 + wn * WTN + j * 16 + (lane & 15);
-        if (row < g.M && col < g.N) {
-          double v = alpha * acc[i][j][r];
-          double* cp = C + row + (int64_t)col * g.ldc;
-          if (beta != 0.0) v = fma(beta, *cp, v);
-          *cp = v;
+        for (int r = 0; r < 4; ++r) {
+          const int row = rowb + i * 16, col = colb + j * 16 + mfma_f64_row(lane, r);
+          cv[j][r] = (row < g.M && col < g.N) ? C[row + (int64_t)col * g.ldc] : 0.0;
         }
-      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rowb + i * 16, col = colb + j * 16 + mfma_f64_row(lane, r);
+          if (row < g.M && col < g.N) C[row + (int64_t)col * g.ldc] = fma(beta, cv[j][r], alpha * acc[i][j][r]);
+        }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rowb + i * 16, col = colb + j * 16 + mfma_f64_row(lane, r);
+          if (row < g.M && col < g.N) C[row + (int64_t)col * g.ldc] = alpha * acc[i][j][r];
+        }
+  }
 }
 
 template <int BM, int BN, int BK, int WM, int WN, int OPA, int OPB>
@@ -130,10 +196,12 @@ __global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
   const int lane = tid & 63;
   const int w = tid >> 6;
   const int wm = w % WM, wn = w / WM;
+  int tmi, tni;
+  tile_coords(tmi, tni);
   const int bz = blockIdx.z;
   const int bat = bz / g.ksplit;
   const int ks = bz % g.ksplit;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int m0 = tmi * BM, n0 = tni * BN;
   const int kbeg = ks * g.kchunk;
   const int kend = min(g.K, kbeg + g.kchunk);
   const zd* A = reinterpret_cast<const zd*>(g.A) + (int64_t)bat * g.strideA;
@@ -187,10 +255,10 @@ __global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          cr[i][j] = mfma_f64(ar[i], br[j], cr[i][j]);
-          cr[i][j] = mfma_f64(nai[i], bi[j], cr[i][j]);
-          ci[i][j] = mfma_f64(ar[i], bi[j], ci[i][j]);
-          ci[i][j] = mfma_f64(ai[i], br[j], ci[i][j]);
+          cr[i][j] = mfma_f64(br[j], ar[i], cr[i][j]);
+          cr[i][j] = mfma_f64(bi[j], nai[i], cr[i][j]);
+          ci[i][j] = mfma_f64(bi[j], ar[i], ci[i][j]);
+          ci[i][j] = mfma_f64(br[j], ai[i], ci[i][j]);
         }
     }
     __syncthreads();
@@ -203,8 +271,8 @@ __global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * WTM + i * 16 + mfma_f64_row(lane, r);
-        const int col = n0 + wn * WTN + j * 16 + (lane & 15);
+        const int row = m0 + wm * WTM + i * 16 + (lane & 15);
+        const int col = n0 + wn * WTN + j * 16 + mfma_f64_row(lane, r);
         if (row < g.M && col < g.N) {
           zd v = s_mul(alpha, mkz(cr[i][j][r], ci[i][j][r]));
           zd* cp = C + row + (int64_t)col * g.ldc;
@@ -269,7 +337,7 @@ template void gemm<zd>(int, int, GemmArgs, hipStream_t);
 
 void splitk_plan(int N, int K, int bn, int& ksplit, int& kchunk) {
   const int tiles = cdiv(N, bn);
-  int ks = cdiv(512, tiles);
+  int ks = cdiv(256, tiles);
   ks = std::min(ks, std::max(1, K / 256));
   ks = std::max(ks, 1);
   kchunk = cdiv(cdiv(K, ks), 16) * 16;
@@ -283,32 +351,45 @@ template <typename T>
 __global__ void __launch_bounds__(256) wy_reduce_kernel(const T* __restrict__ Wp, int ksplit, int64_t slice,
                                                         const T* __restrict__ Tm, int transT, int nbc,
                                                         int ncols, T* __restrict__ W2) {
-  __shared__ T S[32][65];
+  // 16 columns per workgroup; thread (r, cc) sums the ksplit partials of
+  // two outputs with four independent accumulators each.
+  __shared__ T S[32][17];
   __shared__ T Tl[32][33];
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * 64;
+  const int c0 = blockIdx.x * 16;
   for (int i = tid; i < 32 * 32; i += 256) {
     const int r = i % 32, c = i / 32;
     T v = s_zero(T());
     if (r < nbc && c < nbc) v = transT ? s_conj(Tm[c + r * 32]) : Tm[r + c * 32];
     Tl[r][c] = v;
   }
-  for (int i = tid; i < 32 * 64; i += 256) {
-    const int r = i % 32, c = i / 32;
-    T s = s_zero(T());
-    if (r < nbc && c0 + c < ncols) {
-      const T* p = Wp + r + (int64_t)(c0 + c) * nbc;
-      for (int z = 0; z < ksplit; ++z) s = s_add(s, p[(int64_t)z * slice]);
+  const int r = tid % 32;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int cc = tid / 32 + 8 * q;
+    const int c = c0 + cc;
+    T acc[4] = {s_zero(T()), s_zero(T()), s_zero(T()), s_zero(T())};
+    if (r < nbc && c < ncols) {
+      const T* p = Wp + r + (int64_t)c * nbc;
+      int z = 0;
+      for (; z + 4 <= ksplit; z += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = s_add(acc[u], p[(int64_t)(z + u) * slice]);
+      }
+      for (; z < ksplit; ++z) acc[0] = s_add(acc[0], p[(int64_t)z * slice]);
     }
-    S[r][c] = s;
+    S[r][cc] = s_add(s_add(acc[0], acc[1]), s_add(acc[2], acc[3]));
   }
   __syncthreads();
-  for (int i = tid; i < 32 * 64; i += 256) {
-    const int r = i % 32, c = i / 32;
-    if (r < nbc && c0 + c < ncols) {
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int cc = tid / 32 + 8 * q;
+    const int c = c0 + cc;
+    if (r < nbc && c < ncols) {
       T s = s_zero(T());
-      for (int k = 0; k < nbc; ++k) s = s_fma(Tl[r][k], S[k][c], s);
-      W2[r + (int64_t)(c0 + c) * nbc] = s;
+#pragma unroll 8
+      for (int k = 0; k < 32; ++k) s = s_fma(Tl[r][k], S[k][cc], s);
+      W2[r + (int64_t)c * nbc] = s;
     }
   }
 }
@@ -317,7 +398,7 @@ template <typename T>
 void wy_reduce(const void* Wp, int ksplit, int64_t slice, const void* Tm, int transT, int nbc, int ncols,
                void* W2, hipStream_t s) {
   if (ncols <= 0) return;
-  hipLaunchKernelGGL((wy_reduce_kernel<T>), dim3(cdiv(ncols, 64)), dim3(256), 0, s,
+  hipLaunchKernelGGL((wy_reduce_kernel<T>), dim3(cdiv(ncols, 16)), dim3(256), 0, s,
                      reinterpret_cast<const T*>(Wp), ksplit, slice, reinterpret_cast<const T*>(Tm), transT,
                      nbc, ncols, reinterpret_cast<T*>(W2));
 }

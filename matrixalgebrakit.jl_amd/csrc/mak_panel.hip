@@ -43,11 +43,40 @@ template <int NB> __device__ inline int col_of_lane(int l) {
   return c;
 }
 
+// Halving step across lane bit 5 (mask 32) / bit 4 (mask 16) with the gfx950
+// lane-swap instructions: after v_permlane32_swap(x, y) the lower half-wave
+// holds (x_l, x_{l+32}) and the upper half-wave (y_{l-32}, y_l), so x + y is
+// the pair sum of column k (lower lanes) and column k + half (upper lanes).
+__device__ inline double swap_add32(double x, double y) {
+  const uint64_t xb = __builtin_bit_cast(uint64_t, x), yb = __builtin_bit_cast(uint64_t, y);
+  auto lo = __builtin_amdgcn_permlane32_swap((unsigned)xb, (unsigned)yb, false, false);
+  auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(xb >> 32), (unsigned)(yb >> 32), false, false);
+  const double xs = __builtin_bit_cast(double, ((uint64_t)hi[0] << 32) | lo[0]);
+  const double ys = __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
+  return xs + ys;
+}
+__device__ inline double swap_add16(double x, double y) {
+  const uint64_t xb = __builtin_bit_cast(uint64_t, x), yb = __builtin_bit_cast(uint64_t, y);
+  auto lo = __builtin_amdgcn_permlane16_swap((unsigned)xb, (unsigned)yb, false, false);
+  auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(xb >> 32), (unsigned)(yb >> 32), false, false);
+  const double xs = __builtin_bit_cast(double, ((uint64_t)hi[0] << 32) | lo[0]);
+  const double ys = __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
+  return xs + ys;
+}
+__device__ inline zd swap_add32(zd x, zd y) { return mkz(swap_add32(x.x, y.x), swap_add32(x.y, y.y)); }
+__device__ inline zd swap_add16(zd x, zd y) { return mkz(swap_add16(x.x, y.x), swap_add16(x.y, y.y)); }
+
 template <typename T, int NB> __device__ inline T wave_reduce_transposed(T (&p)[NB]) {
   const int l = lane_id();
-  // halving steps
+  static_assert(NB == 32, "reduction levels are laid out for 32 slots");
+  // masks 32 and 16: lane-swap instructions, no selects
 #pragma unroll
-  for (int len = NB, m = 32; len > 1; len >>= 1, m >>= 1) {
+  for (int k = 0; k < 16; ++k) p[k] = swap_add32(p[k], p[k + 16]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) p[k] = swap_add16(p[k], p[k + 8]);
+  // masks 8, 4, 2: shuffles with value selects
+#pragma unroll
+  for (int len = 8, m = 8; len > 1; len >>= 1, m >>= 1) {
     const bool up = (l & m) != 0;
 #pragma unroll
     for (int k = 0; k < len / 2; ++k) {
@@ -58,11 +87,8 @@ template <typename T, int NB> __device__ inline T wave_reduce_transposed(T (&p)[
       p[k] = s_add(keep, shfl_xor_s(send, m));
     }
   }
-  T v = p[0];
-  // remaining full-reduction steps over the lanes that share a column
-#pragma unroll
-  for (int m = 32 / NB; m >= 1; m >>= 1) v = s_add(v, shfl_xor_s(v, m));
-  return v;
+  // mask 1: the two lanes that share a column
+  return s_add(p[0], shfl_xor_s(p[0], 1));
 }
 
 // Rows of node `i` out of `nodes` over `H` stacked rows: fixed chunks when
@@ -77,18 +103,121 @@ __device__ __host__ inline void node_rows(int i, int nodes, int H, int chunk, in
   }
 }
 
+// Make LDS writes of this wave visible to its own later LDS reads (LDS ops of
+// one wave execute in order; this only stops the compiler from reordering).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Shared state of geqr2_node_kernel.
+template <typename T, int NB, int NW> struct NodeLds {
+  T red_w[2][NW][NB];
+  T rowj[2][NB];
+  T gsh[NW][NB];
+  T Ysh[NB][NB + 1];   // Ysh[j][c] = V_c^H v_j for c < j
+  T tau_s[NB];
+};
+
+// One column step j of the node factorisation, with the current column held
+// in register slot OFF (slot i holds column (i + j - OFF) mod NB).
+template <typename T, int NB, int RPT, int NT, int OFF>
+__device__ inline void node_step(T (&a)[RPT][NB], NodeLds<T, NB, NT / WAVE>& S, int j, int h, int nbc) {
+  constexpr int NW = NT / WAVE;
+  const int tid = threadIdx.x;
+  const int wid = tid / WAVE;
+  const int l = lane_id();
+  const int par = j & 1;
+  // partial sums over rows r > j of conj(a_rj) a_r(slot i)
+  T p[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) p[i] = s_zero(T());
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = tid + NT * q;
+    const T x = (r > j && r < h) ? a[q][OFF] : s_zero(T());
+#pragma unroll
+    for (int i = 0; i < NB; ++i) p[i] = s_cfma(x, a[q][i], p[i]);
+  }
+  const T tot = wave_reduce_transposed<T, NB>(p);
+  if ((l & (64 / NB - 1)) == 0) S.red_w[par][wid][col_of_lane<NB>(l)] = tot;
+  __syncthreads();
+
+  // every wave: lane i < NB finishes slot i
+  T ri = s_zero(T());
+  double nu2 = 0.0;
+  const int is = l & (NB - 1);
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    ri = s_add(ri, S.red_w[par][w][is]);
+    nu2 += s_re(S.red_w[par][w][OFF]);
+  }
+  const T alpha = S.rowj[par][OFF];
+  const T rji = S.rowj[par][is];
+  T tau, beta, scale;
+  if (nu2 == 0.0 && s_im(alpha) == 0.0) {
+    tau = s_zero(T());
+    beta = alpha;
+    scale = s_zero(T());
+  } else {
+    const double bn = sqrt(s_abs2(alpha) + nu2);
+    const double br = s_re(alpha) >= 0.0 ? -bn : bn;
+    beta = s_from_real<T>(br);
+    tau = s_scale(s_sub(beta, alpha), 1.0 / br);
+    scale = s_div(s_one(T()), s_sub(alpha, beta));
+  }
+  {
+    int c = is + j - OFF;  // column held by slot is
+    if (c >= NB) c -= NB;
+    if (c < 0) c += NB;
+    T g = s_zero(T());
+    if (c > j && c < nbc) g = s_mul(s_conj(tau), s_cfma(scale, ri, rji));   // conj(tau) v^H a_c
+    if (l < NB) S.gsh[wid][is] = g;
+    if (wid == 0 && l < NB && c < j) S.Ysh[j][c] = s_fma(scale, s_conj(ri), s_conj(rji));  // V_c^H v_j
+    if (tid == 0) S.tau_s[j] = tau;
+  }
+  wave_lds_sync();
+  T g[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) g[i] = (i == OFF) ? s_zero(T()) : S.gsh[wid][i];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = tid + NT * q;
+    const bool below = (r > j && r < h);
+    const bool onrow = (r == j);
+    const T v = s_mul(scale, a[q][OFF]);
+    const T coef = below ? v : (onrow ? s_one(T()) : s_zero(T()));
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+      if (i != OFF) a[q][i] = s_sub(a[q][i], s_mul(coef, g[i]));
+    a[q][OFF] = below ? v : (onrow ? beta : a[q][OFF]);
+  }
+}
+
+template <typename T, int NB, int RPT, int NT>
+__device__ inline void publish_row(const T (&a)[RPT][NB], NodeLds<T, NB, NT / WAVE>& S, int row, int kmax) {
+  // the owner of `row` (thread row, register row 0) publishes it for step `row`
+  if (threadIdx.x == row && row < kmax) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) S.rowj[row & 1][i] = a[0][i];
+  }
+}
+
 template <typename T, int NB, int RPT, int NT>
 __global__ void __launch_bounds__(NT) geqr2_node_kernel(Geqr2NodeArgs args) {
-  // Register layout: thread t holds rows t + NT*q (q < RPT).  The row is kept
-  // ROTATED: at step j, slot i holds column (i + j) mod NB, so the current
-  // column is always slot 0 and no register is indexed by the runtime j.
+  // Register layout: thread t holds rows t + NT*q (q < RPT).  Rows are kept
+  // ROTATED in registers: steps go in pairs with the current column in slot 0
+  // then slot 1, and the row is rotated by two slots after each pair, so no
+  // register is indexed by the runtime column.
+  //
+  // One workgroup barrier per column: the cross-wave partials and row j are
+  // double-buffered by step parity, and every wave finishes the reduction for
+  // itself (lane i owns slot i), broadcasting the update coefficients through
+  // a wave-private LDS row.  T is formed after the loop from the stored
+  // products Y(c, j) = V_c^H v_j (LAPACK xLARFT recurrence).
   constexpr int NW = NT / WAVE;
-  __shared__ T red_w[NW][NB];
-  __shared__ T rowj[NB];   // row j, rotated order
-  __shared__ T gvec[NB];   // conj(tau) * v^H a_c, rotated order (0 unless trailing)
-  __shared__ T yvec[NB];   // V_c^H v_j, natural order
-  __shared__ T scal[3];    // scale, tau, beta
-  __shared__ T Tl[NB][NB + 1];
+  __shared__ NodeLds<T, NB, NW> S;
 
   const int node = blockIdx.x;
   int r0, r1;
@@ -109,126 +238,66 @@ __global__ void __launch_bounds__(NT) geqr2_node_kernel(Geqr2NodeArgs args) {
     for (int c = 0; c < NB; ++c)
       a[q][c] = (r < h && c < nbc) ? X[r + c * ldx] : s_zero(T());
   }
-  for (int i = tid; i < NB * (NB + 1); i += NT) (&Tl[0][0])[i] = s_zero(T());
-  __syncthreads();
-
+  for (int i = tid; i < NB * (NB + 1); i += NT) (&S.Ysh[0][0])[i] = s_zero(T());
+  if (tid < NB) S.tau_s[tid] = s_zero(T());
   const int kmax = min(h, nbc);
-  for (int j = 0; j < kmax; ++j) {
-    // partial sums t_i = sum_{rows r > j} conj(a_rj) a_r(slot i)
-    T p[NB];
-#pragma unroll
-    for (int i = 0; i < NB; ++i) p[i] = s_zero(T());
+  publish_row<T, NB, RPT, NT>(a, S, 0, kmax);
+
+  for (int j = 0; j < kmax; j += 2) {
+    node_step<T, NB, RPT, NT, 0>(a, S, j, h, nbc);
+    publish_row<T, NB, RPT, NT>(a, S, j + 1, kmax);
+    if (j + 1 < kmax) node_step<T, NB, RPT, NT, 1>(a, S, j + 1, h, nbc);
+    // rotate by two slots: slot i <- slot i+2
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-      const int r = tid + NT * q;
-      const T x = (r > j && r < h) ? a[q][0] : s_zero(T());
+      const T f0 = a[q][0], f1 = a[q][1];
 #pragma unroll
-      for (int i = 0; i < NB; ++i) p[i] = s_cfma(x, a[q][i], p[i]);
+      for (int i = 0; i < NB - 2; ++i) a[q][i] = a[q][i + 2];
+      a[q][NB - 2] = f0;
+      a[q][NB - 1] = f1;
     }
-    T tot = wave_reduce_transposed<T, NB>(p);
-    if ((l & (64 / NB - 1)) == 0) red_w[wid][col_of_lane<NB>(l)] = tot;
-    if (tid == j) {
-#pragma unroll
-      for (int i = 0; i < NB; ++i) rowj[i] = a[0][i];
-    }
-    __syncthreads();
-    if (tid < NB) {
-      const int i = tid;             // slot
-      int c = i + j;                 // column held by slot i
-      if (c >= NB) c -= NB;
-      T ri = s_zero(T());
-      double nu2 = 0.0;
-#pragma unroll
-      for (int w = 0; w < NW; ++w) {
-        ri = s_add(ri, red_w[w][i]);
-        nu2 += s_re(red_w[w][0]);
-      }
-      const T alpha = rowj[0];
-      T tau, beta, scale;
-      if (nu2 == 0.0 && s_im(alpha) == 0.0) {
-        tau = s_zero(T());
-        beta = alpha;
-        scale = s_zero(T());
-      } else {
-        const double bn = sqrt(s_abs2(alpha) + nu2);
-        const double br = s_re(alpha) >= 0.0 ? -bn : bn;
-        beta = s_from_real<T>(br);
-        tau = s_scale(s_sub(beta, alpha), 1.0 / br);
-        scale = s_div(s_one(T()), s_sub(alpha, beta));
-      }
-      T g = s_zero(T());
-      if (c > j && c < nbc) {
-        // w_c = a_jc + conj(scale) red ; g = conj(tau) w_c
-        g = s_mul(s_conj(tau), s_cfma(scale, ri, rowj[i]));
-      } else if (c < j) {
-        // y_c = conj(a_jc) + scale conj(red)
-        yvec[c] = s_fma(scale, s_conj(ri), s_conj(rowj[i]));
-      }
-      gvec[i] = g;
-      if (i == 0) {
-        scal[0] = scale;
-        scal[1] = tau;
-        scal[2] = beta;
-      }
-    }
-    __syncthreads();
-    {
-      const T scale = scal[0];
-      const T tau = scal[1];
-      const T beta = scal[2];
-      T g[NB];
-#pragma unroll
-      for (int i = 1; i < NB; ++i) g[i] = gvec[i];
-#pragma unroll
-      for (int q = 0; q < RPT; ++q) {
-        const int r = tid + NT * q;
-        const bool below = (r > j && r < h);
-        const bool onrow = (r == j);
-        const T v = s_mul(scale, a[q][0]);
-        const T coef = below ? v : (onrow ? s_one(T()) : s_zero(T()));
-#pragma unroll
-        for (int i = 1; i < NB; ++i) a[q][i] = s_sub(a[q][i], s_mul(coef, g[i]));
-        a[q][0] = below ? v : (onrow ? beta : a[q][0]);
-      }
-      // T(0:j, j) = -tau T(0:j, 0:j) y ; T(j, j) = tau
-      if (tid < j) {
-        T s = s_zero(T());
-        for (int k = tid; k < j; ++k) s = s_fma(Tl[tid][k], yvec[k], s);
-        Tl[tid][j] = s_neg(s_mul(tau, s));
-      } else if (tid == j) {
-        Tl[j][j] = tau;
-      }
-      // rotate: slot i <- slot i+1
-#pragma unroll
-      for (int q = 0; q < RPT; ++q) {
-        const T first = a[q][0];
-#pragma unroll
-        for (int i = 0; i < NB - 1; ++i) a[q][i] = a[q][i + 1];
-        a[q][NB - 1] = first;
-      }
-    }
-    __syncthreads();
+    publish_row<T, NB, RPT, NT>(a, S, j + 2, kmax);
   }
+  __syncthreads();
 
-  // slot i now holds column (i + kmax) mod NB
-  // write back the factored block (V strictly below, R on and above)
+  // slot i now holds column (i + krot) mod NB
+  const int krot = 2 * ((kmax + 1) / 2);
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     const int r = tid + NT * q;
     if (r < h) {
 #pragma unroll
       for (int i = 0; i < NB; ++i) {
-        int c = i + kmax;
-        if (c >= NB) c -= NB;
+        int c = (i + krot) % NB;
         if (c < nbc) X[r + c * ldx] = a[q][i];
       }
     }
   }
-  if (args.Tout) {
+  if (args.Tout && wid == 0) {
+    // T(0:j, j) = -tau_j T(0:j, 0:j) Y(0:j, j); T(j, j) = tau_j.  Lane r keeps
+    // row r of T in registers.
+    const int r = l;
+    T Treg[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) Treg[k] = s_zero(T());
+    for (int j = 0; j < kmax; ++j) {
+      T acc[4] = {s_zero(T()), s_zero(T()), s_zero(T()), s_zero(T())};
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        const T y = S.Ysh[j][k];
+        const T t = s_mul(Treg[k], y);
+        acc[k & 3] = s_add(acc[k & 3], (k >= r && k < j) ? t : s_zero(T()));
+      }
+      const T tj = S.tau_s[j];
+      const T val = (r < j) ? s_neg(s_mul(tj, s_add(s_add(acc[0], acc[1]), s_add(acc[2], acc[3]))))
+                            : ((r == j) ? tj : s_zero(T()));
+#pragma unroll
+      for (int k = 0; k < NB; ++k) Treg[k] = (k == j) ? val : Treg[k];
+    }
     T* To = reinterpret_cast<T*>(args.Tout) + (int64_t)node * args.tstride;
-    for (int i = tid; i < NB * NB; i += NT) {
-      const int r = i % NB, c = i / NB;
-      To[r + c * NB] = (r <= c) ? Tl[r][c] : s_zero(T());
+    if (r < NB) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) To[r + c * NB] = (r <= c) ? Treg[c] : s_zero(T());
     }
   }
   if (args.Rnext) {
@@ -238,8 +307,7 @@ __global__ void __launch_bounds__(NT) geqr2_node_kernel(Geqr2NodeArgs args) {
     if (r < nbc) {
 #pragma unroll
       for (int i = 0; i < NB; ++i) {
-        int c = i + kmax;
-        if (c >= NB) c -= NB;
+        const int c = (i + krot) % NB;
         if (c < nbc) Rn[r + (int64_t)c * args.ldr] = (r <= c && r < h) ? a[0][i] : s_zero(T());
       }
     }
@@ -250,12 +318,53 @@ __global__ void __launch_bounds__(NT) geqr2_node_kernel(Geqr2NodeArgs args) {
     if (r < NB) {
 #pragma unroll
       for (int i = 0; i < NB; ++i) {
-        int c = i + kmax;
-        if (c >= NB) c -= NB;
+        const int c = (i + krot) % NB;
         V1[r + c * NB] = (r > c && r < h && c < nbc) ? a[0][i] : (r == c ? s_one(T()) : s_zero(T()));
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// 32x32x32 products on FP64 MFMA for one 256-thread workgroup:
+//   C = alpha op(A) op(B) (+ C when acc), op = identity or conjugate transpose.
+// Wave w computes the 16x16 block (w % 2, w / 2).  The result is written after
+// a workgroup barrier, so C may alias A or B.  All 256 threads must call it.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline void mm32(T (*C)[33], const T (*A)[33], bool cA, const T (*B)[33], bool cB, double alpha,
+                            bool acc) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = tid >> 6;
+  const int bi = w & 1, bj = w >> 1;
+  v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kk = 0; kk < 32; kk += 4) {
+    const int k = kk + (lane >> 4);
+    const int i = bi * 16 + (lane & 15);
+    const int j = bj * 16 + (lane & 15);
+    const T a = cA ? s_conj(A[k][i]) : A[i][k];
+    const T b = cB ? s_conj(B[j][k]) : B[k][j];
+    // transposed block: lane l holds row (l % 16) of the output
+    cr = mfma_f64(s_re(b), s_re(a), cr);
+    if constexpr (ScalarTraits<T>::is_complex) {
+      cr = mfma_f64(-s_im(b), s_im(a), cr);
+      ci = mfma_f64(s_im(b), s_re(a), ci);
+      ci = mfma_f64(s_re(b), s_im(a), ci);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = bi * 16 + (lane & 15);
+    const int col = bj * 16 + mfma_f64_row(lane, r);
+    T v;
+    if constexpr (ScalarTraits<T>::is_complex) v = mkz(alpha * cr[r], alpha * ci[r]);
+    else v = alpha * cr[r];
+    C[row][col] = acc ? s_add(C[row][col], v) : v;
+  }
+  __syncthreads();
 }
 
 // Element (r, c) of a unit-lower-trapezoidal V stored strictly below the
@@ -272,8 +381,12 @@ __device__ inline T vload(const T* V, int64_t ldv, int r, int c) {
 //          every lower level),
 //   LU of Qt - D with d_j = -sign(pivot_j)  ->  U (stored as U^{-1}), L1, D,
 //   T = -U D^H L1^{-H},  R = D R_top,  panel top rows <- [R ; L1].
+// Every operand is staged in LDS first; triangular inverses run as one
+// rank-1 update per step over all (row, col) pairs.
 template <typename T, int NB, int NT>
 __global__ void __launch_bounds__(NT) tsqr_root_finish_kernel(TsqrRootArgs args) {
+  __shared__ T Vt[NB][NB + 1];   // top node's V1 (unit lower) ; R_top above
+  __shared__ T Rt[NB][NB + 1];
   __shared__ T M[NB][NB + 1];
   __shared__ T Qt[NB][NB + 1];
   __shared__ T W[NB][NB + 1];
@@ -282,117 +395,93 @@ __global__ void __launch_bounds__(NT) tsqr_root_finish_kernel(TsqrRootArgs args)
   __shared__ T dvec[NB];
   const int tid = threadIdx.x;
   const int nbc = args.nbc;
-  const T* Vt = reinterpret_cast<const T*>(args.Xtop);
+  const T* Vg = reinterpret_cast<const T*>(args.Xtop);
   const int64_t ldt = args.ldxtop;
+  const T* Tt = reinterpret_cast<const T*>(args.Ttop);
 
+  for (int i = tid; i < NB * NB; i += NT) {
+    const int r = i % NB, c = i / NB;
+    const bool in = r < nbc && c < nbc;
+    Vt[r][c] = in ? vload(Vg, ldt, r, c) : s_zero(T());
+    Rt[r][c] = (in && r <= c) ? Vg[r + c * ldt] : s_zero(T());
+    Tn[r][c] = in ? Tt[r + c * NB] : s_zero(T());
+  }
+  __syncthreads();
+  // M = Tn Vt^H ; Qt = I - Vt M
+  mm32<T>(M, Tn, false, Vt, true, 1.0, false);
   {
-    const T* Tt = reinterpret_cast<const T*>(args.Ttop);
-    for (int i = tid; i < NB * NB; i += NT) {
-      const int r = i % NB, c = i / NB;
-      Tn[r][c] = (r < nbc && c < nbc) ? Tt[r + c * NB] : s_zero(T());
-    }
-    __syncthreads();
     T* Mo = reinterpret_cast<T*>(args.Mtop);
     for (int i = tid; i < NB * NB; i += NT) {
       const int r = i % NB, c = i / NB;
-      T s = s_zero(T());
-      if (r < nbc && c < nbc) {
-        for (int k = r; k < nbc; ++k) s = s_fma(Tn[r][k], s_conj(vload(Vt, ldt, c, k)), s);
-      }
-      M[r][c] = s;
-      Mo[r + c * NB] = s;
+      Mo[r + c * NB] = M[r][c];
+      Qt[r][c] = (r == c) ? s_one(T()) : s_zero(T());
     }
-    __syncthreads();
-    for (int i = tid; i < NB * NB; i += NT) {
-      const int r = i % NB, c = i / NB;
-      T s = (r == c) ? s_one(T()) : s_zero(T());
-      if (r < nbc && c < nbc) {
-        for (int k = 0; k <= r; ++k) s = s_sub(s, s_mul(vload(Vt, ldt, r, k), M[k][c]));
-      }
-      Qt[r][c] = s;
-    }
-    __syncthreads();
   }
+  __syncthreads();
+  mm32<T>(Qt, Vt, false, M, false, -1.0, true);
   // walk down the first node of each lower level: Qt <- Qt - V1 (Tn (V1^H Qt))
   for (int lev = args.nchain - 1; lev >= 0; --lev) {
-    const T* V1 = reinterpret_cast<const T*>(args.chainV1[lev]);
+    const T* V1g = reinterpret_cast<const T*>(args.chainV1[lev]);
     const int64_t ld1 = args.chainLd[lev];
     const T* Tc = reinterpret_cast<const T*>(args.chainT[lev]);
     for (int i = tid; i < NB * NB; i += NT) {
       const int r = i % NB, c = i / NB;
-      Tn[r][c] = (r < nbc && c < nbc) ? Tc[r + c * NB] : s_zero(T());
-      T s = s_zero(T());
-      if (r < nbc && c < nbc)
-        for (int k = r; k < nbc; ++k) s = s_cfma(vload(V1, ld1, k, r), Qt[k][c], s);
-      W[r][c] = s;
+      const bool in = r < nbc && c < nbc;
+      Vt[r][c] = in ? vload(V1g, ld1, r, c) : s_zero(T());
+      Tn[r][c] = in ? Tc[r + c * NB] : s_zero(T());
     }
     __syncthreads();
-    for (int i = tid; i < NB * NB; i += NT) {
-      const int r = i % NB, c = i / NB;
-      T s = s_zero(T());
-      if (r < nbc && c < nbc)
-        for (int k = r; k < nbc; ++k) s = s_fma(Tn[r][k], W[k][c], s);
-      M[r][c] = s;
-    }
-    __syncthreads();
-    for (int i = tid; i < NB * NB; i += NT) {
-      const int r = i % NB, c = i / NB;
-      T s = Qt[r][c];
-      if (r < nbc && c < nbc)
-        for (int k = 0; k <= r; ++k) s = s_sub(s, s_mul(vload(V1, ld1, r, k), M[k][c]));
-      W[r][c] = s;
-    }
-    __syncthreads();
-    for (int i = tid; i < NB * NB; i += NT) {
-      const int r = i % NB, c = i / NB;
-      Qt[r][c] = W[r][c];
-    }
-    __syncthreads();
+    mm32<T>(W, Vt, true, Qt, false, 1.0, false);
+    mm32<T>(M, Tn, false, W, false, 1.0, false);
+    mm32<T>(Qt, Vt, false, M, false, -1.0, true);
   }
   // LU of Qt - D without pivoting, d_j = -sign(pivot_j)
   for (int j = 0; j < nbc; ++j) {
+    const T piv = Qt[j][j];
+    const T d = s_neg(sign_safe(piv));
+    const T pj = s_sub(piv, d);
+    __syncthreads();
     if (tid == 0) {
-      const T piv = Qt[j][j];
-      const T d = s_neg(sign_safe(piv));
       dvec[j] = d;
-      Qt[j][j] = s_sub(piv, d);
+      Qt[j][j] = pj;
     }
+    for (int r = j + 1 + tid; r < nbc; r += NT) Qt[r][j] = s_div(Qt[r][j], pj);
     __syncthreads();
-    for (int r = j + 1 + tid; r < nbc; r += NT) Qt[r][j] = s_div(Qt[r][j], Qt[j][j]);
-    __syncthreads();
-    for (int i = tid; i < nbc * nbc; i += NT) {
-      const int r = i % nbc, c = i / nbc;
-      if (r > j && c > j) Qt[r][c] = s_sub(Qt[r][c], s_mul(Qt[r][j], Qt[j][c]));
+    for (int i = tid; i < NB * NB; i += NT) {
+      const int r = i % NB, c = i / NB;
+      if (r > j && c > j && r < nbc && c < nbc) Qt[r][c] = s_sub(Qt[r][c], s_mul(Qt[r][j], Qt[j][c]));
     }
     __syncthreads();
   }
-  // Li = L1^{-1} (unit lower), one column per thread
-  if (tid < NB) {
-    const int c = tid;
-    for (int r = 0; r < NB; ++r) Li[r][c] = s_zero(T());
-    if (c < nbc) {
-      Li[c][c] = s_one(T());
-      for (int r = c + 1; r < nbc; ++r) {
-        T s = s_zero(T());
-        for (int k = c; k < r; ++k) s = s_fma(Qt[r][k], Li[k][c], s);
-        Li[r][c] = s_neg(s);
-      }
-    }
-  }
-  // W = U^{-1} (upper), one column per thread, back substitution
-  if (tid < NB) {
-    const int c = tid;
-    for (int r = 0; r < NB; ++r) W[r][c] = s_zero(T());
-    if (c < nbc) {
-      W[c][c] = s_div(s_one(T()), Qt[c][c]);
-      for (int r = c - 1; r >= 0; --r) {
-        T s = s_zero(T());
-        for (int k = r + 1; k <= c; ++k) s = s_fma(Qt[r][k], W[k][c], s);
-        W[r][c] = s_neg(s_div(s, Qt[r][r]));
-      }
-    }
+  // Li = L1^{-1} (unit lower) and W = U^{-1} (upper), one rank-1 step per j
+  for (int i = tid; i < NB * NB; i += NT) {
+    const int r = i % NB, c = i / NB;
+    Li[r][c] = (r == c) ? s_one(T()) : s_zero(T());
+    W[r][c] = (r == c) ? (r < nbc ? s_div(s_one(T()), Qt[r][r]) : s_one(T())) : s_zero(T());
   }
   __syncthreads();
+  for (int j = 0; j < nbc; ++j) {
+    // forward: rows r > j of Li:  Li[r][:] -= L[r][j] Li[j][:]
+    // backward (column nbc-1-j of U): rows r < jj of W: W[r][:] -= U[r][jj]/U[r][r] W[jj][:]
+    const int jj = nbc - 1 - j;
+    for (int i = tid; i < NB * NB; i += NT) {
+      const int r = i % NB, c = i / NB;
+      if (r > j && r < nbc && c <= j) Li[r][c] = s_sub(Li[r][c], s_mul(Qt[r][j], Li[j][c]));
+    }
+    for (int i = tid; i < NB * NB; i += NT) {
+      const int r = i % NB, c = i / NB;
+      if (r < jj && c >= jj && c < nbc)
+        W[r][c] = s_sub(W[r][c], s_mul(s_div(Qt[r][jj], Qt[r][r]), W[jj][c]));
+    }
+    __syncthreads();
+  }
+  // T = -(U D^H) Li^H : UD in M (upper), result in Tn
+  for (int i = tid; i < NB * NB; i += NT) {
+    const int r = i % NB, c = i / NB;
+    M[r][c] = (r <= c && c < nbc) ? s_mul(Qt[r][c], s_conj(dvec[c])) : s_zero(T());
+  }
+  __syncthreads();
+  mm32<T>(Tn, M, false, Li, true, -1.0, false);
   {
     T* To = reinterpret_cast<T*>(args.Tout);
     T* Uo = reinterpret_cast<T*>(args.Uout);
@@ -400,16 +489,10 @@ __global__ void __launch_bounds__(NT) tsqr_root_finish_kernel(TsqrRootArgs args)
     const int64_t lda = args.lda;
     for (int i = tid; i < NB * NB; i += NT) {
       const int r = i % NB, c = i / NB;
-      T t = s_zero(T());
-      if (r <= c && c < nbc) {
-        for (int k = r; k <= c; ++k)
-          t = s_fma(s_mul(Qt[r][k], s_conj(dvec[k])), s_conj(Li[c][k]), t);
-        t = s_neg(t);
-      }
-      To[r + c * NB] = t;
+      To[r + c * NB] = (r <= c && c < nbc) ? Tn[r][c] : s_zero(T());
       Uo[r + c * NB] = (r <= c && c < nbc) ? W[r][c] : s_zero(T());
       if (r < nbc && c < nbc) {
-        if (r <= c) Ap[r + c * lda] = s_mul(dvec[r], Vt[r + c * ldt]);
+        if (r <= c) Ap[r + c * lda] = s_mul(dvec[r], Rt[r][c]);
         else Ap[r + c * lda] = Qt[r][c];
       }
     }
@@ -482,6 +565,7 @@ __global__ void __launch_bounds__(NT) tsqr_expand_kernel(TsqrExpandArgs args) {
   __shared__ T V1[NB][NB + 1];
   __shared__ T Qb[NB][NB + 1];
   __shared__ T W[NB][NB + 1];
+  __shared__ T Qb2[NB][NB + 1];
   __shared__ ExpandLds<T> L;
   const int node = blockIdx.x;
   const int tid = threadIdx.x;
@@ -518,32 +602,31 @@ __global__ void __launch_bounds__(NT) tsqr_expand_kernel(TsqrExpandArgs args) {
     const int64_t ldt = args.ldxtop;
     for (int i = tid; i < NB * NB; i += NT) {
       const int r = i % NB, c = i / NB;
-      T s = s_zero(T());
-      if (r < nbc && c < nbc) {
-        const int g = node * nbc + r;
-        s = (g == c) ? s_one(T()) : s_zero(T());
-        const int kend = min(nbc, g + 1);
-        for (int k = 0; k < kend; ++k) s = s_sub(s, s_mul(vload(Vt, ldt, g, k), W[k][c]));
-      }
-      Qb[r][c] = s;
+      const int g = node * nbc + r;
+      Tn[r][c] = s_zero(T());  // Tn is reloaded below; reuse it as staging
+      V1[r][c] = (r < nbc && c < nbc) ? vload(Vt, ldt, g, c) : s_zero(T());
+    }
+    for (int i = tid; i < NB * NB; i += NT) {
+      const int r = i % NB, c = i / NB;
+      const int g = node * nbc + r;
+      Qb[r][c] = (r < nbc && c < nbc && g == c) ? s_one(T()) : s_zero(T());
+    }
+    __syncthreads();
+    mm32<T>(Qb, V1, false, W, false, -1.0, true);
+    const T* V1src2 = reinterpret_cast<const T*>(args.V1base) + (int64_t)node * args.v1stride;
+    const T* Tsrc2 = reinterpret_cast<const T*>(args.Tnodes) + (int64_t)node * NB * NB;
+    for (int i = tid; i < NB * NB; i += NT) {
+      const int r = i % NB, c = i / NB;
+      const bool in = r < nbc && c < nbc;
+      Tn[r][c] = in ? Tsrc2[r + c * NB] : s_zero(T());
+      V1[r][c] = in ? vload(V1src2, (int64_t)args.v1ld, r, c) : s_zero(T());
     }
     __syncthreads();
   }
-  // W = V1^H Qb
-  for (int i = tid; i < NB * NB; i += NT) {
-    const int r = i % NB, c = i / NB;
-    T s = s_zero(T());
-    for (int k = r; k < nbc; ++k) s = s_cfma(V1[k][r], Qb[k][c], s);
-    W[r][c] = s;
-  }
-  __syncthreads();
-  // M = Tn W  -> B operand of the first product
-  for (int i = tid; i < NB * NB; i += NT) {
-    const int r = i % NB, c = i / NB;
-    T s = s_zero(T());
-    for (int k = r; k < nbc; ++k) s = s_fma(Tn[r][k], W[k][c], s);
-    lds_put<T>(L, 1, r, c, s);
-  }
+  // W = V1^H Qb ; M = Tn W -> B operand of the first product
+  mm32<T>(W, V1, true, Qb, false, 1.0, false);
+  mm32<T>(Qb2, Tn, false, W, false, 1.0, false);
+  for (int i = tid; i < NB * NB; i += NT) lds_put<T>(L, 1, i % NB, i / NB, Qb2[i % NB][i / NB]);
   // A operand: 64 rows of V (unit lower in node-local rows), k-major
   T* X = reinterpret_cast<T*>(args.X);
   const int64_t ldx = args.ldx;
@@ -611,7 +694,7 @@ __global__ void __launch_bounds__(NT) tsqr_expand_kernel(TsqrExpandArgs args) {
 // Host launchers
 // ---------------------------------------------------------------------------
 template <typename T> int geqr2_node_capacity() {
-  return ScalarTraits<T>::is_complex ? 256 : 1024;
+  return ScalarTraits<T>::is_complex ? 256 : 512;
 }
 template int geqr2_node_capacity<double>();
 template int geqr2_node_capacity<zd>();
@@ -622,7 +705,7 @@ void launch_geqr2_node(const Geqr2NodeArgs& a, int leaf_rows_max, hipStream_t s)
   if constexpr (ScalarTraits<T>::is_complex) {
     hipLaunchKernelGGL((geqr2_node_kernel<T, 32, 1, 256>), dim3(a.nodes), dim3(256), 0, s, a);
   } else {
-    hipLaunchKernelGGL((geqr2_node_kernel<T, 32, 2, 512>), dim3(a.nodes), dim3(512), 0, s, a);
+    hipLaunchKernelGGL((geqr2_node_kernel<T, 32, 2, 256>), dim3(a.nodes), dim3(256), 0, s, a);
   }
   (void)leaf_rows_max;
 }

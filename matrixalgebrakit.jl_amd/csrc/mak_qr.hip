@@ -95,7 +95,7 @@ template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
   splitk_plan(ncmax, m, 64, ks, kc);
   L.wpElems = (size_t)NB * ncmax * std::max(ks, 1);
   // ksplit grows as N shrinks: bound ksplit * N over every N <= ncmax
-  L.wpElems = std::max(L.wpElems, (size_t)NB * (size_t)(512 * 64 + ncmax));
+  L.wpElems = std::max(L.wpElems, (size_t)NB * (size_t)(256 * 64 + ncmax));
   L.offWp = off;
   off += align256(L.wpElems * sizeof(T));
   L.w2Elems = (size_t)NB * ncmax;
