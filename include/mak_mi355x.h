@@ -60,6 +60,47 @@ int mak_qr_full_d(void* stream, int m, int n, double* A, int lda, double* Q, int
 int mak_qr_full_z(void* stream, int m, int n, void* A, int lda, void* Q, int ldq, void* R, int ldr,
                   int positive, void* work, size_t work_bytes);
 
+/* --------------------------------------------------------------- polar --- */
+
+/* Diagnostics of one polar call (optional output, may be NULL). */
+typedef struct mak_polar_info {
+  int iterations;   /* QDWH iterations run */
+  int qr_steps;     /* iterations with c > 100 (QR of [sqrt(c) X; I]) */
+  int chol_steps;   /* iterations with c <= 100 (Cholesky form) */
+  int ns_steps;     /* FP64 Newton-Schulz refinement steps (precision 32 / 16) */
+  double alpha;     /* ||A||_F, the scaling of X_0 */
+  double l0;        /* lower bound used for sigma_min(X_0) */
+  double l_final;   /* lower bound after the last iteration */
+  double last_diff; /* ||X_k - X_{k-1}||_F / ||X_k||_F of the last iteration */
+} mak_polar_info;
+
+/* Device workspace bytes for the polar entry points on an m x n input. */
+int mak_polar_workspace_size(int dtype, int m, int n, size_t* bytes);
+
+/* left_polar!(A, (W, P), alg) for m >= n: A = W P, W m x n with orthonormal
+ * columns, P = herm(W^H A) n x n Hermitian positive semidefinite, or NULL
+ * (P not computed).  A is left untouched.
+ *   precision: 64 (FP64 throughout), 32 (QDWH bulk GEMMs in FP32) or 16 (BF16
+ *   in the Cholesky iterations, FP32 in the QR iterations), all with FP32
+ *   accumulation; Gram matrices, Cholesky and weights stay FP64, followed by
+ *   FP64 Newton-Schulz refinement.  Complex data only supports 64.
+ *   maxiter <= 0 selects 20; l0 outside (0, 1) selects 1e-10.
+ * Replaces left_polar!(A, WP, ::PolarViaSVD / ::PolarNewton)
+ * (src/implementations/polar.jl:59-70, :99-111). */
+int mak_left_polar_d(void* stream, int m, int n, const double* A, int lda, double* W, int ldw, double* P, int ldp,
+                     int precision, int maxiter, double l0, void* work, size_t work_bytes, mak_polar_info* info);
+int mak_left_polar_z(void* stream, int m, int n, const void* A, int lda, void* W, int ldw, void* P, int ldp,
+                     int precision, int maxiter, double l0, void* work, size_t work_bytes, mak_polar_info* info);
+
+/* right_polar!(A, (P, Wh), alg) for n >= m: A = P Wh, P m x m (or NULL),
+ * Wh m x n with orthonormal rows.
+ * Replaces right_polar!(A, PWh, ::PolarViaSVD / ::PolarNewton)
+ * (src/implementations/polar.jl:71-82, :113-125). */
+int mak_right_polar_d(void* stream, int m, int n, const double* A, int lda, double* P, int ldp, double* Wh, int ldw,
+                      int precision, int maxiter, double l0, void* work, size_t work_bytes, mak_polar_info* info);
+int mak_right_polar_z(void* stream, int m, int n, const void* A, int lda, void* P, int ldp, void* Wh, int ldw,
+                      int precision, int maxiter, double l0, void* work, size_t work_bytes, mak_polar_info* info);
+
 #ifdef __cplusplus
 }
 #endif

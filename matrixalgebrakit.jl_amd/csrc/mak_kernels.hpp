@@ -99,6 +99,26 @@ template <typename T>
 void wy_reduce(const void* Wp, int ksplit, int64_t slice, const void* Tm, int transT, int nbc,
                int ncols, void* W2, hipStream_t s);
 
+// Precision-dispatched GEMM on FP64 data: prec 64 (FP64 MFMA), 32 (FP32 MFMA),
+// 16 (BF16 MFMA), FP32 accumulation for 32 / 16.  Complex data runs in FP64.
+template <typename T> void gemm_prec(int prec, int opA, int opB, GemmArgs g, hipStream_t s);
+
+// ---------------------------------------------------------------- dense blocks
+template <typename T>
+void potrf_upper(T* Z, int64_t ldz, int n, T* Winv, int64_t ldw, T* tmp, int* dinfo, hipStream_t s);
+template <typename T>
+void triangular_right_inplace(T* X, int64_t ldx, int m, int kb, const T* U, int64_t ldu, double alpha,
+                              hipStream_t s);
+template <typename T>
+void sumsq(const T* X, int64_t ldx, const T* Y, int64_t ldy, int m, int n, double* partial, double* out,
+           hipStream_t s);
+template <typename T>
+void axpby(int m, int n, double a, const T* X, int64_t ldx, double b, T* Y, int64_t ldy, hipStream_t s);
+template <typename T> void herm_part(T* C, int64_t ldc, int n, hipStream_t s);
+template <typename T> void add_diag(T* A, int64_t lda, int n, double v, hipStream_t s);
+template <typename T> void fill_zero(T* A, int64_t lda, int m, int n, hipStream_t s);
+template <typename T> void copy_mat(int m, int n, const T* X, int64_t ldx, T* Y, int64_t ldy, hipStream_t s);
+
 // ---------------------------------------------------------------- elementwise
 template <typename T> void set_identity(void* Q, int64_t ldq, int m, int n, hipStream_t s);
 // R <- triu(A[0:minmn, :]) row-scaled by conj(sign_safe(diag)) when positive

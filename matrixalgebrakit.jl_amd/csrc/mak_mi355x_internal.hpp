@@ -18,4 +18,10 @@ template <typename T>
 int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, int64_t ldq, T* R, int64_t ldr,
               int positive, void* work, size_t work_bytes);
 
+template <typename T> size_t polar_workspace_bytes(int m, int n);
+template <typename T>
+int polar_driver(hipStream_t s, int right, int m, int n, const T* A, int64_t lda, T* W, int64_t ldw, T* P,
+                 int64_t ldp, int precision, int maxiter, double l0, void* work, size_t work_bytes,
+                 mak_polar_info* info);
+
 }  // namespace mak

@@ -1,0 +1,323 @@
+// Polar decomposition by QDWH (Nakatsukasa, Bai, Gygi, "Optimizing Halley's
+// iteration for computing the matrix polar decomposition", SIAM J. Matrix
+// Anal. Appl. 31, 2010; Nakatsukasa and Higham, SIAM J. Sci. Comput. 35, 2013)
+// behind left_polar! / right_polar! (MatrixAlgebraKit
+// src/implementations/polar.jl:59-125).
+//
+// X_0 = A / ||A||_F, l_0 a lower bound of sigma_min(X_0).  With the dynamic
+// weights (a, b, c) of l_k:
+//   c > 100 : X+ = (b/c) X + (a - b/c)/sqrt(c) Q1 Q2^H,  [sqrt(c) X; I] = [Q1; Q2] R
+//   c <= 100: X+ = (b/c) X + (a - b/c) X (I + c X^H X)^{-1}
+// The QR of the stacked matrix is a CholeskyQR2 (two Gram + Cholesky +
+// triangular-solve passes): the stacked matrix has condition number at most
+// sqrt(1 + c), so for c <= kMaxCholQrC CholeskyQR2 is as accurate as
+// Householder QR and runs entirely on MFMA GEMMs.  l_0 defaults to 1e-10 (the
+// iteration converges for any l_0; a smaller true sigma_min only costs
+// iterations).  P = herm(W^H A) as in the reference's Newton path
+// (src/implementations/polar.jl:108).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "mak_common.hpp"
+#include "mak_kernels.hpp"
+#include "mak_mi355x_internal.hpp"
+
+namespace mak {
+
+namespace {
+
+constexpr double kMaxCholQrC = 1e14;  // sqrt(1 + c) <= 1e7: (1e7)^2 u << 1
+
+inline size_t al256(size_t b) { return ((b + 255) / 256) * 256; }
+
+struct PolarLayout {
+  size_t offX1, offX2, offQ1, offG, offY1, offY2, offTmp, offPart, offScal, offInfo, bytes;
+};
+
+template <typename T> PolarLayout polar_layout(int m, int n) {
+  PolarLayout L{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) { size_t o = off; off += al256(bytes); return o; };
+  const size_t mn = (size_t)m * n * sizeof(T), nn = (size_t)n * n * sizeof(T);
+  L.offX1 = take(mn);
+  L.offX2 = take(mn);
+  L.offQ1 = take(mn);
+  L.offG = take(nn);
+  L.offY1 = take(nn);
+  L.offY2 = take(nn);
+  L.offTmp = take((size_t)64 * std::max(n, 1) * sizeof(T));
+  L.offPart = take(1024 * sizeof(double));
+  L.offScal = take(16 * sizeof(double));
+  L.offInfo = take(16 * sizeof(int));
+  L.bytes = off;
+  return L;
+}
+
+void qdwh_params(double l, double& a, double& b, double& c) {
+  const double l2 = l * l;
+  const double g = std::cbrt(4.0 * (1.0 - l2) / (l2 * l2));
+  a = std::sqrt(1.0 + g) + 0.5 * std::sqrt(8.0 - 4.0 * g + 8.0 * (2.0 - l2) / (l2 * std::sqrt(1.0 + g)));
+  b = (a - 1.0) * (a - 1.0) / 4.0;
+  c = a + b - 1.0;
+}
+
+template <typename T> GemmArgs ga(int M, int N, int K, const T* A, int64_t lda, const T* B, int64_t ldb, T* C,
+                                  int64_t ldc, double alpha, double beta) {
+  GemmArgs g{};
+  g.M = M; g.N = N; g.K = K;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.alpha_re = alpha; g.beta_re = beta;
+  g.ksplit = 1; g.kchunk = K; g.batch = 1;
+  return g;
+}
+
+template <typename T>
+__global__ void conj_transpose_scale_kernel(int m, int n, double a, const T* A, int64_t lda, T* B, int64_t ldb) {
+  // B (n x m) = a A^H, A m x n
+  __shared__ T tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;  // bx over rows of A, by over cols of A
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const int r = bx + tx, c = by + k;
+    tile[k][tx] = (r < m && c < n) ? A[r + (int64_t)c * lda] : s_zero(T());
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int r = by + tx, c = bx + k;  // B row = A col, B col = A row
+    if (r < n && c < m) B[r + (int64_t)c * ldb] = s_scale(s_conj(tile[tx][k]), a);
+  }
+}
+template <typename T>
+void conj_transpose_scale(int m, int n, double a, const T* A, int64_t lda, T* B, int64_t ldb, hipStream_t s) {
+  if (m <= 0 || n <= 0) return;
+  dim3 grid(cdiv(m, 32), cdiv(n, 32));
+  hipLaunchKernelGGL((conj_transpose_scale_kernel<T>), grid, dim3(256), 0, s, m, n, a, A, lda, B, ldb);
+}
+
+template <typename T> double read_scalar(const double* d, hipStream_t s) {
+  double h = 0.0;
+  hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, s);
+  hipStreamSynchronize(s);
+  return h;
+}
+
+// The iteration on an m x n (m >= n) matrix held in Xw (ld ldx), already
+// scaled to ||X||_2 <= 1.  On return Xw holds the unitary polar factor.
+template <typename T>
+int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision, int maxiter, double l0,
+                 char* ws, const PolarLayout& L, mak_polar_info* info) {
+  T* X2 = reinterpret_cast<T*>(ws + L.offX2);
+  T* Q1 = reinterpret_cast<T*>(ws + L.offQ1);
+  T* G = reinterpret_cast<T*>(ws + L.offG);
+  T* Y1 = reinterpret_cast<T*>(ws + L.offY1);
+  T* Y2 = reinterpret_cast<T*>(ws + L.offY2);
+  T* tmp = reinterpret_cast<T*>(ws + L.offTmp);
+  double* part = reinterpret_cast<double*>(ws + L.offPart);
+  double* scal = reinterpret_cast<double*>(ws + L.offScal);
+  int* dinfo = reinterpret_cast<int*>(ws + L.offInfo);
+  const int64_t ldm = m, ldn = n;
+  const int prec = (precision == 32 || precision == 16) ? precision : 64;
+  // BF16 bulk only in the well-conditioned Cholesky iterations; the CholeskyQR2
+  // iterations (c > 100, ill-conditioned iterate) use FP32 products.
+  const int prec_qr = (prec == 16) ? 32 : prec;
+
+  // three m x n buffers rotate roles: the current iterate and two spares
+  T* buf[3] = {Xw, X2, Q1};
+  const int64_t ldb[3] = {ldx, ldm, ldm};
+  int cur = 0;
+  double l = l0;
+  const double u = 1.1102230246251565e-16;
+  const double tol = std::cbrt(5.0 * u);
+  int it = 0, nqr = 0, nch = 0;
+  double diff = 1.0;
+  int status = 0;
+  for (it = 0; it < maxiter; ++it) {
+    double a, b, c;
+    if (1.0 - l < 1e-15) { a = 3.0; b = 1.0; c = 3.0; }  // Halley's limit (l = 1)
+    else qdwh_params(l, a, b, c);
+    // keep the stacked matrix within CholeskyQR2's range: a larger l only costs
+    // iterations (QDWH converges for any l in (0, 1])
+    while (c > kMaxCholQrC) {
+      l *= 2.0;
+      qdwh_params(l, a, b, c);
+    }
+    const int s1 = (cur + 1) % 3, s2 = (cur + 2) % 3;
+    T* Xc = buf[cur];
+    const int64_t ldc = ldb[cur];
+    // G = I + c X^H X  (FP64: it feeds the Cholesky factorisation)
+    set_identity<T>(G, ldn, n, n, s);
+    gemm<T>(OP_C, OP_N, ga<T>(n, n, m, Xc, ldc, Xc, ldc, G, ldn, c, 1.0), s);
+    potrf_upper<T>(G, ldn, n, Y1, ldn, tmp, dinfo, s);
+    int nxt;
+    if (c > 100.0) {
+      // pass 1: Q1 = sqrt(c) X W1^{-1} (buf[s2]), Q2 = W1^{-1} (Y1)
+      gemm_prec<T>(prec_qr, OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y1, ldn, buf[s2], ldb[s2], std::sqrt(c), 0.0), s);
+      // pass 2: G = Q2^H Q2 + Q1^H Q1 = W2^H W2
+      gemm<T>(OP_C, OP_N, ga<T>(n, n, n, Y1, ldn, Y1, ldn, G, ldn, 1.0, 0.0), s);
+      gemm<T>(OP_C, OP_N, ga<T>(n, n, m, buf[s2], ldb[s2], buf[s2], ldb[s2], G, ldn, 1.0, 1.0), s);
+      potrf_upper<T>(G, ldn, n, Y2, ldn, tmp, dinfo, s);
+      // Q1' = Q1 W2^{-1} (buf[s1]), Q2' = Q2 W2^{-1} (G)
+      gemm_prec<T>(prec_qr, OP_N, OP_N, ga<T>(m, n, n, buf[s2], ldb[s2], Y2, ldn, buf[s1], ldb[s1], 1.0, 0.0), s);
+      gemm<T>(OP_N, OP_N, ga<T>(n, n, n, Y1, ldn, Y2, ldn, G, ldn, 1.0, 0.0), s);
+      // X+ = (b/c) X + (a - b/c)/sqrt(c) Q1' Q2'^H  (buf[s2])
+      copy_mat<T>(m, n, Xc, ldc, buf[s2], ldb[s2], s);
+      gemm_prec<T>(prec_qr, OP_N, OP_C,
+                   ga<T>(m, n, n, buf[s1], ldb[s1], G, ldn, buf[s2], ldb[s2], (a - b / c) / std::sqrt(c), b / c), s);
+      nxt = s2;
+      ++nqr;
+    } else {
+      // Z^{-1} = W^{-1} W^{-H};  X+ = (b/c) X + (a - b/c) X Z^{-1}  (buf[s1])
+      gemm_prec<T>(prec, OP_N, OP_C, ga<T>(n, n, n, Y1, ldn, Y1, ldn, Y2, ldn, 1.0, 0.0), s);
+      copy_mat<T>(m, n, Xc, ldc, buf[s1], ldb[s1], s);
+      gemm_prec<T>(prec, OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y2, ldn, buf[s1], ldb[s1], a - b / c, b / c), s);
+      nxt = s1;
+      ++nch;
+    }
+    // convergence: ||X+ - X||_F / ||X+||_F, plus the Cholesky status
+    sumsq<T>(buf[nxt], ldb[nxt], Xc, ldc, m, n, part, scal, s);
+    sumsq<T>(buf[nxt], ldb[nxt], nullptr, 0, m, n, part + 512, scal + 1, s);
+    double hs[2] = {0, 0};
+    int pinfo = 0;
+    hipMemcpyAsync(hs, scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(&pinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, s);
+    hipStreamSynchronize(s);
+    cur = nxt;
+    if (pinfo != 0) { status = 2; ++it; break; }
+    diff = std::sqrt(hs[0]) / std::max(std::sqrt(hs[1]), 1e-300);
+    l = l * (a + b * l * l) / (1.0 + c * l * l);
+    if (1.0 - l < 10.0 * u && diff <= (prec == 64 ? tol : std::sqrt(tol))) { ++it; break; }
+    if (it + 1 == maxiter) status = 6;  // not converged within maxiter (a warning)
+  }
+  // low-precision bulk: FP64 Newton-Schulz refinement X <- X (3I - X^H X) / 2
+  int nns = 0;
+  if ((status == 0 || status == 6) && prec != 64) {
+    const int steps = (prec == 16) ? 3 : 2;
+    for (int k = 0; k < steps; ++k) {
+      const int nx = (cur + 1) % 3;
+      gemm<T>(OP_C, OP_N, ga<T>(n, n, m, buf[cur], ldb[cur], buf[cur], ldb[cur], G, ldn, 1.0, 0.0), s);
+      copy_mat<T>(m, n, buf[cur], ldb[cur], buf[nx], ldb[nx], s);
+      gemm<T>(OP_N, OP_N, ga<T>(m, n, n, buf[cur], ldb[cur], G, ldn, buf[nx], ldb[nx], -0.5, 1.5), s);
+      cur = nx;
+      ++nns;
+    }
+  }
+  if (cur != 0) copy_mat<T>(m, n, buf[cur], ldb[cur], Xw, ldx, s);
+  if (info) {
+    info->iterations = it;
+    info->qr_steps = nqr;
+    info->chol_steps = nch;
+    info->ns_steps = nns;
+    info->last_diff = diff;
+    info->l_final = l;
+  }
+  return status;
+}
+
+}  // namespace
+
+template <typename T> size_t polar_workspace_bytes(int m, int n) {
+  const int mm = std::max(m, n), nn = std::min(m, n);
+  return polar_layout<T>(mm, nn).bytes;
+}
+
+template <typename T>
+int polar_driver(hipStream_t s, int right, int m, int n, const T* A, int64_t lda, T* W, int64_t ldw, T* P,
+                 int64_t ldp, int precision, int maxiter, double l0, void* work, size_t work_bytes,
+                 mak_polar_info* info) {
+  if (m < 0) return -3;
+  if (n < 0) return -4;
+  if (!right && m < n) return -3;  // left_polar! needs m >= n (src/implementations/polar.jl:9-10)
+  if (right && n < m) return -4;   // right_polar! needs n >= m (:21-22)
+  if (lda < std::max(1, m)) return -6;
+  if (ldw < std::max(1, m)) return -8;
+  const int k = std::min(m, n), big = std::max(m, n);
+  if (P && ldp < std::max(1, k)) return -10;
+  if (precision != 64 && precision != 32 && precision != 16) return -11;
+  if (ScalarTraits<T>::is_complex && precision != 64) return -11;
+  if (maxiter <= 0) maxiter = 20;
+  if (!(l0 > 0.0 && l0 < 1.0)) l0 = 1e-10;
+  const size_t need = polar_workspace_bytes<T>(m, n);
+  if (work == nullptr || work_bytes < need) return -14;
+  if (k == 0) return 0;
+  PolarLayout L = polar_layout<T>(big, k);
+  char* ws = reinterpret_cast<char*>(work);
+  double* part = reinterpret_cast<double*>(ws + L.offPart);
+  double* scal = reinterpret_cast<double*>(ws + L.offScal);
+  // alpha = ||A||_F
+  sumsq<T>(A, lda, nullptr, 0, m, n, part, scal, s);
+  const double alpha = std::sqrt(read_scalar<T>(scal, s));
+  if (info) { info->alpha = alpha; info->l0 = l0; }
+  if (!(alpha > 0.0) || !std::isfinite(alpha)) return alpha == 0.0 ? 4 : 5;
+  int st;
+  if (!right) {
+    // X0 = A / alpha, iterated in W
+    copy_mat<T>(m, n, A, lda, W, ldw, s);
+    axpby<T>(m, n, 0.0, A, lda, 1.0 / alpha, W, ldw, s);
+    st = qdwh_iterate<T>(s, m, n, W, ldw, precision, maxiter, l0, ws, L, info);
+    if ((st == 0 || st == 6) && P) {
+      // P = herm(W^H A)
+      gemm<T>(OP_C, OP_N, ga<T>(n, n, m, W, ldw, A, lda, P, ldp, 1.0, 0.0), s);
+      herm_part<T>(P, ldp, n, s);
+    }
+  } else {
+    // right_polar!(A) via the left polar of A^H (n x m): A^H = W' P'
+    T* Xh = reinterpret_cast<T*>(ws + L.offX1);
+    conj_transpose_scale<T>(m, n, 1.0 / alpha, A, lda, Xh, n, s);
+    st = qdwh_iterate<T>(s, n, m, Xh, n, precision, maxiter, l0, ws, L, info);
+    if (st == 0 || st == 6) {
+      conj_transpose_scale<T>(n, m, 1.0, Xh, n, W, ldw, s);   // Wh = (W')^H
+      if (P) {
+        // P = herm(A Wh^H)  (src/implementations/polar.jl:122)
+        gemm<T>(OP_N, OP_C, ga<T>(m, m, n, A, lda, W, ldw, P, ldp, 1.0, 0.0), s);
+        herm_part<T>(P, ldp, m, s);
+      }
+    }
+  }
+  if (hipGetLastError() != hipSuccess) return 1;
+  return st;
+}
+
+template size_t polar_workspace_bytes<double>(int, int);
+template size_t polar_workspace_bytes<zd>(int, int);
+template int polar_driver<double>(hipStream_t, int, int, int, const double*, int64_t, double*, int64_t, double*,
+                                  int64_t, int, int, double, void*, size_t, mak_polar_info*);
+template int polar_driver<zd>(hipStream_t, int, int, int, const zd*, int64_t, zd*, int64_t, zd*, int64_t, int, int,
+                              double, void*, size_t, mak_polar_info*);
+
+}  // namespace mak
+
+extern "C" {
+
+int mak_polar_workspace_size(int dtype, int m, int n, size_t* bytes) {
+  if (bytes == nullptr) return -4;
+  if (m < 0) return -2;
+  if (n < 0) return -3;
+  if (dtype == MAK_F64) *bytes = mak::polar_workspace_bytes<double>(m, n);
+  else if (dtype == MAK_C64) *bytes = mak::polar_workspace_bytes<mak::zd>(m, n);
+  else return -1;
+  return 0;
+}
+
+int mak_left_polar_d(void* s, int m, int n, const double* A, int lda, double* W, int ldw, double* P, int ldp,
+                     int precision, int maxiter, double l0, void* work, size_t work_bytes, mak_polar_info* info) {
+  return mak::polar_driver<double>((hipStream_t)s, 0, m, n, A, lda, W, ldw, P, ldp, precision, maxiter, l0, work,
+                                   work_bytes, info);
+}
+int mak_left_polar_z(void* s, int m, int n, const void* A, int lda, void* W, int ldw, void* P, int ldp,
+                     int precision, int maxiter, double l0, void* work, size_t work_bytes, mak_polar_info* info) {
+  return mak::polar_driver<mak::zd>((hipStream_t)s, 0, m, n, (const mak::zd*)A, lda, (mak::zd*)W, ldw,
+                                    (mak::zd*)P, ldp, precision, maxiter, l0, work, work_bytes, info);
+}
+int mak_right_polar_d(void* s, int m, int n, const double* A, int lda, double* P, int ldp, double* Wh, int ldw,
+                      int precision, int maxiter, double l0, void* work, size_t work_bytes, mak_polar_info* info) {
+  return mak::polar_driver<double>((hipStream_t)s, 1, m, n, A, lda, Wh, ldw, P, ldp, precision, maxiter, l0, work,
+                                   work_bytes, info);
+}
+int mak_right_polar_z(void* s, int m, int n, const void* A, int lda, void* P, int ldp, void* Wh, int ldw,
+                      int precision, int maxiter, double l0, void* work, size_t work_bytes, mak_polar_info* info) {
+  return mak::polar_driver<mak::zd>((hipStream_t)s, 1, m, n, (const mak::zd*)A, lda, (mak::zd*)Wh, ldw,
+                                    (mak::zd*)P, ldp, precision, maxiter, l0, work, work_bytes, info);
+}
+
+}  // extern "C"

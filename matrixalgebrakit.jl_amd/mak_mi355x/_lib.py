@@ -25,7 +25,22 @@ SIGNATURES = {
     "mak_qr_compact_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
     "mak_qr_full_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
     "mak_qr_full_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
+    "mak_polar_workspace_size": (_i, [_i, _i, _i, C.POINTER(_sz)]),
+    "mak_left_polar_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp, _sz, _vp]),
+    "mak_left_polar_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp, _sz, _vp]),
+    "mak_right_polar_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp, _sz, _vp]),
+    "mak_right_polar_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp, _sz, _vp]),
 }
+
+
+class PolarInfo(C.Structure):
+    """mak_polar_info (include/mak_mi355x.h)."""
+    _fields_ = [("iterations", C.c_int), ("qr_steps", C.c_int), ("chol_steps", C.c_int),
+                ("ns_steps", C.c_int), ("alpha", C.c_double), ("l0", C.c_double),
+                ("l_final", C.c_double), ("last_diff", C.c_double)]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class LibraryNotBuilt(RuntimeError):

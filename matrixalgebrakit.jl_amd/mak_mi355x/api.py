@@ -19,6 +19,7 @@ work runs on the current torch stream; no implicit synchronisation.
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 from dataclasses import dataclass, field
 
 import torch
@@ -26,9 +27,12 @@ import torch
 from . import _lib
 
 __all__ = [
-    "DimensionMismatch", "ArgumentError", "MI355XHouseholder", "colmajor", "similar_colmajor",
+    "DimensionMismatch", "ArgumentError", "NumericalError", "MI355XHouseholder", "MI355XPolar",
+    "colmajor", "similar_colmajor",
     "qr_compact", "qr_compact_", "qr_full", "qr_full_",
     "check_input_qr_compact", "check_input_qr_full", "initialize_output_qr",
+    "left_polar", "left_polar_", "right_polar", "right_polar_",
+    "check_input_left_polar", "check_input_right_polar", "initialize_output_polar", "last_polar_info",
 ]
 
 
@@ -218,3 +222,138 @@ def qr_compact(A, alg=None):
 def qr_full(A, alg=None):
     """``qr_full(A)``: works on a column-major copy of A (``copy_input``)."""
     return qr_full_(colmajor(A.clone() if A.stride(0) == 1 else A), None, alg)
+
+
+# ---------------------------------------------------------------------------
+# Polar (src/implementations/polar.jl)
+# ---------------------------------------------------------------------------
+@dataclass(frozen=True)
+class MI355XPolar:
+    """Algorithm type of this backend for ``left_polar!`` / ``right_polar!``:
+    QDWH on the GPU.
+
+    ``precision`` 64 runs every product in FP64; 32 / 16 run the bulk products
+    of the iteration in FP32 / BF16 (FP32 accumulation) with FP64 Gram
+    matrices, Cholesky factorisations and weights, followed by FP64
+    Newton-Schulz refinement.  ``maxiter`` (default 20) and ``l0`` (lower bound
+    of sigma_min(A) / ||A||_F, default 1e-10) tune the iteration; like
+    ``PolarNewton`` (src/interface/decompositions.jl:305) non-convergence only
+    warns.
+    """
+    precision: int = 64
+    maxiter: int = 20
+    l0: float = 1e-10
+
+    def __post_init__(self):
+        if self.precision not in (64, 32, 16):
+            raise ArgumentError(f"precision must be 64, 32 or 16 (got {self.precision})")
+
+
+def check_input_left_polar(A, WP):
+    """``check_input(left_polar!, A, (W, P), alg)`` (src/implementations/polar.jl:6-17)."""
+    m, n = A.shape
+    W, P = WP
+    if m < n:
+        raise ArgumentError(f"input matrix needs at least as many rows ({m}) as columns ({n})")
+    _check_size(W, (m, n), "W")
+    _check_scalar(W, A, "W", "A")
+    if P.numel() > 0:
+        _check_size(P, (n, n), "P")
+    _check_scalar(P, A, "P", "A")
+
+
+def check_input_right_polar(A, PWh):
+    """``check_input(right_polar!, A, (P, Wh), alg)`` (src/implementations/polar.jl:18-29)."""
+    m, n = A.shape
+    P, Wh = PWh
+    if n < m:
+        raise ArgumentError(f"input matrix needs at least as many columns ({n}) as rows ({m})")
+    if P.numel() > 0:
+        _check_size(P, (m, m), "P")
+    _check_scalar(P, A, "P", "A")
+    _check_size(Wh, (m, n), "Wh")
+    _check_scalar(Wh, A, "Wh", "A")
+
+
+def initialize_output_polar(A, right: bool):
+    """``initialize_output(left_polar! / right_polar!, A, alg)`` (src/implementations/polar.jl:33-44)."""
+    m, n = A.shape
+    if right:
+        return similar_colmajor(A, (m, m)), similar_colmajor(A, (m, n))
+    return similar_colmajor(A, (m, n)), similar_colmajor(A, (n, n))
+
+
+_LAST_POLAR_INFO: dict = {}
+
+
+def last_polar_info() -> dict:
+    """Diagnostics (iterations, steps, alpha, l0, last difference) of the last
+    polar call made through this module."""
+    return dict(_LAST_POLAR_INFO)
+
+
+def _polar_call(A, out, alg, right: bool):
+    if alg is None:
+        alg = MI355XPolar()
+    if A.dtype not in _DT:
+        raise ArgumentError(f"unsupported element type {A.dtype} (Float64 / ComplexF64)")
+    if right:
+        check_input_right_polar(A, out)
+        P, W = out
+    else:
+        check_input_left_polar(A, out)
+        W, P = out
+    _check_device(A, W, P)
+    if A.dtype == torch.complex128 and alg.precision != 64:
+        raise ArgumentError("reduced-precision polar is only available for Float64")
+    lib = _lib.load()
+    m, n = A.shape
+    dt = _DT[A.dtype]
+    nbytes = C.c_size_t(0)
+    _status(lib.mak_polar_workspace_size(dt, m, n, C.byref(nbytes)), "mak_polar_workspace_size")
+    dev = A.device
+    ws = _workspace(dev, nbytes.value)
+    info = _lib.PolarInfo()
+    Parg = P if P.numel() > 0 else None
+    fname = f"mak_{'right' if right else 'left'}_polar_{'z' if dt else 'd'}"
+    fn = getattr(lib, fname)
+    if right:
+        code = fn(_stream(dev), m, n, _ptr(A), _ld(A), _ptr(Parg), _ld(P) if Parg is not None else 1,
+                  _ptr(W), _ld(W), int(alg.precision), int(alg.maxiter), float(alg.l0), _ptr(ws),
+                  nbytes.value, C.byref(info))
+    else:
+        code = fn(_stream(dev), m, n, _ptr(A), _ld(A), _ptr(W), _ld(W), _ptr(Parg),
+                  _ld(P) if Parg is not None else 1, int(alg.precision), int(alg.maxiter), float(alg.l0),
+                  _ptr(ws), nbytes.value, C.byref(info))
+    _LAST_POLAR_INFO.clear()
+    _LAST_POLAR_INFO.update(info.asdict())
+    if code == 6:
+        warnings.warn(f"`{'right' if right else 'left'}_polar!` via QDWH did not converge within "
+                      f"{alg.maxiter} iterations (final difference: {info.last_diff})")
+        code = 0
+    _status(code, fname)
+    return out
+
+
+def left_polar_(A, WP=None, alg=None):
+    """``left_polar!(A, (W, P), alg)``: returns the given (W, P); empty P skips P."""
+    if WP is None:
+        WP = initialize_output_polar(A, False)
+    return _polar_call(A, WP, alg, False)
+
+
+def right_polar_(A, PWh=None, alg=None):
+    """``right_polar!(A, (P, Wh), alg)``: returns the given (P, Wh); empty P skips P."""
+    if PWh is None:
+        PWh = initialize_output_polar(A, True)
+    return _polar_call(A, PWh, alg, True)
+
+
+def left_polar(A, alg=None):
+    """``left_polar(A)``."""
+    return left_polar_(colmajor(A), None, alg)
+
+
+def right_polar(A, alg=None):
+    """``right_polar(A)``."""
+    return right_polar_(colmajor(A), None, alg)

@@ -1,0 +1,149 @@
+"""Parity of the HIP polar path (QDWH) with the CPU oracle (the reference's
+PolarViaSVD path: gesdd -> W = U V^H, P = (sqrt(S) V^H)^H (sqrt(S) V^H)).
+
+The polar decomposition of a full-rank matrix is unique, so W and P are
+compared elementwise.  Tolerances (relative Frobenius):
+  * FP64, Gaussian inputs (condition <= ~1e4):  W, P within 1e-11 of the oracle;
+  * FP64, condition 1e6: W within 1e-9 (the perturbation bound of the polar
+    factor is ~ cond * u, so both the oracle and this path sit ~1e-11 from the
+    exact W), backward error ||A - W P|| / ||A|| <= 1e-14;
+  * FP32 / BF16 bulk (precision 32 / 16) + FP64 Newton-Schulz: orthogonality
+    ||W^H W - I||_F <= 1e-12, backward error <= 1e-5 (FP32) / 1e-1 (BF16); the
+    W error is bounded below by cond * u_bulk and is reported, not asserted.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import mak_mi355x as mak
+from oracle import mak_oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = np.load(Path(__file__).parent / "golden" / "golden_small.npz")
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def to_dev(A, dev):
+    return mak.colmajor(torch.from_numpy(np.asfortranarray(A)).to(dev))
+
+
+def run_left(A, dev, alg=None, with_p=True):
+    At = to_dev(A, dev)
+    W, P = mak.initialize_output_polar(At, False)
+    if not with_p:
+        P = torch.empty((0, 0), dtype=At.dtype, device=dev)
+    W2, P2 = mak.left_polar_(At, (W, P), alg)
+    assert W2 is W and P2 is P
+    torch.cuda.synchronize()
+    return W.cpu().numpy(), (P.cpu().numpy() if with_p else None)
+
+
+SIZES = [(54, 37), (54, 54), (20, 1), (300, 200), (512, 512), (1000, 130)]
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+@pytest.mark.parametrize("m,n", SIZES, ids=[f"{m}x{n}" for m, n in SIZES])
+def test_left_polar_matches_oracle(m, n, dt, dev):
+    A = O.gaussian(m, n, dt, seed=500 + m + n)
+    W, P = run_left(A, dev)
+    Wo, Po = O.left_polar_svd(A)
+    assert rel(W, Wo) < 1e-11, rel(W, Wo)
+    assert rel(P, Po) < 1e-11, rel(P, Po)
+    assert O.isisometric(W)
+    assert np.allclose(P, P.conj().T)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+@pytest.mark.parametrize("m,n", [(37, 54), (54, 54), (130, 1000)])
+def test_right_polar_matches_oracle(m, n, dt, dev):
+    A = O.gaussian(m, n, dt, seed=600 + m + n)
+    At = to_dev(A, dev)
+    P, Wh = mak.initialize_output_polar(At, True)
+    P2, Wh2 = mak.right_polar_(At, (P, Wh))
+    assert P2 is P and Wh2 is Wh
+    torch.cuda.synchronize()
+    Po, Who = O.right_polar_svd(A)
+    assert rel(Wh.cpu().numpy(), Who) < 1e-11
+    assert rel(P.cpu().numpy(), Po) < 1e-11
+
+
+@pytest.mark.parametrize("key", [k[:-2] for k in GOLD.files if k.startswith("lpolar_") and k.endswith("_A")])
+def test_polar_golden_fixtures(key, dev):
+    W, P = run_left(GOLD[key + "_A"], dev)
+    assert rel(W, GOLD[key + "_W"]) < 1e-11 and rel(P, GOLD[key + "_P"]) < 1e-11
+
+
+def test_left_polar_without_P_and_input_untouched(dev):
+    A = O.gaussian(90, 60, np.float64, seed=3)
+    At = to_dev(A, dev)
+    A0 = At.clone()
+    W = mak.similar_colmajor(At, (90, 60))
+    noP = torch.empty((0, 0), dtype=At.dtype, device=dev)
+    W2, P2 = mak.left_polar_(At, (W, noP))
+    torch.cuda.synchronize()
+    assert W2 is W and P2 is noP
+    assert torch.equal(At, A0)
+    Wo, _ = O.left_polar_svd(A)
+    assert rel(W.cpu().numpy(), Wo) < 1e-11
+
+
+def test_polar_argument_errors(dev):
+    A = to_dev(O.gaussian(10, 20, np.float64, seed=1), dev)
+    with pytest.raises(mak.ArgumentError):
+        mak.left_polar_(A)          # m < n
+    B = to_dev(O.gaussian(20, 10, np.float64, seed=1), dev)
+    with pytest.raises(mak.ArgumentError):
+        mak.right_polar_(B)         # n < m
+    W, P = mak.initialize_output_polar(B, False)
+    with pytest.raises(mak.DimensionMismatch):
+        mak.left_polar_(B, (W, P[:5, :5]))
+    with pytest.raises(mak.ArgumentError):
+        mak.MI355XPolar(precision=8)
+    C = to_dev(O.gaussian(20, 10, np.complex128, seed=1), dev)
+    with pytest.raises(mak.ArgumentError):
+        mak.left_polar_(C, None, mak.MI355XPolar(precision=32))
+
+
+def test_polar_conditioned_1e6_fp64(dev):
+    A, _ = O.conditioned_matrix(512, 1e6, seed=0)
+    W, P = run_left(A, dev)
+    info = mak.last_polar_info()
+    Wo, Po = O.left_polar_svd(A)
+    bwd = np.linalg.norm(A - W @ P) / np.linalg.norm(A)
+    orth = np.linalg.norm(W.T @ W - np.eye(512))
+    print(f"cond 1e6 512^2 FP64: relW {rel(W, Wo):.2e} relP {rel(P, Po):.2e} bwd {bwd:.2e} orth {orth:.2e} "
+          f"info {info}", flush=True)
+    assert rel(W, Wo) < 1e-9 and rel(P, Po) < 1e-12
+    assert bwd < 1e-14 and orth < 1e-13
+    assert info["qr_steps"] >= 1 and info["iterations"] <= 8
+
+
+@pytest.mark.parametrize("prec,bwd_tol", [(32, 1e-5), (16, 1e-2)])
+def test_polar_mixed_precision(prec, bwd_tol, dev):
+    A, _ = O.conditioned_matrix(512, 1e6, seed=0)
+    W, P = run_left(A, dev, mak.MI355XPolar(precision=prec))
+    info = mak.last_polar_info()
+    Wo, _ = O.left_polar_svd(A)
+    bwd = np.linalg.norm(A - W @ P) / np.linalg.norm(A)
+    orth = np.linalg.norm(W.T @ W - np.eye(512))
+    print(f"cond 1e6 512^2 precision {prec}: relW {rel(W, Wo):.2e} bwd {bwd:.2e} orth {orth:.2e} info {info}",
+          flush=True)
+    assert orth < 1e-12 and bwd < bwd_tol
+    assert info["ns_steps"] >= 2
+
+
+def test_polar_full_size_gaussian_properties(dev):
+    n = 4096
+    A = O.gaussian(n, n, np.float64, seed=7)
+    W, P = run_left(A, dev)
+    info = mak.last_polar_info()
+    bwd = np.linalg.norm(A - W @ P) / np.linalg.norm(A)
+    orth = np.linalg.norm(W.T @ W - np.eye(n)) / np.sqrt(n)
+    lmin = np.linalg.eigvalsh(P).min()
+    print(f"4096^2 Gaussian FP64: bwd {bwd:.2e} orth/sqrt(n) {orth:.2e} min eig P {lmin:.2e} info {info}", flush=True)
+    assert bwd < 1e-13 and orth < 1e-13 and lmin > 0
