@@ -60,6 +60,17 @@ int mak_qr_full_d(void* stream, int m, int n, double* A, int lda, double* Q, int
 int mak_qr_full_z(void* stream, int m, int n, void* A, int lda, void* Q, int ldq, void* R, int ldr,
                   int positive, void* work, size_t work_bytes);
 
+/* qr_compact! as the primal of the reverse rule: like mak_qr_compact_{d,z}, and
+ * additionally keeps the compact WY factors the adjoint re-forms Q from: on
+ * return A holds V (strictly below the diagonal) and R' (Householder signs),
+ * Tout (32 x 32 x ceil(min(m,n)/32), column-major per panel) the T blocks.
+ * Replaces the primal call inside rrule(::typeof(qr_compact!), A, QR, alg)
+ * (ext/MatrixAlgebraKitChainRulesCoreExt.jl:26-43). */
+int mak_qr_compact_wy_d(void* stream, int m, int n, double* A, int lda, double* Q, int ldq, double* R, int ldr,
+                        double* Tout, int positive, void* work, size_t work_bytes);
+int mak_qr_compact_wy_z(void* stream, int m, int n, void* A, int lda, void* Q, int ldq, void* R, int ldr, void* Tout,
+                        int positive, void* work, size_t work_bytes);
+
 /* --------------------------------------------------------------- polar --- */
 
 /* Diagnostics of one polar call (optional output, may be NULL). */
@@ -100,6 +111,46 @@ int mak_right_polar_d(void* stream, int m, int n, const double* A, int lda, doub
                       int precision, int maxiter, double l0, void* work, size_t work_bytes, mak_polar_info* info);
 int mak_right_polar_z(void* stream, int m, int n, const void* A, int lda, void* P, int ldp, void* Wh, int ldw,
                       int precision, int maxiter, double l0, void* work, size_t work_bytes, mak_polar_info* info);
+
+/* ----------------------------------------------------------- adjoints --- */
+
+/* Device workspace of the reverse rules for `batch` matrices of size m x n:
+ * kind 0 = mak_qr_pullback_*, kind 1 = mak_left_polar_pullback_*. */
+int mak_pullback_workspace_size(int kind, int dtype, int batch, int m, int n, size_t* bytes);
+
+/* qr_pullback!(dA, A, (Q, R), (dQ, dR)) for `batch` equally sized matrices
+ * (matrix b at pointer + b * stride), accumulating into dA (dA += ...).  Q is
+ * re-formed from the compact WY factors (A_wy, T) kept by
+ * mak_qr_compact_wy_*, with the gauge signs when `positive` matches the
+ * primal's positive = true; R is the primal's R (min(m,n) x n, only its upper
+ * triangle is read); p is the rank (qr_rank, src/pullbacks/qr.jl:1-2).
+ * dQ (m x min(m,n)) / dR may be NULL (absent cotangent).
+ * Replaces qr_pullback! (src/pullbacks/qr.jl:73-120). */
+int mak_qr_pullback_d(void* stream, int batch, int m, int n, int p, int positive, const double* A, int lda,
+                      long long strideA, const double* T, long long strideT, const double* R, int ldr,
+                      long long strideR, const double* dQ, int lddq, long long strideDQ, const double* dR, int lddr,
+                      long long strideDR, double* dA, int ldda, long long strideDA, void* work, size_t work_bytes);
+int mak_qr_pullback_z(void* stream, int batch, int m, int n, int p, int positive, const void* A, int lda,
+                      long long strideA, const void* T, long long strideT, const void* R, int ldr, long long strideR,
+                      const void* dQ, int lddq, long long strideDQ, const void* dR, int lddr, long long strideDR,
+                      void* dA, int ldda, long long strideDA, void* work, size_t work_bytes);
+
+/* left_polar_pullback!(dA, A, (W, P), (dW, dP)) for `batch` equally sized
+ * matrices (m >= n), accumulating into dA.  The Sylvester equation
+ * P C + C P = M - M^H is solved by preconditioned conjugate gradients
+ * (maxiter <= 0: 200, tol <= 0: 1e-14 relative residual); iters receives the
+ * iteration count (may be NULL).  Returns 6 when CG did not reach tol.
+ * Replaces left_polar_pullback! (src/pullbacks/polar.jl:7-29). */
+int mak_left_polar_pullback_d(void* stream, int batch, int m, int n, const double* W, int ldw, long long strideW,
+                              const double* P, int ldp, long long strideP, const double* dW, int lddw,
+                              long long strideDW, const double* dP, int lddp, long long strideDP, double* dA,
+                              int ldda, long long strideDA, int maxiter, double tol, void* work, size_t work_bytes,
+                              int* iters);
+int mak_left_polar_pullback_z(void* stream, int batch, int m, int n, const void* W, int ldw, long long strideW,
+                              const void* P, int ldp, long long strideP, const void* dW, int lddw,
+                              long long strideDW, const void* dP, int lddp, long long strideDP, void* dA, int ldda,
+                              long long strideDA, int maxiter, double tol, void* work, size_t work_bytes,
+                              int* iters);
 
 #ifdef __cplusplus
 }

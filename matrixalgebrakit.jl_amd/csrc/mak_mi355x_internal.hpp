@@ -16,7 +16,7 @@ template <typename T>
 void orgqr_blocked(const T* A, int64_t lda, int m, int n, T* Q, int64_t ldq, int qcols, char* ws, hipStream_t s);
 template <typename T>
 int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, int64_t ldq, T* R, int64_t ldr,
-              int positive, void* work, size_t work_bytes);
+              int positive, void* work, size_t work_bytes, T* Tout);
 
 template <typename T> size_t polar_workspace_bytes(int m, int n);
 template <typename T>

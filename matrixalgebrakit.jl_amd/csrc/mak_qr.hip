@@ -242,7 +242,7 @@ void orgqr_blocked(const T* A, int64_t lda, int m, int n, T* Q, int64_t ldq, int
 
 template <typename T>
 int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, int64_t ldq, T* R, int64_t ldr,
-              int positive, void* work, size_t work_bytes) {
+              int positive, void* work, size_t work_bytes, T* Tout) {
   if (m < 0) return -2;
   if (n < 0) return -3;
   if (m > 0 && A == nullptr) return -4;
@@ -262,6 +262,10 @@ int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, in
     return hipGetLastError() == hipSuccess ? 0 : 1;
   }
   geqrf_blocked<T>(A, lda, m, n, ws, qcols, s);
+  if (Tout) {  // keep the compact WY factors' T blocks (32 x 32 per panel) for the adjoint
+    const int npan = cdiv(minmn, NB);
+    hipMemcpyAsync(Tout, ws, sizeof(T) * NB * NB * npan, hipMemcpyDeviceToDevice, s);
+  }
   if (R) qr_extract_r<T>(A, lda, R, ldr, rrows, n, minmn, positive, s);
   orgqr_blocked<T>(A, lda, m, n, Q, ldq, qcols, ws, s);
   if (positive) qr_gauge_q<T>(A, lda, Q, ldq, m, minmn, s);
@@ -273,9 +277,9 @@ template void geqrf_blocked<zd>(zd*, int64_t, int, int, char*, size_t, hipStream
 template void orgqr_blocked<double>(const double*, int64_t, int, int, double*, int64_t, int, char*, hipStream_t);
 template void orgqr_blocked<zd>(const zd*, int64_t, int, int, zd*, int64_t, int, char*, hipStream_t);
 template int qr_driver<double>(hipStream_t, int, int, int, double*, int64_t, double*, int64_t, double*, int64_t,
-                               int, void*, size_t);
+                               int, void*, size_t, double*);
 template int qr_driver<zd>(hipStream_t, int, int, int, zd*, int64_t, zd*, int64_t, zd*, int64_t, int, void*,
-                           size_t);
+                           size_t, zd*);
 
 }  // namespace mak
 
@@ -296,21 +300,33 @@ int mak_qr_workspace_size(int dtype, int full, int m, int n, size_t* bytes) {
 
 int mak_qr_compact_d(void* s, int m, int n, double* A, int lda, double* Q, int ldq, double* R, int ldr,
                      int positive, void* work, size_t work_bytes) {
-  return mak::qr_driver<double>((hipStream_t)s, 0, m, n, A, lda, Q, ldq, R, ldr, positive, work, work_bytes);
+  return mak::qr_driver<double>((hipStream_t)s, 0, m, n, A, lda, Q, ldq, R, ldr, positive, work, work_bytes, nullptr);
 }
 int mak_qr_compact_z(void* s, int m, int n, void* A, int lda, void* Q, int ldq, void* R, int ldr,
                      int positive, void* work, size_t work_bytes) {
   return mak::qr_driver<mak::zd>((hipStream_t)s, 0, m, n, (mak::zd*)A, lda, (mak::zd*)Q, ldq, (mak::zd*)R, ldr, positive,
-                                 work, work_bytes);
+                                 work, work_bytes, nullptr);
 }
 int mak_qr_full_d(void* s, int m, int n, double* A, int lda, double* Q, int ldq, double* R, int ldr,
                   int positive, void* work, size_t work_bytes) {
-  return mak::qr_driver<double>((hipStream_t)s, 1, m, n, A, lda, Q, ldq, R, ldr, positive, work, work_bytes);
+  return mak::qr_driver<double>((hipStream_t)s, 1, m, n, A, lda, Q, ldq, R, ldr, positive, work, work_bytes, nullptr);
 }
 int mak_qr_full_z(void* s, int m, int n, void* A, int lda, void* Q, int ldq, void* R, int ldr, int positive,
                   void* work, size_t work_bytes) {
   return mak::qr_driver<mak::zd>((hipStream_t)s, 1, m, n, (mak::zd*)A, lda, (mak::zd*)Q, ldq, (mak::zd*)R, ldr, positive,
-                                 work, work_bytes);
+                                 work, work_bytes, nullptr);
+}
+
+int mak_qr_compact_wy_d(void* s, int m, int n, double* A, int lda, double* Q, int ldq, double* R, int ldr,
+                        double* Tout, int positive, void* work, size_t work_bytes) {
+  if (Tout == nullptr) return -10;
+  return mak::qr_driver<double>((hipStream_t)s, 0, m, n, A, lda, Q, ldq, R, ldr, positive, work, work_bytes, Tout);
+}
+int mak_qr_compact_wy_z(void* s, int m, int n, void* A, int lda, void* Q, int ldq, void* R, int ldr, void* Tout,
+                        int positive, void* work, size_t work_bytes) {
+  if (Tout == nullptr) return -10;
+  return mak::qr_driver<mak::zd>((hipStream_t)s, 0, m, n, (mak::zd*)A, lda, (mak::zd*)Q, ldq, (mak::zd*)R, ldr,
+                                 positive, work, work_bytes, (mak::zd*)Tout);
 }
 
 }  // extern "C"

@@ -9,3 +9,7 @@ from .api import __all__ as _api_all
 from ._lib import LIB_PATH, LibraryNotBuilt, declared_symbols, load
 
 __all__ = list(_api_all) + ["LIB_PATH", "LibraryNotBuilt", "declared_symbols", "load"]
+from .pullbacks import *  # noqa: F401,F403,E402
+from .pullbacks import __all__ as _pb_all  # noqa: E402
+
+__all__ += list(_pb_all)

@@ -17,6 +17,7 @@ _vp = C.c_void_p
 _i = C.c_int
 _sz = C.c_size_t
 _d = C.c_double
+_ll = C.c_longlong
 
 # name -> (restype, argtypes); kept in the order of include/mak_mi355x.h
 SIGNATURES = {
@@ -25,11 +26,22 @@ SIGNATURES = {
     "mak_qr_compact_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
     "mak_qr_full_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
     "mak_qr_full_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
+    "mak_qr_compact_wy_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
+    "mak_qr_compact_wy_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
     "mak_polar_workspace_size": (_i, [_i, _i, _i, C.POINTER(_sz)]),
     "mak_left_polar_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp, _sz, _vp]),
     "mak_left_polar_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp, _sz, _vp]),
     "mak_right_polar_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp, _sz, _vp]),
     "mak_right_polar_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp, _sz, _vp]),
+    "mak_pullback_workspace_size": (_i, [_i, _i, _i, _i, _i, C.POINTER(_sz)]),
+    "mak_qr_pullback_d": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _i, _ll, _vp, _ll, _vp, _i, _ll, _vp, _i, _ll,
+                               _vp, _i, _ll, _vp, _i, _ll, _vp, _sz]),
+    "mak_qr_pullback_z": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _i, _ll, _vp, _ll, _vp, _i, _ll, _vp, _i, _ll,
+                               _vp, _i, _ll, _vp, _i, _ll, _vp, _sz]),
+    "mak_left_polar_pullback_d": (_i, [_vp, _i, _i, _i, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll,
+                                       _vp, _i, _ll, _i, _d, _vp, _sz, _vp]),
+    "mak_left_polar_pullback_z": (_i, [_vp, _i, _i, _i, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll,
+                                       _vp, _i, _ll, _i, _d, _vp, _sz, _vp]),
 }
 
 
