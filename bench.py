@@ -216,6 +216,23 @@ def bench_pullbacks(mak, dev, stream, steps, batch=256, m=2048, n=256):
     return out
 
 
+def run_extras(mak, dev, stream, groups, xs):
+    import torch
+    m, n = 8192, 1024
+    ex = {}
+    if "qr" in groups:
+        ex["qr_full_f64_8192x1024"] = bench_qr_extra(mak, dev, stream, m, n, torch.float64, True, xs)
+        ex["qr_compact_c64_8192x1024"] = bench_qr_extra(mak, dev, stream, m, n, torch.complex128, False, xs)
+        ex["qr_full_c64_8192x1024"] = bench_qr_extra(mak, dev, stream, m, n, torch.complex128, True, xs)
+        torch.cuda.empty_cache()
+    if "polar" in groups:
+        ex.update(bench_polar(mak, dev, stream, xs))
+        torch.cuda.empty_cache()
+    if "pullback" in groups:
+        ex.update(bench_pullbacks(mak, dev, stream, xs))
+    return ex
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,6 +240,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
+    ap.add_argument("--extras", default="qr,polar,pullback",
+                    help="comma-separated extra workload groups: qr, polar, pullback")
+    ap.add_argument("--skip-headline", action="store_true",
+                    help="profiling aid: run only the extra workloads (the JSON line then has no headline value)")
     ap.add_argument("--traffic-bytes", type=float, default=None,
                     help="HBM bytes per qr_compact! call from the rocprofv3 counter runs (default: the "
                          "committed counter summary in profiles/, if present)")
@@ -254,6 +275,11 @@ def main():
         return
 
     m, n = 8192, 1024
+    groups = set() if args.no_extras else {g for g in args.extras.split(",") if g}
+    if args.skip_headline:
+        ex = run_extras(mak, dev, stream, groups, max(2, args.steps // 3))
+        print(json.dumps({"metric": METRIC, "value": None, "extras": ex}), flush=True)
+        return
     rng = np.random.default_rng(0)
     A0 = torch.from_numpy(np.asfortranarray(rng.standard_normal((m, n)))).to(dev)
     A0 = mak.colmajor(A0)
@@ -304,17 +330,8 @@ def main():
                      "ms_per_launch": round(ms, 4)},
         "checks": {"rel_residual": res, "orth": orth},
     }
-    if not args.no_extras:
-        ex = {}
-        xs = max(2, args.steps // 3)
-        ex["qr_full_f64_8192x1024"] = bench_qr_extra(mak, dev, stream, m, n, torch.float64, True, xs)
-        ex["qr_compact_c64_8192x1024"] = bench_qr_extra(mak, dev, stream, m, n, torch.complex128, False, xs)
-        ex["qr_full_c64_8192x1024"] = bench_qr_extra(mak, dev, stream, m, n, torch.complex128, True, xs)
-        torch.cuda.empty_cache()
-        ex.update(bench_polar(mak, dev, stream, xs))
-        torch.cuda.empty_cache()
-        ex.update(bench_pullbacks(mak, dev, stream, xs))
-        out["extras"] = ex
+    if groups:
+        out["extras"] = run_extras(mak, dev, stream, groups, max(2, args.steps // 3))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, n)
     print(json.dumps(out), flush=True)

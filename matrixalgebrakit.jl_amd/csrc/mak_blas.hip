@@ -1,8 +1,7 @@
 // Dense building blocks for the polar iterations (FP64 / ComplexF64):
-//   * potrf_upper: blocked right-looking Cholesky Z = W^H W (W upper) that also
-//     returns W^{-1}; the 64x64 diagonal blocks are factored in LDS by one
-//     workgroup (one barrier per column), panels and trailing updates are MFMA
-//     GEMMs;
+//   * potrf_upper: recursive Cholesky Z = W^H W (W upper) that also returns
+//     W^{-1}; 64 x 64 leaves are factored and inverted in registers by one
+//     workgroup, every off-diagonal product is an MFMA GEMM;
 //   * Frobenius norms (deterministic two-pass reductions);
 //   * elementwise helpers (identity fill, axpby, Hermitian part).
 #include <algorithm>
@@ -17,64 +16,106 @@ constexpr int DB = 64;  // diagonal block of the blocked Cholesky
 
 template <typename T> void fill_zero(T* A, int64_t lda, int m, int n, hipStream_t s);
 
-// Factor the kb x kb diagonal block Z (upper part read) into W (upper, written
-// over Z's upper part; strict lower part set to zero) and write W^{-1} (upper)
-// to Y.  info[0] receives j+1 (plus the block offset) when the block is not
-// positive definite; the factorisation then stops.
+// Factor the kb x kb (kb <= 64) diagonal block Z (upper part read) into W
+// (upper, written over Z; strict lower part set to zero) and write W^{-1}
+// (upper, zero below) to Y.  info[0] receives offset + j + 1 when the leading
+// (j+1) x (j+1) minor is not positive definite; the kernel then stops.
+//
+// Layout: wave q holds rows 16q .. 16q + 15 of the block, lane c column c, in
+// registers.  Cholesky (right-looking, one barrier per column): the wave
+// holding row j publishes it to LDS; every lane forms W(j, c) = Z(j, c) / d_j
+// and updates its rows r > j.  Inverse (one barrier per column): column k of
+// Y = W^{-1} is final once columns < k have been added in,
+// Y(r, k) = (delta_rk - sum_{i<k} Y(r, i) W(i, k)) / W(k, k); the lanes of
+// column k publish it and the lanes of columns > k accumulate it.
+// Rows/columns >= kb are padded with the identity.
 template <typename T>
-__global__ void __launch_bounds__(256) potrf_diag_kernel(T* Z, int64_t ldz, int kb, T* Y, int64_t ldy,
-                                                         int* info, int offset) {
-  __shared__ T S[DB][DB + 1];
-  __shared__ T Wl[DB][DB + 1];
-  __shared__ int bad;
+__global__ void __launch_bounds__(256) potrf_inv64_kernel(T* Z, int64_t ldz, int kb, T* Y, int64_t ldy,
+                                                          int* info, int offset) {
+  __shared__ T Wsh[DB][DB + 1];
+  __shared__ T rowb[2][DB];
+  __shared__ T colb[2][DB];
   const int tid = threadIdx.x;
-  for (int i = tid; i < DB * DB; i += 256) {
-    const int r = i % DB, c = i / DB;
-    S[r][c] = (r <= c && c < kb) ? Z[r + c * ldz] : s_zero(T());
-    Wl[r][c] = s_zero(T());
+  const int c = tid & 63;
+  const int q = tid >> 6;
+  T s[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int r = 16 * q + i;
+    s[i] = (r <= c && c < kb) ? Z[r + c * ldz] : ((r == c) ? s_one(T()) : s_zero(T()));
   }
-  if (tid == 0) bad = 0;
+  if (q == 0) rowb[0][c] = s[0];
   __syncthreads();
-  for (int j = 0; j < kb; ++j) {
-    const double d2 = s_re(S[j][j]);
-    if (!(d2 > 0.0)) {  // uniform: every thread reads the same value
-      if (tid == 0) {
-        bad = 1;
-        if (info && info[0] == 0) info[0] = offset + j + 1;
+  int bad = 0;
+  for (int jq = 0; jq < 4; ++jq) {
+#pragma unroll
+    for (int ji = 0; ji < 16; ++ji) {
+      const int j = 16 * jq + ji;
+      const int par = ji & 1;
+      const double d2 = s_re(rowb[par][j]);
+      // uniform: every thread read the same value after the same barrier
+      if (!bad && !(d2 > 0.0)) bad = j + 1;
+      if (!bad) {
+        const double d = sqrt(d2);
+        const double rd = 1.0 / d;
+        const T wjc = sel(c > j, s_scale(rowb[par][c], rd), sel(c == j, s_from_real<T>(d), s_zero(T())));
+        if (q == jq) {
+          s[ji] = wjc;
+          Wsh[j][c] = wjc;
+        }
+        if (q >= jq) {
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            const int r = 16 * q + i;
+            const T wjr = s_scale(rowb[par][r], rd);
+            s[i] = sel(r > j && r <= c, s_sub(s[i], s_mul(s_conj(wjr), wjc)), s[i]);
+          }
+        }
+        const int j1 = j + 1;
+        if (j1 < DB && q == (j1 >> 4)) rowb[par ^ 1][c] = s[(ji + 1) & 15];
+        __syncthreads();
       }
-      break;
     }
-    const double d = sqrt(d2), rd = 1.0 / d, rd2 = 1.0 / d2;
-    for (int c = j + tid; c < kb; c += 256) Wl[j][c] = (c == j) ? s_from_real<T>(d) : s_scale(S[j][c], rd);
-    // trailing update of the upper part: rows j < r <= c
-    for (int i = tid; i < DB * DB; i += 256) {
-      const int r = i % DB, c = i / DB;
-      if (r > j && r <= c && c < kb)
-        S[r][c] = s_sub(S[r][c], s_scale(s_mul(s_conj(S[j][r]), S[j][c]), rd2));
+  }
+  if (bad) {
+    if (tid == 0 && info && info[0] == 0) info[0] = offset + bad;
+    return;
+  }
+  // inverse: s[i] accumulates sum_{k < c} Y(16q + i, k) W(k, c), then holds Y
+#pragma unroll
+  for (int i = 0; i < 16; ++i) s[i] = s_zero(T());
+  for (int kq = 0; kq < 4; ++kq) {
+#pragma unroll
+    for (int ki = 0; ki < 16; ++ki) {
+      const int k = 16 * kq + ki;
+      const int par = ki & 1;
+      const T invk = s_div(s_one(T()), Wsh[k][k]);
+      if (c == k) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int r = 16 * q + i;
+          const T y = sel(r < k, s_neg(s_mul(s[i], invk)), sel(r == k, invk, s_zero(T())));
+          s[i] = y;
+          colb[par][r] = y;
+        }
+      }
+      __syncthreads();
+      if (c > k && 16 * q <= k) {
+        const T wkc = Wsh[k][c];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int r = 16 * q + i;
+          s[i] = sel(r <= k, s_fma(colb[par][r], wkc, s[i]), s[i]);
+        }
+      }
     }
-    __syncthreads();
   }
-  __syncthreads();
-  if (bad) return;
-  // Y = W^{-1}: init diag(1/W_rr), then for jj = kb-1..0, rows r < jj:
-  // Y[r][:] -= (W[r][jj] / W[r][r]) Y[jj][:]
-  for (int i = tid; i < DB * DB; i += 256) {
-    const int r = i % DB, c = i / DB;
-    S[r][c] = (r == c && r < kb) ? s_div(s_one(T()), Wl[r][r]) : s_zero(T());
-  }
-  __syncthreads();
-  for (int jj = kb - 1; jj > 0; --jj) {
-    for (int i = tid; i < DB * DB; i += 256) {
-      const int r = i % DB, c = i / DB;
-      if (r < jj && c >= jj && c < kb) S[r][c] = s_sub(S[r][c], s_mul(s_div(Wl[r][jj], Wl[r][r]), S[jj][c]));
-    }
-    __syncthreads();
-  }
-  for (int i = tid; i < DB * DB; i += 256) {
-    const int r = i % DB, c = i / DB;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int r = 16 * q + i;
     if (r < kb && c < kb) {
-      Z[r + c * ldz] = (r <= c) ? Wl[r][c] : s_zero(T());
-      Y[r + c * ldy] = (r <= c) ? S[r][c] : s_zero(T());
+      Z[r + c * ldz] = (r <= c) ? Wsh[r][c] : s_zero(T());
+      Y[r + c * ldy] = (r <= c) ? s[i] : s_zero(T());
     }
   }
 }
@@ -114,38 +155,44 @@ template <typename T> void copy_mat(int m, int n, const T* X, int64_t ldx, T* Y,
   hipLaunchKernelGGL((copy_kernel<T>), dim3((unsigned)cdiv64(tot, 256)), dim3(256), 0, s, m, n, X, ldx, Y, ldy);
 }
 
+// Recursive Cholesky with inverse on an n x n block (n a multiple of DB except
+// possibly the last diagonal block):
+//   Z = [Z11 Z12; . Z22] = W^H W,  W = [W11 W12; 0 W22],  Y = W^{-1}:
+//   W11, Y11 <- recursion;  W12 = Y11^H Z12;  Z22 -= W12^H W12;
+//   W22, Y22 <- recursion;  Y12 = -(Y11 W12) Y22;  Y21 = 0.
+// Every product is one MFMA GEMM; the 64 x 64 leaves run potrf_inv64_kernel.
+template <typename T>
+static void potrf_inv_rec(T* Z, int64_t ldz, T* Y, int64_t ldy, int n, int offset, T* tmp, int* dinfo,
+                          hipStream_t s) {
+  if (n <= DB) {
+    hipLaunchKernelGGL((potrf_inv64_kernel<T>), dim3(1), dim3(256), 0, s, Z, ldz, n, Y, ldy, dinfo, offset);
+    return;
+  }
+  const int n1 = ((n / 2 + DB - 1) / DB) * DB;
+  const int n2 = n - n1;
+  T* Z12 = Z + (int64_t)n1 * ldz;
+  T* Z22 = Z + n1 + (int64_t)n1 * ldz;
+  T* Y12 = Y + (int64_t)n1 * ldy;
+  T* Y22 = Y + n1 + (int64_t)n1 * ldy;
+  potrf_inv_rec<T>(Z, ldz, Y, ldy, n1, offset, tmp, dinfo, s);
+  gemm<T>(OP_C, OP_N, gargs<T>(n1, n2, n1, Y, ldy, Z12, ldz, tmp, n1, 1.0, 0.0), s);
+  copy_mat<T>(n1, n2, tmp, n1, Z12, ldz, s);
+  gemm<T>(OP_C, OP_N, gargs<T>(n2, n2, n1, Z12, ldz, Z12, ldz, Z22, ldz, -1.0, 1.0), s);
+  potrf_inv_rec<T>(Z22, ldz, Y22, ldy, n2, offset + n1, tmp, dinfo, s);
+  gemm<T>(OP_N, OP_N, gargs<T>(n1, n2, n1, Y, ldy, Z12, ldz, tmp, n1, 1.0, 0.0), s);
+  gemm<T>(OP_N, OP_N, gargs<T>(n1, n2, n2, tmp, n1, Y22, ldy, Y12, ldy, -1.0, 0.0), s);
+  fill_zero<T>(Y + n1, ldy, n2, n1, s);
+}
+
 // Z (n x n, upper part) -> W (upper, in place, strict lower zeroed) with
 // Z = W^H W, and Winv = W^{-1} (upper, n x n, fully overwritten).  tmp holds
-// DB x n scratch.  dinfo (device) receives j > 0 when the leading j x j minor
-// is not positive definite.
+// potrf_tmp_elems(n) elements.  dinfo (device) receives j > 0 when the leading
+// j x j minor is not positive definite.
 template <typename T>
 void potrf_upper(T* Z, int64_t ldz, int n, T* Winv, int64_t ldw, T* tmp, int* dinfo, hipStream_t s) {
   hipMemsetAsync(dinfo, 0, sizeof(int), s);
-  fill_zero<T>(Winv, ldw, n, n, s);
-  for (int k = 0; k < n; k += DB) {
-    const int kb = std::min(DB, n - k);
-    T* Zkk = Z + k + (int64_t)k * ldz;
-    T* Ykk = Winv + k + (int64_t)k * ldw;
-    hipLaunchKernelGGL((potrf_diag_kernel<T>), dim3(1), dim3(256), 0, s, Zkk, ldz, kb, Ykk, ldw, dinfo, k);
-    const int rest = n - k - kb;
-    if (rest > 0) {
-      // W[k, k+kb:] = W_kk^{-H} Z[k, k+kb:]  (through tmp)
-      T* Zrow = Z + k + (int64_t)(k + kb) * ldz;
-      gemm<T>(OP_C, OP_N, gargs<T>(kb, rest, kb, Ykk, ldw, Zrow, ldz, tmp, DB, 1.0, 0.0), s);
-      copy_mat<T>(kb, rest, tmp, DB, Zrow, ldz, s);
-      // Z[k+kb:, k+kb:] -= W[k, k+kb:]^H W[k, k+kb:]
-      T* Z22 = Z + (k + kb) + (int64_t)(k + kb) * ldz;
-      gemm<T>(OP_C, OP_N, gargs<T>(rest, rest, kb, Zrow, ldz, Zrow, ldz, Z22, ldz, -1.0, 1.0), s);
-    }
-  }
-  // off-diagonal blocks of W^{-1}, block column by block column:
-  //   Y[0:k, k:k+kb] = -(Y[0:k, 0:k] W[0:k, k:k+kb]) Y_kk
-  for (int k = DB; k < n; k += DB) {
-    const int kb = std::min(DB, n - k);
-    T* Yc = Winv + (int64_t)k * ldw;
-    gemm<T>(OP_N, OP_N, gargs<T>(k, kb, k, Winv, ldw, Z + (int64_t)k * ldz, ldz, Yc, ldw, 1.0, 0.0), s);
-    triangular_right_inplace<T>(Yc, ldw, k, kb, Winv + k + (int64_t)k * ldw, ldw, -1.0, s);
-  }
+  if (n <= 0) return;
+  potrf_inv_rec<T>(Z, ldz, Winv, ldw, n, 0, tmp, dinfo, s);
   hipLaunchKernelGGL((zero_lower_kernel<T>), dim3((unsigned)cdiv64((int64_t)n * n, 256)), dim3(256), 0, s, Z, ldz, n);
 }
 

@@ -65,6 +65,7 @@ struct TsqrExpandArgs {
 };
 
 template <typename T> int geqr2_node_capacity();
+extern int g_node_variant;  // node kernel selection (test hook; 0 = default per element type)
 template <typename T> void launch_geqr2_node(const Geqr2NodeArgs& a, int leaf_rows_max, hipStream_t s);
 template <typename T> void launch_tsqr_root_finish(const TsqrRootArgs& a, hipStream_t s);
 template <typename T> void launch_tsqr_expand(const TsqrExpandArgs& a, int max_node_rows, hipStream_t s);

@@ -19,6 +19,8 @@
 // Nguyen, Solomonik, "Reconstructing Householder vectors from tall-skinny QR",
 // 2014): Q - [D; 0] = V U is an LU factorisation without pivoting when each
 // d_j = -sign(pivot_j); then T = -U D^H V1^{-H} and R = D R_tree.
+#include <utility>
+
 #include "mak_common.hpp"
 #include "mak_kernels.hpp"
 
@@ -321,6 +323,281 @@ __global__ void __launch_bounds__(NT) geqr2_node_kernel(Geqr2NodeArgs args) {
         const int c = (i + krot) % NB;
         V1[r + c * NB] = (r > c && r < h && c < nbc) ? a[0][i] : (r == c ? s_one(T()) : s_zero(T()));
       }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Node factorisation with a 2-D lane layout (the production node kernel).
+//
+// A wave holds WR = 4 KR rows of the 32-column block: lane l owns columns
+// cl = l & 15 and cl + 16 of rows 4k + (l >> 4), k < KR.  Per column j:
+//   * the 4 lanes holding column j copy its rows below j into wave-private LDS,
+//     every lane reads back the KR entries of its own rows;
+//   * each lane forms conj(x)^T a for its two columns over its KR rows, the
+//     four row groups are summed with two lane-swap steps (permlane32/16), so
+//     lane l ends with column (l & 15) + 16 (l >> 5) of the wave's sum;
+//   * the NW wave sums meet in LDS behind ONE workgroup barrier per column;
+//   * reflector (xLARFG convention), update coefficients and the xLARFT
+//     products are formed redundantly per lane; the rank-1 update is 2 FMAs
+//     per row and lane.
+// The column loop runs two steps per iteration (step parity is a template
+// argument); registers are only ever selected by value, never indexed.
+// ---------------------------------------------------------------------------
+template <typename T, int NW, int WR> struct Node2Lds {
+  T red[2][32][NW];  // wave partial sums per column, double-buffered by step parity
+  T rowj[2][32];     // row j of the block, published at the end of step j - 1
+  T xs[NW][WR];      // wave-private copy of the current column (rows of the wave)
+  T Ysh[32][33];     // Ysh[j][c] = V_c^H v_j, c < j
+  T tau_s[32];
+  T scale_s[32];     // v_j = scale_j x_j below the diagonal (applied after the loop)
+  T beta_s[32];      // R(j, j)
+};
+
+template <typename T, int NW, int KR, int PAR>
+__device__ __forceinline__ void node2_step(T (&a)[KR][2], Node2Lds<T, NW, 4 * KR>& S, const int J, int kmax,
+                                           int nbc) {
+  // J is uniform (the column loop counter, J & 1 == PAR); J < 32 <= WR, so
+  // rows <= J all live in registers k < 8 of wave 0.  Finished columns keep
+  // their unscaled x below the diagonal until the loop ends.
+  constexpr int WR = 4 * KR;
+  const int l = lane_id();
+  const int w = threadIdx.x / WAVE;
+  const int g = l >> 4;
+  const int cl = l & 15;
+  const int d = w * WR + g - J;     // row of register k minus J: d + 4 k
+  const bool hi = J >= 16;           // register half holding column J (uniform)
+
+  // 1. the column-J lanes publish the column below row J
+  if (cl == (J & 15)) {
+    if (w == 0) {
+#pragma unroll
+      for (int k = 0; k < KR; ++k)
+        S.xs[0][g * KR + k] = sel(k >= 8 || d + 4 * k > 0, sel(hi, a[k][1], a[k][0]), s_zero(T()));
+    } else {
+#pragma unroll
+      for (int k = 0; k < KR; ++k) S.xs[w][g * KR + k] = sel(hi, a[k][1], a[k][0]);
+    }
+  }
+  wave_lds_sync();
+  T x[KR];
+#pragma unroll
+  for (int k = 0; k < KR; ++k) x[k] = S.xs[w][g * KR + k];
+
+  // 2. partial products conj(x)^T a over this lane's rows
+  T p0a = s_zero(T()), p0b = s_zero(T()), p1a = s_zero(T()), p1b = s_zero(T());
+#pragma unroll
+  for (int k = 0; k < KR; k += 2) {
+    p0a = s_cfma(x[k], a[k][0], p0a);
+    p1a = s_cfma(x[k], a[k][1], p1a);
+    if (k + 1 < KR) {
+      p0b = s_cfma(x[k + 1], a[k + 1][0], p0b);
+      p1b = s_cfma(x[k + 1], a[k + 1][1], p1b);
+    }
+  }
+  // 3. sum the four row groups: lanes < 32 -> column cl, lanes >= 32 -> cl + 16
+  T sw = swap_add32(s_add(p0a, p0b), s_add(p1a, p1b));
+  sw = swap_add16(sw, sw);
+  if ((l & 16) == 0) S.red[PAR][cl + ((l >> 5) << 4)][w] = sw;
+  __syncthreads();
+
+  // 4. workgroup totals of this lane's columns and of column J
+  T t0 = s_zero(T()), t1 = s_zero(T());
+  double nu2 = 0.0;
+#pragma unroll
+  for (int v = 0; v < NW; ++v) {
+    t0 = s_add(t0, S.red[PAR][cl][v]);
+    t1 = s_add(t1, S.red[PAR][cl + 16][v]);
+    nu2 += s_re(S.red[PAR][J][v]);
+  }
+  const T alpha = S.rowj[PAR][J];
+  const T r0 = S.rowj[PAR][cl];
+  const T r1 = S.rowj[PAR][cl + 16];
+  T tau, beta, scale;
+  if (nu2 == 0.0 && s_im(alpha) == 0.0) {
+    tau = s_zero(T());
+    beta = alpha;
+    scale = s_zero(T());
+  } else {
+    const double bn = sqrt(s_abs2(alpha) + nu2);
+    const double br = s_re(alpha) >= 0.0 ? -bn : bn;
+    beta = s_from_real<T>(br);
+    tau = s_scale(s_sub(beta, alpha), 1.0 / br);
+    scale = s_div(s_one(T()), s_sub(alpha, beta));
+  }
+  // update coefficients conj(tau) v^H a_c for J < c < nbc
+  const T ctau = s_conj(tau);
+  const T g0 = sel(cl > J && cl < nbc, s_mul(ctau, s_cfma(scale, t0, r0)), s_zero(T()));
+  const T g1 = sel(cl + 16 > J && cl + 16 < nbc, s_mul(ctau, s_cfma(scale, t1, r1)), s_zero(T()));
+  if (w == 0 && l < 16) {
+    // V_c^H v_J for c < J: column c still holds x_c (v_c = scale_c x_c below c)
+    if (cl < J) S.Ysh[J][cl] = s_mul(s_conj(S.scale_s[cl]), s_fma(scale, s_conj(t0), s_conj(r0)));
+    if (cl + 16 < J) S.Ysh[J][cl + 16] = s_mul(s_conj(S.scale_s[cl + 16]), s_fma(scale, s_conj(t1), s_conj(r1)));
+    if (l == 0) {
+      S.tau_s[J] = tau;
+      S.scale_s[J] = scale;
+      S.beta_s[J] = beta;
+    }
+  }
+
+  // 5. rank-1 update a_rc -= v_r g_c with v_r = scale x_r below row J and
+  //    v_J = 1 (x was zeroed on rows <= J); g = 0 on columns <= J.
+  if (w == 0) {
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      const T coef = (k < 8) ? sel(d + 4 * k == 0, s_one(T()), s_mul(scale, x[k])) : s_mul(scale, x[k]);
+      a[k][0] = s_sub(a[k][0], s_mul(coef, g0));
+      a[k][1] = s_sub(a[k][1], s_mul(coef, g1));
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      const T v = s_mul(scale, x[k]);
+      a[k][0] = s_sub(a[k][0], s_mul(v, g0));
+      a[k][1] = s_sub(a[k][1], s_mul(v, g1));
+    }
+  }
+  // 6. publish row J + 1 (register k1 = (J + 1) / 4 of row group (J + 1) & 3, wave 0)
+  const int J1 = J + 1;
+  if (J1 < kmax && w == 0 && g == (J1 & 3)) {
+    const int k1 = J1 >> 2;
+    T q0 = a[0][0], q1 = a[0][1];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+      q0 = sel(k == k1, a[k][0], q0);
+      q1 = sel(k == k1, a[k][1], q1);
+    }
+    S.rowj[PAR ^ 1][cl] = q0;
+    S.rowj[PAR ^ 1][cl + 16] = q1;
+  }
+}
+
+template <typename T, int NW, int KR>
+__device__ __forceinline__ void node2_all_steps(T (&a)[KR][2], Node2Lds<T, NW, 4 * KR>& S, int kmax, int nbc) {
+#pragma unroll 1
+  for (int j = 0; j < kmax; j += 2) {
+    node2_step<T, NW, KR, 0>(a, S, j, kmax, nbc);
+    if (j + 1 < kmax) node2_step<T, NW, KR, 1>(a, S, j + 1, kmax, nbc);
+  }
+}
+
+template <typename T, int J>
+__device__ __forceinline__ void node2_tcol(T (&Treg)[32], const T (*Ysh)[33], const T* tau_s, int r, int kmax) {
+  if (J >= kmax) return;
+  T acc0 = s_zero(T()), acc1 = s_zero(T());
+#pragma unroll
+  for (int k = 0; k < J; ++k) {
+    if (k & 1) acc1 = s_fma(Treg[k], Ysh[J][k], acc1);
+    else acc0 = s_fma(Treg[k], Ysh[J][k], acc0);
+  }
+  const T tj = tau_s[J];
+  Treg[J] = sel(r < J, s_neg(s_mul(tj, s_add(acc0, acc1))), sel(r == J, tj, s_zero(T())));
+}
+template <typename T, int... Js>
+__device__ __forceinline__ void node2_tcols(T (&Treg)[32], const T (*Ysh)[33], const T* tau_s, int r, int kmax,
+                                            std::integer_sequence<int, Js...>) {
+  (node2_tcol<T, Js>(Treg, Ysh, tau_s, r, kmax), ...);
+}
+
+template <typename T, int NW, int KR>
+__global__ void __launch_bounds__(NW * WAVE) geqr2_node2_kernel(Geqr2NodeArgs args) {
+  constexpr int WR = 4 * KR;
+  __shared__ Node2Lds<T, NW, WR> S;
+  const int node = blockIdx.x;
+  int r0, r1;
+  node_rows(node, args.nodes, args.H, args.chunk, r0, r1);
+  const int h = r1 - r0;
+  const int nbc = args.nbc;
+  T* X = reinterpret_cast<T*>(args.X) + r0;
+  const int64_t ldx = args.ldx;
+  const int tid = threadIdx.x;
+  const int l = lane_id();
+  const int w = tid / WAVE;
+  const int g = l >> 4;
+  const int cl = l & 15;
+  const int rbase = w * WR + g;
+
+  T a[KR][2];
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int r = rbase + 4 * k;
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+      const int c = cl + 16 * hh;
+      a[k][hh] = (r < h && c < nbc) ? X[r + c * ldx] : s_zero(T());
+    }
+  }
+  for (int i = tid; i < 32 * 33; i += NW * WAVE) (&S.Ysh[0][0])[i] = s_zero(T());
+  if (tid < 32) {
+    S.tau_s[tid] = s_zero(T());
+    S.scale_s[tid] = s_zero(T());
+    S.beta_s[tid] = s_zero(T());
+  }
+  const int kmax = min(h, nbc);
+  // row 0 is register 0 of row group 0 in wave 0
+  if (w == 0 && g == 0) {
+    S.rowj[0][cl] = a[0][0];
+    S.rowj[0][cl + 16] = a[0][1];
+  }
+  __syncthreads();
+  node2_all_steps<T, NW, KR>(a, S, kmax, nbc);
+  __syncthreads();
+  // finished columns: v_c = scale_c x_c below the diagonal, R(c, c) = beta_c
+#pragma unroll
+  for (int hh = 0; hh < 2; ++hh) {
+    const int c = cl + 16 * hh;
+    if (c < kmax) {
+      const T sc = S.scale_s[c];
+      const T bt = S.beta_s[c];
+#pragma unroll
+      for (int k = 0; k < KR; ++k) {
+        const int r = rbase + 4 * k;
+        a[k][hh] = sel(r > c, s_mul(sc, a[k][hh]), sel(r == c, bt, a[k][hh]));
+      }
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int r = rbase + 4 * k;
+    if (r < h) {
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        const int c = cl + 16 * hh;
+        if (c < nbc) X[r + c * ldx] = a[k][hh];
+      }
+    }
+  }
+  if (w == 0) {
+    // rows < 32 live in wave 0 (WR >= 32): registers k < 8
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int r = g + 4 * k;
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        const int c = cl + 16 * hh;
+        if (args.Rnext && r < nbc && c < nbc) {
+          T* Rn = reinterpret_cast<T*>(args.Rnext) + (int64_t)node * nbc;
+          Rn[r + (int64_t)c * args.ldr] = (r <= c && r < h) ? a[k][hh] : s_zero(T());
+        }
+        if (args.V1out) {
+          T* V1 = reinterpret_cast<T*>(args.V1out) + (int64_t)node * 32 * 32;
+          V1[r + c * 32] = (r > c && r < h && c < nbc) ? a[k][hh] : (r == c ? s_one(T()) : s_zero(T()));
+        }
+      }
+    }
+  }
+  if (args.Tout && w == 0) {
+    // T(0:j, j) = -tau_j T(0:j, 0:j) Y(0:j, j); T(j, j) = tau_j (xLARFT).  Lane r
+    // keeps row r of T; T(r, k) = 0 for k < r, so the sums need no masks.
+    T Treg[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c) Treg[c] = s_zero(T());
+    node2_tcols<T>(Treg, S.Ysh, S.tau_s, l, kmax, std::make_integer_sequence<int, 32>{});
+    T* To = reinterpret_cast<T*>(args.Tout) + (int64_t)node * args.tstride;
+    if (l < 32) {
+#pragma unroll
+      for (int c = 0; c < 32; ++c) To[l + c * 32] = Treg[c];
     }
   }
 }
@@ -693,6 +970,16 @@ __global__ void __launch_bounds__(NT) tsqr_expand_kernel(TsqrExpandArgs args) {
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
+// node kernel configuration: waves per node, rows per lane (4 * KR rows per wave)
+template <typename T> struct NodeCfg;
+template <> struct NodeCfg<double> { static constexpr int NW = 4, KR = 32; };
+template <> struct NodeCfg<zd> { static constexpr int NW = 4, KR = 16; };
+
+// 1: per-thread-row kernel, 2: 2-D lane layout (NodeCfg), 3: 2-D lane layout
+// with another wave count; 0: the measured-fastest per element type (real:
+// 1, 78 us per 512 x 32 node; complex: 2, 137 us per 256 x 32 node, on MI355X)
+int g_node_variant = 0;
+
 template <typename T> int geqr2_node_capacity() {
   return ScalarTraits<T>::is_complex ? 256 : 512;
 }
@@ -702,7 +989,17 @@ template int geqr2_node_capacity<zd>();
 template <typename T>
 void launch_geqr2_node(const Geqr2NodeArgs& a, int leaf_rows_max, hipStream_t s) {
   // the register layout holds geqr2_node_capacity<T>() rows per workgroup
-  if constexpr (ScalarTraits<T>::is_complex) {
+  static_assert(NodeCfg<T>::NW * 4 * NodeCfg<T>::KR == (ScalarTraits<T>::is_complex ? 256 : 512),
+                "node capacity");
+  const int variant = g_node_variant != 0 ? g_node_variant : (ScalarTraits<T>::is_complex ? 2 : 1);
+  if (variant == 2) {
+    constexpr int NW = NodeCfg<T>::NW, KR = NodeCfg<T>::KR;
+    hipLaunchKernelGGL((geqr2_node2_kernel<T, NW, KR>), dim3(a.nodes), dim3(NW * WAVE), 0, s, a);
+  } else if (variant == 3) {
+    // alternative shape of the same kernel (same capacity), for tuning
+    constexpr int NW = ScalarTraits<T>::is_complex ? 8 : 8, KR = ScalarTraits<T>::is_complex ? 8 : 16;
+    hipLaunchKernelGGL((geqr2_node2_kernel<T, NW, KR>), dim3(a.nodes), dim3(NW * WAVE), 0, s, a);
+  } else if constexpr (ScalarTraits<T>::is_complex) {
     hipLaunchKernelGGL((geqr2_node_kernel<T, 32, 1, 256>), dim3(a.nodes), dim3(256), 0, s, a);
   } else {
     hipLaunchKernelGGL((geqr2_node_kernel<T, 32, 2, 256>), dim3(a.nodes), dim3(256), 0, s, a);

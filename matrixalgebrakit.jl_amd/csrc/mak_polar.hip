@@ -46,7 +46,7 @@ template <typename T> PolarLayout polar_layout(int m, int n) {
   L.offG = take(nn);
   L.offY1 = take(nn);
   L.offY2 = take(nn);
-  L.offTmp = take((size_t)64 * std::max(n, 1) * sizeof(T));
+  L.offTmp = take(((size_t)64 * std::max(n, 1) + (size_t)n * n / 4) * sizeof(T));  // Cholesky scratch
   L.offPart = take(1024 * sizeof(double));
   L.offScal = take(16 * sizeof(double));
   L.offInfo = take(16 * sizeof(int));

@@ -32,6 +32,40 @@ int maktest_geqr2(void* stream, int dtype, int H, int nbc, void* X, int ldx, voi
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+// Select the node kernel used by every later launch (1: per-thread-row layout,
+// 2: 2-D lane layout, 3: 2-D lane layout with another wave count, 0: default
+// per element type); returns the previous selection.
+int maktest_set_node_variant(int v) {
+  const int prev = mak::g_node_variant;
+  if (v >= 0 && v <= 3) mak::g_node_variant = v;
+  return prev;
+}
+
+// One TSQR level: `nodes` workgroups over H stacked rows (balanced split),
+// each node factored in place; node i's T at Tout + i * 32 * 32.
+int maktest_geqr2_level(void* stream, int dtype, int H, int nodes, int nbc, void* X, int ldx, void* Tout) {
+  const int cap = dtype == MAK_F64 ? mak::geqr2_node_capacity<double>() : mak::geqr2_node_capacity<mak::zd>();
+  if (nodes < 1 || H < nodes || nbc < 1 || nbc > 32) return -4;
+  if ((H + nodes - 1) / nodes > cap) return -3;
+  mak::Geqr2NodeArgs g{};
+  g.X = X; g.ldx = ldx; g.H = H; g.nodes = nodes; g.chunk = 0; g.nbc = nbc;
+  g.Tout = Tout; g.tstride = 32 * 32;
+  if (dtype == MAK_F64) mak::launch_geqr2_node<double>(g, 0, (hipStream_t)stream);
+  else mak::launch_geqr2_node<mak::zd>(g, 0, (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// Cholesky with inverse: Z (n x n, upper part) -> W (upper) with Z = W^H W and
+// Y = W^{-1}; info receives j > 0 when the leading j x j minor is not positive
+// definite.  tmp: n * n / 4 + 64 * n elements.
+int maktest_potrf(void* stream, int dtype, int n, void* Z, int ldz, void* Y, int ldy, void* tmp, int* info) {
+  if (dtype == MAK_F64)
+    mak::potrf_upper<double>((double*)Z, ldz, n, (double*)Y, ldy, (double*)tmp, info, (hipStream_t)stream);
+  else
+    mak::potrf_upper<mak::zd>((mak::zd*)Z, ldz, n, (mak::zd*)Y, ldy, (mak::zd*)tmp, info, (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 // Blocked factorisation only: A <- compact WY (V below, R above), one 32x32 T
 // per panel written to Tpanels (32 x 32 x npanels); ws from the QR query.
 int maktest_geqrf(void* stream, int dtype, int m, int n, void* A, int lda, void* Tpanels, void* ws) {
