@@ -21,7 +21,9 @@ HIP events on the stream the library launches on:
   left_polar! 4096x4096 with sigma log-spaced 1 .. 1e-6 in FP64 / FP32 / BF16
   bulk (speedups over FP64, W error against the FP64 GPU result and against
   the exact U V^T of the construction),
-  qr and left-polar pullbacks on 256 x (2048 x 256) Float64.
+  qr and left-polar pullbacks on 256 x (2048 x 256) Float64,
+  batched qr_compact! on 4096 blocks (m log-uniform in [16, 1024], n uniform in
+  [8, m]) in Float64 and ComplexF64, one captured-graph execution per step.
 
 cpu_baseline runs the reference's LAPACK sequence (dgeqrt nb=36 -> dgemqrt on
 I -> triu + sign gauge, restated in oracle/mak_oracle.py through scipy's
@@ -216,6 +218,44 @@ def bench_pullbacks(mak, dev, stream, steps, batch=256, m=2048, n=256):
     return out
 
 
+def bench_batched(mak, dev, stream, steps, dtype, count=4096):
+    """4096 blocks, m log-uniform in [16, 1024], n uniform in [8, m], one
+    captured-graph execution per step (inputs refilled before each, untimed)."""
+    import torch
+    shapes = mak.block_shapes(count, seed=0)
+    g = torch.Generator(device="cpu").manual_seed(1)
+    A0 = []
+    for m, n in shapes:
+        x = torch.randn((n, m), generator=g, dtype=torch.float64)
+        if dtype == torch.complex128:
+            x = torch.complex(x, torch.randn((n, m), generator=g, dtype=torch.float64))
+        A0.append(x.to(dev).t())
+    As = [a.clone() for a in A0]
+    t0 = time.perf_counter()
+    plan = mak.BatchedQRPlan(As, tune=True)
+    t_plan = time.perf_counter() - t0
+    cplx = dtype == torch.complex128
+    fl = sum(qr_flops(m, n, False, cplx) for m, n in shapes)
+    ev = []
+    for i in range(steps + 1):
+        for a, a0 in zip(As, A0):
+            a.copy_(a0)
+        e = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        e[0].record(stream)
+        plan.execute()
+        e[1].record(stream)
+        ev.append(e)
+    torch.cuda.synchronize()
+    t = sorted(a.elapsed_time(b) for a, b in ev[1:])
+    ms = t[len(t) // 2]
+    Q, R = plan.Qs[0], plan.Rs[0]
+    res = (torch.linalg.norm(A0[0] - Q @ R) / torch.linalg.norm(A0[0])).item()
+    info = plan.info()
+    plan.close()
+    return {"ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 3), "gflop": round(fl / 1e9, 2),
+            "blocks": count, "plan_s": round(t_plan, 3), "plan": info, "rel_residual_block0": res}
+
+
 def run_extras(mak, dev, stream, groups, xs):
     import torch
     m, n = 8192, 1024
@@ -230,6 +270,11 @@ def run_extras(mak, dev, stream, groups, xs):
         torch.cuda.empty_cache()
     if "pullback" in groups:
         ex.update(bench_pullbacks(mak, dev, stream, xs))
+        torch.cuda.empty_cache()
+    if "batched" in groups:
+        ex["qr_batched_f64_4096_blocks"] = bench_batched(mak, dev, stream, xs, torch.float64)
+        torch.cuda.empty_cache()
+        ex["qr_batched_c64_4096_blocks"] = bench_batched(mak, dev, stream, xs, torch.complex128)
     return ex
 
 
@@ -240,8 +285,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
-    ap.add_argument("--extras", default="qr,polar,pullback",
-                    help="comma-separated extra workload groups: qr, polar, pullback")
+    ap.add_argument("--extras", default="qr,polar,pullback,batched",
+                    help="comma-separated extra workload groups: qr, polar, pullback, batched")
     ap.add_argument("--skip-headline", action="store_true",
                     help="profiling aid: run only the extra workloads (the JSON line then has no headline value)")
     ap.add_argument("--traffic-bytes", type=float, default=None,

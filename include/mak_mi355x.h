@@ -152,6 +152,38 @@ int mak_left_polar_pullback_z(void* stream, int batch, int m, int n, const void*
                               long long strideDA, int maxiter, double tol, void* work, size_t work_bytes,
                               int* iters);
 
+/* ------------------------------------------------------- batched QR --- */
+
+/* A plan for qr_compact!(A_i, (Q_i, R_i)) over `count` independent problems of
+ * arbitrary sizes m_i x n_i (Float64 or ComplexF64, all of one type).  All
+ * size, leading-dimension and pointer arrays are HOST arrays; the matrices are
+ * device memory that must stay valid for the plan's lifetime.  Plan creation
+ * builds the task lists in the caller's workspace, optionally autotunes the
+ * apply kernel (tune != 0: the workspace then also holds a scratch copy of the
+ * shapes; results are cached per shape set for the process), and captures the
+ * whole launch sequence in a hipGraph.  A_i is overwritten (compact WY form),
+ * R may be NULL (no R) or hold NULL entries.
+ * No counterpart in the reference, which has no batched API: it replaces a
+ * caller's loop of qr_compact! over blocks (src/implementations/qr.jl:110-113). */
+typedef struct mak_qr_batched_plan mak_qr_batched_plan;
+
+int mak_qr_batched_workspace_size(int dtype, int count, const int* m, const int* n, int tune, size_t* bytes);
+int mak_qr_batched_plan_create(mak_qr_batched_plan** plan, int dtype, int count, const int* m, const int* n,
+                               void* const* A, const int* lda, void* const* Q, const int* ldq, void* const* R,
+                               const int* ldr, int positive, int tune, void* work, size_t work_bytes);
+/* Replays the captured graph on `stream` (asynchronous). */
+int mak_qr_batched_execute(mak_qr_batched_plan* plan, void* stream);
+/* Chosen chunk width, persistent workgroups per CU, launches in the graph and
+ * problems routed to the single-matrix driver (any pointer may be NULL). */
+int mak_qr_batched_plan_info(const mak_qr_batched_plan* plan, int* chunk_cols, int* grid_per_cu, int* launches,
+                             int* fallback_problems);
+int mak_qr_batched_plan_destroy(mak_qr_batched_plan* plan);
+/* One-shot form: create (no tuning unless cached), execute, synchronise the
+ * stream, destroy. */
+int mak_qr_compact_batched(void* stream, int dtype, int count, const int* m, const int* n, void* const* A,
+                           const int* lda, void* const* Q, const int* ldq, void* const* R, const int* ldr,
+                           int positive, void* work, size_t work_bytes);
+
 #ifdef __cplusplus
 }
 #endif

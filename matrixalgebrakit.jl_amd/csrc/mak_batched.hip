@@ -1,0 +1,909 @@
+// Batched qr_compact! for thousands of independent, unevenly sized problems
+// (the block-sparse tensor-network workload), in one replayable launch graph.
+//
+// MatrixAlgebraKit has no batched API: a caller loops qr_compact! over its
+// blocks (src/implementations/qr.jl:110-113 per block).  Here every problem
+// follows the same blocked Householder algorithm as the single-matrix driver
+// (32-column panels, compact WY, backward orgqr, positive gauge), and the
+// panel steps of all problems run together:
+//   for panel step s: one node launch per row-size class factors panel s of
+//                     every problem that has it (geqr2_node2_tasks_kernel);
+//                     one persistent launch applies the block reflectors
+//                     (apply_wy_tasks_kernel, one task per 32/64-column chunk);
+//   Q: identity, then the same persistent apply launch backwards over s;
+//   R: triu + gauge, Q gauge.
+// The whole sequence is captured once into a hipGraph at plan creation and
+// replayed by mak_qr_batched_execute.  The chunk width and the persistent
+// grid size come from a small autotuner that times every candidate on a
+// scratch copy of the plan's shapes and caches the winner per shape set.
+//
+// Panels taller than one node (512 rows ComplexF64, 1024 Float64) are factored
+// by a two-leaf TSQR tree with Householder reconstruction (the single-matrix
+// panel kernels, launched over task lists); problems taller than two nodes run
+// through the single-matrix driver inside the same graph.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "mak_common.hpp"
+#include "mak_kernels.hpp"
+#include "mak_mi355x_internal.hpp"
+
+namespace mak {
+
+namespace {
+
+constexpr int NB = 32;
+
+inline size_t al256(size_t b) { return ((b + 255) / 256) * 256; }
+
+struct BProb {
+  void* A;
+  int64_t lda;
+  void* Q;
+  int64_t ldq;
+  void* R;
+  int64_t ldr;
+  int m, n;
+};
+
+struct ApplyTask {
+  const void* V;   // rows x nbc unit-lower panel (its first row is the diagonal row)
+  int64_t ldv;
+  const void* Tm;  // 32 x 32 upper (ld 32)
+  void* C;         // rows x ncols chunk
+  int64_t ldc;
+  int rows, ncols, nbc, pad;
+};
+
+// ---------------------------------------------------------------------------
+// C <- (I - V op(T) V^H) C  for one chunk of at most NC columns per task:
+//   W = V^H C (32 x NC, MFMA over KB-row blocks staged in LDS)
+//   W2 = op(T) W, op = conj-transpose when TRANS (geqrf) or identity (orgqr)
+//   C -= V W2 (MFMA per KB-row block; operands swapped so that each lane
+//              owns one row of C and the column stores are coalesced)
+// Persistent: workgroup b handles tasks b, b + gridDim.x, ...
+// ---------------------------------------------------------------------------
+template <typename T> struct ApplyCfg;
+template <> struct ApplyCfg<double> { static constexpr int KB = 64; };
+template <> struct ApplyCfg<zd> { static constexpr int KB = 32; };
+
+template <typename T> __device__ inline T vload_unit(const T* V, int64_t ldv, int k, int i, int rows, int nbc) {
+  // V(k, i) of a unit-lower panel: 1 on the diagonal, 0 above, stored below
+  if (k >= rows || i >= nbc || k < i) return s_zero(T());
+  if (k == i) return s_one(T());
+  return V[k + (int64_t)i * ldv];
+}
+
+// 16x16x4 complex product step on FP64 MFMA: acc += a b (conj(a) b when CA)
+template <bool CA>
+__device__ inline void zmfma(v4d& cr, v4d& ci, zd a, zd b) {
+  const double ai = CA ? -a.y : a.y;
+  cr = mfma_f64(a.x, b.x, cr);
+  cr = mfma_f64(-ai, b.y, cr);
+  ci = mfma_f64(a.x, b.y, ci);
+  ci = mfma_f64(ai, b.x, ci);
+}
+
+template <typename T, int NC, int TRANS>
+__global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* tasks, int ntasks) {
+  constexpr int KB = ApplyCfg<T>::KB;
+  constexpr bool CPLX = ScalarTraits<T>::is_complex;
+  __shared__ T Vs[KB][NB + 1];
+  __shared__ T Cs[KB][NC + 1];
+  __shared__ T Ws[NB][NC + 1];
+  __shared__ T Ts[NB][NB + 1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  for (int it = blockIdx.x; it < ntasks; it += gridDim.x) {
+    const ApplyTask t = tasks[it];
+    const T* V = reinterpret_cast<const T*>(t.V);
+    const T* Tg = reinterpret_cast<const T*>(t.Tm);
+    T* C = reinterpret_cast<T*>(t.C);
+    const int rows = t.rows, ncols = t.ncols, nbc = t.nbc;
+
+    // ---- phase 1: W = V^H C.  Tiles (2 x NC/16), NC/32 per wave: id = w + 4u
+    constexpr int TPW1 = NC / 32;
+    v4d acc_r[TPW1], acc_i[TPW1];
+#pragma unroll
+    for (int u = 0; u < TPW1; ++u) {
+      acc_r[u] = v4d{0.0, 0.0, 0.0, 0.0};
+      acc_i[u] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
+    for (int k0 = 0; k0 < rows; k0 += KB) {
+      __syncthreads();
+      for (int e = tid; e < KB * NB; e += 256) {
+        const int k = e % KB, i = e / KB;
+        Vs[k][i] = vload_unit(V, t.ldv, k0 + k, i, rows, nbc);
+      }
+      for (int e = tid; e < KB * NC; e += 256) {
+        const int k = e % KB, j = e / KB;
+        Cs[k][j] = (k0 + k < rows && j < ncols) ? C[(k0 + k) + (int64_t)j * t.ldc] : s_zero(T());
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < TPW1; ++u) {
+        const int id = w + 4 * u;
+        const int ti = id & 1, tj = id >> 1;
+#pragma unroll 4
+        for (int kk = 0; kk < KB; kk += 4) {
+          const T a = Vs[kk + lk][ti * 16 + lr];   // V^H(i, k) = conj(V(k, i))
+          const T b = Cs[kk + lk][tj * 16 + lr];
+          if constexpr (CPLX) zmfma<true>(acc_r[u], acc_i[u], a, b);
+          else acc_r[u] = mfma_f64(a, b, acc_r[u]);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < TPW1; ++u) {
+      const int id = w + 4 * u;
+      const int ti = id & 1, tj = id >> 1;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = ti * 16 + mfma_f64_row(lane, r), j = tj * 16 + lr;
+        if constexpr (CPLX) Ws[i][j] = mkz(acc_r[u][r], acc_i[u][r]);
+        else Ws[i][j] = acc_r[u][r];
+      }
+    }
+    for (int e = tid; e < NB * NB; e += 256) {
+      const int r = e % NB, c = e / NB;
+      Ts[r][c] = (r <= c && c < nbc) ? Tg[r + c * NB] : s_zero(T());
+    }
+    __syncthreads();
+    // ---- phase 2: W2 = op(T) W (same tile split), kept in registers, then LDS
+#pragma unroll
+    for (int u = 0; u < TPW1; ++u) {
+      acc_r[u] = v4d{0.0, 0.0, 0.0, 0.0};
+      acc_i[u] = v4d{0.0, 0.0, 0.0, 0.0};
+      const int id = w + 4 * u;
+      const int ti = id & 1, tj = id >> 1;
+#pragma unroll
+      for (int kk = 0; kk < NB; kk += 4) {
+        const int i = ti * 16 + lr, c = kk + lk;
+        const T a = TRANS ? Ts[c][i] : Ts[i][c];   // conjugated below when TRANS
+        const T b = Ws[c][tj * 16 + lr];
+        if constexpr (CPLX) {
+          if (TRANS) zmfma<true>(acc_r[u], acc_i[u], a, b);
+          else zmfma<false>(acc_r[u], acc_i[u], a, b);
+        } else {
+          acc_r[u] = mfma_f64(a, b, acc_r[u]);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < TPW1; ++u) {
+      const int id = w + 4 * u;
+      const int ti = id & 1, tj = id >> 1;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = ti * 16 + mfma_f64_row(lane, r), j = tj * 16 + lr;
+        if constexpr (CPLX) Ws[i][j] = mkz(acc_r[u][r], acc_i[u][r]);
+        else Ws[i][j] = acc_r[u][r];
+      }
+    }
+    // ---- phase 3: C -= V W2 per KB-row block; tiles (KB/16 x NC/16) over 4 waves
+    constexpr int TPW3 = (KB / 16) * (NC / 16) / 4;
+    for (int k0 = 0; k0 < rows; k0 += KB) {
+      __syncthreads();
+      for (int e = tid; e < KB * NB; e += 256) {
+        const int k = e % KB, i = e / KB;
+        Vs[k][i] = vload_unit(V, t.ldv, k0 + k, i, rows, nbc);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < TPW3; ++u) {
+        const int id = w + 4 * u;
+        const int ti = id % (KB / 16), tj = id / (KB / 16);
+        v4d dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < NB; kk += 4) {
+          // swapped operands: D' = W2^T V^T, lane owns row ti*16 + lr of C
+          const T a = Ws[kk + lk][tj * 16 + lr];  // (W2^T)(j, c)
+          const T b = Vs[ti * 16 + lr][kk + lk];  // (V^T)(c, i)
+          if constexpr (CPLX) zmfma<false>(dr, di, a, b);
+          else dr = mfma_f64(a, b, dr);
+        }
+        const int row = k0 + ti * 16 + lr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int col = tj * 16 + mfma_f64_row(lane, r);
+          if (row < rows && col < ncols) {
+            T* cp = C + row + (int64_t)col * t.ldc;
+            if constexpr (CPLX) *cp = s_sub(*cp, mkz(dr[r], di[r]));
+            else *cp = *cp - dr[r];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Per-problem elementwise passes (grid.y = problem, grid.x strides elements).
+template <typename T> __global__ void batched_identity_kernel(const BProb* P) {
+  const BProb p = P[blockIdx.y];
+  const int k = min(p.m, p.n);
+  T* Q = reinterpret_cast<T*>(p.Q);
+  const int64_t tot = (int64_t)p.m * k;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % p.m), c = (int)(e / p.m);
+    Q[r + c * p.ldq] = (r == c) ? s_one(T()) : s_zero(T());
+  }
+}
+
+template <typename T> __global__ void batched_finalize_kernel(const BProb* P, int positive) {
+  const BProb p = P[blockIdx.y];
+  const int k = min(p.m, p.n);
+  const T* A = reinterpret_cast<const T*>(p.A);
+  T* Q = reinterpret_cast<T*>(p.Q);
+  T* R = reinterpret_cast<T*>(p.R);
+  const int64_t totr = R ? (int64_t)k * p.n : 0;
+  const int64_t totq = positive ? (int64_t)p.m * k : 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < totr + totq;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (e < totr) {
+      const int r = (int)(e % k), c = (int)(e / k);
+      T v = s_zero(T());
+      if (r <= c) {
+        v = A[r + c * p.lda];
+        if (positive) v = (r == c) ? s_from_real<T>(s_absv(v)) : s_mul(s_conj(sign_safe(A[r + r * p.lda])), v);
+      }
+      R[r + c * p.ldr] = v;
+    } else {
+      const int64_t f = e - totr;
+      const int r = (int)(f % p.m), c = (int)(f / p.m);
+      Q[r + c * p.ldq] = s_mul(Q[r + c * p.ldq], sign_safe(A[c + c * p.lda]));
+    }
+  }
+}
+
+// deterministic pseudo-random fill of a tuning arena (values only steer no branch)
+template <typename T> __global__ void arena_fill_kernel(T* X, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t h = (uint64_t)e * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 29;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 32;
+    const double v = (double)(h & 0xFFFFF) / (double)0x100000 - 0.5;
+    X[e] = s_from_real<T>(v);
+  }
+}
+
+template <typename T>
+void launch_apply(const ApplyTask* tasks, int ntasks, int nc, int trans, int grid, hipStream_t s) {
+  if (ntasks <= 0) return;
+  const int g = std::max(1, std::min(grid, ntasks));
+  if (nc == 32) {
+    if (trans) hipLaunchKernelGGL((apply_wy_tasks_kernel<T, 32, 1>), dim3(g), dim3(256), 0, s, tasks, ntasks);
+    else hipLaunchKernelGGL((apply_wy_tasks_kernel<T, 32, 0>), dim3(g), dim3(256), 0, s, tasks, ntasks);
+  } else {
+    if (trans) hipLaunchKernelGGL((apply_wy_tasks_kernel<T, 64, 1>), dim3(g), dim3(256), 0, s, tasks, ntasks);
+    else hipLaunchKernelGGL((apply_wy_tasks_kernel<T, 64, 0>), dim3(g), dim3(256), 0, s, tasks, ntasks);
+  }
+}
+
+int device_cus() {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  return std::max(cus, 1);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Plan
+// ---------------------------------------------------------------------------
+struct BatchedQrPlan {
+  int dtype = 0, count = 0, positive = 1;
+  std::vector<BProb> probs;        // caller's matrices
+  std::vector<int> batched;        // problems on the batched path
+  std::vector<int> fallback;       // problems on the single-matrix driver
+  // workspace carve-up
+  char* ws = nullptr;
+  size_t ws_bytes = 0;
+  size_t offProbs = 0, offT = 0, offNode = 0, offApply = 0, offFallback = 0, offArena = 0, arenaBytes = 0;
+  size_t offTree = 0, offRf = 0, offEx = 0;
+  size_t fallbackBytes = 0;
+  std::vector<size_t> tOff;        // per problem: element offset of its T blocks
+  std::vector<size_t> treeOff;     // per problem with panels taller than a node: TSQR scratch offset
+  // launch schedule
+  struct Step {
+    int nodeOff[3], nodeCnt[3];    // leaves and whole-panel nodes, per size class
+    int rootOff, rootCnt;          // TSQR roots (class 0 node tasks) of tall panels
+    int rfOff, rfCnt;              // TSQR root finish tasks
+    int exOff, exCnt, exRows;      // TSQR expansion tasks, largest leaf
+    int applyOff, applyCnt;        // geqrf (trans = 1)
+  };
+  std::vector<Step> steps;
+  std::vector<int> qOff, qCnt;     // orgqr per step (trans = 0)
+  int nc = 64, gridMult = 2, cus = 256;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int launches = 0;
+  double tuned_ms = -1.0;
+};
+
+namespace {
+
+template <typename T> size_t elem() { return sizeof(T); }
+
+// Task lists for problem set P (pointers into A/Q/R of P) with chunk width nc.
+// Largest panel one node holds, and the TSQR scratch of one tall problem
+// (two leaves + one root): leaf T (2), leaf V1 (2), stacked R (64 x 32),
+// root T, M, U -- in elements.
+template <typename T> int node_cap() {
+  int c = 0;
+  for (int cls = 0; cls < 3; ++cls) c = std::max(c, node_class_rows<T>(cls));
+  return c;
+}
+constexpr size_t kTreeElems = 2 * NB * NB + 2 * NB * NB + 2 * NB * NB + 3 * NB * NB;
+
+template <typename T>
+void build_tasks(const std::vector<BProb>& P, const std::vector<int>& idx, const std::vector<size_t>& tOff,
+                 const std::vector<size_t>& treeOff, const T* Tbase, T* treeBase, int nc, std::vector<NodeTask>& node,
+                 std::vector<TsqrRootArgs>& rf, std::vector<TsqrExpandArgs>& ex, std::vector<ApplyTask>& apply,
+                 std::vector<BatchedQrPlan::Step>& steps, std::vector<int>& qOff, std::vector<int>& qCnt) {
+  int S = 0;
+  for (int i : idx) S = std::max(S, cdiv(std::min(P[i].m, P[i].n), NB));
+  steps.assign(S, BatchedQrPlan::Step{});
+  node.clear();
+  rf.clear();
+  ex.clear();
+  apply.clear();
+  const int cap = node_cap<T>();
+  auto cls_of = [&](int H) {
+    for (int cls = 0; cls < 3; ++cls)
+      if (H <= node_class_rows<T>(cls)) return cls;
+    return 3;
+  };
+  for (int s = 0; s < S; ++s) {
+    const int j0 = s * NB;
+    // leaves of tall panels and whole panels, grouped by row class
+    for (int cls = 0; cls < 3; ++cls) {
+      steps[s].nodeOff[cls] = (int)node.size();
+      for (int i : idx) {
+        const BProb& p = P[i];
+        const int k = std::min(p.m, p.n);
+        if (j0 >= k) continue;
+        const int H = p.m - j0;
+        const int nbc = std::min(NB, k - j0);
+        T* A = reinterpret_cast<T*>(p.A);
+        T* Ap = A + j0 + (int64_t)j0 * p.lda;
+        if (H <= cap) {
+          if (cls_of(H) != cls) continue;
+          NodeTask t{};
+          t.X = Ap;
+          t.ldx = p.lda;
+          t.Tout = const_cast<T*>(Tbase) + tOff[i] + (size_t)s * NB * NB;
+          t.H = H;
+          t.nbc = nbc;
+          node.push_back(t);
+        } else {
+          T* tb = treeBase + treeOff[i];
+          const int H1 = 2 * nbc;
+          for (int leaf = 0; leaf < 2; ++leaf) {
+            const int r0 = (leaf * H) / 2, r1 = ((leaf + 1) * H) / 2;
+            if (cls_of(r1 - r0) != cls) continue;
+            NodeTask t{};
+            t.X = Ap + r0;
+            t.ldx = p.lda;
+            t.Tout = tb + (size_t)leaf * NB * NB;                 // leaf T
+            t.V1 = tb + (size_t)(2 + leaf) * NB * NB;            // leaf V1
+            t.Rn = tb + (size_t)4 * NB * NB + (size_t)leaf * nbc;  // stacked R (ld H1)
+            t.ldr = H1;
+            t.H = r1 - r0;
+            t.nbc = nbc;
+            node.push_back(t);
+          }
+        }
+      }
+      steps[s].nodeCnt[cls] = (int)node.size() - steps[s].nodeOff[cls];
+    }
+    // roots, root finishes and expansions of the tall panels
+    steps[s].rootOff = (int)node.size();
+    steps[s].rfOff = (int)rf.size();
+    steps[s].exOff = (int)ex.size();
+    steps[s].exRows = 0;
+    for (int i : idx) {
+      const BProb& p = P[i];
+      const int k = std::min(p.m, p.n);
+      if (j0 >= k) continue;
+      const int H = p.m - j0;
+      if (H <= cap) continue;
+      const int nbc = std::min(NB, k - j0);
+      const int H1 = 2 * nbc;
+      T* A = reinterpret_cast<T*>(p.A);
+      T* Ap = A + j0 + (int64_t)j0 * p.lda;
+      T* tb = treeBase + treeOff[i];
+      T* T0 = tb;
+      T* V1 = tb + (size_t)2 * NB * NB;
+      T* X1 = tb + (size_t)4 * NB * NB;
+      T* T1 = tb + (size_t)6 * NB * NB;
+      T* M = tb + (size_t)7 * NB * NB;
+      T* U = tb + (size_t)8 * NB * NB;
+      NodeTask t{};
+      t.X = X1;
+      t.ldx = H1;
+      t.Tout = T1;
+      t.H = H1;
+      t.nbc = nbc;
+      node.push_back(t);
+      TsqrRootArgs r{};
+      r.Xtop = X1; r.ldxtop = H1; r.htop = H1; r.nbc = nbc; r.Ttop = T1;
+      r.nchain = 1; r.chainV1[0] = V1; r.chainLd[0] = NB; r.chainT[0] = T0;
+      r.Mtop = M; r.Tout = const_cast<T*>(Tbase) + tOff[i] + (size_t)s * NB * NB; r.Uout = U;
+      r.Apanel = Ap; r.lda = p.lda;
+      rf.push_back(r);
+      TsqrExpandArgs e{};
+      e.H = H; e.nodes = 2; e.chunk = 0; e.nbc = nbc;
+      e.X = Ap; e.ldx = p.lda; e.V1base = V1; e.v1stride = NB * NB; e.v1ld = NB;
+      e.Tnodes = T0; e.Mtop = M; e.Xtop = X1; e.ldxtop = H1; e.Umat = U;
+      ex.push_back(e);
+      steps[s].exRows = std::max(steps[s].exRows, cdiv(H, 2) + 1);
+    }
+    steps[s].rootCnt = (int)node.size() - steps[s].rootOff;
+    steps[s].rfCnt = (int)rf.size() - steps[s].rfOff;
+    steps[s].exCnt = (int)ex.size() - steps[s].exOff;
+    steps[s].applyOff = (int)apply.size();
+    for (int i : idx) {
+      const BProb& p = P[i];
+      const int k = std::min(p.m, p.n);
+      if (j0 >= k) continue;
+      const int nbc = std::min(NB, k - j0);
+      const int c0 = j0 + nbc;
+      T* A = reinterpret_cast<T*>(p.A);
+      for (int c = c0; c < p.n; c += nc) {
+        ApplyTask t{};
+        t.V = A + j0 + (int64_t)j0 * p.lda;
+        t.ldv = p.lda;
+        t.Tm = Tbase + tOff[i] + (size_t)s * NB * NB;
+        t.C = A + j0 + (int64_t)c * p.lda;
+        t.ldc = p.lda;
+        t.rows = p.m - j0;
+        t.ncols = std::min(nc, p.n - c);
+        t.nbc = nbc;
+        apply.push_back(t);
+      }
+    }
+    steps[s].applyCnt = (int)apply.size() - steps[s].applyOff;
+  }
+  qOff.assign(S, 0);
+  qCnt.assign(S, 0);
+  for (int s = S - 1; s >= 0; --s) {
+    const int j0 = s * NB;
+    qOff[s] = (int)apply.size();
+    for (int i : idx) {
+      const BProb& p = P[i];
+      const int k = std::min(p.m, p.n);
+      if (j0 >= k) continue;
+      const int nbc = std::min(NB, k - j0);
+      T* A = reinterpret_cast<T*>(p.A);
+      T* Q = reinterpret_cast<T*>(p.Q);
+      for (int c = j0; c < k; c += nc) {
+        ApplyTask t{};
+        t.V = A + j0 + (int64_t)j0 * p.lda;
+        t.ldv = p.lda;
+        t.Tm = Tbase + tOff[i] + (size_t)s * NB * NB;
+        t.C = Q + j0 + (int64_t)c * p.ldq;
+        t.ldc = p.ldq;
+        t.rows = p.m - j0;
+        t.ncols = std::min(nc, k - c);
+        t.nbc = nbc;
+        apply.push_back(t);
+      }
+    }
+    qCnt[s] = (int)apply.size() - qOff[s];
+  }
+}
+
+// Number of apply tasks for chunk width nc (to size the task buffer).
+size_t count_apply(const std::vector<BProb>& P, const std::vector<int>& idx, int nc) {
+  size_t cnt = 0;
+  for (int i : idx) {
+    const BProb& p = P[i];
+    const int k = std::min(p.m, p.n);
+    for (int j0 = 0; j0 < k; j0 += NB) {
+      const int nbc = std::min(NB, k - j0);
+      cnt += (size_t)cdiv(std::max(p.n - j0 - nbc, 0), nc) + (size_t)cdiv(k - j0, nc);
+    }
+  }
+  return cnt;
+}
+
+// Enqueue the whole schedule on stream s; returns the number of launches.
+template <typename T>
+int enqueue(const BatchedQrPlan& pl, const std::vector<BatchedQrPlan::Step>& steps, const std::vector<int>& qOff,
+            const std::vector<int>& qCnt, const BProb* dprobs, const std::vector<BProb>& P, int nprob_batched,
+            const NodeTask* dnode, const TsqrRootArgs* drf, const TsqrExpandArgs* dex, const ApplyTask* dapply,
+            int nc, int grid, int positive, bool with_fallback, hipStream_t s) {
+  int launches = 0;
+  for (size_t st = 0; st < steps.size(); ++st) {
+    for (int cls = 0; cls < 3; ++cls) {
+      if (steps[st].nodeCnt[cls] == 0) continue;
+      launch_geqr2_node_tasks<T>(dnode + steps[st].nodeOff[cls], steps[st].nodeCnt[cls], cls, s);
+      ++launches;
+    }
+    if (steps[st].rootCnt) {
+      launch_geqr2_node_tasks<T>(dnode + steps[st].rootOff, steps[st].rootCnt, 0, s);
+      launch_tsqr_root_finish_tasks<T>(drf + steps[st].rfOff, steps[st].rfCnt, s);
+      launch_tsqr_expand_tasks<T>(dex + steps[st].exOff, steps[st].exCnt, 2, steps[st].exRows, s);
+      launches += 3;
+    }
+    if (steps[st].applyCnt) {
+      launch_apply<T>(dapply + steps[st].applyOff, steps[st].applyCnt, nc, 1, grid, s);
+      ++launches;
+    }
+  }
+  if (nprob_batched > 0) {
+    hipLaunchKernelGGL((batched_identity_kernel<T>), dim3(64, nprob_batched), dim3(256), 0, s, dprobs);
+    ++launches;
+  }
+  for (int st = (int)steps.size() - 1; st >= 0; --st) {
+    if (qCnt[st]) {
+      launch_apply<T>(dapply + qOff[st], qCnt[st], nc, 0, grid, s);
+      ++launches;
+    }
+  }
+  if (nprob_batched > 0) {
+    hipLaunchKernelGGL((batched_finalize_kernel<T>), dim3(64, nprob_batched), dim3(256), 0, s, dprobs, positive);
+    ++launches;
+  }
+  if (with_fallback) {
+    for (int i : pl.fallback) {
+      const BProb& p = P[i];
+      qr_driver<T>(s, 0, p.m, p.n, reinterpret_cast<T*>(p.A), p.lda, reinterpret_cast<T*>(p.Q), p.ldq,
+                   reinterpret_cast<T*>(p.R), p.ldr, positive, pl.ws + pl.offFallback, pl.fallbackBytes, nullptr);
+      launches += 1;
+    }
+  }
+  return launches;
+}
+
+std::mutex g_tune_mu;
+std::map<uint64_t, std::pair<int, int>> g_tune_cache;  // shape-set hash -> (nc, grid multiplier)
+
+uint64_t shape_hash(int dtype, const std::vector<BProb>& P, const std::vector<int>& idx) {
+  uint64_t h = 1469598103934665603ull ^ (uint64_t)dtype;
+  auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+  for (int i : idx) {
+    mix((uint64_t)P[i].m);
+    mix((uint64_t)P[i].n);
+  }
+  return h;
+}
+
+struct Sizes {
+  size_t probs, tblocks, nodeTasks, applyTasks, fallback, arena, tree, rfTasks, exTasks;
+};
+
+template <typename T>
+Sizes plan_sizes(const std::vector<BProb>& P, const std::vector<int>& batched, const std::vector<int>& fallback,
+                 int tune) {
+  Sizes z{};
+  z.probs = al256(sizeof(BProb) * std::max<size_t>(P.size(), 1)) * 2;  // caller set + arena set
+  size_t tb = 0, nt = 0, tall = 0, ntall = 0;
+  const int cap = node_cap<T>();
+  for (int i : batched) {
+    const int k = std::min(P[i].m, P[i].n);
+    const int npan = cdiv(k, NB);
+    tb += (size_t)npan * NB * NB;
+    nt += npan;
+    if (P[i].m > cap) {
+      ++ntall;
+      for (int j0 = 0; j0 < k; j0 += NB)
+        if (P[i].m - j0 > cap) ++tall;
+    }
+  }
+  z.tblocks = al256(tb * sizeof(T));
+  z.nodeTasks = al256(sizeof(NodeTask) * std::max<size_t>(nt + 2 * tall, 1));
+  z.tree = al256(sizeof(T) * kTreeElems * std::max<size_t>(ntall, 1));
+  z.rfTasks = al256(sizeof(TsqrRootArgs) * std::max<size_t>(tall, 1));
+  z.exTasks = al256(sizeof(TsqrExpandArgs) * std::max<size_t>(tall, 1));
+  z.applyTasks = al256(sizeof(ApplyTask) * std::max<size_t>(count_apply(P, batched, 32), 1));
+  size_t fb = 0;
+  for (int i : fallback) fb = std::max(fb, qr_workspace_bytes<T>(P[i].m, P[i].n, 0));
+  z.fallback = al256(fb);
+  size_t ar = 0;
+  if (tune)
+    for (int i : batched) {
+      const int k = std::min(P[i].m, P[i].n);
+      ar += (size_t)P[i].m * P[i].n + (size_t)P[i].m * k + (size_t)k * P[i].n;
+    }
+  z.arena = al256(ar * sizeof(T));
+  return z;
+}
+
+template <typename T>
+void split_problems(const std::vector<BProb>& P, std::vector<int>& batched, std::vector<int>& fallback) {
+  batched.clear();
+  fallback.clear();
+  const int maxrows = 2 * node_cap<T>();  // two TSQR leaves per panel at most
+  for (int i = 0; i < (int)P.size(); ++i) {
+    if (std::min(P[i].m, P[i].n) == 0) {
+      fallback.push_back(i);  // empty factorisations: the driver's identity / empty-R path
+      continue;
+    }
+    if (P[i].m > maxrows) fallback.push_back(i);
+    else batched.push_back(i);
+  }
+  // larger problems first: the persistent apply launches then finish on small tasks
+  std::stable_sort(batched.begin(), batched.end(), [&](int a, int b) {
+    return (double)P[a].m * P[a].n * std::min(P[a].m, P[a].n) > (double)P[b].m * P[b].n * std::min(P[b].m, P[b].n);
+  });
+}
+
+bool upload_tasks(NodeTask* dnode, const std::vector<NodeTask>& node, TsqrRootArgs* drf,
+                  const std::vector<TsqrRootArgs>& rf, TsqrExpandArgs* dex, const std::vector<TsqrExpandArgs>& ex,
+                  ApplyTask* dapply, const std::vector<ApplyTask>& apply) {
+  bool ok = true;
+  if (!node.empty()) ok &= hipMemcpy(dnode, node.data(), sizeof(NodeTask) * node.size(), hipMemcpyHostToDevice) == hipSuccess;
+  if (!rf.empty()) ok &= hipMemcpy(drf, rf.data(), sizeof(TsqrRootArgs) * rf.size(), hipMemcpyHostToDevice) == hipSuccess;
+  if (!ex.empty()) ok &= hipMemcpy(dex, ex.data(), sizeof(TsqrExpandArgs) * ex.size(), hipMemcpyHostToDevice) == hipSuccess;
+  if (!apply.empty())
+    ok &= hipMemcpy(dapply, apply.data(), sizeof(ApplyTask) * apply.size(), hipMemcpyHostToDevice) == hipSuccess;
+  return ok;
+}
+
+template <typename T>
+int plan_create(BatchedQrPlan* pl, int count, const int* m, const int* n, void* const* A, const int* lda,
+                void* const* Q, const int* ldq, void* const* R, const int* ldr, int positive, int tune, void* work,
+                size_t work_bytes) {
+  pl->count = count;
+  pl->positive = positive ? 1 : 0;
+  pl->probs.resize(count);
+  for (int i = 0; i < count; ++i) {
+    if (m[i] < 0 || n[i] < 0) return -4;
+    const int k = std::min(m[i], n[i]);
+    if (lda[i] < std::max(1, m[i])) return -7;
+    if (ldq[i] < std::max(1, m[i])) return -9;
+    if (R && R[i] && ldr[i] < std::max(1, k)) return -11;
+    if ((int64_t)m[i] * n[i] > 0 && A[i] == nullptr) return -6;
+    if ((int64_t)m[i] * k > 0 && Q[i] == nullptr) return -8;
+    pl->probs[i] = BProb{A[i], lda[i], Q[i], ldq[i], R ? R[i] : nullptr, R ? ldr[i] : 1, m[i], n[i]};
+  }
+  split_problems<T>(pl->probs, pl->batched, pl->fallback);
+  const Sizes z = plan_sizes<T>(pl->probs, pl->batched, pl->fallback, tune);
+  size_t off = 0;
+  pl->offProbs = off; off += z.probs;
+  pl->offT = off; off += z.tblocks;
+  pl->offNode = off; off += z.nodeTasks;
+  pl->offApply = off; off += z.applyTasks;
+  pl->offFallback = off; off += z.fallback;
+  pl->fallbackBytes = z.fallback;
+  pl->offArena = off; off += z.arena;
+  pl->arenaBytes = z.arena;
+  pl->offTree = off; off += z.tree;
+  pl->offRf = off; off += z.rfTasks;
+  pl->offEx = off; off += z.exTasks;
+  if (work == nullptr || work_bytes < off) return -14;
+  pl->ws = reinterpret_cast<char*>(work);
+  pl->ws_bytes = work_bytes;
+  pl->cus = device_cus();
+  // T block offsets
+  pl->tOff.assign(count, 0);
+  size_t to = 0;
+  for (int i : pl->batched) {
+    pl->tOff[i] = to;
+    to += (size_t)cdiv(std::min(pl->probs[i].m, pl->probs[i].n), NB) * NB * NB;
+  }
+  pl->treeOff.assign(count, 0);
+  size_t tro = 0;
+  for (int i : pl->batched)
+    if (pl->probs[i].m > node_cap<T>()) {
+      pl->treeOff[i] = tro;
+      tro += kTreeElems;
+    }
+  T* Tbase = reinterpret_cast<T*>(pl->ws + pl->offT);
+  T* treeBase = reinterpret_cast<T*>(pl->ws + pl->offTree);
+  TsqrRootArgs* drf = reinterpret_cast<TsqrRootArgs*>(pl->ws + pl->offRf);
+  TsqrExpandArgs* dex = reinterpret_cast<TsqrExpandArgs*>(pl->ws + pl->offEx);
+  BProb* dprobs = reinterpret_cast<BProb*>(pl->ws + pl->offProbs);
+  BProb* dprobs_arena = dprobs + std::max<size_t>(pl->probs.size(), 1);
+  NodeTask* dnode = reinterpret_cast<NodeTask*>(pl->ws + pl->offNode);
+  ApplyTask* dapply = reinterpret_cast<ApplyTask*>(pl->ws + pl->offApply);
+
+  std::vector<BProb> bp;
+  for (int i : pl->batched) bp.push_back(pl->probs[i]);
+  if (!bp.empty() && hipMemcpy(dprobs, bp.data(), sizeof(BProb) * bp.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return 1;
+
+  // ---- autotuning on an arena copy of the shapes (values are synthetic)
+  std::vector<std::pair<int, int>> cands = {{64, 2}, {64, 1}, {64, 4}, {32, 2}, {32, 4}, {32, 1}};
+  const uint64_t key = shape_hash(std::is_same<T, double>::value ? 0 : 1, pl->probs, pl->batched);
+  bool cached = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      pl->nc = it->second.first;
+      pl->gridMult = it->second.second;
+      cached = true;
+    }
+  }
+  std::vector<NodeTask> node;
+  std::vector<TsqrRootArgs> rf;
+  std::vector<TsqrExpandArgs> ex;
+  std::vector<ApplyTask> apply;
+  std::vector<BatchedQrPlan::Step> steps;
+  std::vector<int> qOff, qCnt;
+  if (tune && !cached && !pl->batched.empty()) {
+    // arena problems: packed copies with the same shapes
+    std::vector<BProb> AP = pl->probs;
+    T* ar = reinterpret_cast<T*>(pl->ws + pl->offArena);
+    size_t ao = 0;
+    std::vector<BProb> bpa;
+    for (int i : pl->batched) {
+      BProb& p = AP[i];
+      const int k = std::min(p.m, p.n);
+      p.A = ar + ao; p.lda = std::max(1, p.m); ao += (size_t)p.m * p.n;
+      p.Q = ar + ao; p.ldq = std::max(1, p.m); ao += (size_t)p.m * k;
+      p.R = ar + ao; p.ldr = std::max(1, k); ao += (size_t)k * p.n;
+      bpa.push_back(p);
+    }
+    hipMemcpy(dprobs_arena, bpa.data(), sizeof(BProb) * bpa.size(), hipMemcpyHostToDevice);
+    hipStream_t ts;
+    hipStreamCreateWithFlags(&ts, hipStreamNonBlocking);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    double best = 1e30;
+    for (auto c : cands) {
+      build_tasks<T>(AP, pl->batched, pl->tOff, pl->treeOff, Tbase, treeBase, c.first, node, rf, ex, apply, steps,
+                     qOff, qCnt);
+      upload_tasks(dnode, node, drf, rf, dex, ex, dapply, apply);
+      float ms = 0.f;
+      for (int rep = 0; rep < 2; ++rep) {
+        hipLaunchKernelGGL((arena_fill_kernel<T>), dim3(1024), dim3(256), 0, ts, ar, (int64_t)(pl->arenaBytes / sizeof(T)));
+        hipEventRecord(e0, ts);
+        enqueue<T>(*pl, steps, qOff, qCnt, dprobs_arena, AP, (int)pl->batched.size(), dnode, drf, dex, dapply,
+                   c.first, c.second * pl->cus, pl->positive, false, ts);
+        hipEventRecord(e1, ts);
+        hipEventSynchronize(e1);
+        hipEventElapsedTime(&ms, e0, e1);
+      }
+      if (ms < best) {
+        best = ms;
+        pl->nc = c.first;
+        pl->gridMult = c.second;
+      }
+    }
+    pl->tuned_ms = best;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(ts);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = {pl->nc, pl->gridMult};
+  }
+  // ---- final task lists on the caller's matrices
+  build_tasks<T>(pl->probs, pl->batched, pl->tOff, pl->treeOff, Tbase, treeBase, pl->nc, node, rf, ex, apply, steps,
+                 qOff, qCnt);
+  if (!upload_tasks(dnode, node, drf, rf, dex, ex, dapply, apply)) return 1;
+  pl->steps = steps;
+  pl->qOff = qOff;
+  pl->qCnt = qCnt;
+  // ---- capture the schedule once
+  hipStream_t cs;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return 1;
+  int st = 0;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) st = 1;
+  if (!st) {
+    pl->launches = enqueue<T>(*pl, steps, qOff, qCnt, dprobs, pl->probs, (int)pl->batched.size(), dnode, drf, dex,
+                              dapply, pl->nc, pl->gridMult * pl->cus, pl->positive, true, cs);
+    if (hipStreamEndCapture(cs, &pl->graph) != hipSuccess) st = 1;
+  }
+  if (!st && hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0) != hipSuccess) st = 1;
+  hipStreamDestroy(cs);
+  return st;
+}
+
+}  // namespace
+
+size_t batched_qr_workspace_bytes(int dtype, int count, const int* m, const int* n, int tune) {
+  std::vector<BProb> P(count);
+  for (int i = 0; i < count; ++i) P[i] = BProb{nullptr, 1, nullptr, 1, nullptr, 1, std::max(m[i], 0), std::max(n[i], 0)};
+  std::vector<int> b, f;
+  Sizes z{};
+  if (dtype == MAK_F64) {
+    split_problems<double>(P, b, f);
+    z = plan_sizes<double>(P, b, f, tune);
+  } else {
+    split_problems<zd>(P, b, f);
+    z = plan_sizes<zd>(P, b, f, tune);
+  }
+  return z.probs + z.tblocks + z.nodeTasks + z.applyTasks + z.fallback + z.arena + z.tree + z.rfTasks + z.exTasks;
+}
+
+}  // namespace mak
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+struct mak_qr_batched_plan {
+  mak::BatchedQrPlan p;
+};
+
+extern "C" {
+
+int mak_qr_batched_workspace_size(int dtype, int count, const int* m, const int* n, int tune, size_t* bytes) {
+  if (dtype != MAK_F64 && dtype != MAK_C64) return -1;
+  if (count < 0) return -2;
+  if (count > 0 && (m == nullptr)) return -3;
+  if (count > 0 && (n == nullptr)) return -4;
+  if (bytes == nullptr) return -6;
+  *bytes = mak::batched_qr_workspace_bytes(dtype, count, m, n, tune);
+  return 0;
+}
+
+int mak_qr_batched_plan_create(mak_qr_batched_plan** plan, int dtype, int count, const int* m, const int* n,
+                               void* const* A, const int* lda, void* const* Q, const int* ldq, void* const* R,
+                               const int* ldr, int positive, int tune, void* work, size_t work_bytes) {
+  if (plan == nullptr) return -1;
+  *plan = nullptr;
+  if (dtype != MAK_F64 && dtype != MAK_C64) return -2;
+  if (count < 0 || count > 65535) return -3;  // one grid row per problem in the elementwise passes
+  if (count > 0 && (m == nullptr || n == nullptr || A == nullptr || lda == nullptr || Q == nullptr ||
+                    ldq == nullptr))
+    return -4;
+  if (R != nullptr && ldr == nullptr) return -11;
+  auto* h = new mak_qr_batched_plan();
+  h->p.dtype = dtype;
+  const int st = dtype == MAK_F64
+                     ? mak::plan_create<double>(&h->p, count, m, n, A, lda, Q, ldq, R, ldr, positive, tune, work,
+                                                work_bytes)
+                     : mak::plan_create<mak::zd>(&h->p, count, m, n, A, lda, Q, ldq, R, ldr, positive, tune, work,
+                                                 work_bytes);
+  if (st != 0) {
+    if (h->p.exec) hipGraphExecDestroy(h->p.exec);
+    if (h->p.graph) hipGraphDestroy(h->p.graph);
+    delete h;
+    return st;
+  }
+  *plan = h;
+  return 0;
+}
+
+int mak_qr_batched_execute(mak_qr_batched_plan* plan, void* stream) {
+  if (plan == nullptr) return -1;
+  if (plan->p.exec == nullptr) return 1;
+  return hipGraphLaunch(plan->p.exec, (hipStream_t)stream) == hipSuccess ? 0 : 1;
+}
+
+int mak_qr_batched_plan_info(const mak_qr_batched_plan* plan, int* chunk_cols, int* grid_per_cu, int* launches,
+                             int* fallback_problems) {
+  if (plan == nullptr) return -1;
+  if (chunk_cols) *chunk_cols = plan->p.nc;
+  if (grid_per_cu) *grid_per_cu = plan->p.gridMult;
+  if (launches) *launches = plan->p.launches;
+  if (fallback_problems) *fallback_problems = (int)plan->p.fallback.size();
+  return 0;
+}
+
+int mak_qr_batched_plan_destroy(mak_qr_batched_plan* plan) {
+  if (plan == nullptr) return 0;
+  if (plan->p.exec) hipGraphExecDestroy(plan->p.exec);
+  if (plan->p.graph) hipGraphDestroy(plan->p.graph);
+  delete plan;
+  return 0;
+}
+
+int mak_qr_compact_batched(void* stream, int dtype, int count, const int* m, const int* n, void* const* A,
+                           const int* lda, void* const* Q, const int* ldq, void* const* R, const int* ldr,
+                           int positive, void* work, size_t work_bytes) {
+  mak_qr_batched_plan* plan = nullptr;
+  int st = mak_qr_batched_plan_create(&plan, dtype, count, m, n, A, lda, Q, ldq, R, ldr, positive, 0, work,
+                                      work_bytes);
+  if (st != 0) return st == -14 ? -13 : st;  // argument positions match except the absent `tune`
+  st = mak_qr_batched_execute(plan, stream);
+  if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess && st == 0) st = 1;
+  mak_qr_batched_plan_destroy(plan);
+  return st;
+}
+
+}  // extern "C"

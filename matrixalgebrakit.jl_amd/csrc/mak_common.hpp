@@ -37,6 +37,7 @@ __device__ inline double s_conj(double a) { return a; }
 __device__ inline double s_re(double a) { return a; }
 __device__ inline double s_im(double) { return 0.0; }
 __device__ inline double s_abs2(double a) { return a * a; }
+__device__ inline double s_absv(double a) { return fabs(a); }
 __device__ inline double s_add(double a, double b) { return a + b; }
 __device__ inline double s_sub(double a, double b) { return a - b; }
 __device__ inline double s_mul(double a, double b) { return a * b; }
@@ -56,6 +57,7 @@ __device__ inline zd s_conj(zd a) { return mkz(a.x, -a.y); }
 __device__ inline double s_re(zd a) { return a.x; }
 __device__ inline double s_im(zd a) { return a.y; }
 __device__ inline double s_abs2(zd a) { return a.x * a.x + a.y * a.y; }
+__device__ inline double s_absv(zd a) { return hypot(a.x, a.y); }
 __device__ inline zd s_add(zd a, zd b) { return mkz(a.x + b.x, a.y + b.y); }
 __device__ inline zd s_sub(zd a, zd b) { return mkz(a.x - b.x, a.y - b.y); }
 __device__ inline zd s_mul(zd a, zd b) {

@@ -336,6 +336,11 @@ template void gemm<double>(int, int, GemmArgs, hipStream_t);
 template void gemm<zd>(int, int, GemmArgs, hipStream_t);
 
 void splitk_plan(int N, int K, int bn, int& ksplit, int& kchunk) {
+  if (N <= 0 || K <= 0) {  // empty products: one (empty) slice
+    ksplit = 1;
+    kchunk = std::max(K, 1);
+    return;
+  }
   const int tiles = cdiv(N, bn);
   int ks = cdiv(256, tiles);
   ks = std::min(ks, std::max(1, K / 256));
@@ -433,7 +438,8 @@ __global__ void qr_extract_r_kernel(const T* A, int64_t lda, T* R, int64_t ldr, 
   T v = s_zero(T());
   if (r < minmn && r <= c) {
     v = A[r + c * lda];
-    if (positive) v = s_mul(s_conj(sign_safe(A[r + r * lda])), v);
+    // positive gauge: row r times conj(sign(R'_rr)); the diagonal is |R'_rr|, exactly real
+    if (positive) v = (r == c) ? s_from_real<T>(s_absv(v)) : s_mul(s_conj(sign_safe(A[r + r * lda])), v);
   }
   R[r + c * ldr] = v;
 }

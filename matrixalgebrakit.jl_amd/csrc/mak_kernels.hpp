@@ -24,6 +24,18 @@ struct Geqr2NodeArgs {
   void* V1out;      // copy of the top 32x32 of V per node (unit lower), or null
 };
 
+// One independent panel node of a batch (geqr2_node2_tasks_kernel).
+struct NodeTask {
+  void* X;          // H x nbc panel, factored in place
+  int64_t ldx;
+  void* Tout;       // 32 x 32 T factor (ld 32)
+  void* Rn;         // optional: R (nbc x nbc) of the node, ld ldr (TSQR leaves)
+  int64_t ldr;
+  void* V1;         // optional: top 32 x 32 of V (unit lower), ld 32
+  int H;
+  int nbc;
+};
+
 constexpr int kMaxTreeLevels = 8;
 
 struct TsqrRootArgs {
@@ -65,6 +77,13 @@ struct TsqrExpandArgs {
 };
 
 template <typename T> int geqr2_node_capacity();
+// Node capacity of size class cls (0: 256, 1: 512, 2: 1024 rows; class 2 is
+// real only) for geqr2_node_tasks launches.
+template <typename T> int node_class_rows(int cls);
+template <typename T> void launch_geqr2_node_tasks(const NodeTask* tasks, int ntasks, int cls, hipStream_t s);
+template <typename T> void launch_tsqr_root_finish_tasks(const TsqrRootArgs* tasks, int ntasks, hipStream_t s);
+template <typename T>
+void launch_tsqr_expand_tasks(const TsqrExpandArgs* tasks, int ntasks, int nodes, int max_node_rows, hipStream_t s);
 extern int g_node_variant;  // node kernel selection (test hook; 0 = default per element type)
 template <typename T> void launch_geqr2_node(const Geqr2NodeArgs& a, int leaf_rows_max, hipStream_t s);
 template <typename T> void launch_tsqr_root_finish(const TsqrRootArgs& a, hipStream_t s);

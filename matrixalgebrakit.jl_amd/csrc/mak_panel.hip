@@ -499,17 +499,12 @@ __device__ __forceinline__ void node2_tcols(T (&Treg)[32], const T (*Ysh)[33], c
   (node2_tcol<T, Js>(Treg, Ysh, tau_s, r, kmax), ...);
 }
 
+// Factor one node: X (h x nbc, ld ldx) in place; T (32 x 32, ld 32) to Tout,
+// R (nbc x nbc) to Rn (ld ldr) and the top 32 x 32 of V to V1 when non-null.
 template <typename T, int NW, int KR>
-__global__ void __launch_bounds__(NW * WAVE) geqr2_node2_kernel(Geqr2NodeArgs args) {
+__device__ __forceinline__ void node2_body(Node2Lds<T, NW, 4 * KR>& S, T* X, int64_t ldx, int h, int nbc,
+                                           T* Tout, T* Rn, int64_t ldr, T* V1) {
   constexpr int WR = 4 * KR;
-  __shared__ Node2Lds<T, NW, WR> S;
-  const int node = blockIdx.x;
-  int r0, r1;
-  node_rows(node, args.nodes, args.H, args.chunk, r0, r1);
-  const int h = r1 - r0;
-  const int nbc = args.nbc;
-  T* X = reinterpret_cast<T*>(args.X) + r0;
-  const int64_t ldx = args.ldx;
   const int tid = threadIdx.x;
   const int l = lane_id();
   const int w = tid / WAVE;
@@ -576,30 +571,44 @@ __global__ void __launch_bounds__(NW * WAVE) geqr2_node2_kernel(Geqr2NodeArgs ar
 #pragma unroll
       for (int hh = 0; hh < 2; ++hh) {
         const int c = cl + 16 * hh;
-        if (args.Rnext && r < nbc && c < nbc) {
-          T* Rn = reinterpret_cast<T*>(args.Rnext) + (int64_t)node * nbc;
-          Rn[r + (int64_t)c * args.ldr] = (r <= c && r < h) ? a[k][hh] : s_zero(T());
-        }
-        if (args.V1out) {
-          T* V1 = reinterpret_cast<T*>(args.V1out) + (int64_t)node * 32 * 32;
-          V1[r + c * 32] = (r > c && r < h && c < nbc) ? a[k][hh] : (r == c ? s_one(T()) : s_zero(T()));
-        }
+        if (Rn && r < nbc && c < nbc) Rn[r + (int64_t)c * ldr] = (r <= c && r < h) ? a[k][hh] : s_zero(T());
+        if (V1) V1[r + c * 32] = (r > c && r < h && c < nbc) ? a[k][hh] : (r == c ? s_one(T()) : s_zero(T()));
       }
     }
   }
-  if (args.Tout && w == 0) {
+  if (Tout && w == 0) {
     // T(0:j, j) = -tau_j T(0:j, 0:j) Y(0:j, j); T(j, j) = tau_j (xLARFT).  Lane r
     // keeps row r of T; T(r, k) = 0 for k < r, so the sums need no masks.
     T Treg[32];
 #pragma unroll
     for (int c = 0; c < 32; ++c) Treg[c] = s_zero(T());
     node2_tcols<T>(Treg, S.Ysh, S.tau_s, l, kmax, std::make_integer_sequence<int, 32>{});
-    T* To = reinterpret_cast<T*>(args.Tout) + (int64_t)node * args.tstride;
     if (l < 32) {
 #pragma unroll
-      for (int c = 0; c < 32; ++c) To[l + c * 32] = Treg[c];
+      for (int c = 0; c < 32; ++c) Tout[l + c * 32] = Treg[c];
     }
   }
+}
+
+template <typename T, int NW, int KR>
+__global__ void __launch_bounds__(NW * WAVE) geqr2_node2_kernel(Geqr2NodeArgs args) {
+  __shared__ Node2Lds<T, NW, 4 * KR> S;
+  const int node = blockIdx.x;
+  int r0, r1;
+  node_rows(node, args.nodes, args.H, args.chunk, r0, r1);
+  T* Tout = args.Tout ? reinterpret_cast<T*>(args.Tout) + (int64_t)node * args.tstride : nullptr;
+  T* Rn = args.Rnext ? reinterpret_cast<T*>(args.Rnext) + (int64_t)node * args.nbc : nullptr;
+  T* V1 = args.V1out ? reinterpret_cast<T*>(args.V1out) + (int64_t)node * 32 * 32 : nullptr;
+  node2_body<T, NW, KR>(S, reinterpret_cast<T*>(args.X) + r0, args.ldx, r1 - r0, args.nbc, Tout, Rn, args.ldr, V1);
+}
+
+// One node per workgroup from a task list (independent problems of a batch).
+template <typename T, int NW, int KR>
+__global__ void __launch_bounds__(NW * WAVE) geqr2_node2_tasks_kernel(const NodeTask* tasks) {
+  __shared__ Node2Lds<T, NW, 4 * KR> S;
+  const NodeTask t = tasks[blockIdx.x];
+  node2_body<T, NW, KR>(S, reinterpret_cast<T*>(t.X), t.ldx, t.H, t.nbc, reinterpret_cast<T*>(t.Tout),
+                        reinterpret_cast<T*>(t.Rn), t.ldr, reinterpret_cast<T*>(t.V1));
 }
 
 // ---------------------------------------------------------------------------
@@ -661,7 +670,7 @@ __device__ inline T vload(const T* V, int64_t ldv, int r, int c) {
 // Every operand is staged in LDS first; triangular inverses run as one
 // rank-1 update per step over all (row, col) pairs.
 template <typename T, int NB, int NT>
-__global__ void __launch_bounds__(NT) tsqr_root_finish_kernel(TsqrRootArgs args) {
+__device__ __forceinline__ void tsqr_root_finish_body(const TsqrRootArgs& args) {
   __shared__ T Vt[NB][NB + 1];   // top node's V1 (unit lower) ; R_top above
   __shared__ T Rt[NB][NB + 1];
   __shared__ T M[NB][NB + 1];
@@ -781,6 +790,16 @@ __global__ void __launch_bounds__(NT) tsqr_root_finish_kernel(TsqrRootArgs args)
 // rows become Householder vectors V = Q U^{-1} (rows at or below nbc of the
 // panel).  Both products run on FP64 MFMA: a workgroup owns 64 rows of one
 // node, each wave 16 rows x 32 columns (two 16x16 blocks, K = 32).
+
+template <typename T, int NB, int NT>
+__global__ void __launch_bounds__(NT) tsqr_root_finish_kernel(TsqrRootArgs args) {
+  tsqr_root_finish_body<T, NB, NT>(args);
+}
+template <typename T, int NB, int NT>
+__global__ void __launch_bounds__(NT) tsqr_root_finish_tasks_kernel(const TsqrRootArgs* tasks) {
+  tsqr_root_finish_body<T, NB, NT>(tasks[blockIdx.x]);
+}
+
 template <typename T> struct ExpandLds;
 template <> struct ExpandLds<double> {
   double Ar[32][64 + 2];   // A operand, k-major: Ar[k][row]
@@ -836,7 +855,7 @@ template <> __device__ inline double mk_from<double>(double re, double) { return
 template <> __device__ inline zd mk_from<zd>(double re, double im) { return mkz(re, im); }
 
 template <typename T, int NB, int NT>
-__global__ void __launch_bounds__(NT) tsqr_expand_kernel(TsqrExpandArgs args) {
+__device__ __forceinline__ void tsqr_expand_body(const TsqrExpandArgs& args, const int node, const int tile_y) {
   static_assert(NB == 32 && NT == 256, "expand layout");
   __shared__ T Tn[NB][NB + 1];
   __shared__ T V1[NB][NB + 1];
@@ -844,7 +863,6 @@ __global__ void __launch_bounds__(NT) tsqr_expand_kernel(TsqrExpandArgs args) {
   __shared__ T W[NB][NB + 1];
   __shared__ T Qb2[NB][NB + 1];
   __shared__ ExpandLds<T> L;
-  const int node = blockIdx.x;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = tid >> 6;
@@ -852,7 +870,7 @@ __global__ void __launch_bounds__(NT) tsqr_expand_kernel(TsqrExpandArgs args) {
   int r0, r1;
   node_rows(node, args.nodes, args.H, args.chunk, r0, r1);
   const int hnode = r1 - r0;
-  const int tile0 = blockIdx.y * 64;  // first local row of this workgroup
+  const int tile0 = tile_y * 64;  // first local row of this workgroup
   if (tile0 >= hnode) return;         // uniform per workgroup
 
   const T* Tsrc = reinterpret_cast<const T*>(args.Tnodes) + (int64_t)node * NB * NB;
@@ -967,6 +985,16 @@ __global__ void __launch_bounds__(NT) tsqr_expand_kernel(TsqrExpandArgs args) {
     }
 }
 
+
+template <typename T, int NB, int NT>
+__global__ void __launch_bounds__(NT) tsqr_expand_kernel(TsqrExpandArgs args) {
+  tsqr_expand_body<T, NB, NT>(args, blockIdx.x, blockIdx.y);
+}
+template <typename T, int NB, int NT>
+__global__ void __launch_bounds__(NT) tsqr_expand_tasks_kernel(const TsqrExpandArgs* tasks) {
+  tsqr_expand_body<T, NB, NT>(tasks[blockIdx.z], blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -1006,6 +1034,42 @@ void launch_geqr2_node(const Geqr2NodeArgs& a, int leaf_rows_max, hipStream_t s)
   }
   (void)leaf_rows_max;
 }
+
+template <typename T> int node_class_rows(int cls) {
+  if (cls == 0) return 256;
+  if (cls == 1) return 512;
+  return ScalarTraits<T>::is_complex ? 0 : 1024;
+}
+
+template <typename T> void launch_geqr2_node_tasks(const NodeTask* tasks, int ntasks, int cls, hipStream_t s) {
+  if (ntasks <= 0) return;
+  if constexpr (ScalarTraits<T>::is_complex) {
+    if (cls == 0) hipLaunchKernelGGL((geqr2_node2_tasks_kernel<T, 4, 16>), dim3(ntasks), dim3(256), 0, s, tasks);
+    else hipLaunchKernelGGL((geqr2_node2_tasks_kernel<T, 8, 16>), dim3(ntasks), dim3(512), 0, s, tasks);
+  } else {
+    if (cls == 0) hipLaunchKernelGGL((geqr2_node2_tasks_kernel<T, 4, 16>), dim3(ntasks), dim3(256), 0, s, tasks);
+    else if (cls == 1) hipLaunchKernelGGL((geqr2_node2_tasks_kernel<T, 4, 32>), dim3(ntasks), dim3(256), 0, s, tasks);
+    else hipLaunchKernelGGL((geqr2_node2_tasks_kernel<T, 8, 32>), dim3(ntasks), dim3(512), 0, s, tasks);
+  }
+}
+template <typename T> void launch_tsqr_root_finish_tasks(const TsqrRootArgs* tasks, int ntasks, hipStream_t s) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL((tsqr_root_finish_tasks_kernel<T, 32, 256>), dim3(ntasks), dim3(256), 0, s, tasks);
+}
+template <typename T>
+void launch_tsqr_expand_tasks(const TsqrExpandArgs* tasks, int ntasks, int nodes, int max_node_rows, hipStream_t s) {
+  if (ntasks <= 0) return;
+  dim3 grid(nodes, cdiv(max_node_rows, 64), ntasks);
+  hipLaunchKernelGGL((tsqr_expand_tasks_kernel<T, 32, 256>), grid, dim3(256), 0, s, tasks);
+}
+template void launch_tsqr_root_finish_tasks<double>(const TsqrRootArgs*, int, hipStream_t);
+template void launch_tsqr_root_finish_tasks<zd>(const TsqrRootArgs*, int, hipStream_t);
+template void launch_tsqr_expand_tasks<double>(const TsqrExpandArgs*, int, int, int, hipStream_t);
+template void launch_tsqr_expand_tasks<zd>(const TsqrExpandArgs*, int, int, int, hipStream_t);
+template int node_class_rows<double>(int);
+template int node_class_rows<zd>(int);
+template void launch_geqr2_node_tasks<double>(const NodeTask*, int, int, hipStream_t);
+template void launch_geqr2_node_tasks<zd>(const NodeTask*, int, int, hipStream_t);
 
 template <typename T> void launch_tsqr_root_finish(const TsqrRootArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((tsqr_root_finish_kernel<T, 32, 256>), dim3(1), dim3(256), 0, s, a);

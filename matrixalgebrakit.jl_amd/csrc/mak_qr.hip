@@ -245,7 +245,7 @@ int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, in
               int positive, void* work, size_t work_bytes, T* Tout) {
   if (m < 0) return -2;
   if (n < 0) return -3;
-  if (m > 0 && A == nullptr) return -4;
+  if ((int64_t)m * n > 0 && A == nullptr) return -4;
   if (lda < std::max(1, m)) return -5;
   const int minmn = std::min(m, n);
   const int qcols = full ? m : minmn;

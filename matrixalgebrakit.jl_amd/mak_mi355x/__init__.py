@@ -13,3 +13,7 @@ from .pullbacks import *  # noqa: F401,F403,E402
 from .pullbacks import __all__ as _pb_all  # noqa: E402
 
 __all__ += list(_pb_all)
+from .batched import *  # noqa: F401,F403,E402
+from .batched import __all__ as _b_all  # noqa: E402
+
+__all__ += list(_b_all)

@@ -42,6 +42,13 @@ SIGNATURES = {
                                        _vp, _i, _ll, _i, _d, _vp, _sz, _vp]),
     "mak_left_polar_pullback_z": (_i, [_vp, _i, _i, _i, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll,
                                        _vp, _i, _ll, _i, _d, _vp, _sz, _vp]),
+    "mak_qr_batched_workspace_size": (_i, [_i, _i, _vp, _vp, _i, C.POINTER(_sz)]),
+    "mak_qr_batched_plan_create": (_i, [C.POINTER(_vp), _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i,
+                                        _vp, _sz]),
+    "mak_qr_batched_execute": (_i, [_vp, _vp]),
+    "mak_qr_batched_plan_info": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "mak_qr_batched_plan_destroy": (_i, [_vp]),
+    "mak_qr_compact_batched": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _sz]),
 }
 
 

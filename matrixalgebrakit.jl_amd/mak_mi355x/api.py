@@ -78,7 +78,9 @@ def colmajor(x: torch.Tensor) -> torch.Tensor:
     """Column-major copy (stride(0) == 1) of a 2-D tensor (no copy if already)."""
     if x.dim() != 2:
         raise DimensionMismatch(f"expected a matrix, got shape {tuple(x.shape)}")
-    if x.stride(0) == 1 and x.stride(1) >= max(1, x.shape[0]):
+    m, n = x.shape
+    # strides of size-1 dimensions carry no layout information
+    if (m <= 1 or x.stride(0) == 1) and (n <= 1 or x.stride(1) >= max(1, m)):
         return x
     return x.t().contiguous().t()
 

@@ -73,3 +73,16 @@ def test_initialize_output_layout():
     assert Q.shape == (7, 5) and R.shape == (5, 5) and Q.stride(0) == 1
     Q, R = mak.initialize_output_qr(A, True)
     assert Q.shape == (7, 7) and R.shape == (7, 5)
+
+
+def test_workspace_queries_accept_empty_shapes():
+    """Host-side size queries (no GPU work) for empty factorisations."""
+    lib = _lib.load()
+    b = C.c_size_t(0)
+    for dt in (0, 1):
+        for full in (0, 1):
+            for m, n in ((4, 0), (0, 5), (0, 0), (1, 1)):
+                assert lib.mak_qr_workspace_size(dt, full, m, n, C.byref(b)) == 0
+        ms = (C.c_int * 5)(0, 4, 1, 3000, 700)
+        ns = (C.c_int * 5)(5, 0, 1, 40, 700)
+        assert lib.mak_qr_batched_workspace_size(dt, 5, ms, ns, 1, C.byref(b)) == 0 and b.value > 0

@@ -40,7 +40,7 @@ def run_qr(A, dev, full=False, positive=True, with_r=True):
     return Q.cpu().numpy(), R.cpu().numpy()
 
 
-SIZES = [(54, 37), (54, 54), (54, 63), (1, 1), (5, 1), (1, 5), (33, 33), (100, 32), (64, 100),
+SIZES = [(54, 37), (54, 54), (54, 63), (1, 1), (5, 1), (1, 5), (4, 0), (0, 5), (33, 33), (100, 32), (64, 100),
          (1024, 512), (1025, 64), (2100, 64), (3000, 200), (200, 300), (700, 700)]
 
 
