@@ -7,6 +7,7 @@
 #include <algorithm>
 
 #include "mak_common.hpp"
+#include "mak_dense32.hpp"
 #include "mak_kernels.hpp"
 #include "mak_mi355x_internal.hpp"
 
@@ -19,103 +20,66 @@ template <typename T> void fill_zero(T* A, int64_t lda, int m, int n, hipStream_
 // Factor the kb x kb (kb <= 64) diagonal block Z (upper part read) into W
 // (upper, written over Z; strict lower part set to zero) and write W^{-1}
 // (upper, zero below) to Y.  info[0] receives offset + j + 1 when the leading
-// (j+1) x (j+1) minor is not positive definite; the kernel then stops.
+// (j+1) x (j+1) minor is not positive definite.
 //
-// Layout: wave q holds rows 16q .. 16q + 15 of the block, lane c column c, in
-// registers.  Cholesky (right-looking, one barrier per column): the wave
-// holding row j publishes it to LDS; every lane forms W(j, c) = Z(j, c) / d_j
-// and updates its rows r > j.  Inverse (one barrier per column): column k of
-// Y = W^{-1} is final once columns < k have been added in,
-// Y(r, k) = (delta_rk - sum_{i<k} Y(r, i) W(i, k)) / W(k, k); the lanes of
-// column k publish it and the lanes of columns > k accumulate it.
+// 2 x 2 blocks of 32, one workgroup of 4 waves, everything in LDS:
+//   W11, Y11 = chol_inv(Z11)                 (wave 0, no workgroup barrier)
+//   W12 = Y11^H Z12 ; Z22 -= W12^H W12       (MFMA, 4 waves)
+//   W22, Y22 = chol_inv(Z22)                 (wave 0)
+//   Y12 = -(Y11 W12) Y22                     (MFMA)
 // Rows/columns >= kb are padded with the identity.
 template <typename T>
 __global__ void __launch_bounds__(256) potrf_inv64_kernel(T* Z, int64_t ldz, int kb, T* Y, int64_t ldy,
                                                           int* info, int offset) {
-  __shared__ T Wsh[DB][DB + 1];
-  __shared__ T rowb[2][DB];
-  __shared__ T colb[2][DB];
+  __shared__ T Z11[32][33], Z12[32][33], Z22[32][33];
+  __shared__ T Y11[32][33], Y12[32][33], Y22[32][33];
+  __shared__ int bad_s;
   const int tid = threadIdx.x;
-  const int c = tid & 63;
-  const int q = tid >> 6;
-  T s[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int r = 16 * q + i;
-    s[i] = (r <= c && c < kb) ? Z[r + c * ldz] : ((r == c) ? s_one(T()) : s_zero(T()));
+  const int w = tid >> 6;
+  for (int e = tid; e < 32 * 32; e += 256) {
+    const int r = e % 32, c = e / 32;
+    auto ld = [&](int rr, int cc) {
+      return (rr <= cc && cc < kb) ? Z[rr + (int64_t)cc * ldz] : ((rr == cc) ? s_one(T()) : s_zero(T()));
+    };
+    Z11[r][c] = ld(r, c);
+    Z12[r][c] = ld(r, c + 32);
+    Z22[r][c] = ld(r + 32, c + 32);
   }
-  if (q == 0) rowb[0][c] = s[0];
+  if (tid == 0) bad_s = 0;
   __syncthreads();
-  int bad = 0;
-  for (int jq = 0; jq < 4; ++jq) {
-#pragma unroll
-    for (int ji = 0; ji < 16; ++ji) {
-      const int j = 16 * jq + ji;
-      const int par = ji & 1;
-      const double d2 = s_re(rowb[par][j]);
-      // uniform: every thread read the same value after the same barrier
-      if (!bad && !(d2 > 0.0)) bad = j + 1;
-      if (!bad) {
-        const double d = sqrt(d2);
-        const double rd = 1.0 / d;
-        const T wjc = sel(c > j, s_scale(rowb[par][c], rd), sel(c == j, s_from_real<T>(d), s_zero(T())));
-        if (q == jq) {
-          s[ji] = wjc;
-          Wsh[j][c] = wjc;
-        }
-        if (q >= jq) {
-#pragma unroll
-          for (int i = 0; i < 16; ++i) {
-            const int r = 16 * q + i;
-            const T wjr = s_scale(rowb[par][r], rd);
-            s[i] = sel(r > j && r <= c, s_sub(s[i], s_mul(s_conj(wjr), wjc)), s[i]);
-          }
-        }
-        const int j1 = j + 1;
-        if (j1 < DB && q == (j1 >> 4)) rowb[par ^ 1][c] = s[(ji + 1) & 15];
-        __syncthreads();
-      }
-    }
+  if (w == 0) {
+    const int b = wave_chol_inv32<T>(Z11, Y11);
+    if (lane_id() == 0) bad_s = b;
   }
-  if (bad) {
-    if (tid == 0 && info && info[0] == 0) info[0] = offset + bad;
+  __syncthreads();
+  if (bad_s) {
+    if (tid == 0 && info && info[0] == 0) info[0] = offset + bad_s;
     return;
   }
-  // inverse: s[i] accumulates sum_{k < c} Y(16q + i, k) W(k, c), then holds Y
-#pragma unroll
-  for (int i = 0; i < 16; ++i) s[i] = s_zero(T());
-  for (int kq = 0; kq < 4; ++kq) {
-#pragma unroll
-    for (int ki = 0; ki < 16; ++ki) {
-      const int k = 16 * kq + ki;
-      const int par = ki & 1;
-      const T invk = s_div(s_one(T()), Wsh[k][k]);
-      if (c == k) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int r = 16 * q + i;
-          const T y = sel(r < k, s_neg(s_mul(s[i], invk)), sel(r == k, invk, s_zero(T())));
-          s[i] = y;
-          colb[par][r] = y;
-        }
-      }
-      __syncthreads();
-      if (c > k && 16 * q <= k) {
-        const T wkc = Wsh[k][c];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int r = 16 * q + i;
-          s[i] = sel(r <= k, s_fma(colb[par][r], wkc, s[i]), s[i]);
-        }
-      }
-    }
+  mm32<T>(Z12, Y11, true, Z12, false, 1.0, false);    // W12 = Y11^H Z12
+  mm32<T>(Z22, Z12, true, Z12, false, -1.0, true);    // Z22 -= W12^H W12
+  if (w == 0) {
+    const int b = wave_chol_inv32<T>(Z22, Y22);
+    if (lane_id() == 0) bad_s = b;
   }
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int r = 16 * q + i;
+  __syncthreads();
+  if (bad_s) {
+    if (tid == 0 && info && info[0] == 0) info[0] = offset + 32 + bad_s;
+    return;
+  }
+  mm32<T>(Y12, Y11, false, Z12, false, 1.0, false);   // Y11 W12
+  mm32<T>(Y12, Y12, false, Y22, false, -1.0, false);  // -(Y11 W12) Y22
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int r = e % 64, c = e / 64;
     if (r < kb && c < kb) {
-      Z[r + c * ldz] = (r <= c) ? Wsh[r][c] : s_zero(T());
-      Y[r + c * ldy] = (r <= c) ? s[i] : s_zero(T());
+      T wv = s_zero(T()), yv = s_zero(T());
+      if (r <= c) {
+        if (c < 32) { wv = Z11[r][c]; yv = Y11[r][c]; }
+        else if (r < 32) { wv = Z12[r][c - 32]; yv = Y12[r][c - 32]; }
+        else { wv = Z22[r - 32][c - 32]; yv = Y22[r - 32][c - 32]; }
+      }
+      Z[r + (int64_t)c * ldz] = wv;
+      Y[r + (int64_t)c * ldy] = yv;
     }
   }
 }

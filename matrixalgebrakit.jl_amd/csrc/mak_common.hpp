@@ -149,6 +149,14 @@ __device__ inline v4f mfma_bf16(v8bf a, v8bf b, v4f c) {
 __device__ inline int mfma_f64_row(int lane, int r) { return 4 * r + (lane >> 4); }
 __device__ inline int mfma_f32_row(int lane, int r) { return 4 * (lane >> 4) + r; }
 
+// Make LDS writes of this wave visible to its own later LDS reads (LDS ops of
+// one wave execute in order; this only stops the compiler from reordering).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 __host__ __device__ inline int64_t cdiv64(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

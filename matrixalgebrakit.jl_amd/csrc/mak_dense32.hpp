@@ -1,0 +1,130 @@
+// Dense 32 x 32 building blocks in LDS for one workgroup / one wave (gfx950).
+#pragma once
+
+#include <utility>
+
+#include "mak_common.hpp"
+
+namespace mak {
+
+// ---------------------------------------------------------------------------
+// 32x32x32 products on FP64 MFMA for one 256-thread workgroup:
+//   C = alpha op(A) op(B) (+ C when acc), op = identity or conjugate transpose.
+// Wave w computes the 16x16 block (w % 2, w / 2).  The result is written after
+// a workgroup barrier, so C may alias A or B.  All 256 threads must call it.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline void mm32(T (*C)[33], const T (*A)[33], bool cA, const T (*B)[33], bool cB, double alpha,
+                            bool acc) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = tid >> 6;
+  const int bi = w & 1, bj = w >> 1;
+  v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kk = 0; kk < 32; kk += 4) {
+    const int k = kk + (lane >> 4);
+    const int i = bi * 16 + (lane & 15);
+    const int j = bj * 16 + (lane & 15);
+    const T a = cA ? s_conj(A[k][i]) : A[i][k];
+    const T b = cB ? s_conj(B[j][k]) : B[k][j];
+    // transposed block: lane l holds row (l % 16) of the output
+    cr = mfma_f64(s_re(b), s_re(a), cr);
+    if constexpr (ScalarTraits<T>::is_complex) {
+      cr = mfma_f64(-s_im(b), s_im(a), cr);
+      ci = mfma_f64(s_im(b), s_re(a), ci);
+      ci = mfma_f64(s_re(b), s_im(a), ci);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = bi * 16 + (lane & 15);
+    const int col = bj * 16 + mfma_f64_row(lane, r);
+    T v;
+    if constexpr (ScalarTraits<T>::is_complex) v = mkz(alpha * cr[r], alpha * ci[r]);
+    else v = alpha * cr[r];
+    C[row][col] = acc ? s_add(C[row][col], v) : v;
+  }
+  __syncthreads();
+}
+
+// Value of v in lane `src` (a constant) broadcast to the wave through a scalar register.
+__device__ inline double rdl(double v, int src) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, src);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), src);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ inline zd rdl(zd v, int src) { return mkz(rdl(v.x, src), rdl(v.y, src)); }
+
+// Cholesky with inverse of a 32 x 32 Hermitian positive definite block, by
+// ONE wave (no workgroup barrier, no LDS traffic inside the factorisation):
+//   S (LDS, 32 x 33) holds the upper triangle on entry (rows/columns beyond the
+//   active size padded with the identity by the caller); on exit S = W (upper,
+//   zero below) with S_in = W^H W, and Y = W^{-1} (upper, zero below).
+//   Returns 0, or j + 1 for the first pivot j that is not positive (uniform
+//   over the wave; S and Y are then unspecified).
+// Lane c (c = lane % 32; lanes 32..63 mirror lanes 0..31) keeps column c of W
+// and of Y in registers; row entries of other columns come from lane
+// broadcasts (v_readlane).  Cholesky step j: W(j, c) = S(j, c) / d_j, then
+// S(r, c) -= conj(W(j, r)) W(j, c) for r > j (entries below the diagonal only
+// receive the mirrored update and are ignored).  Inverse, rows from the
+// bottom: Y(r, c) = (delta_rc - sum_{k > r} W(r, k) Y(k, c)) / W(r, r).
+template <typename T, int J>
+__device__ __forceinline__ void chol32_step(T (&s)[32], int c, int& bad) {
+  const double d2 = s_re(rdl(s[J], J));
+  if (!bad && !(d2 > 0.0)) bad = J + 1;
+  const double d = sqrt(fmax(d2, 0.0));
+  const double rd = (d2 > 0.0) ? 1.0 / d : 0.0;
+  const T w = sel(c > J, s_scale(s[J], rd), sel(c == J, s_from_real<T>(d), s_zero(T())));
+  s[J] = w;
+#pragma unroll
+  for (int r = J + 1; r < 32; ++r) s[r] = s_sub(s[r], s_mul(s_conj(rdl(w, r)), w));
+}
+template <typename T, int... Js>
+__device__ __forceinline__ void chol32_steps(T (&s)[32], int c, int& bad, std::integer_sequence<int, Js...>) {
+  (chol32_step<T, Js>(s, c, bad), ...);
+}
+template <typename T, int I>
+__device__ __forceinline__ void trinv32_row(T (&y)[32], const T (&s)[32], int c) {
+  constexpr int r = 31 - I;  // rows from the bottom
+  T acc0 = s_zero(T()), acc1 = s_zero(T());
+#pragma unroll
+  for (int k = r + 1; k < 32; ++k) {
+    if ((k - r) & 1) acc0 = s_fma(rdl(s[r], k), y[k], acc0);   // W(r, k) Y(k, c)
+    else acc1 = s_fma(rdl(s[r], k), y[k], acc1);
+  }
+  const T inv = s_div(s_one(T()), rdl(s[r], r));
+  y[r] = sel(r < c, s_neg(s_mul(s_add(acc0, acc1), inv)), sel(r == c, inv, s_zero(T())));
+}
+template <typename T, int... Is>
+__device__ __forceinline__ void trinv32_rows(T (&y)[32], const T (&s)[32], int c, std::integer_sequence<int, Is...>) {
+  (trinv32_row<T, Is>(y, s, c), ...);
+}
+
+template <typename T>
+__device__ inline int wave_chol_inv32(T (*S)[33], T (*Y)[33]) {
+  const int l = lane_id();
+  const int c = l & 31;
+  const bool writer = l < 32;
+  T s[32];
+#pragma unroll
+  for (int r = 0; r < 32; ++r) s[r] = (r <= c) ? S[r][c] : s_zero(T());
+  int bad = 0;
+  chol32_steps<T>(s, c, bad, std::make_integer_sequence<int, 32>{});
+#pragma unroll
+  for (int r = 0; r < 32; ++r)
+    if (writer) S[r][c] = (r <= c) ? s[r] : s_zero(T());
+  T y[32];
+#pragma unroll
+  for (int r = 0; r < 32; ++r) y[r] = s_zero(T());
+  trinv32_rows<T>(y, s, c, std::make_integer_sequence<int, 32>{});
+#pragma unroll
+  for (int r = 0; r < 32; ++r)
+    if (writer) Y[r][c] = y[r];
+  wave_lds_sync();
+  return bad;
+}
+
+}  // namespace mak

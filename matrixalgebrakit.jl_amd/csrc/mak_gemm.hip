@@ -295,15 +295,23 @@ static void launch_gemm_tile(const GemmArgs& g, hipStream_t s) {
     hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, WM, WN, OPA, OPB>), grid, dim3(256), 0, s, g);
 }
 
+// The largest tile that still gives the launch enough workgroups to cover the
+// chip (256 CUs): 128x128 (real) / 64x64 while that yields >= 192 / 128
+// workgroups, then 32x32 -- a 512^3 product on 128x128 tiles runs 16
+// workgroups and is ~10x slower than on 32x32 tiles.
 template <typename T, int OPA, int OPB>
 static void gemm_shape(const GemmArgs& g, hipStream_t s) {
   constexpr bool cplx = ScalarTraits<T>::is_complex;
+  const int64_t slabs = (int64_t)std::max(g.batch, 1) * std::max(g.ksplit, 1);
+  auto wgs = [&](int bm, int bn) { return (int64_t)cdiv(g.M, bm) * cdiv(g.N, bn) * slabs; };
   if (g.M <= 32) {
     launch_gemm_tile<32, 64, 16, 2, 2, OPA, OPB, T>(g, s);
-  } else if (cplx) {
+  } else if (!cplx && wgs(128, 128) >= 192) {
+    launch_gemm_tile<128, 128, 16, 2, 2, OPA, OPB, T>(g, s);
+  } else if (wgs(64, 64) >= 128) {
     launch_gemm_tile<64, 64, 16, 2, 2, OPA, OPB, T>(g, s);
   } else {
-    launch_gemm_tile<128, 128, 16, 2, 2, OPA, OPB, T>(g, s);
+    launch_gemm_tile<32, 32, 16, 2, 2, OPA, OPB, T>(g, s);
   }
 }
 

@@ -22,6 +22,7 @@
 #include <utility>
 
 #include "mak_common.hpp"
+#include "mak_dense32.hpp"
 #include "mak_kernels.hpp"
 
 namespace mak {
@@ -105,13 +106,6 @@ __device__ __host__ inline void node_rows(int i, int nodes, int H, int chunk, in
   }
 }
 
-// Make LDS writes of this wave visible to its own later LDS reads (LDS ops of
-// one wave execute in order; this only stops the compiler from reordering).
-__device__ inline void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Shared state of geqr2_node_kernel.
 template <typename T, int NB, int NW> struct NodeLds {
@@ -609,48 +603,6 @@ __global__ void __launch_bounds__(NW * WAVE) geqr2_node2_tasks_kernel(const Node
   const NodeTask t = tasks[blockIdx.x];
   node2_body<T, NW, KR>(S, reinterpret_cast<T*>(t.X), t.ldx, t.H, t.nbc, reinterpret_cast<T*>(t.Tout),
                         reinterpret_cast<T*>(t.Rn), t.ldr, reinterpret_cast<T*>(t.V1));
-}
-
-// ---------------------------------------------------------------------------
-// 32x32x32 products on FP64 MFMA for one 256-thread workgroup:
-//   C = alpha op(A) op(B) (+ C when acc), op = identity or conjugate transpose.
-// Wave w computes the 16x16 block (w % 2, w / 2).  The result is written after
-// a workgroup barrier, so C may alias A or B.  All 256 threads must call it.
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ inline void mm32(T (*C)[33], const T (*A)[33], bool cA, const T (*B)[33], bool cB, double alpha,
-                            bool acc) {
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int w = tid >> 6;
-  const int bi = w & 1, bj = w >> 1;
-  v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int kk = 0; kk < 32; kk += 4) {
-    const int k = kk + (lane >> 4);
-    const int i = bi * 16 + (lane & 15);
-    const int j = bj * 16 + (lane & 15);
-    const T a = cA ? s_conj(A[k][i]) : A[i][k];
-    const T b = cB ? s_conj(B[j][k]) : B[k][j];
-    // transposed block: lane l holds row (l % 16) of the output
-    cr = mfma_f64(s_re(b), s_re(a), cr);
-    if constexpr (ScalarTraits<T>::is_complex) {
-      cr = mfma_f64(-s_im(b), s_im(a), cr);
-      ci = mfma_f64(s_im(b), s_re(a), ci);
-      ci = mfma_f64(s_re(b), s_im(a), ci);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int row = bi * 16 + (lane & 15);
-    const int col = bj * 16 + mfma_f64_row(lane, r);
-    T v;
-    if constexpr (ScalarTraits<T>::is_complex) v = mkz(alpha * cr[r], alpha * ci[r]);
-    else v = alpha * cr[r];
-    C[row][col] = acc ? s_add(C[row][col], v) : v;
-  }
-  __syncthreads();
 }
 
 // Element (r, c) of a unit-lower-trapezoidal V stored strictly below the
