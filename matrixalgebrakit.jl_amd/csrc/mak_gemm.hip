@@ -66,6 +66,7 @@ __global__ void __launch_bounds__(256, 2) gemm_f64_kernel(GemmArgs g) {
   static_assert(NA * 256 == BM * BK && NB_ * 256 == BN * BK, "tile / thread count");
   __shared__ double As[BK][BM + 2];
   __shared__ double Bs[BK][BN + 2];
+  if (gate_closed(g.gate, g.gate_want)) return;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = tid >> 6;
@@ -192,6 +193,7 @@ __global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
   static_assert(WM * WN == 4, "4 waves");
   __shared__ double Ar[BK][BM + 2], Ai[BK][BM + 2];
   __shared__ double Br[BK][BN + 2], Bi[BK][BN + 2];
+  if (gate_closed(g.gate, g.gate_want)) return;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = tid >> 6;

@@ -22,7 +22,12 @@ struct Geqr2NodeArgs {
   void* Rnext;      // R of node i -> rows [i*nbc, i*nbc + nbc) of the next level
   int64_t ldr;
   void* V1out;      // copy of the top 32x32 of V per node (unit lower), or null
+  const int* gate;  // when non-null the launch runs only if *gate == gate_want
+  int gate_want;
 };
+
+// Device-side predication of a launch (all threads of a workgroup agree).
+__device__ inline bool gate_closed(const int* gate, int want) { return gate != nullptr && *gate != want; }
 
 // One independent panel node of a batch (geqr2_node2_tasks_kernel).
 struct NodeTask {
@@ -53,6 +58,8 @@ struct TsqrRootArgs {
   void* Uout;        // out: U of the LU of Q - D (32 x 32)
   void* Apanel;      // out: top nbc rows of the panel <- [D R ; L1]
   int64_t lda;
+  const int* gate;   // run only if *gate == gate_want (when non-null)
+  int gate_want;
 };
 
 struct TsqrExpandArgs {
@@ -74,6 +81,8 @@ struct TsqrExpandArgs {
   void* Qout;              // inner levels: explicit Q rows of this level
   int64_t ldqo;
   const void* Umat;        // leaves: U of the LU; rows >= nbc become V = Q U^{-1}
+  const int* gate;         // run only if *gate == gate_want (when non-null)
+  int gate_want;
 };
 
 template <typename T> int geqr2_node_capacity();
@@ -106,6 +115,8 @@ struct GemmArgs {
   int64_t splitStride;
   int batch;
   int unitLowerA;         // stored A is unit lower trapezoidal (strict lower part valid)
+  const int* gate;        // run only if *gate == gate_want (when non-null)
+  int gate_want;
 };
 
 // C = alpha op(A) op(B) + beta C on FP64 MFMA (real) or 4 real MFMAs (complex).

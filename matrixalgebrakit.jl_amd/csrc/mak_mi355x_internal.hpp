@@ -18,6 +18,15 @@ template <typename T>
 int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, int64_t ldq, T* R, int64_t ldr,
               int positive, void* work, size_t work_bytes, T* Tout);
 
+// CholeskyQR2 panel with Householder reconstruction (mak_cholqr.hip): factors
+// the H x nbc panel Ap in place into compact WY form (T to Tpan) unless the
+// device-side check sets *flag, in which case Ap is untouched and the caller's
+// flag-gated Householder launches factor it.  cw: cholqr_workspace_elems(H).
+size_t cholqr_workspace_elems(int m);
+template <typename T>
+void cholqr_panel(T* Ap, int64_t lda, int H, int nbc, T* Tpan, T* cw, int* flag, int* counter, hipStream_t s);
+void zero_ints(int* p, int n, hipStream_t s);
+
 template <typename T> size_t polar_workspace_bytes(int m, int n);
 template <typename T>
 int polar_driver(hipStream_t s, int right, int m, int n, const T* A, int64_t lda, T* W, int64_t ldw, T* P,

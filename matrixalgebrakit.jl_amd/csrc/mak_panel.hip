@@ -202,6 +202,7 @@ __device__ inline void publish_row(const T (&a)[RPT][NB], NodeLds<T, NB, NT / WA
 
 template <typename T, int NB, int RPT, int NT>
 __global__ void __launch_bounds__(NT) geqr2_node_kernel(Geqr2NodeArgs args) {
+  if (gate_closed(args.gate, args.gate_want)) return;
   // Register layout: thread t holds rows t + NT*q (q < RPT).  Rows are kept
   // ROTATED in registers: steps go in pairs with the current column in slot 0
   // then slot 1, and the row is rotated by two slots after each pair, so no
@@ -587,6 +588,7 @@ __device__ __forceinline__ void node2_body(Node2Lds<T, NW, 4 * KR>& S, T* X, int
 template <typename T, int NW, int KR>
 __global__ void __launch_bounds__(NW * WAVE) geqr2_node2_kernel(Geqr2NodeArgs args) {
   __shared__ Node2Lds<T, NW, 4 * KR> S;
+  if (gate_closed(args.gate, args.gate_want)) return;
   const int node = blockIdx.x;
   int r0, r1;
   node_rows(node, args.nodes, args.H, args.chunk, r0, r1);
@@ -623,6 +625,7 @@ __device__ inline T vload(const T* V, int64_t ldv, int r, int c) {
 // rank-1 update per step over all (row, col) pairs.
 template <typename T, int NB, int NT>
 __device__ __forceinline__ void tsqr_root_finish_body(const TsqrRootArgs& args) {
+  if (gate_closed(args.gate, args.gate_want)) return;
   __shared__ T Vt[NB][NB + 1];   // top node's V1 (unit lower) ; R_top above
   __shared__ T Rt[NB][NB + 1];
   __shared__ T M[NB][NB + 1];
@@ -824,6 +827,7 @@ __device__ __forceinline__ void tsqr_expand_body(const TsqrExpandArgs& args, con
   const int hnode = r1 - r0;
   const int tile0 = tile_y * 64;  // first local row of this workgroup
   if (tile0 >= hnode) return;         // uniform per workgroup
+  if (gate_closed(args.gate, args.gate_want)) return;
 
   const T* Tsrc = reinterpret_cast<const T*>(args.Tnodes) + (int64_t)node * NB * NB;
   const T* V1src = reinterpret_cast<const T*>(args.V1base) + (int64_t)node * args.v1stride;

@@ -68,8 +68,12 @@ template <typename T> TreePlan plan_tree(int H, int nbc, size_t base) {
   return p;
 }
 
+// Panels at least this tall go through the CholeskyQR2 path first.
+constexpr int kCholQrMinRows = 64;
+
 template <typename T> struct QrLayout {
   size_t offTpanels = 0;  // NB*NB per panel
+  size_t offCq = 0, offFlags = 0;  // CholeskyQR2 scratch; flag + counter
   size_t offTree = 0;
   size_t offWp = 0, offW2 = 0;
   size_t wpElems = 0, w2Elems = 0;
@@ -85,6 +89,12 @@ template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
   size_t off = 0;
   L.offTpanels = off;
   off += align256((size_t)npan * NB * NB * sizeof(T));
+  if (m >= kCholQrMinRows) {
+    L.offCq = off;
+    off += align256(cholqr_workspace_elems(m) * sizeof(T));
+    L.offFlags = off;
+    off += 256;
+  }
   L.offTree = off;
   if (m > geqr2_node_capacity<T>()) {
     TreePlan tp = plan_tree<T>(m, NB, 0);
@@ -111,10 +121,19 @@ void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrL
   const int H = m - j0;
   T* Tpan = reinterpret_cast<T*>(ws + L.offTpanels) + (size_t)(j0 / NB) * NB * NB;
   const int cap = geqr2_node_capacity<T>();
+  // CholeskyQR2 first; the Householder launches below then run only if its
+  // device-side check rejected the panel (gate: flag == 1)
+  const int* gate = nullptr;
+  if (H >= kCholQrMinRows && L.offFlags) {
+    int* flag = reinterpret_cast<int*>(ws + L.offFlags);
+    cholqr_panel<T>(Ap, lda, H, nbc, Tpan, reinterpret_cast<T*>(ws + L.offCq), flag, flag + 1, s);
+    gate = flag;
+  }
   if (H <= cap) {
     Geqr2NodeArgs g{};
     g.X = Ap; g.ldx = lda; g.H = H; g.nodes = 1; g.chunk = 0; g.nbc = nbc;
     g.Tout = Tpan; g.tstride = NB * NB;
+    g.gate = gate; g.gate_want = 1;
     launch_geqr2_node<T>(g, cap, s);
     return;
   }
@@ -132,6 +151,7 @@ void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrL
     g.Tout = ptr(lv.offT); g.tstride = NB * NB;
     if (lev < top) { g.Rnext = ptr(tp.levels[lev + 1].offX); g.ldr = tp.levels[lev + 1].H; }
     if (lev == 0) g.V1out = ptr(tp.offV1);
+    g.gate = gate; g.gate_want = 1;
     launch_geqr2_node<T>(g, cap, s);
   }
   // root finish
@@ -147,6 +167,7 @@ void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrL
       r.chainT[lev] = ptr(tp.levels[lev].offT);
     }
     r.Mtop = ptr(tp.offM); r.Tout = Tpan; r.Uout = ptr(tp.offU); r.Apanel = Ap; r.lda = lda;
+    r.gate = gate; r.gate_want = 1;
     launch_tsqr_root_finish<T>(r, s);
   }
   // expansion top-down
@@ -166,6 +187,7 @@ void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrL
     if (lev == 0) e.Umat = ptr(tp.offU);
     else { e.Qout = ptr(lv.offQ); e.ldqo = lv.H; }
     const int maxrows = (lev == 0) ? cdiv(lv.H, lv.nodes) + 1 : lv.chunk;
+    e.gate = gate; e.gate_want = 1;
     launch_tsqr_expand<T>(e, maxrows, s);
   }
 }
@@ -213,6 +235,7 @@ template <typename T>
 void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s) {
   const int minmn = std::min(m, n);
   QrLayout<T> L = qr_layout<T>(m, n, (int)qcols);
+  if (L.offFlags) zero_ints(reinterpret_cast<int*>(ws + L.offFlags), 2, s);  // flag, Gram ticket counter
   for (int j0 = 0; j0 < minmn; j0 += NB) {
     const int nbc = std::min(NB, minmn - j0);
     factor_panel<T>(A, lda, j0, m, nbc, ws, L, s);
