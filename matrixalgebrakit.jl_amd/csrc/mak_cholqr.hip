@@ -4,15 +4,17 @@
 // at once unless the gate opens; no host synchronisation).
 //
 //   pass 1: G1 = X^H X, G1 = R1^H R1            (gram_chol_kernel)
-//           Q1 = X R1^{-1}                       (MFMA GEMM)
-//   pass 2: G2 = Q1^H Q1, G2 = R2^H R2, accepted only if ||G2 - I||_F <= 0.1
-//           (then kappa(Q1) <= 1.1 and the second pass is orthogonal to
-//           working precision; Yamamoto, Nakatsukasa, Yanagisawa, Fukaya,
-//           ETNA 44, 2015), R = R2 R1, Q = Q1 R2^{-1}
+//   pass 2: Q1 = X R1^{-1}, G2 = Q1^H Q1, G2 = R2^H R2, accepted only if
+//           ||G2 - I||_F <= 0.1 (then kappa(Q1) <= 1.1 and the second pass is
+//           orthogonal to working precision; Yamamoto, Nakatsukasa,
+//           Yanagisawa, Fukaya, ETNA 44, 2015), R = R2 R1, Q = Q1 R2^{-1}
 //   reconstruction (Ballard, Demmel, Grigori, Jacquelin, Nguyen, Solomonik,
 //   IPDPS 2014): Q - [D; 0] = V U, LU without pivoting with
 //   d_j = -sign(pivot_j); T = -U D^H V1^{-H}, the panel's R = D R and
 //   V = [V1; Q_rest U^{-1}] -- the compact WY form LAPACK's geqrt produces.
+//   Pass 2 and the reconstruction run in one launch (cholqr2_finish_kernel):
+//   only the top 32 rows of Q are formed there, and V_rest = Q1_rest M with
+//   M = R2^{-1} U^{-1} is one GEMM afterwards.
 // A panel whose first Gram is not numerically positive definite or whose
 // second Gram fails the check (kappa(X) beyond ~1e7) is factored by the TSQR
 // path instead, so rank-deficient inputs keep Householder's stability.
@@ -28,40 +30,108 @@ namespace mak {
 namespace {
 
 constexpr int NB = 32;
+constexpr int kMaxGramBlocks = 32;  // partial Grams summed by the last workgroup
 
-// G = X^H X over the row blocks of this workgroup -> partial[blockIdx.x];
-// the last workgroup to finish sums the partials in a fixed order, factors G
-// and writes the factors.  pass 1: always runs, sets *flag = (G not PD).
-// pass 2: runs only if *flag == 0; sets *flag when G is not PD or far from I;
-// otherwise writes Rout = W2 * R1 and Rinv = W2^{-1}.
-template <typename T>
-__global__ void __launch_bounds__(256) gram_chol_kernel(const T* X, int64_t ldx, int H, int nbc, T* partial,
-                                                        int* counter, int pass, const T* R1, T* Rout, T* Rinv,
-                                                        int* flag) {
-  __shared__ T Xs[64][NB + 1];
-  __shared__ T G[NB][NB + 1];
-  __shared__ T Yl[NB][NB + 1];
-  __shared__ T Rl[NB][NB + 1];
-  __shared__ int last_s, bad_s;
-  __shared__ double dev_s[4];
-  if (pass == 2 && *flag != 0) return;
+// rows of X staged per tile: 16 elements per thread in both cases
+template <typename T> struct CqTile;
+template <> struct CqTile<double> { static constexpr int TR = 128; };
+template <> struct CqTile<zd> { static constexpr int TR = 64; };
+
+// G += Y^H Y over the row tiles blockIdx.x, blockIdx.x + gridDim.x, ... of X
+// (H x nbc, columns nbc..31 read as zero), with Y = X, or Y = X B when XFORM
+// (B: 32 x 32 in LDS; Y is then also written to Yout, H x 32, ld H).
+// The next tile's loads are issued before the current tile is used.
+// Wave w accumulates the 16 x 16 block (w % 2, w / 2) of G in (cr, ci).
+template <typename T, bool XFORM>
+__device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*Xs)[NB + 1], const T (*Bs)[NB + 1],
+                                  T* Yout, v4d& cr, v4d& ci) {
+  constexpr int TR = CqTile<T>::TR;
+  constexpr int PER = TR * NB / 256;
+  constexpr bool CPLX = ScalarTraits<T>::is_complex;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int ti = w & 1, tj = w >> 1;
-  constexpr bool CPLX = ScalarTraits<T>::is_complex;
-  v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
-  for (int r0 = blockIdx.x * 64; r0 < H; r0 += gridDim.x * 64) {
-    __syncthreads();
-    for (int e = tid; e < 64 * NB; e += 256) {
-      const int r = e % 64, c = e / 64;
-      Xs[r][c] = (r0 + r < H && c < nbc) ? X[(r0 + r) + (int64_t)c * ldx] : s_zero(T());
+  const int ntile = cdiv(H, TR);
+  T cur[PER];
+  auto load = [&](int t) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + 256 * q;
+      const int r = e % TR, c = e / TR;
+      const int row = t * TR + r;
+      cur[q] = (row < H && c < nbc) ? X[row + (int64_t)c * ldx] : s_zero(T());
     }
+  };
+  int t = blockIdx.x;
+  if (t < ntile) load(t);
+  for (; t < ntile; t += gridDim.x) {
     __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + 256 * q;
+      Xs[e % TR][e / TR] = cur[q];
+    }
+    if (t + (int)gridDim.x < ntile) load(t + gridDim.x);
+    __syncthreads();
+    if constexpr (XFORM) {
+      // Y = X B as transposed 16 x 16 blocks (Y^T = B^T X^T): lane l, register
+      // r holds Y(i0 + l % 16, j0 + 4 r + l / 16), so stores are row-contiguous.
+      // Wave w owns rows [w TR / 4, (w + 1) TR / 4).
+      constexpr int RB = TR / 64;
+      v4d yr[RB][2], yi[RB][2];
+#pragma unroll
+      for (int ib = 0; ib < RB; ++ib)
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb) {
+          yr[ib][jb] = v4d{0.0, 0.0, 0.0, 0.0};
+          yi[ib][jb] = v4d{0.0, 0.0, 0.0, 0.0};
+        }
+#pragma unroll
+      for (int kk = 0; kk < NB; kk += 4) {
+        const int k = kk + lk;
+        T a[2];
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb) a[jb] = Bs[k][jb * 16 + lr];
+#pragma unroll
+        for (int ib = 0; ib < RB; ++ib) {
+          const T b = Xs[w * (TR / 4) + ib * 16 + lr][k];
+#pragma unroll
+          for (int jb = 0; jb < 2; ++jb) {
+            if constexpr (CPLX) {
+              yr[ib][jb] = mfma_f64(s_re(a[jb]), s_re(b), yr[ib][jb]);
+              yr[ib][jb] = mfma_f64(-s_im(a[jb]), s_im(b), yr[ib][jb]);
+              yi[ib][jb] = mfma_f64(s_im(a[jb]), s_re(b), yi[ib][jb]);
+              yi[ib][jb] = mfma_f64(s_re(a[jb]), s_im(b), yi[ib][jb]);
+            } else {
+              yr[ib][jb] = mfma_f64(a[jb], b, yr[ib][jb]);
+            }
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ib = 0; ib < RB; ++ib) {
+        const int i = w * (TR / 4) + ib * 16 + lr;
+        const int row = t * TR + i;
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int j = jb * 16 + 4 * r + lk;
+            T v;
+            if constexpr (CPLX) v = mkz(yr[ib][jb][r], yi[ib][jb][r]);
+            else v = yr[ib][jb][r];
+            Xs[i][j] = v;
+            if (row < H) Yout[row + (int64_t)j * H] = v;
+          }
+      }
+      __syncthreads();
+    }
 #pragma unroll 4
-    for (int kk = 0; kk < 64; kk += 4) {
-      const T a = Xs[kk + lk][ti * 16 + lr];  // (X^H)(i, k) = conj(X(k, i))
+    for (int kk = 0; kk < TR; kk += 4) {
+      const T a = Xs[kk + lk][ti * 16 + lr];  // (Y^H)(i, k) = conj(Y(k, i))
       const T b = Xs[kk + lk][tj * 16 + lr];
       if constexpr (CPLX) {
         cr = mfma_f64(s_re(a), s_re(b), cr);
@@ -73,72 +143,104 @@ __global__ void __launch_bounds__(256) gram_chol_kernel(const T* X, int64_t ldx,
       }
     }
   }
+}
+
+// Stores this workgroup's partial Gram (32 x 32 at partial + blockIdx.x * 1024)
+// and takes a ticket; returns true (uniformly) only in the last workgroup to
+// finish, after an acquire fence, so every partial is visible to it.
+template <typename T>
+__device__ inline bool partial_is_last(T* partial, const v4d& cr, const v4d& ci, int* counter, int* last_s) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = tid >> 6;
+  const int ti = w & 1, tj = w >> 1;
   T* P = partial + (int64_t)blockIdx.x * NB * NB;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int i = ti * 16 + mfma_f64_row(lane, r), j = tj * 16 + lr;
-    if constexpr (CPLX) P[i + j * NB] = mkz(cr[r], ci[r]);
+    const int i = ti * 16 + mfma_f64_row(lane, r), j = tj * 16 + (lane & 15);
+    if constexpr (ScalarTraits<T>::is_complex) P[i + j * NB] = mkz(cr[r], ci[r]);
     else P[i + j * NB] = cr[r];
   }
-  // last workgroup: reduce, factor
   __threadfence();
   __syncthreads();
-  if (tid == 0) last_s = (atomicAdd(counter, 1) == (int)gridDim.x - 1);
+  if (tid == 0) *last_s = (atomicAdd(counter, 1) == (int)gridDim.x - 1);
   __syncthreads();
-  if (!last_s) return;
+  if (!*last_s) return false;
   __threadfence();
+  return true;
+}
+
+// G = sum over the nb partials in workgroup order (rows / columns >= nbc:
+// identity).  32 loads per thread are in flight per batch of 8 partials.
+// Resets *counter for the next use; returns ||G - I||_F^2.
+template <typename T>
+__device__ inline double reduce_partials(const T* partial, int nb, int nbc, T (*G)[NB + 1], int* counter,
+                                         double* dev_s) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = tid >> 6;
+  T acc[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) acc[u] = s_zero(T());
+  int b = 0;
+  for (; b + 8 <= nb; b += 8) {
+    T v[8][4];
+#pragma unroll
+    for (int p = 0; p < 8; ++p)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[p][u] = partial[(int64_t)(b + p) * NB * NB + tid + 256 * u];
+#pragma unroll
+    for (int p = 0; p < 8; ++p)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] = s_add(acc[u], v[p][u]);
+  }
+  for (; b < nb; ++b)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = s_add(acc[u], partial[(int64_t)b * NB * NB + tid + 256 * u]);
   double dev = 0.0;
-  for (int e = tid; e < NB * NB; e += 256) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = tid + 256 * u;
     const int i = e % NB, j = e / NB;
-    // eight independent partial sums keep eight loads in flight
-    T a8[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) a8[u] = s_zero(T());
-    const int nb = (int)gridDim.x;
-    int b = 0;
-    for (; b + 8 <= nb; b += 8) {
-#pragma unroll
-      for (int u = 0; u < 8; ++u) a8[u] = s_add(a8[u], partial[(int64_t)(b + u) * NB * NB + e]);
-    }
-    for (; b < nb; ++b) a8[0] = s_add(a8[0], partial[(int64_t)b * NB * NB + e]);
-    T acc = s_add(s_add(s_add(a8[0], a8[1]), s_add(a8[2], a8[3])), s_add(s_add(a8[4], a8[5]), s_add(a8[6], a8[7])));
-    if (i >= nbc || j >= nbc) acc = (i == j) ? s_one(T()) : s_zero(T());
-    G[i][j] = acc;
-    const T dlt = s_sub(acc, (i == j) ? s_one(T()) : s_zero(T()));
-    dev += s_abs2(dlt);
+    const T id = (i == j) ? s_one(T()) : s_zero(T());
+    const T g = (i >= nbc || j >= nbc) ? id : acc[u];
+    G[i][j] = g;
+    dev += s_abs2(s_sub(g, id));
   }
   dev = wave_sum(dev);
   if (lane == 0) dev_s[w] = dev;
   if (tid == 0) *counter = 0;  // ready for the next use
   __syncthreads();
-  if (w == 0) {
+  return dev_s[0] + dev_s[1] + dev_s[2] + dev_s[3];
+}
+
+// Pass 1: G1 = X^H X over nblk workgroups; the last one factors G1 = W^H W and
+// writes R1 = W, R1inv = W^{-1} (32 x 32, ld 32).  Sets *flag = 1 when G1 is
+// not numerically positive definite, else 0.
+template <typename T>
+__global__ void __launch_bounds__(256) gram_chol_kernel(const T* X, int64_t ldx, int H, int nbc, T* partial,
+                                                        int* counter, T* R1, T* R1inv, int* flag) {
+  constexpr int TR = CqTile<T>::TR;
+  __shared__ T Xs[TR][NB + 1];
+  __shared__ T Yl[NB][NB + 1];
+  __shared__ int last_s, bad_s;
+  __shared__ double dev_s[4];
+  const int tid = threadIdx.x;
+  v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+  gram_tiles<T, false>(X, ldx, H, nbc, Xs, nullptr, nullptr, cr, ci);
+  if (!partial_is_last<T>(partial, cr, ci, counter, &last_s)) return;
+  T (*G)[NB + 1] = Xs;
+  reduce_partials<T>(partial, (int)gridDim.x, nbc, G, counter, dev_s);
+  if ((tid >> 6) == 0) {
     const int b = wave_chol_inv32<T>(G, Yl);
-    if (lane == 0) bad_s = b;
+    if (tid == 0) bad_s = b;
   }
   __syncthreads();
-  const double devF2 = dev_s[0] + dev_s[1] + dev_s[2] + dev_s[3];
-  const bool fail = bad_s != 0 || (pass == 2 && !(devF2 <= 0.01));
-  if (pass == 2) {
-    if (fail) {
-      if (tid == 0) *flag = 1;
-      return;
-    }
-    for (int e = tid; e < NB * NB; e += 256) Rl[e % NB][e / NB] = R1[e];
-    __syncthreads();
-    mm32<T>(Rl, G, false, Rl, false, 1.0, false);  // R = W2 R1 (W2 in G)
-    for (int e = tid; e < NB * NB; e += 256) {
-      const int i = e % NB, j = e / NB;
-      Rout[e] = Rl[i][j];
-      Rinv[e] = Yl[i][j];
-    }
-  } else {
-    if (tid == 0) *flag = fail ? 1 : 0;
-    if (fail) return;
-    for (int e = tid; e < NB * NB; e += 256) {
-      const int i = e % NB, j = e / NB;
-      Rout[e] = G[i][j];
-      Rinv[e] = Yl[i][j];
-    }
+  if (tid == 0) *flag = bad_s != 0 ? 1 : 0;
+  if (bad_s != 0) return;
+  for (int e = tid; e < NB * NB; e += 256) {
+    R1[e] = G[e % NB][e / NB];
+    R1inv[e] = Yl[e % NB][e / NB];
   }
 }
 
@@ -146,91 +248,133 @@ __global__ void __launch_bounds__(256) gram_chol_kernel(const T* X, int64_t ldx,
 // d_j = -sign(M(j, j)) subtracted from each pivot first (|pivot| >= 1 for the
 // top of an orthonormal Q).  Lane c keeps column c: U(r, c) for r <= c,
 // L(r, c) (unit lower) for r > c; other lanes' entries come from lane
-// broadcasts.  dv receives d.
+// broadcasts, four at a time.  dv receives d (when dv is not null), `pinv` of
+// lane j receives 1 / U(j, j).
 template <typename T, int J>
-__device__ __forceinline__ void lu32_step(T (&s)[32], int c, T* dv) {
+__device__ __forceinline__ void lu32_step(T (&s)[32], int c, T* dv, T& pinv) {
   const T piv = rdl(s[J], J);
   const T d = s_neg(sign_safe(piv));
   const T p = s_sub(piv, d);
-  if (c == 0) dv[J] = d;
-  const T inv = s_div(s_one(T()), p);
+  if (dv != nullptr && c == 0) dv[J] = d;
+  const T inv = s_recip(p);
+  pinv = sel(c == J, inv, pinv);
   const T mjc = s[J];
   s[J] = sel(c == J, p, mjc);
+  T dep = p;
 #pragma unroll
-  for (int r = J + 1; r < 32; ++r) {
-    const T lrj = s_mul(rdl(s[r], J), inv);  // L(r, J), from lane J before its scaling
-    s[r] = sel(c == J, lrj, sel(c > J, s_sub(s[r], s_mul(lrj, mjc)), s[r]));
+  for (int r0 = J + 1; r0 < 32; r0 += kBc<T>) {
+    const int lj = lane_after(J, dep);
+    T b[kBc<T>];
+#pragma unroll
+    for (int u = 0; u < kBc<T>; ++u) b[u] = (r0 + u < 32) ? rdl(s[r0 + u], lj) : s_zero(T());   // M(r, J)
+#pragma unroll
+    for (int u = 0; u < kBc<T>; ++u) {
+      if (r0 + u < 32) {
+        const T lrj = s_mul(b[u], inv);  // L(r, J), from lane J before its scaling
+        s[r0 + u] = sel(c == J, lrj, sel(c > J, s_sub(s[r0 + u], s_mul(lrj, mjc)), s[r0 + u]));
+      }
+    }
+    dep = s[r0];
   }
 }
 template <typename T, int... Js>
-__device__ __forceinline__ void lu32_steps(T (&s)[32], int c, T* dv, std::integer_sequence<int, Js...>) {
-  (lu32_step<T, Js>(s, c, dv), ...);
+__device__ __forceinline__ void lu32_steps(T (&s)[32], int c, T* dv, T& pinv, std::integer_sequence<int, Js...>) {
+  (lu32_step<T, Js>(s, c, dv, pinv), ...);
 }
 
-// Householder reconstruction of a CholeskyQR2 panel (see the file comment).
-// Runs only if *flag == 0.  Q: H x nbc (ld ldq) explicit Q; R: 32 x 32.
-// Writes T (32 x 32) to Tout, U^{-1} (32 x 32) to Uinv, and the top nbc rows
-// of the panel Ap: D R on and above the diagonal, V1 below.
+// Pass 2 and the reconstruction (runs only if *flag == 0 after pass 1):
+// every workgroup forms its tiles of Q1 = X R1^{-1} (to Q1, H x 32, ld H) and
+// their partial Gram; the last one factors G2 = W2^H W2 and checks it (sets
+// *flag = 1 and returns when it fails), then forms R = W2 R1, the top of
+// Q = Q1 W2^{-1}, its LU with signs D, T = -U D^H L^{-H}, M = W2^{-1} U^{-1}
+// (to Mout, 32 x 32) and the top nbc rows of the panel Ap (D R on and above
+// the diagonal, L below).  T goes to Tout (32 x 32, zero outside nbc x nbc).
 template <typename T>
-__global__ void __launch_bounds__(256) cholqr_recon_kernel(const T* Q, int64_t ldq, int nbc, const T* R, T* Tout,
-                                                           T* Uinv, T* Ap, int64_t lda, const int* flag) {
-  __shared__ T Ls[NB][NB + 1];   // L (unit lower), U on and above
-  __shared__ T Li[NB][NB + 1];   // L^{-1}
-  __shared__ T Ms[NB][NB + 1];   // U D^H, then T
-  __shared__ T Ui[NB][NB + 1];   // U^{-1}
+__global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t ldx, int H, int nbc, const T* R1,
+                                                             const T* R1inv, T* Q1, T* partial, int* counter,
+                                                             T* Tout, T* Mout, T* Ap, int64_t lda, int* flag) {
+  constexpr int TR = CqTile<T>::TR;
+  constexpr int XM = TR / NB;  // 32-row matrices spanned by the staged tile
+  static_assert(XM + 1 <= 8, "LDS pool too small for the staged tile");
+  __shared__ T pool[8][NB][NB + 1];
   __shared__ T dv[NB];
+  __shared__ int last_s, bad_s;
+  __shared__ double dev_s[4];
   if (*flag != 0) return;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = tid >> 6;
   const int c = lane & 31;
+  {
+    T (*Bs)[NB + 1] = pool[XM];
+    for (int e = tid; e < NB * NB; e += 256) Bs[e % NB][e / NB] = R1inv[e];
+    // (gram_tiles begins each tile with a workgroup barrier)
+    v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+    gram_tiles<T, true>(X, ldx, H, nbc, pool[0], Bs, Q1, cr, ci);
+    if (!partial_is_last<T>(partial, cr, ci, counter, &last_s)) return;
+  }
+  T (*G)[NB + 1] = pool[0];   // G2, then W2
+  T (*Yl)[NB + 1] = pool[1];  // W2^{-1}
+  T (*Rl)[NB + 1] = pool[2];  // R1, then R = W2 R1
+  T (*Qt)[NB + 1] = pool[3];  // top of Q1, then of Q; later M
+  T (*Ls)[NB + 1] = pool[4];  // L (unit lower), U on and above
+  T (*Li)[NB + 1] = pool[5];  // L^{-1}
+  T (*Ms)[NB + 1] = pool[6];  // U D^H, then T
+  T (*Ui)[NB + 1] = pool[7];  // U^{-1}
+  const double devF2 = reduce_partials<T>(partial, (int)gridDim.x, nbc, G, counter, dev_s);
   if (w == 0) {
-    T s[32];
-#pragma unroll
-    for (int r = 0; r < 32; ++r) {
-      const T q = Q[(int64_t)min(r, nbc - 1) + (int64_t)min(c, nbc - 1) * ldq];
-      s[r] = sel(r < nbc && c < nbc, q, sel(r == c, s_one(T()), s_zero(T())));
-    }
-    lu32_steps<T>(s, c, dv, std::make_integer_sequence<int, 32>{});
-#pragma unroll
-    for (int r = 0; r < 32; ++r)
-      if (lane < 32) Ls[r][c] = s[r];
-    // U^{-1} (upper): rows from the bottom, U(r, k) of lane k
-    T y[32];
-#pragma unroll
-    for (int r = 0; r < 32; ++r) y[r] = s_zero(T());
-    trinv32_rows<T>(y, s, c, std::make_integer_sequence<int, 32>{});
-#pragma unroll
-    for (int r = 0; r < 32; ++r)
-      if (lane < 32) Ui[r][c] = y[r];
+    const int b = wave_chol_inv32<T>(G, Yl);
+    if (lane == 0) bad_s = b;
+  }
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e % NB, j = e / NB;
+    Rl[i][j] = R1[e];
+    Qt[i][j] = Q1[i + (int64_t)j * H];  // H >= 64: rows 0..31 exist
   }
   __syncthreads();
-  if (w == 1) {
-    // L^{-1} (unit lower), column c: z(c) = 1, z(r) = -sum_{c<=k<r} L(r, k) z(k)
-    T z[32];
-#pragma unroll
-    for (int r = 0; r < 32; ++r) {
-      T acc = s_zero(T());
-#pragma unroll
-      for (int k = 0; k < r; ++k) acc = s_fma(Ls[r][k], z[k], acc);   // z(k) = 0 for k < c
-      z[r] = sel(r == c, s_one(T()), sel(r > c, s_neg(acc), s_zero(T())));
-    }
-#pragma unroll
-    for (int r = 0; r < 32; ++r)
-      if (lane < 32) Li[r][c] = z[r];
+  if (bad_s != 0 || !(devF2 <= 0.01)) {
+    if (tid == 0) *flag = 1;
+    return;
   }
+  mm32<T>(Rl, G, false, Rl, false, 1.0, false);   // R = W2 R1
+  mm32<T>(Qt, Qt, false, Yl, false, 1.0, false);  // top of Q = Q1 W2^{-1}
+  if (w < 2) {
+    // both waves factor the same block; wave 0 then inverts U, wave 1 L
+    T s[32];
+#pragma unroll
+    for (int r = 0; r < 32; ++r) s[r] = sel(r < nbc && c < nbc, Qt[r][c], sel(r == c, s_one(T()), s_zero(T())));
+    T pinv = s_zero(T());
+    lu32_steps<T>(s, c, w == 0 ? dv : (T*)nullptr, pinv, std::make_integer_sequence<int, 32>{});
+    T y[32];
+    if (w == 0) {
+#pragma unroll
+      for (int r = 0; r < 32; ++r)
+        if (lane < 32) Ls[r][c] = s[r];
+      trinv32_rows<T>(y, s, pinv, c, std::make_integer_sequence<int, 32>{});
+#pragma unroll
+      for (int r = 0; r < 32; ++r)
+        if (lane < 32) Ui[r][c] = y[r];
+    } else {
+      trilinv32_rows<T>(y, s, c, std::make_integer_sequence<int, 32>{});
+#pragma unroll
+      for (int r = 0; r < 32; ++r)
+        if (lane < 32) Li[r][c] = y[r];
+    }
+  }
+  __syncthreads();
   for (int e = tid; e < NB * NB; e += 256) {
     const int r = e % NB, cc = e / NB;
     Ms[r][cc] = (r <= cc) ? s_mul(Ls[r][cc], s_conj(dv[cc])) : s_zero(T());
   }
   __syncthreads();
   mm32<T>(Ms, Ms, false, Li, true, -1.0, false);  // T = -(U D^H) L^{-H}
+  mm32<T>(Qt, Yl, false, Ui, false, 1.0, false);  // M = W2^{-1} U^{-1}
   for (int e = tid; e < NB * NB; e += 256) {
     const int r = e % NB, cc = e / NB;
     const bool in = r < nbc && cc < nbc;
     Tout[e] = (in && r <= cc) ? Ms[r][cc] : s_zero(T());
-    Uinv[e] = (in && r <= cc) ? Ui[r][cc] : s_zero(T());
-    if (in) Ap[r + (int64_t)cc * lda] = (r <= cc) ? s_mul(dv[r], R[r + cc * NB]) : Ls[r][cc];
+    Mout[e] = in ? Qt[r][cc] : s_zero(T());
+    if (in) Ap[r + (int64_t)cc * lda] = (r <= cc) ? s_mul(dv[r], Rl[r][cc]) : Ls[r][cc];
   }
 }
 
@@ -256,30 +400,23 @@ void zero_ints(int* p, int n, hipStream_t s) {
 }
 
 size_t cholqr_workspace_elems(int m) {
-  // Q1, Q2 (m x 32), 256 partial Grams, R1, R1inv, R, R2inv, Uinv
-  return (size_t)2 * std::max(m, 1) * NB + (size_t)256 * NB * NB + (size_t)5 * NB * NB;
+  // Q1 (m x 32), the partial Grams, R1, R1^{-1}, M
+  return (size_t)std::max(m, 1) * NB + (size_t)kMaxGramBlocks * NB * NB + (size_t)3 * NB * NB;
 }
 
 template <typename T>
 void cholqr_panel(T* Ap, int64_t lda, int H, int nbc, T* Tpan, T* cw, int* flag, int* counter, hipStream_t s) {
   T* Q1 = cw;
-  T* Q2 = Q1 + (size_t)H * NB;
-  T* part = Q2 + (size_t)H * NB;
-  T* R1 = part + (size_t)256 * NB * NB;
+  T* part = Q1 + (size_t)H * NB;
+  T* R1 = part + (size_t)kMaxGramBlocks * NB * NB;
   T* R1i = R1 + NB * NB;
-  T* R = R1i + NB * NB;
-  T* R2i = R + NB * NB;
-  T* Ui = R2i + NB * NB;
-  const int nblk = std::max(1, std::min(64, cdiv(H, 256)));  // 64 partials at most: a short final sum
-  hipLaunchKernelGGL((gram_chol_kernel<T>), dim3(nblk), dim3(256), 0, s, Ap, lda, H, nbc, part, counter, 1,
-                     (const T*)nullptr, R1, R1i, flag);
-  gemm<T>(OP_N, OP_N, gg<T>(H, nbc, nbc, Ap, lda, R1i, NB, Q1, H, flag), s);
-  hipLaunchKernelGGL((gram_chol_kernel<T>), dim3(nblk), dim3(256), 0, s, Q1, (int64_t)H, H, nbc, part, counter, 2,
-                     (const T*)R1, R, R2i, flag);
-  gemm<T>(OP_N, OP_N, gg<T>(H, nbc, nbc, Q1, H, R2i, NB, Q2, H, flag), s);
-  hipLaunchKernelGGL((cholqr_recon_kernel<T>), dim3(1), dim3(256), 0, s, (const T*)Q2, (int64_t)H, nbc,
-                     (const T*)R, Tpan, Ui, Ap, lda, (const int*)flag);
-  if (H > nbc) gemm<T>(OP_N, OP_N, gg<T>(H - nbc, nbc, nbc, Q2 + nbc, H, Ui, NB, Ap + nbc, lda, flag), s);
+  T* M = R1i + NB * NB;
+  const int nblk = std::max(1, std::min(kMaxGramBlocks, cdiv(H, 256)));
+  hipLaunchKernelGGL((gram_chol_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc, part, counter,
+                     R1, R1i, flag);
+  hipLaunchKernelGGL((cholqr2_finish_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc,
+                     (const T*)R1, (const T*)R1i, Q1, part, counter, Tpan, M, Ap, lda, flag);
+  if (H > nbc) gemm<T>(OP_N, OP_N, gg<T>(H - nbc, nbc, nbc, Q1 + nbc, H, M, NB, Ap + nbc, lda, flag), s);
 }
 
 template void cholqr_panel<double>(double*, int64_t, int, int, double*, double*, int*, int*, hipStream_t);

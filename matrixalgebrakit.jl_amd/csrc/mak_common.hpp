@@ -84,6 +84,36 @@ __device__ inline zd s_mkreal(zd, double r) { return mkz(r, 0.0); }
 __device__ inline double s_mkreal(double, double r) { return r; }
 __device__ inline zd s_neg(zd a) { return mkz(-a.x, -a.y); }
 
+// 1 / sqrt(x) and 1 / x for finite x > 0 (x != 0) to about one ulp: the
+// hardware estimates (v_rsq_f64, v_rcp_f64, relative error about 2^-23)
+// refined by two Newton-Raphson steps each, with the residual formed by fma:
+//   rsqrt: e = 1 - x y^2, y <- y + (y / 2) e;   recip: e = 1 - x y, y <- y + y e.
+// The error roughly squares per step.  They replace IEEE sqrt / division on
+// the serial pivot chain of the in-wave factorisations, where latency counts.
+__device__ inline double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+__device__ inline double recip_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+  }
+  return y;
+}
+__device__ inline double s_recip(double a) { return recip_nr(a); }
+__device__ inline zd s_recip(zd a) {  // conj(a) / |a|^2
+  const double r = recip_nr(fma(a.x, a.x, a.y * a.y));
+  return mkz(a.x * r, -a.y * r);
+}
+
 template <typename T> __device__ inline T s_from_real(double r);
 template <> __device__ inline double s_from_real<double>(double r) { return r; }
 template <> __device__ inline zd s_from_real<zd>(double r) { return mkz(r, 0.0); }

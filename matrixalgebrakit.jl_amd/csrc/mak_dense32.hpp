@@ -49,7 +49,7 @@ __device__ inline void mm32(T (*C)[33], const T (*A)[33], bool cA, const T (*B)[
   __syncthreads();
 }
 
-// Value of v in lane `src` (a constant) broadcast to the wave through a scalar register.
+// Value of v in lane `src` (uniform over the wave) broadcast to the wave through a scalar register.
 __device__ inline double rdl(double v, int src) {
   const uint64_t u = __builtin_bit_cast(uint64_t, v);
   const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, src);
@@ -57,6 +57,21 @@ __device__ inline double rdl(double v, int src) {
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 __device__ inline zd rdl(zd v, int src) { return mkz(rdl(v.x, src), rdl(v.y, src)); }
+
+// Broadcasts issued together before their arithmetic: four real or two
+// complex values (eight scalar registers).
+template <typename T> constexpr int kBc = ScalarTraits<T>::is_complex ? 2 : 4;
+
+// Lane index `src` made to depend on `dep` through an empty asm: broadcasts
+// that read lane `src + u` cannot be scheduled before `dep` is computed.
+// Without it the compiler hoists all broadcasts of a factorisation step (or
+// of a whole triangular inverse) into scalar registers at once and spills
+// them to VGPR lanes, which costs more than the arithmetic.
+__device__ __forceinline__ int lane_after(int src, double dep) {
+  asm volatile("" : "+s"(src) : "v"(dep));
+  return src;
+}
+__device__ __forceinline__ int lane_after(int src, zd dep) { return lane_after(src, dep.x); }
 
 // Cholesky with inverse of a 32 x 32 Hermitian positive definite block, by
 // ONE wave (no workgroup barrier, no LDS traffic inside the factorisation):
@@ -67,44 +82,101 @@ __device__ inline zd rdl(zd v, int src) { return mkz(rdl(v.x, src), rdl(v.y, src
 //   over the wave; S and Y are then unspecified).
 // Lane c (c = lane % 32; lanes 32..63 mirror lanes 0..31) keeps column c of W
 // and of Y in registers; row entries of other columns come from lane
-// broadcasts (v_readlane).  Cholesky step j: W(j, c) = S(j, c) / d_j, then
-// S(r, c) -= conj(W(j, r)) W(j, c) for r > j (entries below the diagonal only
-// receive the mirrored update and are ignored).  Inverse, rows from the
-// bottom: Y(r, c) = (delta_rc - sum_{k > r} W(r, k) Y(k, c)) / W(r, r).
+// broadcasts (v_readlane), four at a time.  Cholesky step j: W(j, c) =
+// S(j, c) / d_j, then S(r, c) -= conj(W(j, r)) W(j, c) for r > j (entries
+// below the diagonal only receive the mirrored update and are ignored).
+// `dinv` of lane j receives 1 / W(j, j).
 template <typename T, int J>
-__device__ __forceinline__ void chol32_step(T (&s)[32], int c, int& bad) {
+__device__ __forceinline__ void chol32_step(T (&s)[32], int c, int& bad, T& dinv) {
   const double d2 = s_re(rdl(s[J], J));
   if (!bad && !(d2 > 0.0)) bad = J + 1;
-  const double d = sqrt(fmax(d2, 0.0));
-  const double rd = (d2 > 0.0) ? 1.0 / d : 0.0;
+  const double rd = (d2 > 0.0) ? rsqrt_nr(d2) : 0.0;
+  const double d = d2 * rd;
+  dinv = sel(c == J, s_from_real<T>(rd), dinv);
   const T w = sel(c > J, s_scale(s[J], rd), sel(c == J, s_from_real<T>(d), s_zero(T())));
   s[J] = w;
+  T dep = w;
 #pragma unroll
-  for (int r = J + 1; r < 32; ++r) s[r] = s_sub(s[r], s_mul(s_conj(rdl(w, r)), w));
+  for (int r0 = J + 1; r0 < 32; r0 += kBc<T>) {
+    const int base = lane_after(r0, dep);
+    T b[kBc<T>];
+#pragma unroll
+    for (int u = 0; u < kBc<T>; ++u) b[u] = (r0 + u < 32) ? rdl(w, base + u) : s_zero(T());
+#pragma unroll
+    for (int u = 0; u < kBc<T>; ++u)
+      if (r0 + u < 32) s[r0 + u] = s_sub(s[r0 + u], s_mul(s_conj(b[u]), w));
+    dep = s[r0];
+  }
 }
 template <typename T, int... Js>
-__device__ __forceinline__ void chol32_steps(T (&s)[32], int c, int& bad, std::integer_sequence<int, Js...>) {
-  (chol32_step<T, Js>(s, c, bad), ...);
+__device__ __forceinline__ void chol32_steps(T (&s)[32], int c, int& bad, T& dinv,
+                                             std::integer_sequence<int, Js...>) {
+  (chol32_step<T, Js>(s, c, bad, dinv), ...);
 }
+
+// Inverse Y of an upper triangular 32 x 32 block U held by lanes as above
+// (lane c: s[r] = U(r, c)), `dinv` of lane r = 1 / U(r, r).  Rows from the
+// bottom: Y(r, c) = (delta_rc - sum_{k > r} U(r, k) Y(k, c)) / U(r, r).
 template <typename T, int I>
-__device__ __forceinline__ void trinv32_row(T (&y)[32], const T (&s)[32], int c) {
-  constexpr int r = 31 - I;  // rows from the bottom
+__device__ __forceinline__ void trinv32_row(T (&y)[32], const T (&s)[32], const T& dinv, int c) {
+  constexpr int r = 31 - I;
   T acc0 = s_zero(T()), acc1 = s_zero(T());
+  T dep = (r < 31) ? y[r < 31 ? r + 1 : r] : s[r];
 #pragma unroll
-  for (int k = r + 1; k < 32; ++k) {
-    if ((k - r) & 1) acc0 = s_fma(rdl(s[r], k), y[k], acc0);   // W(r, k) Y(k, c)
-    else acc1 = s_fma(rdl(s[r], k), y[k], acc1);
+  for (int k0 = r + 1; k0 < 32; k0 += kBc<T>) {
+    const int base = lane_after(k0, dep);
+    T b[kBc<T>];
+#pragma unroll
+    for (int u = 0; u < kBc<T>; ++u) b[u] = (k0 + u < 32) ? rdl(s[r], base + u) : s_zero(T());   // U(r, k)
+#pragma unroll
+    for (int u = 0; u < kBc<T>; ++u) {
+      if (k0 + u < 32) {
+        if (u & 1) acc1 = s_fma(b[u], y[k0 + u], acc1);
+        else acc0 = s_fma(b[u], y[k0 + u], acc0);
+      }
+    }
+    dep = acc0;
   }
-  const T inv = s_div(s_one(T()), rdl(s[r], r));
+  const T inv = rdl(dinv, r);
   y[r] = sel(r < c, s_neg(s_mul(s_add(acc0, acc1), inv)), sel(r == c, inv, s_zero(T())));
 }
 template <typename T, int... Is>
-__device__ __forceinline__ void trinv32_rows(T (&y)[32], const T (&s)[32], int c, std::integer_sequence<int, Is...>) {
-  (trinv32_row<T, Is>(y, s, c), ...);
+__device__ __forceinline__ void trinv32_rows(T (&y)[32], const T (&s)[32], const T& dinv, int c,
+                                             std::integer_sequence<int, Is...>) {
+  (trinv32_row<T, Is>(y, s, dinv, c), ...);
+}
+
+// Inverse Z of a unit lower triangular 32 x 32 block L (lane c: s[r] = L(r, c)
+// for r > c), rows from the top: Z(r, c) = delta_rc - sum_{c <= k < r} L(r, k) Z(k, c).
+template <typename T, int R>
+__device__ __forceinline__ void trilinv32_row(T (&z)[32], const T (&s)[32], int c) {
+  T acc0 = s_zero(T()), acc1 = s_zero(T());
+  T dep = (R > 0) ? z[R > 0 ? R - 1 : 0] : s[R];
+#pragma unroll
+  for (int k0 = 0; k0 < R; k0 += kBc<T>) {
+    const int base = lane_after(k0, dep);
+    T b[kBc<T>];
+#pragma unroll
+    for (int u = 0; u < kBc<T>; ++u) b[u] = (k0 + u < R) ? rdl(s[R], base + u) : s_zero(T());   // L(R, k)
+#pragma unroll
+    for (int u = 0; u < kBc<T>; ++u) {
+      if (k0 + u < R) {
+        if (u & 1) acc1 = s_fma(b[u], z[k0 + u], acc1);
+        else acc0 = s_fma(b[u], z[k0 + u], acc0);
+      }
+    }
+    dep = acc0;
+  }
+  z[R] = sel(R == c, s_one(T()), sel(R > c, s_neg(s_add(acc0, acc1)), s_zero(T())));
+}
+template <typename T, int... Rs>
+__device__ __forceinline__ void trilinv32_rows(T (&z)[32], const T (&s)[32], int c,
+                                               std::integer_sequence<int, Rs...>) {
+  (trilinv32_row<T, Rs>(z, s, c), ...);
 }
 
 template <typename T>
-__device__ inline int wave_chol_inv32(T (*S)[33], T (*Y)[33]) {
+__device__ __forceinline__ int wave_chol_inv32(T (*S)[33], T (*Y)[33]) {
   const int l = lane_id();
   const int c = l & 31;
   const bool writer = l < 32;
@@ -112,14 +184,15 @@ __device__ inline int wave_chol_inv32(T (*S)[33], T (*Y)[33]) {
 #pragma unroll
   for (int r = 0; r < 32; ++r) s[r] = (r <= c) ? S[r][c] : s_zero(T());
   int bad = 0;
-  chol32_steps<T>(s, c, bad, std::make_integer_sequence<int, 32>{});
+  T dinv = s_zero(T());
+  chol32_steps<T>(s, c, bad, dinv, std::make_integer_sequence<int, 32>{});
 #pragma unroll
   for (int r = 0; r < 32; ++r)
     if (writer) S[r][c] = (r <= c) ? s[r] : s_zero(T());
   T y[32];
 #pragma unroll
   for (int r = 0; r < 32; ++r) y[r] = s_zero(T());
-  trinv32_rows<T>(y, s, c, std::make_integer_sequence<int, 32>{});
+  trinv32_rows<T>(y, s, dinv, c, std::make_integer_sequence<int, 32>{});
 #pragma unroll
   for (int r = 0; r < 32; ++r)
     if (writer) Y[r][c] = y[r];
