@@ -214,12 +214,17 @@ __device__ inline double reduce_partials(const T* partial, int nb, int nbc, T (*
   return dev_s[0] + dev_s[1] + dev_s[2] + dev_s[3];
 }
 
+// status[0]: panel rejected (the flag), status[1]: ticket counter,
+// status[2]: first column of the first rejected panel (sticky mode).
+//
 // Pass 1: G1 = X^H X over nblk workgroups; the last one factors G1 = W^H W and
-// writes R1 = W, R1inv = W^{-1} (32 x 32, ld 32).  Sets *flag = 1 when G1 is
-// not numerically positive definite, else 0.
+// writes R1 = W, R1inv = W^{-1} (32 x 32, ld 32).  When G1 is not numerically
+// positive definite the flag is set (with status[2] = j0 if it was clear);
+// otherwise it is cleared, unless `sticky` (then a set flag stays set).
 template <typename T>
 __global__ void __launch_bounds__(256) gram_chol_kernel(const T* X, int64_t ldx, int H, int nbc, T* partial,
-                                                        int* counter, T* R1, T* R1inv, int* flag) {
+                                                        int* status, int j0, int sticky, T* R1, T* R1inv) {
+  int* counter = status + 1;
   constexpr int TR = CqTile<T>::TR;
   __shared__ T Xs[TR][NB + 1];
   __shared__ T Yl[NB][NB + 1];
@@ -236,7 +241,11 @@ __global__ void __launch_bounds__(256) gram_chol_kernel(const T* X, int64_t ldx,
     if (tid == 0) bad_s = b;
   }
   __syncthreads();
-  if (tid == 0) *flag = bad_s != 0 ? 1 : 0;
+  if (tid == 0) {
+    const int prev = status[0];
+    if (bad_s != 0 && prev == 0) status[2] = j0;
+    status[0] = (bad_s != 0 || (sticky && prev != 0)) ? 1 : 0;
+  }
   if (bad_s != 0) return;
   for (int e = tid; e < NB * NB; e += 256) {
     R1[e] = G[e % NB][e / NB];
@@ -263,10 +272,12 @@ __device__ __forceinline__ void lu32_step(T (&s)[32], int c, T* dv, T& pinv) {
   T dep = p;
 #pragma unroll
   for (int r0 = J + 1; r0 < 32; r0 += kBc<T>) {
-    const int lj = lane_after(J, dep);
     T b[kBc<T>];
 #pragma unroll
-    for (int u = 0; u < kBc<T>; ++u) b[u] = (r0 + u < 32) ? rdl(s[r0 + u], lj) : s_zero(T());   // M(r, J)
+    for (int u = 0; u < kBc<T>; ++u) {
+      if (r0 + u < 32) order_after(s[r0 + u], dep);
+      b[u] = (r0 + u < 32) ? rdl(s[r0 + u], J) : s_zero(T());   // M(r, J)
+    }
 #pragma unroll
     for (int u = 0; u < kBc<T>; ++u) {
       if (r0 + u < 32) {
@@ -282,17 +293,17 @@ __device__ __forceinline__ void lu32_steps(T (&s)[32], int c, T* dv, T& pinv, st
   (lu32_step<T, Js>(s, c, dv, pinv), ...);
 }
 
-// Pass 2 and the reconstruction (runs only if *flag == 0 after pass 1):
+// Pass 2 and the reconstruction (runs only if the flag is clear after pass 1):
 // every workgroup forms its tiles of Q1 = X R1^{-1} (to Q1, H x 32, ld H) and
 // their partial Gram; the last one factors G2 = W2^H W2 and checks it (sets
-// *flag = 1 and returns when it fails), then forms R = W2 R1, the top of
+// the flag and status[2] = j0 and returns when it fails), then forms R = W2 R1, the top of
 // Q = Q1 W2^{-1}, its LU with signs D, T = -U D^H L^{-H}, M = W2^{-1} U^{-1}
 // (to Mout, 32 x 32) and the top nbc rows of the panel Ap (D R on and above
 // the diagonal, L below).  T goes to Tout (32 x 32, zero outside nbc x nbc).
 template <typename T>
 __global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t ldx, int H, int nbc, const T* R1,
-                                                             const T* R1inv, T* Q1, T* partial, int* counter,
-                                                             T* Tout, T* Mout, T* Ap, int64_t lda, int* flag) {
+                                                             const T* R1inv, T* Q1, T* partial, int* status,
+                                                             int j0, T* Tout, T* Mout, T* Ap, int64_t lda) {
   constexpr int TR = CqTile<T>::TR;
   constexpr int XM = TR / NB;  // 32-row matrices spanned by the staged tile
   static_assert(XM + 1 <= 8, "LDS pool too small for the staged tile");
@@ -300,7 +311,8 @@ __global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t
   __shared__ T dv[NB];
   __shared__ int last_s, bad_s;
   __shared__ double dev_s[4];
-  if (*flag != 0) return;
+  if (status[0] != 0) return;
+  int* counter = status + 1;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = tid >> 6;
@@ -333,7 +345,10 @@ __global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t
   }
   __syncthreads();
   if (bad_s != 0 || !(devF2 <= 0.01)) {
-    if (tid == 0) *flag = 1;
+    if (tid == 0) {
+      status[2] = j0;
+      status[0] = 1;
+    }
     return;
   }
   mm32<T>(Rl, G, false, Rl, false, 1.0, false);   // R = W2 R1
@@ -405,21 +420,22 @@ size_t cholqr_workspace_elems(int m) {
 }
 
 template <typename T>
-void cholqr_panel(T* Ap, int64_t lda, int H, int nbc, T* Tpan, T* cw, int* flag, int* counter, hipStream_t s) {
+void cholqr_panel(T* Ap, int64_t lda, int H, int nbc, T* Tpan, T* cw, int* status, int j0, int sticky,
+                  hipStream_t s) {
   T* Q1 = cw;
   T* part = Q1 + (size_t)H * NB;
   T* R1 = part + (size_t)kMaxGramBlocks * NB * NB;
   T* R1i = R1 + NB * NB;
   T* M = R1i + NB * NB;
   const int nblk = std::max(1, std::min(kMaxGramBlocks, cdiv(H, 256)));
-  hipLaunchKernelGGL((gram_chol_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc, part, counter,
-                     R1, R1i, flag);
+  hipLaunchKernelGGL((gram_chol_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc, part, status,
+                     j0, sticky, R1, R1i);
   hipLaunchKernelGGL((cholqr2_finish_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc,
-                     (const T*)R1, (const T*)R1i, Q1, part, counter, Tpan, M, Ap, lda, flag);
-  if (H > nbc) gemm<T>(OP_N, OP_N, gg<T>(H - nbc, nbc, nbc, Q1 + nbc, H, M, NB, Ap + nbc, lda, flag), s);
+                     (const T*)R1, (const T*)R1i, Q1, part, status, j0, Tpan, M, Ap, lda);
+  if (H > nbc) gemm<T>(OP_N, OP_N, gg<T>(H - nbc, nbc, nbc, Q1 + nbc, H, M, NB, Ap + nbc, lda, status), s);
 }
 
-template void cholqr_panel<double>(double*, int64_t, int, int, double*, double*, int*, int*, hipStream_t);
-template void cholqr_panel<zd>(zd*, int64_t, int, int, zd*, zd*, int*, int*, hipStream_t);
+template void cholqr_panel<double>(double*, int64_t, int, int, double*, double*, int*, int, int, hipStream_t);
+template void cholqr_panel<zd>(zd*, int64_t, int, int, zd*, zd*, int*, int, int, hipStream_t);
 
 }  // namespace mak

@@ -62,16 +62,15 @@ __device__ inline zd rdl(zd v, int src) { return mkz(rdl(v.x, src), rdl(v.y, src
 // complex values (eight scalar registers).
 template <typename T> constexpr int kBc = ScalarTraits<T>::is_complex ? 2 : 4;
 
-// Lane index `src` made to depend on `dep` through an empty asm: broadcasts
-// that read lane `src + u` cannot be scheduled before `dep` is computed.
-// Without it the compiler hoists all broadcasts of a factorisation step (or
-// of a whole triangular inverse) into scalar registers at once and spills
-// them to VGPR lanes, which costs more than the arithmetic.
-__device__ __forceinline__ int lane_after(int src, double dep) {
-  asm volatile("" : "+s"(src) : "v"(dep));
-  return src;
+// Makes `v` appear to depend on `dep` through an empty asm (no instruction is
+// emitted): broadcasts of `v` cannot then be scheduled before `dep` is
+// computed.  Without it the compiler hoists all broadcasts of a factorisation
+// step (or of a whole triangular inverse) into scalar registers at once and
+// spills them to VGPR lanes, which costs more than the arithmetic.
+__device__ __forceinline__ void order_after(double& v, double dep) { asm volatile("" : "+v"(v) : "v"(dep)); }
+__device__ __forceinline__ void order_after(zd& v, const zd& dep) {
+  asm volatile("" : "+v"(v.x), "+v"(v.y) : "v"(dep.x));
 }
-__device__ __forceinline__ int lane_after(int src, zd dep) { return lane_after(src, dep.x); }
 
 // Cholesky with inverse of a 32 x 32 Hermitian positive definite block, by
 // ONE wave (no workgroup barrier, no LDS traffic inside the factorisation):
@@ -95,17 +94,16 @@ __device__ __forceinline__ void chol32_step(T (&s)[32], int c, int& bad, T& dinv
   dinv = sel(c == J, s_from_real<T>(rd), dinv);
   const T w = sel(c > J, s_scale(s[J], rd), sel(c == J, s_from_real<T>(d), s_zero(T())));
   s[J] = w;
-  T dep = w;
+  T src = w;
 #pragma unroll
   for (int r0 = J + 1; r0 < 32; r0 += kBc<T>) {
-    const int base = lane_after(r0, dep);
     T b[kBc<T>];
 #pragma unroll
-    for (int u = 0; u < kBc<T>; ++u) b[u] = (r0 + u < 32) ? rdl(w, base + u) : s_zero(T());
+    for (int u = 0; u < kBc<T>; ++u) b[u] = (r0 + u < 32) ? rdl(src, r0 + u) : s_zero(T());
 #pragma unroll
     for (int u = 0; u < kBc<T>; ++u)
       if (r0 + u < 32) s[r0 + u] = s_sub(s[r0 + u], s_mul(s_conj(b[u]), w));
-    dep = s[r0];
+    order_after(src, s[r0]);
   }
 }
 template <typename T, int... Js>
@@ -121,13 +119,13 @@ template <typename T, int I>
 __device__ __forceinline__ void trinv32_row(T (&y)[32], const T (&s)[32], const T& dinv, int c) {
   constexpr int r = 31 - I;
   T acc0 = s_zero(T()), acc1 = s_zero(T());
-  T dep = (r < 31) ? y[r < 31 ? r + 1 : r] : s[r];
+  T src = s[r];
+  if constexpr (r < 31) order_after(src, y[r < 31 ? r + 1 : r]);
 #pragma unroll
   for (int k0 = r + 1; k0 < 32; k0 += kBc<T>) {
-    const int base = lane_after(k0, dep);
     T b[kBc<T>];
 #pragma unroll
-    for (int u = 0; u < kBc<T>; ++u) b[u] = (k0 + u < 32) ? rdl(s[r], base + u) : s_zero(T());   // U(r, k)
+    for (int u = 0; u < kBc<T>; ++u) b[u] = (k0 + u < 32) ? rdl(src, k0 + u) : s_zero(T());   // U(r, k)
 #pragma unroll
     for (int u = 0; u < kBc<T>; ++u) {
       if (k0 + u < 32) {
@@ -135,7 +133,7 @@ __device__ __forceinline__ void trinv32_row(T (&y)[32], const T (&s)[32], const 
         else acc0 = s_fma(b[u], y[k0 + u], acc0);
       }
     }
-    dep = acc0;
+    order_after(src, acc0);
   }
   const T inv = rdl(dinv, r);
   y[r] = sel(r < c, s_neg(s_mul(s_add(acc0, acc1), inv)), sel(r == c, inv, s_zero(T())));
@@ -151,13 +149,13 @@ __device__ __forceinline__ void trinv32_rows(T (&y)[32], const T (&s)[32], const
 template <typename T, int R>
 __device__ __forceinline__ void trilinv32_row(T (&z)[32], const T (&s)[32], int c) {
   T acc0 = s_zero(T()), acc1 = s_zero(T());
-  T dep = (R > 0) ? z[R > 0 ? R - 1 : 0] : s[R];
+  T src = s[R];
+  if constexpr (R > 0) order_after(src, z[R > 0 ? R - 1 : 0]);
 #pragma unroll
   for (int k0 = 0; k0 < R; k0 += kBc<T>) {
-    const int base = lane_after(k0, dep);
     T b[kBc<T>];
 #pragma unroll
-    for (int u = 0; u < kBc<T>; ++u) b[u] = (k0 + u < R) ? rdl(s[R], base + u) : s_zero(T());   // L(R, k)
+    for (int u = 0; u < kBc<T>; ++u) b[u] = (k0 + u < R) ? rdl(src, k0 + u) : s_zero(T());   // L(R, k)
 #pragma unroll
     for (int u = 0; u < kBc<T>; ++u) {
       if (k0 + u < R) {
@@ -165,7 +163,7 @@ __device__ __forceinline__ void trilinv32_row(T (&z)[32], const T (&s)[32], int 
         else acc0 = s_fma(b[u], z[k0 + u], acc0);
       }
     }
-    dep = acc0;
+    order_after(src, acc0);
   }
   z[R] = sel(R == c, s_one(T()), sel(R > c, s_neg(s_add(acc0, acc1)), s_zero(T())));
 }
@@ -182,7 +180,10 @@ __device__ __forceinline__ int wave_chol_inv32(T (*S)[33], T (*Y)[33]) {
   const bool writer = l < 32;
   T s[32];
 #pragma unroll
-  for (int r = 0; r < 32; ++r) s[r] = (r <= c) ? S[r][c] : s_zero(T());
+  for (int r = 0; r < 32; ++r) {
+    const T v = S[r][c];
+    s[r] = sel(r <= c, v, s_zero(T()));
+  }
   int bad = 0;
   T dinv = s_zero(T());
   chol32_steps<T>(s, c, bad, dinv, std::make_integer_sequence<int, 32>{});

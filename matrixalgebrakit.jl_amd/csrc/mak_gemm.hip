@@ -365,11 +365,12 @@ void splitk_plan(int N, int K, int bn, int& ksplit, int& kchunk) {
 template <typename T>
 __global__ void __launch_bounds__(256) wy_reduce_kernel(const T* __restrict__ Wp, int ksplit, int64_t slice,
                                                         const T* __restrict__ Tm, int transT, int nbc,
-                                                        int ncols, T* __restrict__ W2) {
+                                                        int ncols, T* __restrict__ W2, const int* gate) {
   // 16 columns per workgroup; thread (r, cc) sums the ksplit partials of
   // two outputs with four independent accumulators each.
   __shared__ T S[32][17];
   __shared__ T Tl[32][33];
+  if (gate_closed(gate, 0)) return;
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * 16;
   for (int i = tid; i < 32 * 32; i += 256) {
@@ -411,14 +412,15 @@ __global__ void __launch_bounds__(256) wy_reduce_kernel(const T* __restrict__ Wp
 
 template <typename T>
 void wy_reduce(const void* Wp, int ksplit, int64_t slice, const void* Tm, int transT, int nbc, int ncols,
-               void* W2, hipStream_t s) {
+               void* W2, hipStream_t s, const int* gate) {
   if (ncols <= 0) return;
   hipLaunchKernelGGL((wy_reduce_kernel<T>), dim3(cdiv(ncols, 16)), dim3(256), 0, s,
                      reinterpret_cast<const T*>(Wp), ksplit, slice, reinterpret_cast<const T*>(Tm), transT,
-                     nbc, ncols, reinterpret_cast<T*>(W2));
+                     nbc, ncols, reinterpret_cast<T*>(W2), gate);
 }
-template void wy_reduce<double>(const void*, int, int64_t, const void*, int, int, int, void*, hipStream_t);
-template void wy_reduce<zd>(const void*, int, int64_t, const void*, int, int, int, void*, hipStream_t);
+template void wy_reduce<double>(const void*, int, int64_t, const void*, int, int, int, void*, hipStream_t,
+                                const int*);
+template void wy_reduce<zd>(const void*, int, int64_t, const void*, int, int, int, void*, hipStream_t, const int*);
 
 // ---------------------------------------------------------------------------
 // Elementwise helpers

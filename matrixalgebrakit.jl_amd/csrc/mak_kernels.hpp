@@ -125,10 +125,11 @@ template <typename T> void gemm(int opA, int opB, GemmArgs g, hipStream_t s);
 // Split-K partial count and chunk for an (M x N x K) product with small M.
 void splitk_plan(int N, int K, int bn, int& ksplit, int& kchunk);
 
-// W2 = op(Tm) * sum_z Wp[z]  (nbc x ncols, ld nbc), op = conj-transpose if transT
+// W2 = op(Tm) * sum_z Wp[z]  (nbc x ncols, ld nbc), op = conj-transpose if transT;
+// skipped when gate is given and *gate != 0
 template <typename T>
 void wy_reduce(const void* Wp, int ksplit, int64_t slice, const void* Tm, int transT, int nbc,
-               int ncols, void* W2, hipStream_t s);
+               int ncols, void* W2, hipStream_t s, const int* gate = nullptr);
 
 // Precision-dispatched GEMM on FP64 data: prec 64 (FP64 MFMA), 32 (FP32 MFMA),
 // 16 (BF16 MFMA), FP32 accumulation for 32 / 16.  Complex data runs in FP64.

@@ -115,25 +115,41 @@ template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
   return L;
 }
 
+// How panels are factored:
+//   kHouseholderOnly  Householder node / TSQR launches only;
+//   kCholGated        CholeskyQR2 first, then the Householder launches gated on
+//                     its flag (they exit at once unless the panel was
+//                     rejected): no host synchronisation, so a stream under
+//                     graph capture can record it;
+//   kCholHostChecked  CholeskyQR2 alone with a sticky flag that also skips every
+//                     later launch; the caller reads the flag once on the host
+//                     and refactors from the first rejected panel.
+enum PanelMode { kHouseholderOnly = 0, kCholGated = 1, kCholHostChecked = 2 };
+
 template <typename T>
-void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrLayout<T>& L, hipStream_t s) {
+void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrLayout<T>& L, int mode,
+                  hipStream_t s) {
   T* Ap = A + j0 + (int64_t)j0 * lda;
   const int H = m - j0;
   T* Tpan = reinterpret_cast<T*>(ws + L.offTpanels) + (size_t)(j0 / NB) * NB * NB;
   const int cap = geqr2_node_capacity<T>();
-  // CholeskyQR2 first; the Householder launches below then run only if its
-  // device-side check rejected the panel (gate: flag == 1)
+  int* status = L.offFlags ? reinterpret_cast<int*>(ws + L.offFlags) : nullptr;
   const int* gate = nullptr;
-  if (H >= kCholQrMinRows && L.offFlags) {
-    int* flag = reinterpret_cast<int*>(ws + L.offFlags);
-    cholqr_panel<T>(Ap, lda, H, nbc, Tpan, reinterpret_cast<T*>(ws + L.offCq), flag, flag + 1, s);
-    gate = flag;
+  int want = 1;
+  if (mode != kHouseholderOnly && status != nullptr && H >= kCholQrMinRows) {
+    cholqr_panel<T>(Ap, lda, H, nbc, Tpan, reinterpret_cast<T*>(ws + L.offCq), status, j0,
+                    mode == kCholHostChecked ? 1 : 0, s);
+    if (mode == kCholHostChecked) return;
+    gate = status;  // the Householder launches below run only for a rejected panel
+  } else if (mode == kCholHostChecked && status != nullptr) {
+    gate = status;  // short panel: skipped once an earlier panel was rejected
+    want = 0;
   }
   if (H <= cap) {
     Geqr2NodeArgs g{};
     g.X = Ap; g.ldx = lda; g.H = H; g.nodes = 1; g.chunk = 0; g.nbc = nbc;
     g.Tout = Tpan; g.tstride = NB * NB;
-    g.gate = gate; g.gate_want = 1;
+    g.gate = gate; g.gate_want = want;
     launch_geqr2_node<T>(g, cap, s);
     return;
   }
@@ -151,7 +167,7 @@ void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrL
     g.Tout = ptr(lv.offT); g.tstride = NB * NB;
     if (lev < top) { g.Rnext = ptr(tp.levels[lev + 1].offX); g.ldr = tp.levels[lev + 1].H; }
     if (lev == 0) g.V1out = ptr(tp.offV1);
-    g.gate = gate; g.gate_want = 1;
+    g.gate = gate; g.gate_want = want;
     launch_geqr2_node<T>(g, cap, s);
   }
   // root finish
@@ -167,7 +183,7 @@ void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrL
       r.chainT[lev] = ptr(tp.levels[lev].offT);
     }
     r.Mtop = ptr(tp.offM); r.Tout = Tpan; r.Uout = ptr(tp.offU); r.Apanel = Ap; r.lda = lda;
-    r.gate = gate; r.gate_want = 1;
+    r.gate = gate; r.gate_want = want;
     launch_tsqr_root_finish<T>(r, s);
   }
   // expansion top-down
@@ -187,15 +203,16 @@ void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrL
     if (lev == 0) e.Umat = ptr(tp.offU);
     else { e.Qout = ptr(lv.offQ); e.ldqo = lv.H; }
     const int maxrows = (lev == 0) ? cdiv(lv.H, lv.nodes) + 1 : lv.chunk;
-    e.gate = gate; e.gate_want = 1;
+    e.gate = gate; e.gate_want = want;
     launch_tsqr_expand<T>(e, maxrows, s);
   }
 }
 
 // C (rows x ncols) <- C - V op(T) (V^H C); V is the unit-lower panel at Vp.
+// Skipped when gate is given and *gate != 0.
 template <typename T>
 void apply_wy(const T* Vp, int64_t ldv, int rows, int nbc, const T* Tm, int transT, T* C, int64_t ldc,
-              int ncols, char* ws, const QrLayout<T>& L, hipStream_t s) {
+              int ncols, char* ws, const QrLayout<T>& L, hipStream_t s, const int* gate = nullptr) {
   if (ncols <= 0 || rows <= 0) return;
   T* Wp = reinterpret_cast<T*>(ws + L.offWp);
   T* W2 = reinterpret_cast<T*>(ws + L.offW2);
@@ -208,15 +225,17 @@ void apply_wy(const T* Vp, int64_t ldv, int rows, int nbc, const T* Tm, int tran
     g.A = Vp; g.lda = ldv; g.B = C; g.ldb = ldc; g.C = Wp; g.ldc = nbc;
     g.alpha_re = 1.0; g.beta_re = 0.0;
     g.ksplit = ks; g.kchunk = kc; g.splitStride = slice; g.batch = 1; g.unitLowerA = 1;
+    g.gate = gate; g.gate_want = 0;
     gemm<T>(OP_C, OP_N, g, s);
   }
-  wy_reduce<T>(Wp, ks, slice, Tm, transT, nbc, ncols, W2, s);
+  wy_reduce<T>(Wp, ks, slice, Tm, transT, nbc, ncols, W2, s, gate);
   {
     GemmArgs g{};
     g.M = rows; g.N = ncols; g.K = nbc;
     g.A = Vp; g.lda = ldv; g.B = W2; g.ldb = nbc; g.C = C; g.ldc = ldc;
     g.alpha_re = -1.0; g.beta_re = 1.0;
     g.ksplit = 1; g.kchunk = nbc; g.batch = 1; g.unitLowerA = 1;
+    g.gate = gate; g.gate_want = 0;
     gemm<T>(OP_N, OP_N, g, s);
   }
 }
@@ -231,20 +250,41 @@ template <typename T> size_t qr_workspace_bytes(int m, int n, int full) {
 template size_t qr_workspace_bytes<double>(int, int, int);
 template size_t qr_workspace_bytes<zd>(int, int, int);
 
+// Outside graph capture the panels run in kCholHostChecked mode and the
+// stream is synchronised once, to read the CholeskyQR2 flag; under capture
+// they run in kCholGated mode (no synchronisation).
 template <typename T>
 void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s) {
   const int minmn = std::min(m, n);
   QrLayout<T> L = qr_layout<T>(m, n, (int)qcols);
-  if (L.offFlags) zero_ints(reinterpret_cast<int*>(ws + L.offFlags), 2, s);  // flag, Gram ticket counter
-  for (int j0 = 0; j0 < minmn; j0 += NB) {
-    const int nbc = std::min(NB, minmn - j0);
-    factor_panel<T>(A, lda, j0, m, nbc, ws, L, s);
-    const int nc = n - j0 - nbc;
-    if (nc > 0) {
-      const T* Tpan = reinterpret_cast<const T*>(ws + L.offTpanels) + (size_t)(j0 / NB) * NB * NB;
-      apply_wy<T>(A + j0 + (int64_t)j0 * lda, lda, m - j0, nbc, Tpan, 1, A + j0 + (int64_t)(j0 + nbc) * lda,
-                  lda, nc, ws, L, s);
+  int* status = L.offFlags ? reinterpret_cast<int*>(ws + L.offFlags) : nullptr;
+  int mode = kHouseholderOnly;
+  if (status != nullptr) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) cs = hipStreamCaptureStatusActive;
+    mode = cs == hipStreamCaptureStatusNone ? kCholHostChecked : kCholGated;
+    zero_ints(status, 3, s);  // flag, Gram ticket counter, first rejected column
+  }
+  auto run = [&](int from, int pmode, const int* gate) {
+    for (int j0 = from; j0 < minmn; j0 += NB) {
+      const int nbc = std::min(NB, minmn - j0);
+      factor_panel<T>(A, lda, j0, m, nbc, ws, L, pmode, s);
+      const int nc = n - j0 - nbc;
+      if (nc > 0) {
+        const T* Tpan = reinterpret_cast<const T*>(ws + L.offTpanels) + (size_t)(j0 / NB) * NB * NB;
+        apply_wy<T>(A + j0 + (int64_t)j0 * lda, lda, m - j0, nbc, Tpan, 1, A + j0 + (int64_t)(j0 + nbc) * lda,
+                    lda, nc, ws, L, s, gate);
+      }
     }
+  };
+  run(0, mode, mode == kCholHostChecked ? status : nullptr);
+  if (mode == kCholHostChecked) {
+    // panels before the first rejected one are final and the trailing matrix
+    // is updated up to it; every later launch was skipped
+    int h[3] = {0, 0, 0};
+    if (hipMemcpyAsync(h, status, sizeof(h), hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipStreamSynchronize(s) == hipSuccess && h[0] != 0 && h[2] >= 0 && h[2] < minmn && h[2] % NB == 0)
+      run(h[2], kHouseholderOnly, nullptr);
   }
 }
 

@@ -96,6 +96,32 @@ def test_batched_without_r_and_nonpositive(dev):
     plan.close()
 
 
+def test_batched_tall_block_with_rejected_cholqr_panel(dev):
+    """Blocks taller than two nodes run through the single-matrix driver inside
+    the captured graph, where a panel rejected by the CholeskyQR2 check is
+    refactored by the flag-gated Householder launches (no host check)."""
+    A1 = O.gaussian(2100, 70, np.float64, seed=5)
+    A1[:, 40] = A1[:, 3]          # rank-deficient second panel
+    A2 = O.gaussian(2300, 40, np.float64, seed=6)
+    host = [A1, A2, O.gaussian(300, 20, np.float64, seed=7)]
+    As = [mak.colmajor(torch.from_numpy(np.asfortranarray(h)).to(dev)) for h in host]
+    plan = mak.BatchedQRPlan(As)
+    assert plan.info()["fallback_problems"] == 2
+    for rep in range(2):   # the graph replays on the same (restored) inputs
+        for A, h in zip(As, host):
+            A.copy_(torch.from_numpy(np.asfortranarray(h)).to(dev))
+        Qs, Rs = plan.execute()
+        torch.cuda.synchronize()
+        for h, Q, R in zip(host, Qs, Rs):
+            Qh, Rh = Q.cpu().numpy(), R.cpu().numpy()
+            assert rel(Qh @ Rh, h) < 1e-13, rep
+            assert np.linalg.norm(Qh.T @ Qh - np.eye(Qh.shape[1])) < 1e-12
+            assert np.all(np.diagonal(Rh) >= 0)
+        Qo, Ro = O.qr_compact(host[1])
+        assert rel(Qs[1].cpu().numpy(), Qo) < 1e-12 and rel(Rs[1].cpu().numpy(), Ro) < 1e-12
+    plan.close()
+
+
 @pytest.mark.parametrize("dt", [torch.float64, torch.complex128], ids=["f64", "c64"])
 def test_batched_full_size_properties(dt, dev):
     """4096 blocks, m log-uniform in [16, 1024], n uniform in [8, m]."""

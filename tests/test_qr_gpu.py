@@ -121,6 +121,27 @@ def test_qr_rank_deficient_and_special(dev):
     assert np.allclose(np.diagonal(R), np.abs(d)) and np.allclose(np.diagonal(Q), np.sign(d))
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+@pytest.mark.parametrize("kind", ["repeated_column", "graded_1e10"])
+def test_qr_cholqr_rejected_later_panel(kind, dt, dev):
+    """Panels 0-3 pass the CholeskyQR2 check; a later panel is rejected (an
+    exactly repeated column, or column scales falling to 1e-10), so the driver
+    refactors from that panel with Householder TSQR.  Q is not unique for a
+    rank-deficient A, so the checks are backward error, orthogonality and the
+    gauge of R."""
+    m, n = 2000, 200
+    A = O.gaussian(m, n, dt, seed=21)
+    if kind == "repeated_column":
+        A[:, 150] = A[:, 10]
+    else:  # panel 4 (columns 128..159) spans column scales 1 .. 1e-10
+        A = A * (10.0 ** (-10.0 * np.clip(np.arange(n) - 128, 0, 31) / 31))[None, :]
+    Q, R = run_qr(A, dev)
+    assert rel(Q @ R, A) < 1e-13
+    assert np.linalg.norm(Q.conj().T @ Q - np.eye(n)) < 1e-12
+    d = np.diagonal(R)
+    assert np.all(d.imag == 0) and np.all(d.real >= 0)
+
+
 def test_qr_input_untouched_by_allocating_form(dev):
     A = to_dev(O.gaussian(100, 60, np.float64, seed=3), dev)
     A0 = A.clone()
