@@ -37,12 +37,16 @@ template <typename T> struct CqTile;
 template <> struct CqTile<double> { static constexpr int TR = 128; };
 template <> struct CqTile<zd> { static constexpr int TR = 64; };
 
+// How gram_tiles forms Y from the tile X of each row block:
+//   kPlain   Y = X;
+//   kTimesB  Y = X B (B: 32 x 32 in LDS), Y also written to Yout (H x 32, ld H).
+enum GramMode { kPlain = 0, kTimesB = 1 };
+
 // G += Y^H Y over the row tiles blockIdx.x, blockIdx.x + gridDim.x, ... of X
-// (H x nbc, columns nbc..31 read as zero), with Y = X, or Y = X B when XFORM
-// (B: 32 x 32 in LDS; Y is then also written to Yout, H x 32, ld H).
+// (H x nbc, columns nbc..31 read as zero), with Y formed as MODE says.
 // The next tile's loads are issued before the current tile is used.
 // Wave w accumulates the 16 x 16 block (w % 2, w / 2) of G in (cr, ci).
-template <typename T, bool XFORM>
+template <typename T, int MODE>
 __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*Xs)[NB + 1], const T (*Bs)[NB + 1],
                                   T* Yout, v4d& cr, v4d& ci) {
   constexpr int TR = CqTile<T>::TR;
@@ -75,7 +79,7 @@ __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*X
     }
     if (t + (int)gridDim.x < ntile) load(t + gridDim.x);
     __syncthreads();
-    if constexpr (XFORM) {
+    if constexpr (MODE == kTimesB) {
       // Y = X B as transposed 16 x 16 blocks (Y^T = B^T X^T): lane l, register
       // r holds Y(i0 + l % 16, j0 + 4 r + l / 16), so stores are row-contiguous.
       // Wave w owns rows [w TR / 4, (w + 1) TR / 4).
@@ -171,39 +175,49 @@ __device__ inline bool partial_is_last(T* partial, const v4d& cr, const v4d& ci,
 }
 
 // G = sum over the nb partials in workgroup order (rows / columns >= nbc:
-// identity).  32 loads per thread are in flight per batch of 8 partials.
+// identity).  Thread t owns elements 4t .. 4t + 3 and reads them with 16-byte
+// loads, 32 of them in flight per batch of partials.
 // Resets *counter for the next use; returns ||G - I||_F^2.
 template <typename T>
 __device__ inline double reduce_partials(const T* partial, int nb, int nbc, T (*G)[NB + 1], int* counter,
                                          double* dev_s) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  constexpr bool CPLX = ScalarTraits<T>::is_complex;
+  constexpr int LD = CPLX ? 4 : 2;     // 16-byte loads per partial and thread
+  constexpr int BATCH = 32 / LD;       // partials per batch
+  constexpr int64_t PSTR = (int64_t)NB * NB * sizeof(T) / 16;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = tid >> 6;
-  T acc[4];
+  const d2* base = reinterpret_cast<const d2*>(partial) + tid * LD;
+  d2 acc[LD];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) acc[u] = s_zero(T());
+  for (int q = 0; q < LD; ++q) acc[q] = d2{0.0, 0.0};
   int b = 0;
-  for (; b + 8 <= nb; b += 8) {
-    T v[8][4];
+  for (; b + BATCH <= nb; b += BATCH) {
+    d2 v[BATCH][LD];
 #pragma unroll
-    for (int p = 0; p < 8; ++p)
+    for (int p = 0; p < BATCH; ++p)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) v[p][u] = partial[(int64_t)(b + p) * NB * NB + tid + 256 * u];
+      for (int q = 0; q < LD; ++q) v[p][q] = base[(b + p) * PSTR + q];
 #pragma unroll
-    for (int p = 0; p < 8; ++p)
+    for (int p = 0; p < BATCH; ++p)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc[u] = s_add(acc[u], v[p][u]);
+      for (int q = 0; q < LD; ++q) acc[q] += v[p][q];
   }
   for (; b < nb; ++b)
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc[u] = s_add(acc[u], partial[(int64_t)b * NB * NB + tid + 256 * u]);
+    for (int q = 0; q < LD; ++q) acc[q] += base[b * PSTR + q];
   double dev = 0.0;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int e = tid + 256 * u;
+    const int e = 4 * tid + u;
     const int i = e % NB, j = e / NB;
+    T a;
+    if constexpr (CPLX) a = mkz(acc[u].x, acc[u].y);
+    else a = (u & 1) ? acc[u >> 1].y : acc[u >> 1].x;
     const T id = (i == j) ? s_one(T()) : s_zero(T());
-    const T g = (i >= nbc || j >= nbc) ? id : acc[u];
+    const T g = (i >= nbc || j >= nbc) ? id : a;
     G[i][j] = g;
     dev += s_abs2(s_sub(g, id));
   }
@@ -232,7 +246,7 @@ __global__ void __launch_bounds__(256) gram_chol_kernel(const T* X, int64_t ldx,
   __shared__ double dev_s[4];
   const int tid = threadIdx.x;
   v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
-  gram_tiles<T, false>(X, ldx, H, nbc, Xs, nullptr, nullptr, cr, ci);
+  gram_tiles<T, kPlain>(X, ldx, H, nbc, Xs, nullptr, nullptr, cr, ci);
   if (!partial_is_last<T>(partial, cr, ci, counter, &last_s)) return;
   T (*G)[NB + 1] = Xs;
   reduce_partials<T>(partial, (int)gridDim.x, nbc, G, counter, dev_s);
@@ -255,10 +269,13 @@ __global__ void __launch_bounds__(256) gram_chol_kernel(const T* X, int64_t ldx,
 
 // In-wave LU of a 32 x 32 block without pivoting, with the sign choice
 // d_j = -sign(M(j, j)) subtracted from each pivot first (|pivot| >= 1 for the
-// top of an orthonormal Q).  Lane c keeps column c: U(r, c) for r <= c,
-// L(r, c) (unit lower) for r > c; other lanes' entries come from lane
-// broadcasts, four at a time.  dv receives d (when dv is not null), `pinv` of
-// lane j receives 1 / U(j, j).
+// top of an orthonormal Q).  Lane c keeps column c: U(r, c) for r <= c and,
+// below the diagonal, the eliminated entry M(r, c) (scaled into L(r, c) =
+// M(r, c) / U(c, c) by lu32_scale_l afterwards); other lanes' entries come
+// from lane broadcasts.  Step j: lanes c > j update S(r, c) -= M(r, j) /
+// U(j, j) * S(j, c) for r > j (lanes c <= j multiply by zero, which leaves
+// their finite entries exact).  dv receives d (when dv is not null), `pinv`
+// of lane j receives 1 / U(j, j).
 template <typename T, int J>
 __device__ __forceinline__ void lu32_step(T (&s)[32], int c, T* dv, T& pinv) {
   const T piv = rdl(s[J], J);
@@ -267,8 +284,9 @@ __device__ __forceinline__ void lu32_step(T (&s)[32], int c, T* dv, T& pinv) {
   if (dv != nullptr && c == 0) dv[J] = d;
   const T inv = s_recip(p);
   pinv = sel(c == J, inv, pinv);
-  const T mjc = s[J];
-  s[J] = sel(c == J, p, mjc);
+  const T mjc = sel(c > J, s[J], s_zero(T()));  // U(J, c) for the lanes that update
+  s[J] = sel(c == J, p, s[J]);
+  const T f = s_neg(s_mul(inv, mjc));            // -U(J, c) / U(J, J)
   T dep = p;
 #pragma unroll
   for (int r0 = J + 1; r0 < 32; r0 += kBc<T>) {
@@ -279,18 +297,20 @@ __device__ __forceinline__ void lu32_step(T (&s)[32], int c, T* dv, T& pinv) {
       b[u] = (r0 + u < 32) ? rdl(s[r0 + u], J) : s_zero(T());   // M(r, J)
     }
 #pragma unroll
-    for (int u = 0; u < kBc<T>; ++u) {
-      if (r0 + u < 32) {
-        const T lrj = s_mul(b[u], inv);  // L(r, J), from lane J before its scaling
-        s[r0 + u] = sel(c == J, lrj, sel(c > J, s_sub(s[r0 + u], s_mul(lrj, mjc)), s[r0 + u]));
-      }
-    }
+    for (int u = 0; u < kBc<T>; ++u)
+      if (r0 + u < 32) s[r0 + u] = s_fma(b[u], f, s[r0 + u]);
     dep = s[r0];
   }
 }
 template <typename T, int... Js>
 __device__ __forceinline__ void lu32_steps(T (&s)[32], int c, T* dv, T& pinv, std::integer_sequence<int, Js...>) {
   (lu32_step<T, Js>(s, c, dv, pinv), ...);
+}
+// L(r, c) = M(r, c) / U(c, c) below the diagonal (lane c holds 1 / U(c, c) in pinv).
+template <typename T>
+__device__ __forceinline__ void lu32_scale_l(T (&s)[32], int c, const T& pinv) {
+#pragma unroll
+  for (int r = 0; r < 32; ++r) s[r] = sel(r > c, s_mul(s[r], pinv), s[r]);
 }
 
 // Pass 2 and the reconstruction (runs only if the flag is clear after pass 1):
@@ -322,7 +342,7 @@ __global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t
     for (int e = tid; e < NB * NB; e += 256) Bs[e % NB][e / NB] = R1inv[e];
     // (gram_tiles begins each tile with a workgroup barrier)
     v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
-    gram_tiles<T, true>(X, ldx, H, nbc, pool[0], Bs, Q1, cr, ci);
+    gram_tiles<T, kTimesB>(X, ldx, H, nbc, pool[0], Bs, Q1, cr, ci);
     if (!partial_is_last<T>(partial, cr, ci, counter, &last_s)) return;
   }
   T (*G)[NB + 1] = pool[0];   // G2, then W2
@@ -360,6 +380,7 @@ __global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t
     for (int r = 0; r < 32; ++r) s[r] = sel(r < nbc && c < nbc, Qt[r][c], sel(r == c, s_one(T()), s_zero(T())));
     T pinv = s_zero(T());
     lu32_steps<T>(s, c, w == 0 ? dv : (T*)nullptr, pinv, std::make_integer_sequence<int, 32>{});
+    lu32_scale_l<T>(s, c, pinv);
     T y[32];
     if (w == 0) {
 #pragma unroll

@@ -21,12 +21,12 @@ int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, in
 // CholeskyQR2 panel with Householder reconstruction (mak_cholqr.hip): factors
 // the H x nbc panel Ap (columns j0.. of the matrix) in place into compact WY
 // form (T to Tpan) unless the device-side check sets the flag status[0], in
-// which case Ap is untouched and status[2] = j0 for the first rejection.
-// status[1] is the kernels' ticket counter (zero between launches).  With
-// sticky = 0 the flag is cleared for an accepted panel, so the caller's
-// flag-gated Householder launches factor exactly the rejected panels; with
-// sticky = 1 a set flag stays set (and skips every later panel), for a caller
-// that checks it on the host.  cw: cholqr_workspace_elems(H).
+// which case Ap is untouched (and T not written) and status[2] = j0 for the
+// first rejection.  status[1] is the kernels' ticket counter (zero between
+// launches).  With sticky = 0 the flag is cleared for an accepted panel, so
+// the caller's flag-gated Householder launches factor exactly the rejected
+// panels; with sticky = 1 a set flag stays set (and skips every later panel),
+// for a caller that checks it on the host.  cw: cholqr_workspace_elems(H).
 size_t cholqr_workspace_elems(int m);
 template <typename T>
 void cholqr_panel(T* Ap, int64_t lda, int H, int nbc, T* Tpan, T* cw, int* status, int j0, int sticky,

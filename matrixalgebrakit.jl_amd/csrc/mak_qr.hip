@@ -122,7 +122,7 @@ template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
 //                     rejected): no host synchronisation, so a stream under
 //                     graph capture can record it;
 //   kCholHostChecked  CholeskyQR2 alone with a sticky flag that also skips every
-//                     later launch; the caller reads the flag once on the host
+//                     later panel; the caller reads the flag once on the host
 //                     and refactors from the first rejected panel.
 enum PanelMode { kHouseholderOnly = 0, kCholGated = 1, kCholHostChecked = 2 };
 
@@ -209,10 +209,9 @@ void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrL
 }
 
 // C (rows x ncols) <- C - V op(T) (V^H C); V is the unit-lower panel at Vp.
-// Skipped when gate is given and *gate != 0.
 template <typename T>
 void apply_wy(const T* Vp, int64_t ldv, int rows, int nbc, const T* Tm, int transT, T* C, int64_t ldc,
-              int ncols, char* ws, const QrLayout<T>& L, hipStream_t s, const int* gate = nullptr) {
+              int ncols, char* ws, const QrLayout<T>& L, hipStream_t s) {
   if (ncols <= 0 || rows <= 0) return;
   T* Wp = reinterpret_cast<T*>(ws + L.offWp);
   T* W2 = reinterpret_cast<T*>(ws + L.offW2);
@@ -225,17 +224,15 @@ void apply_wy(const T* Vp, int64_t ldv, int rows, int nbc, const T* Tm, int tran
     g.A = Vp; g.lda = ldv; g.B = C; g.ldb = ldc; g.C = Wp; g.ldc = nbc;
     g.alpha_re = 1.0; g.beta_re = 0.0;
     g.ksplit = ks; g.kchunk = kc; g.splitStride = slice; g.batch = 1; g.unitLowerA = 1;
-    g.gate = gate; g.gate_want = 0;
     gemm<T>(OP_C, OP_N, g, s);
   }
-  wy_reduce<T>(Wp, ks, slice, Tm, transT, nbc, ncols, W2, s, gate);
+  wy_reduce<T>(Wp, ks, slice, Tm, transT, nbc, ncols, W2, s);
   {
     GemmArgs g{};
     g.M = rows; g.N = ncols; g.K = nbc;
     g.A = Vp; g.lda = ldv; g.B = W2; g.ldb = nbc; g.C = C; g.ldc = ldc;
     g.alpha_re = -1.0; g.beta_re = 1.0;
     g.ksplit = 1; g.kchunk = nbc; g.batch = 1; g.unitLowerA = 1;
-    g.gate = gate; g.gate_want = 0;
     gemm<T>(OP_N, OP_N, g, s);
   }
 }
@@ -252,7 +249,11 @@ template size_t qr_workspace_bytes<zd>(int, int, int);
 
 // Outside graph capture the panels run in kCholHostChecked mode and the
 // stream is synchronised once, to read the CholeskyQR2 flag; under capture
-// they run in kCholGated mode (no synchronisation).
+// they run in kCholGated mode with no synchronisation.
+//
+// In kCholHostChecked mode the trailing updates are not gated: the T blocks
+// start at zero and a skipped panel leaves its T at zero, which makes every
+// update with it an exact no-op (C - V 0 V^H C = C).
 template <typename T>
 void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s) {
   const int minmn = std::min(m, n);
@@ -264,8 +265,12 @@ void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipS
     if (hipStreamIsCapturing(s, &cs) != hipSuccess) cs = hipStreamCaptureStatusActive;
     mode = cs == hipStreamCaptureStatusNone ? kCholHostChecked : kCholGated;
     zero_ints(status, 3, s);  // flag, Gram ticket counter, first rejected column
+    if (mode == kCholHostChecked) {
+      const int npan = cdiv(minmn, NB);
+      zero_ints(reinterpret_cast<int*>(ws + L.offTpanels), (int)((size_t)npan * NB * NB * sizeof(T) / sizeof(int)), s);
+    }
   }
-  auto run = [&](int from, int pmode, const int* gate) {
+  auto run = [&](int from, int pmode) {
     for (int j0 = from; j0 < minmn; j0 += NB) {
       const int nbc = std::min(NB, minmn - j0);
       factor_panel<T>(A, lda, j0, m, nbc, ws, L, pmode, s);
@@ -273,18 +278,18 @@ void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipS
       if (nc > 0) {
         const T* Tpan = reinterpret_cast<const T*>(ws + L.offTpanels) + (size_t)(j0 / NB) * NB * NB;
         apply_wy<T>(A + j0 + (int64_t)j0 * lda, lda, m - j0, nbc, Tpan, 1, A + j0 + (int64_t)(j0 + nbc) * lda,
-                    lda, nc, ws, L, s, gate);
+                    lda, nc, ws, L, s);
       }
     }
   };
-  run(0, mode, mode == kCholHostChecked ? status : nullptr);
+  run(0, mode);
   if (mode == kCholHostChecked) {
     // panels before the first rejected one are final and the trailing matrix
-    // is updated up to it; every later launch was skipped
+    // is updated up to it; every later panel was skipped
     int h[3] = {0, 0, 0};
     if (hipMemcpyAsync(h, status, sizeof(h), hipMemcpyDeviceToHost, s) == hipSuccess &&
         hipStreamSynchronize(s) == hipSuccess && h[0] != 0 && h[2] >= 0 && h[2] < minmn && h[2] % NB == 0)
-      run(h[2], kHouseholderOnly, nullptr);
+      run(h[2], kHouseholderOnly);
   }
 }
 
