@@ -12,6 +12,7 @@
 #include <algorithm>
 
 #include "mak_common.hpp"
+#include "mak_dense32.hpp"
 #include "mak_kernels.hpp"
 
 namespace mak {
@@ -421,6 +422,132 @@ void wy_reduce(const void* Wp, int ksplit, int64_t slice, const void* Tm, int tr
 template void wy_reduce<double>(const void*, int, int64_t, const void*, int, int, int, void*, hipStream_t,
                                 const int*);
 template void wy_reduce<zd>(const void*, int, int64_t, const void*, int, int, int, void*, hipStream_t, const int*);
+
+// ---------------------------------------------------------------------------
+// Block reflectors from several 32-column panels (LAPACK xLARFT's recursion
+// applied blockwise): for panels 0..np-1 with compact WY factors (V_p, T_p),
+//   H_0 H_1 ... H_{np-1} = I - V T V^H,  T = [[T_a, -T_a S_ab T_b], [0, T_b]]
+// built one panel at a time, with S = V^H V.
+// ---------------------------------------------------------------------------
+
+// Vc (rows x kg, ld ldvc) = the unit lower trapezoidal V stored in A below the
+// diagonal (ones on the diagonal, zeros above).
+template <typename T>
+__global__ void copy_unit_lower_kernel(const T* A, int64_t lda, int rows, int kg, T* Vc, int64_t ldvc) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)rows * kg) return;
+  const int r = (int)(idx % rows), c = (int)(idx / rows);
+  T v = s_zero(T());
+  if (r > c) v = A[r + (int64_t)c * lda];
+  else if (r == c) v = s_one(T());
+  Vc[r + (int64_t)c * ldvc] = v;
+}
+template <typename T> void copy_unit_lower(const T* A, int64_t lda, int rows, int kg, T* Vc, int64_t ldvc,
+                                           hipStream_t s) {
+  const int64_t tot = (int64_t)rows * kg;
+  if (tot <= 0) return;
+  hipLaunchKernelGGL((copy_unit_lower_kernel<T>), dim3((unsigned)cdiv64(tot, 256)), dim3(256), 0, s, A, lda, rows,
+                     kg, Vc, ldvc);
+}
+template void copy_unit_lower<double>(const double*, int64_t, int, int, double*, int64_t, hipStream_t);
+template void copy_unit_lower<zd>(const zd*, int64_t, int, int, zd*, int64_t, hipStream_t);
+
+// out (M x N, ld ldo) = sum over the ks slices P + z * slice (M x N, ld M), in
+// slice order.
+template <typename T>
+__global__ void sum_slices_kernel(const T* __restrict__ P, int ks, int64_t slice, int M, int N, T* __restrict__ out,
+                                  int64_t ldo) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)M * N) return;
+  const int i = (int)(idx % M), j = (int)(idx / M);
+  const T* p = P + idx;
+  T acc[4] = {s_zero(T()), s_zero(T()), s_zero(T()), s_zero(T())};
+  int z = 0;
+  for (; z + 4 <= ks; z += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = s_add(acc[u], p[(int64_t)(z + u) * slice]);
+  }
+  for (; z < ks; ++z) acc[0] = s_add(acc[0], p[(int64_t)z * slice]);
+  out[i + (int64_t)j * ldo] = s_add(s_add(acc[0], acc[1]), s_add(acc[2], acc[3]));
+}
+template <typename T> void sum_slices(const T* P, int ks, int64_t slice, int M, int N, T* out, int64_t ldo,
+                                      hipStream_t s) {
+  const int64_t tot = (int64_t)M * N;
+  if (tot <= 0) return;
+  hipLaunchKernelGGL((sum_slices_kernel<T>), dim3((unsigned)cdiv64(tot, 256)), dim3(256), 0, s, P, ks, slice, M, N,
+                     out, ldo);
+}
+template void sum_slices<double>(const double*, int, int64_t, int, int, double*, int64_t, hipStream_t);
+template void sum_slices<zd>(const zd*, int, int64_t, int, int, zd*, int64_t, hipStream_t);
+
+// T (kg x kg, ld kg, upper) from S = V^H V (kg x kg, ld kg) and the panels'
+// T_p (32 x 32 each, consecutive, zero outside their nbc x nbc block), one
+// workgroup: Y_c = S(c, b) T_b, T(a, b) = -sum_{a <= c < b} T(a, c) Y_c, block
+// by block on FP64 MFMA (the finished blocks of T are re-read from memory).
+template <typename T>
+__global__ void __launch_bounds__(256) wy_merge_kernel(const T* S, const T* Tp, int np, int kg, T* Tout) {
+  __shared__ T Y[3][32][33];
+  __shared__ T X[32][33];
+  __shared__ T Z[32][33];
+  __shared__ T Acc[32][33];
+  const int tid = threadIdx.x;
+  auto load = [&](T (*D)[33], const T* src, int64_t ld, int r0, int c0) {
+    for (int e = tid; e < 1024; e += 256) {
+      const int i = e % 32, j = e / 32;
+      const int r = r0 + i, c = c0 + j;
+      D[i][j] = (r < kg && c < kg) ? src[r + (int64_t)c * ld] : s_zero(T());
+    }
+  };
+  auto store = [&](T (*D)[33], int r0, int c0) {
+    for (int e = tid; e < 1024; e += 256) {
+      const int i = e % 32, j = e / 32;
+      const int r = r0 + i, c = c0 + j;
+      if (r < kg && c < kg) Tout[r + (int64_t)c * kg] = D[i][j];
+    }
+  };
+  // diagonal blocks and zeros below them
+  for (int b = 0; b < np; ++b) {
+    for (int e = tid; e < 1024; e += 256) {
+      const int i = e % 32, j = e / 32;
+      const int r = 32 * b + i, c = 32 * b + j;
+      if (r < kg && c < kg) Tout[r + (int64_t)c * kg] = Tp[(int64_t)b * 1024 + e];
+    }
+    for (int a = b + 1; a < np; ++a) {
+      for (int e = tid; e < 1024; e += 256) {
+        const int i = e % 32, j = e / 32;
+        const int r = 32 * a + i, c = 32 * b + j;
+        if (r < kg && c < kg) Tout[r + (int64_t)c * kg] = s_zero(T());
+      }
+    }
+  }
+  __syncthreads();
+  for (int b = 1; b < np; ++b) {
+    for (int e = tid; e < 1024; e += 256) X[e % 32][e / 32] = Tp[(int64_t)b * 1024 + e];
+    for (int c = 0; c < b; ++c) {
+      __syncthreads();
+      load(Z, S, kg, 32 * c, 32 * b);
+      __syncthreads();
+      mm32<T>(Y[c], Z, false, X, false, 1.0, false);  // Y_c = S(c, b) T_b
+    }
+    for (int a = 0; a < b; ++a) {
+      for (int c = a; c < b; ++c) {
+        __syncthreads();
+        load(Z, Tout, kg, 32 * a, 32 * c);
+        __syncthreads();
+        mm32<T>(Acc, Z, false, Y[c], false, -1.0, c > a);  // T(a, b) = -sum T(a, c) Y_c
+      }
+      store(Acc, 32 * a, 32 * b);
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+template <typename T> void wy_merge(const T* S, const T* Tp, int np, int kg, T* Tout, hipStream_t s) {
+  if (np <= 0 || np > 4 || kg <= 0) return;
+  hipLaunchKernelGGL((wy_merge_kernel<T>), dim3(1), dim3(256), 0, s, S, Tp, np, kg, Tout);
+}
+template void wy_merge<double>(const double*, const double*, int, int, double*, hipStream_t);
+template void wy_merge<zd>(const zd*, const zd*, int, int, zd*, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Elementwise helpers

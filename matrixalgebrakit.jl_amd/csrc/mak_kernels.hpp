@@ -151,6 +151,17 @@ template <typename T> void add_diag(T* A, int64_t lda, int n, double v, hipStrea
 template <typename T> void fill_zero(T* A, int64_t lda, int m, int n, hipStream_t s);
 template <typename T> void copy_mat(int m, int n, const T* X, int64_t ldx, T* Y, int64_t ldy, hipStream_t s);
 
+// ---------------------------------------------------------------- block reflectors
+// Vc (rows x kg) = unit lower trapezoidal V stored below A's diagonal.
+template <typename T> void copy_unit_lower(const T* A, int64_t lda, int rows, int kg, T* Vc, int64_t ldvc,
+                                           hipStream_t s);
+// out (M x N, ld ldo) = sum of the ks slices P + z * slice (each M x N, ld M).
+template <typename T> void sum_slices(const T* P, int ks, int64_t slice, int M, int N, T* out, int64_t ldo,
+                                      hipStream_t s);
+// T (kg x kg, ld kg) of the block reflector of np <= 4 consecutive 32-column
+// panels, from S = V^H V (kg x kg, ld kg) and the panels' 32 x 32 T blocks.
+template <typename T> void wy_merge(const T* S, const T* Tp, int np, int kg, T* Tout, hipStream_t s);
+
 // ---------------------------------------------------------------- elementwise
 template <typename T> void set_identity(void* Q, int64_t ldq, int m, int n, hipStream_t s);
 // R <- triu(A[0:minmn, :]) row-scaled by conj(sign_safe(diag)) when positive

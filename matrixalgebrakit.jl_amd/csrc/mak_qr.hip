@@ -77,10 +77,29 @@ template <typename T> struct QrLayout {
   size_t offTree = 0;
   size_t offWp = 0, offW2 = 0;
   size_t wpElems = 0, w2Elems = 0;
+  // 128-column block reflectors of Q's formation (kGroup panels):
+  size_t offVc = 0, offS = 0, offTg = 0, offWg = 0;
   size_t bytes = 0;
 };
 
 inline size_t align256(size_t b) { return ((b + 255) / 256) * 256; }
+
+// Q is formed by block reflectors of kGroup panels (kGB columns).
+constexpr int kGroup = 4;
+constexpr int kGB = kGroup * NB;
+
+// Split-K slices (kGB x N x ksplit elements) of the K = m products with kGB
+// rows, maximised over the N <= ncmax that occur (ksplit * N peaks at a
+// column count below ncmax, so every N is tried).
+size_t wp_block_elems(int m, int ncmax) {
+  size_t most = 0;
+  for (int N = 1; N <= std::max(ncmax, kGB); ++N) {
+    int ks, kc;
+    splitk_plan(N, m, 128, ks, kc);
+    most = std::max(most, (size_t)ks * N);
+  }
+  return (size_t)kGB * most;
+}
 
 template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
   QrLayout<T> L;
@@ -106,11 +125,23 @@ template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
   L.wpElems = (size_t)NB * ncmax * std::max(ks, 1);
   // ksplit grows as N shrinks: bound ksplit * N over every N <= ncmax
   L.wpElems = std::max(L.wpElems, (size_t)NB * (size_t)(256 * 64 + ncmax));
+  if (minmn > NB) L.wpElems = std::max(L.wpElems, wp_block_elems(m, ncmax));
   L.offWp = off;
   off += align256(L.wpElems * sizeof(T));
   L.w2Elems = (size_t)NB * ncmax;
+  if (minmn > NB) L.w2Elems = std::max(L.w2Elems, (size_t)kGB * ncmax);
   L.offW2 = off;
   off += align256(L.w2Elems * sizeof(T));
+  if (minmn > NB) {
+    L.offVc = off;
+    off += align256((size_t)m * kGB * sizeof(T));
+    L.offS = off;
+    off += align256((size_t)kGB * kGB * sizeof(T));
+    L.offTg = off;
+    off += align256((size_t)kGB * kGB * sizeof(T));
+    L.offWg = off;
+    off += align256((size_t)kGB * ncmax * sizeof(T));
+  }
   L.bytes = off;
   return L;
 }
@@ -247,15 +278,34 @@ template <typename T> size_t qr_workspace_bytes(int m, int n, int full) {
 template size_t qr_workspace_bytes<double>(int, int, int);
 template size_t qr_workspace_bytes<zd>(int, int, int);
 
-// Outside graph capture the panels run in kCholHostChecked mode and the
-// stream is synchronised once, to read the CholeskyQR2 flag; under capture
-// they run in kCholGated mode with no synchronisation.
+// Panels from column `from` on, each followed by its trailing update.
+template <typename T>
+void geqrf_panels(T* A, int64_t lda, int m, int n, char* ws, const QrLayout<T>& L, int from, int mode,
+                  hipStream_t s) {
+  const int minmn = std::min(m, n);
+  for (int j0 = from; j0 < minmn; j0 += NB) {
+    const int nbc = std::min(NB, minmn - j0);
+    factor_panel<T>(A, lda, j0, m, nbc, ws, L, mode, s);
+    const int nc = n - j0 - nbc;
+    if (nc > 0) {
+      const T* Tpan = reinterpret_cast<const T*>(ws + L.offTpanels) + (size_t)(j0 / NB) * NB * NB;
+      apply_wy<T>(A + j0 + (int64_t)j0 * lda, lda, m - j0, nbc, Tpan, 1, A + j0 + (int64_t)(j0 + nbc) * lda, lda,
+                  nc, ws, L, s);
+    }
+  }
+}
+
+// Enqueues the factorisation.  Outside graph capture the panels run in
+// kCholHostChecked mode and the function returns true: the caller may enqueue
+// more work that reads the factors, and must then call geqrf_finish, which
+// synchronises the stream once to read the CholeskyQR2 flag.  Under capture
+// the panels run in kCholGated mode (nothing to check; returns false).
 //
 // In kCholHostChecked mode the trailing updates are not gated: the T blocks
 // start at zero and a skipped panel leaves its T at zero, which makes every
 // update with it an exact no-op (C - V 0 V^H C = C).
 template <typename T>
-void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s) {
+bool geqrf_enqueue(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s) {
   const int minmn = std::min(m, n);
   QrLayout<T> L = qr_layout<T>(m, n, (int)qcols);
   int* status = L.offFlags ? reinterpret_cast<int*>(ws + L.offFlags) : nullptr;
@@ -270,26 +320,76 @@ void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipS
       zero_ints(reinterpret_cast<int*>(ws + L.offTpanels), (int)((size_t)npan * NB * NB * sizeof(T) / sizeof(int)), s);
     }
   }
-  auto run = [&](int from, int pmode) {
-    for (int j0 = from; j0 < minmn; j0 += NB) {
-      const int nbc = std::min(NB, minmn - j0);
-      factor_panel<T>(A, lda, j0, m, nbc, ws, L, pmode, s);
-      const int nc = n - j0 - nbc;
-      if (nc > 0) {
-        const T* Tpan = reinterpret_cast<const T*>(ws + L.offTpanels) + (size_t)(j0 / NB) * NB * NB;
-        apply_wy<T>(A + j0 + (int64_t)j0 * lda, lda, m - j0, nbc, Tpan, 1, A + j0 + (int64_t)(j0 + nbc) * lda,
-                    lda, nc, ws, L, s);
-      }
-    }
+  geqrf_panels<T>(A, lda, m, n, ws, L, 0, mode, s);
+  return mode == kCholHostChecked;
+}
+
+// Reads the flag (synchronising s).  Panels before the first rejected one are
+// final and the trailing matrix is updated up to it; every later panel was
+// skipped, so the factorisation resumes there with Householder TSQR.  Returns
+// true when it refactored (work enqueued after geqrf_enqueue must be redone).
+template <typename T>
+bool geqrf_finish(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s) {
+  const int minmn = std::min(m, n);
+  QrLayout<T> L = qr_layout<T>(m, n, (int)qcols);
+  if (!L.offFlags) return false;
+  int h[3] = {0, 0, 0};
+  if (hipMemcpyAsync(h, ws + L.offFlags, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess || h[0] == 0 || h[2] < 0 || h[2] >= minmn || h[2] % NB != 0)
+    return false;
+  geqrf_panels<T>(A, lda, m, n, ws, L, h[2], kHouseholderOnly, s);
+  return true;
+}
+
+template <typename T>
+void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s) {
+  if (geqrf_enqueue<T>(A, lda, m, n, ws, qcols, s)) geqrf_finish<T>(A, lda, m, n, ws, qcols, s);
+}
+
+// C (rows x ncols) <- C - V op(Tg) (V^H C) for the block reflector of panels
+// p0 .. p0 + np - 1 (kg columns of compact WY factors at Vp, their T blocks at
+// Tp): V is copied out unit lower, S = V^H V, Tg from S and the T blocks,
+// then three GEMMs with K = rows, kg, kg.
+template <typename T>
+void apply_block_wy(const T* Vp, int64_t ldv, int rows, int kg, int np, const T* Tp, int transT, T* C,
+                    int64_t ldc, int ncols, char* ws, const QrLayout<T>& L, hipStream_t s) {
+  if (ncols <= 0 || rows <= 0) return;
+  T* Vc = reinterpret_cast<T*>(ws + L.offVc);
+  T* S = reinterpret_cast<T*>(ws + L.offS);
+  T* Tg = reinterpret_cast<T*>(ws + L.offTg);
+  T* Wg = reinterpret_cast<T*>(ws + L.offWg);
+  T* Wp = reinterpret_cast<T*>(ws + L.offWp);
+  T* W2 = reinterpret_cast<T*>(ws + L.offW2);
+  copy_unit_lower<T>(Vp, ldv, rows, kg, Vc, rows, s);
+  auto vhx = [&](const T* X, int64_t ldx, int N, T* out) {  // out (kg x N, ld kg) = V^H X
+    int ks, kc;
+    splitk_plan(N, rows, 128, ks, kc);
+    GemmArgs g{};
+    g.M = kg; g.N = N; g.K = rows;
+    g.A = Vc; g.lda = rows; g.B = X; g.ldb = ldx; g.C = Wp; g.ldc = kg;
+    g.alpha_re = 1.0; g.beta_re = 0.0;
+    g.ksplit = ks; g.kchunk = kc; g.splitStride = (int64_t)kg * N; g.batch = 1;
+    gemm<T>(OP_C, OP_N, g, s);
+    sum_slices<T>(Wp, ks, (int64_t)kg * N, kg, N, out, kg, s);
   };
-  run(0, mode);
-  if (mode == kCholHostChecked) {
-    // panels before the first rejected one are final and the trailing matrix
-    // is updated up to it; every later panel was skipped
-    int h[3] = {0, 0, 0};
-    if (hipMemcpyAsync(h, status, sizeof(h), hipMemcpyDeviceToHost, s) == hipSuccess &&
-        hipStreamSynchronize(s) == hipSuccess && h[0] != 0 && h[2] >= 0 && h[2] < minmn && h[2] % NB == 0)
-      run(h[2], kHouseholderOnly);
+  vhx(Vc, rows, kg, S);
+  wy_merge<T>(S, Tp, np, kg, Tg, s);
+  vhx(C, ldc, ncols, Wg);
+  {
+    GemmArgs g{};
+    g.M = kg; g.N = ncols; g.K = kg;
+    g.A = Tg; g.lda = kg; g.B = Wg; g.ldb = kg; g.C = W2; g.ldc = kg;
+    g.alpha_re = 1.0; g.beta_re = 0.0;
+    g.ksplit = 1; g.kchunk = kg; g.batch = 1;
+    gemm<T>(transT ? OP_C : OP_N, OP_N, g, s);
+  }
+  {
+    GemmArgs g{};
+    g.M = rows; g.N = ncols; g.K = kg;
+    g.A = Vc; g.lda = rows; g.B = W2; g.ldb = kg; g.C = C; g.ldc = ldc;
+    g.alpha_re = -1.0; g.beta_re = 1.0;
+    g.ksplit = 1; g.kchunk = kg; g.batch = 1;
+    gemm<T>(OP_N, OP_N, g, s);
   }
 }
 
@@ -299,12 +399,18 @@ void orgqr_blocked(const T* A, int64_t lda, int m, int n, T* Q, int64_t ldq, int
   QrLayout<T> L = qr_layout<T>(m, n, qcols);
   set_identity<T>(Q, ldq, m, qcols, s);
   const int npan = cdiv(minmn, NB);
-  for (int p = npan - 1; p >= 0; --p) {
-    const int j0 = p * NB;
-    const int nbc = std::min(NB, minmn - j0);
-    const T* Tpan = reinterpret_cast<const T*>(ws + L.offTpanels) + (size_t)p * NB * NB;
-    apply_wy<T>(A + j0 + (int64_t)j0 * lda, lda, m - j0, nbc, Tpan, 0, Q + j0 + (int64_t)j0 * ldq, ldq,
-                qcols - j0, ws, L, s);
+  const T* Tpanels = reinterpret_cast<const T*>(ws + L.offTpanels);
+  // Q = H_0 ... H_{npan-1} [I; 0], applied backwards in groups of kGroup panels
+  for (int g = cdiv(npan, kGroup) - 1; g >= 0; --g) {
+    const int p0 = g * kGroup, np = std::min(kGroup, npan - p0);
+    const int j0 = p0 * NB;
+    const int kg = std::min(minmn, (p0 + np) * NB) - j0;
+    const T* Vp = A + j0 + (int64_t)j0 * lda;
+    T* C = Q + j0 + (int64_t)j0 * ldq;
+    if (np == 1)
+      apply_wy<T>(Vp, lda, m - j0, kg, Tpanels + (size_t)p0 * NB * NB, 0, C, ldq, qcols - j0, ws, L, s);
+    else
+      apply_block_wy<T>(Vp, lda, m - j0, kg, np, Tpanels + (size_t)p0 * NB * NB, 0, C, ldq, qcols - j0, ws, L, s);
   }
 }
 
@@ -329,14 +435,21 @@ int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, in
     if (R) qr_extract_r<T>(A, lda, R, ldr, rrows, n, 0, 0, s);
     return hipGetLastError() == hipSuccess ? 0 : 1;
   }
-  geqrf_blocked<T>(A, lda, m, n, ws, qcols, s);
-  if (Tout) {  // keep the compact WY factors' T blocks (32 x 32 per panel) for the adjoint
-    const int npan = cdiv(minmn, NB);
-    hipMemcpyAsync(Tout, ws, sizeof(T) * NB * NB * npan, hipMemcpyDeviceToDevice, s);
-  }
-  if (R) qr_extract_r<T>(A, lda, R, ldr, rrows, n, minmn, positive, s);
-  orgqr_blocked<T>(A, lda, m, n, Q, ldq, qcols, ws, s);
-  if (positive) qr_gauge_q<T>(A, lda, Q, ldq, m, minmn, s);
+  // Q, R and the tape are formed before the CholeskyQR2 flag is read, so the
+  // one synchronisation does not leave the GPU waiting for the host; they are
+  // formed again in the (rare) case that a panel was refactored.
+  auto outputs = [&]() {
+    if (Tout) {  // keep the compact WY factors' T blocks (32 x 32 per panel) for the adjoint
+      const int npan = cdiv(minmn, NB);
+      hipMemcpyAsync(Tout, ws, sizeof(T) * NB * NB * npan, hipMemcpyDeviceToDevice, s);
+    }
+    if (R) qr_extract_r<T>(A, lda, R, ldr, rrows, n, minmn, positive, s);
+    orgqr_blocked<T>(A, lda, m, n, Q, ldq, qcols, ws, s);
+    if (positive) qr_gauge_q<T>(A, lda, Q, ldq, m, minmn, s);
+  };
+  const bool check = geqrf_enqueue<T>(A, lda, m, n, ws, qcols, s);
+  outputs();
+  if (check && geqrf_finish<T>(A, lda, m, n, ws, qcols, s)) outputs();
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
