@@ -483,12 +483,15 @@ template void sum_slices<zd>(const zd*, int, int64_t, int, int, zd*, int64_t, hi
 // T (kg x kg, ld kg, upper) from S = V^H V (kg x kg, ld kg) and the panels'
 // T_p (32 x 32 each, consecutive, zero outside their nbc x nbc block), one
 // workgroup: Y_c = S(c, b) T_b, T(a, b) = -sum_{a <= c < b} T(a, c) Y_c, block
-// by block on FP64 MFMA (the finished blocks of T are re-read from memory).
+// by block on FP64 MFMA.  Float64 keeps every block of T in LDS; ComplexF64
+// (twice the bytes) re-reads the finished blocks from Tout.
 template <typename T>
 __global__ void __launch_bounds__(256) wy_merge_kernel(const T* S, const T* Tp, int np, int kg, T* Tout) {
+  constexpr bool LDS_T = !ScalarTraits<T>::is_complex;
+  __shared__ T Tt[LDS_T ? 10 : 1][32][33];  // block (a, c), a <= c, at c (c + 1) / 2 + a
   __shared__ T Y[3][32][33];
+  __shared__ T Z[3][32][33];
   __shared__ T X[32][33];
-  __shared__ T Z[32][33];
   __shared__ T Acc[32][33];
   const int tid = threadIdx.x;
   auto load = [&](T (*D)[33], const T* src, int64_t ld, int r0, int c0) {
@@ -498,48 +501,62 @@ __global__ void __launch_bounds__(256) wy_merge_kernel(const T* S, const T* Tp, 
       D[i][j] = (r < kg && c < kg) ? src[r + (int64_t)c * ld] : s_zero(T());
     }
   };
-  auto store = [&](T (*D)[33], int r0, int c0) {
+  auto store = [&](const T (*D)[33], int r0, int c0) {
     for (int e = tid; e < 1024; e += 256) {
       const int i = e % 32, j = e / 32;
       const int r = r0 + i, c = c0 + j;
       if (r < kg && c < kg) Tout[r + (int64_t)c * kg] = D[i][j];
     }
   };
-  // diagonal blocks and zeros below them
+  auto tix = [](int a, int c) { return c * (c + 1) / 2 + a; };
+  // diagonal blocks T_b, zeros below them
   for (int b = 0; b < np; ++b) {
-    for (int e = tid; e < 1024; e += 256) {
-      const int i = e % 32, j = e / 32;
-      const int r = 32 * b + i, c = 32 * b + j;
-      if (r < kg && c < kg) Tout[r + (int64_t)c * kg] = Tp[(int64_t)b * 1024 + e];
-    }
-    for (int a = b + 1; a < np; ++a) {
+    if constexpr (LDS_T) {
+      for (int e = tid; e < 1024; e += 256) Tt[tix(b, b)][e % 32][e / 32] = Tp[(int64_t)b * 1024 + e];
+    } else {
       for (int e = tid; e < 1024; e += 256) {
-        const int i = e % 32, j = e / 32;
-        const int r = 32 * a + i, c = 32 * b + j;
+        const int r = 32 * b + e % 32, c = 32 * b + e / 32;
+        if (r < kg && c < kg) Tout[r + (int64_t)c * kg] = Tp[(int64_t)b * 1024 + e];
+      }
+    }
+    for (int a = b + 1; a < np; ++a)
+      for (int e = tid; e < 1024; e += 256) {
+        const int r = 32 * a + e % 32, c = 32 * b + e / 32;
         if (r < kg && c < kg) Tout[r + (int64_t)c * kg] = s_zero(T());
+      }
+  }
+  for (int b = 1; b < np; ++b) {
+    __syncthreads();
+    for (int c = 0; c < b; ++c) load(Z[c], S, kg, 32 * c, 32 * b);
+    if constexpr (!LDS_T)
+      for (int e = tid; e < 1024; e += 256) X[e % 32][e / 32] = Tp[(int64_t)b * 1024 + e];
+    __syncthreads();
+    for (int c = 0; c < b; ++c) {
+      if constexpr (LDS_T) mm32<T>(Y[c], Z[c], false, Tt[tix(b, b)], false, 1.0, false);  // Y_c = S(c, b) T_b
+      else mm32<T>(Y[c], Z[c], false, X, false, 1.0, false);
+    }
+    for (int a = 0; a < b; ++a) {
+      T (*dst)[33] = LDS_T ? Tt[tix(a, b)] : Acc;
+      for (int c = a; c < b; ++c) {
+        if constexpr (LDS_T) {
+          mm32<T>(dst, Tt[tix(a, c)], false, Y[c], false, -1.0, c > a);  // T(a, b) = -sum T(a, c) Y_c
+        } else {
+          __syncthreads();
+          load(Z[0], Tout, kg, 32 * a, 32 * c);
+          __syncthreads();
+          mm32<T>(dst, Z[0], false, Y[c], false, -1.0, c > a);
+        }
+      }
+      if constexpr (!LDS_T) {
+        store(Acc, 32 * a, 32 * b);
+        __syncthreads();
       }
     }
   }
-  __syncthreads();
-  for (int b = 1; b < np; ++b) {
-    for (int e = tid; e < 1024; e += 256) X[e % 32][e / 32] = Tp[(int64_t)b * 1024 + e];
-    for (int c = 0; c < b; ++c) {
-      __syncthreads();
-      load(Z, S, kg, 32 * c, 32 * b);
-      __syncthreads();
-      mm32<T>(Y[c], Z, false, X, false, 1.0, false);  // Y_c = S(c, b) T_b
-    }
-    for (int a = 0; a < b; ++a) {
-      for (int c = a; c < b; ++c) {
-        __syncthreads();
-        load(Z, Tout, kg, 32 * a, 32 * c);
-        __syncthreads();
-        mm32<T>(Acc, Z, false, Y[c], false, -1.0, c > a);  // T(a, b) = -sum T(a, c) Y_c
-      }
-      store(Acc, 32 * a, 32 * b);
-    }
-    __threadfence_block();
+  if constexpr (LDS_T) {
     __syncthreads();
+    for (int b = 0; b < np; ++b)
+      for (int a = 0; a <= b; ++a) store(Tt[tix(a, b)], 32 * a, 32 * b);
   }
 }
 template <typename T> void wy_merge(const T* S, const T* Tp, int np, int kg, T* Tout, hipStream_t s) {
