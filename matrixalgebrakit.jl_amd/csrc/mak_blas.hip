@@ -139,12 +139,22 @@ static void potrf_inv_rec(T* Z, int64_t ldz, T* Y, int64_t ldy, int n, int offse
   T* Y12 = Y + (int64_t)n1 * ldy;
   T* Y22 = Y + n1 + (int64_t)n1 * ldy;
   potrf_inv_rec<T>(Z, ldz, Y, ldy, n1, offset, tmp, dinfo, s);
-  gemm<T>(OP_C, OP_N, gargs<T>(n1, n2, n1, Y, ldy, Z12, ldz, tmp, n1, 1.0, 0.0), s);
+  // Y11, Y22 are upper triangular with zeros below; Z22 is read only above
+  // its diagonal by the recursion
+  GemmArgs g = gargs<T>(n1, n2, n1, Y, ldy, Z12, ldz, tmp, n1, 1.0, 0.0);
+  g.tri = TRI_A_LOWER;  // op(A) = Y11^H
+  gemm<T>(OP_C, OP_N, g, s);
   copy_mat<T>(n1, n2, tmp, n1, Z12, ldz, s);
-  gemm<T>(OP_C, OP_N, gargs<T>(n2, n2, n1, Z12, ldz, Z12, ldz, Z22, ldz, -1.0, 1.0), s);
+  g = gargs<T>(n2, n2, n1, Z12, ldz, Z12, ldz, Z22, ldz, -1.0, 1.0);
+  g.upperC = 1;
+  gemm<T>(OP_C, OP_N, g, s);
   potrf_inv_rec<T>(Z22, ldz, Y22, ldy, n2, offset + n1, tmp, dinfo, s);
-  gemm<T>(OP_N, OP_N, gargs<T>(n1, n2, n1, Y, ldy, Z12, ldz, tmp, n1, 1.0, 0.0), s);
-  gemm<T>(OP_N, OP_N, gargs<T>(n1, n2, n2, tmp, n1, Y22, ldy, Y12, ldy, -1.0, 0.0), s);
+  g = gargs<T>(n1, n2, n1, Y, ldy, Z12, ldz, tmp, n1, 1.0, 0.0);
+  g.tri = TRI_A_UPPER;  // Y11
+  gemm<T>(OP_N, OP_N, g, s);
+  g = gargs<T>(n1, n2, n2, tmp, n1, Y22, ldy, Y12, ldy, -1.0, 0.0);
+  g.tri = TRI_B_UPPER;  // Y22
+  gemm<T>(OP_N, OP_N, g, s);
   fill_zero<T>(Y + n1, ldy, n2, n1, s);
 }
 

@@ -48,13 +48,27 @@ __device__ inline zd load_b(const zd* B, int64_t ldb, int gk, int gj) {
 // Tile order: linear workgroup ids are dealt round-robin to the 8 XCDs; regroup
 // them so every XCD walks a contiguous range of tiles (shared A / B panels stay
 // in that XCD's L2).
-__device__ inline void tile_coords(int& tm, int& tn) {
+// With triangular hints whose K range grows with the tile's row or column
+// index (TRI_A_LOWER: rows, TRI_B_UPPER: columns), that index is reversed so
+// that the longest tiles are dispatched first.
+__device__ inline void tile_coords(int& tm, int& tn, int tri = 0) {
   const int gx = gridDim.x, gy = gridDim.y;
   const int total = gx * gy;
   int L = blockIdx.x + blockIdx.y * gx;
   if ((total & 7) == 0) L = (L & 7) * (total >> 3) + (L >> 3);
   tm = L % gx;
   tn = L / gx;
+  if (tri & TRI_A_LOWER) tm = gx - 1 - tm;
+  if (tri & TRI_B_UPPER) tn = gy - 1 - tn;
+}
+
+// Clip [kbeg, kend) to the terms a tile (rows m0.., cols n0..) can have
+// nonzero under the triangular-operand hints (GemmArgs::tri).
+__device__ inline void tri_k_range(int tri, int m0, int bm, int n0, int bn, int& kbeg, int& kend) {
+  if (tri & TRI_A_LOWER) kend = min(kend, m0 + bm);
+  if (tri & TRI_B_UPPER) kend = min(kend, n0 + bn);
+  if (tri & TRI_A_UPPER) kbeg = max(kbeg, m0);
+  if (tri & TRI_B_LOWER) kbeg = max(kbeg, n0);
 }
 
 // Real FP64 GEMM.  Each thread owns fixed load coordinates; the next K-tile is
@@ -73,13 +87,15 @@ __global__ void __launch_bounds__(256, 2) gemm_f64_kernel(GemmArgs g) {
   const int w = tid >> 6;
   const int wm = w % WM, wn = w / WM;
   int tmi, tni;
-  tile_coords(tmi, tni);
+  tile_coords(tmi, tni, g.tri);
   const int bz = blockIdx.z;
   const int bat = bz / g.ksplit;
   const int ks = bz % g.ksplit;
   const int m0 = tmi * BM, n0 = tni * BN;
-  const int kbeg = ks * g.kchunk;
-  const int kend = min(g.K, kbeg + g.kchunk);
+  if (g.upperC && m0 >= n0 + BN) return;
+  int kbeg = ks * g.kchunk;
+  int kend = min(g.K, kbeg + g.kchunk);
+  tri_k_range(g.tri, m0, BM, n0, BN, kbeg, kend);
   const double* A = reinterpret_cast<const double*>(g.A) + (int64_t)bat * g.strideA;
   const double* B = reinterpret_cast<const double*>(g.B) + (int64_t)bat * g.strideB;
   double* C = reinterpret_cast<double*>(g.C) + (int64_t)bat * g.strideC + (int64_t)ks * g.splitStride;
@@ -200,13 +216,15 @@ __global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
   const int w = tid >> 6;
   const int wm = w % WM, wn = w / WM;
   int tmi, tni;
-  tile_coords(tmi, tni);
+  tile_coords(tmi, tni, g.tri);
   const int bz = blockIdx.z;
   const int bat = bz / g.ksplit;
   const int ks = bz % g.ksplit;
   const int m0 = tmi * BM, n0 = tni * BN;
-  const int kbeg = ks * g.kchunk;
-  const int kend = min(g.K, kbeg + g.kchunk);
+  if (g.upperC && m0 >= n0 + BN) return;
+  int kbeg = ks * g.kchunk;
+  int kend = min(g.K, kbeg + g.kchunk);
+  tri_k_range(g.tri, m0, BM, n0, BN, kbeg, kend);
   const zd* A = reinterpret_cast<const zd*>(g.A) + (int64_t)bat * g.strideA;
   const zd* B = reinterpret_cast<const zd*>(g.B) + (int64_t)bat * g.strideB;
   zd* C = reinterpret_cast<zd*>(g.C) + (int64_t)bat * g.strideC + (int64_t)ks * g.splitStride;

@@ -100,6 +100,7 @@ template <typename T> void launch_tsqr_expand(const TsqrExpandArgs& a, int max_n
 
 // ---------------------------------------------------------------- GEMM
 enum Op { OP_N = 0, OP_T = 1, OP_C = 2 };
+enum TriHint { TRI_A_LOWER = 1, TRI_A_UPPER = 2, TRI_B_UPPER = 4, TRI_B_LOWER = 8 };
 
 struct GemmArgs {
   int M, N, K;
@@ -115,6 +116,15 @@ struct GemmArgs {
   int64_t splitStride;
   int batch;
   int unitLowerA;         // stored A is unit lower trapezoidal (strict lower part valid)
+  // Structure hints (FP64 / complex kernels; the others ignore them, which is
+  // also correct): the K range of a tile is clipped to the terms that can be
+  // nonzero when op(A) is lower (tri & TRI_A_LOWER: k <= i) or upper
+  // (TRI_A_UPPER: k >= i) triangular and op(B) is upper (TRI_B_UPPER: k <= j)
+  // or lower (TRI_B_LOWER: k >= j) triangular; upperC skips tiles strictly
+  // below the diagonal (their C is left as it was), for a consumer that reads
+  // only the upper triangle.
+  int tri;
+  int upperC;
   const int* gate;        // run only if *gate == gate_want (when non-null)
   int gate_want;
 };

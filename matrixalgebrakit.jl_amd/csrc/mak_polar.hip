@@ -62,13 +62,17 @@ void qdwh_params(double l, double& a, double& b, double& c) {
   c = a + b - 1.0;
 }
 
+// tri / upperC: structure hints (GemmArgs): the triangular inverse factors
+// have exact zeros below the diagonal, and the Cholesky factorisation reads
+// only the upper triangle of a Gram matrix.
 template <typename T> GemmArgs ga(int M, int N, int K, const T* A, int64_t lda, const T* B, int64_t ldb, T* C,
-                                  int64_t ldc, double alpha, double beta) {
+                                  int64_t ldc, double alpha, double beta, int tri = 0, int upperC = 0) {
   GemmArgs g{};
   g.M = M; g.N = N; g.K = K;
   g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
   g.alpha_re = alpha; g.beta_re = beta;
   g.ksplit = 1; g.kchunk = K; g.batch = 1;
+  g.tri = tri; g.upperC = upperC;
   return g;
 }
 
@@ -147,28 +151,29 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
     const int64_t ldc = ldb[cur];
     // G = I + c X^H X  (FP64: it feeds the Cholesky factorisation)
     set_identity<T>(G, ldn, n, n, s);
-    gemm<T>(OP_C, OP_N, ga<T>(n, n, m, Xc, ldc, Xc, ldc, G, ldn, c, 1.0), s);
+    gemm<T>(OP_C, OP_N, ga<T>(n, n, m, Xc, ldc, Xc, ldc, G, ldn, c, 1.0, 0, 1), s);
     potrf_upper<T>(G, ldn, n, Y1, ldn, tmp, dinfo, s);
     int nxt;
     if (c > 100.0) {
       // pass 1: Q1 = sqrt(c) X W1^{-1} (buf[s2]), Q2 = W1^{-1} (Y1)
-      gemm_prec<T>(prec_qr, OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y1, ldn, buf[s2], ldb[s2], std::sqrt(c), 0.0), s);
+      gemm_prec<T>(prec_qr, OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y1, ldn, buf[s2], ldb[s2], std::sqrt(c), 0.0, TRI_B_UPPER), s);
       // pass 2: G = Q2^H Q2 + Q1^H Q1 = W2^H W2
-      gemm<T>(OP_C, OP_N, ga<T>(n, n, n, Y1, ldn, Y1, ldn, G, ldn, 1.0, 0.0), s);
-      gemm<T>(OP_C, OP_N, ga<T>(n, n, m, buf[s2], ldb[s2], buf[s2], ldb[s2], G, ldn, 1.0, 1.0), s);
+      gemm<T>(OP_C, OP_N, ga<T>(n, n, n, Y1, ldn, Y1, ldn, G, ldn, 1.0, 0.0, TRI_A_LOWER | TRI_B_UPPER, 1), s);
+      gemm<T>(OP_C, OP_N, ga<T>(n, n, m, buf[s2], ldb[s2], buf[s2], ldb[s2], G, ldn, 1.0, 1.0, 0, 1), s);
       potrf_upper<T>(G, ldn, n, Y2, ldn, tmp, dinfo, s);
       // Q1' = Q1 W2^{-1} (buf[s1]), Q2' = Q2 W2^{-1} (G)
-      gemm_prec<T>(prec_qr, OP_N, OP_N, ga<T>(m, n, n, buf[s2], ldb[s2], Y2, ldn, buf[s1], ldb[s1], 1.0, 0.0), s);
-      gemm<T>(OP_N, OP_N, ga<T>(n, n, n, Y1, ldn, Y2, ldn, G, ldn, 1.0, 0.0), s);
+      gemm_prec<T>(prec_qr, OP_N, OP_N, ga<T>(m, n, n, buf[s2], ldb[s2], Y2, ldn, buf[s1], ldb[s1], 1.0, 0.0, TRI_B_UPPER), s);
+      gemm<T>(OP_N, OP_N, ga<T>(n, n, n, Y1, ldn, Y2, ldn, G, ldn, 1.0, 0.0, TRI_A_UPPER | TRI_B_UPPER), s);
       // X+ = (b/c) X + (a - b/c)/sqrt(c) Q1' Q2'^H  (buf[s2])
       copy_mat<T>(m, n, Xc, ldc, buf[s2], ldb[s2], s);
       gemm_prec<T>(prec_qr, OP_N, OP_C,
-                   ga<T>(m, n, n, buf[s1], ldb[s1], G, ldn, buf[s2], ldb[s2], (a - b / c) / std::sqrt(c), b / c), s);
+                   ga<T>(m, n, n, buf[s1], ldb[s1], G, ldn, buf[s2], ldb[s2], (a - b / c) / std::sqrt(c), b / c,
+                         TRI_B_LOWER), s);
       nxt = s2;
       ++nqr;
     } else {
       // Z^{-1} = W^{-1} W^{-H};  X+ = (b/c) X + (a - b/c) X Z^{-1}  (buf[s1])
-      gemm_prec<T>(prec, OP_N, OP_C, ga<T>(n, n, n, Y1, ldn, Y1, ldn, Y2, ldn, 1.0, 0.0), s);
+      gemm_prec<T>(prec, OP_N, OP_C, ga<T>(n, n, n, Y1, ldn, Y1, ldn, Y2, ldn, 1.0, 0.0, TRI_A_UPPER | TRI_B_LOWER), s);
       copy_mat<T>(m, n, Xc, ldc, buf[s1], ldb[s1], s);
       gemm_prec<T>(prec, OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y2, ldn, buf[s1], ldb[s1], a - b / c, b / c), s);
       nxt = s1;
