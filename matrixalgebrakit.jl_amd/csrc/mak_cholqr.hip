@@ -277,7 +277,8 @@ __global__ void __launch_bounds__(256) gram_chol_kernel(const T* X, int64_t ldx,
 // their finite entries exact).  dv receives d (when dv is not null), `pinv`
 // of lane j receives 1 / U(j, j).
 template <typename T, int J>
-__device__ __forceinline__ void lu32_step(T (&s)[32], int c, T* dv, T& pinv) {
+__device__ __forceinline__ void lu32_step(T (&s)[32], int c0, T* dv, T& pinv) {
+  const int c = step_lane(c0);
   const T piv = rdl(s[J], J);
   const T d = s_neg(sign_safe(piv));
   const T p = s_sub(piv, d);

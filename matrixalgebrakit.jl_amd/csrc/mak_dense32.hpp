@@ -68,6 +68,13 @@ template <typename T> constexpr int kBc = ScalarTraits<T>::is_complex ? 2 : 4;
 // step (or of a whole triangular inverse) into scalar registers at once and
 // spills them to VGPR lanes, which costs more than the arithmetic.
 __device__ __forceinline__ void order_after(double& v, double dep) { asm volatile("" : "+v"(v) : "v"(dep)); }
+// The lane's column index through an empty asm, so that the per-step lane
+// masks (c > j, c == j) are formed in their step instead of all being hoisted
+// into scalar registers up front.
+__device__ __forceinline__ int step_lane(int c) {
+  asm volatile("" : "+v"(c));
+  return c;
+}
 __device__ __forceinline__ void order_after(zd& v, const zd& dep) {
   asm volatile("" : "+v"(v.x), "+v"(v.y) : "v"(dep.x));
 }
@@ -86,7 +93,8 @@ __device__ __forceinline__ void order_after(zd& v, const zd& dep) {
 // below the diagonal only receive the mirrored update and are ignored).
 // `dinv` of lane j receives 1 / W(j, j).
 template <typename T, int J>
-__device__ __forceinline__ void chol32_step(T (&s)[32], int c, int& bad, T& dinv) {
+__device__ __forceinline__ void chol32_step(T (&s)[32], int c0, int& bad, T& dinv) {
+  const int c = step_lane(c0);
   const double d2 = s_re(rdl(s[J], J));
   if (!bad && !(d2 > 0.0)) bad = J + 1;
   const double rd = (d2 > 0.0) ? rsqrt_nr(d2) : 0.0;
@@ -116,8 +124,9 @@ __device__ __forceinline__ void chol32_steps(T (&s)[32], int c, int& bad, T& din
 // (lane c: s[r] = U(r, c)), `dinv` of lane r = 1 / U(r, r).  Rows from the
 // bottom: Y(r, c) = (delta_rc - sum_{k > r} U(r, k) Y(k, c)) / U(r, r).
 template <typename T, int I>
-__device__ __forceinline__ void trinv32_row(T (&y)[32], const T (&s)[32], const T& dinv, int c) {
+__device__ __forceinline__ void trinv32_row(T (&y)[32], const T (&s)[32], const T& dinv, int c0) {
   constexpr int r = 31 - I;
+  const int c = step_lane(c0);
   T acc0 = s_zero(T()), acc1 = s_zero(T());
   T src = s[r];
   if constexpr (r < 31) order_after(src, y[r < 31 ? r + 1 : r]);
@@ -147,7 +156,8 @@ __device__ __forceinline__ void trinv32_rows(T (&y)[32], const T (&s)[32], const
 // Inverse Z of a unit lower triangular 32 x 32 block L (lane c: s[r] = L(r, c)
 // for r > c), rows from the top: Z(r, c) = delta_rc - sum_{c <= k < r} L(r, k) Z(k, c).
 template <typename T, int R>
-__device__ __forceinline__ void trilinv32_row(T (&z)[32], const T (&s)[32], int c) {
+__device__ __forceinline__ void trilinv32_row(T (&z)[32], const T (&s)[32], int c0) {
+  const int c = step_lane(c0);
   T acc0 = s_zero(T()), acc1 = s_zero(T());
   T src = s[R];
   if constexpr (R > 0) order_after(src, z[R > 0 ? R - 1 : 0]);
