@@ -115,16 +115,37 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
       acc_r[u] = v4d{0.0, 0.0, 0.0, 0.0};
       acc_i[u] = v4d{0.0, 0.0, 0.0, 0.0};
     }
+    // the next KB-row block of V and C is loaded into registers while the
+    // current one is used
+    constexpr int PV = KB * NB / 256, PC = KB * NC / 256;
+    T vreg[PV], creg[PC];
+    auto load1 = [&](int k0) {
+#pragma unroll
+      for (int q = 0; q < PV; ++q) {
+        const int e = tid + 256 * q;
+        vreg[q] = vload_unit(V, t.ldv, k0 + e % KB, e / KB, rows, nbc);
+      }
+#pragma unroll
+      for (int q = 0; q < PC; ++q) {
+        const int e = tid + 256 * q;
+        const int k = e % KB, j = e / KB;
+        creg[q] = (k0 + k < rows && j < ncols) ? C[(k0 + k) + (int64_t)j * t.ldc] : s_zero(T());
+      }
+    };
+    load1(0);
     for (int k0 = 0; k0 < rows; k0 += KB) {
       __syncthreads();
-      for (int e = tid; e < KB * NB; e += 256) {
-        const int k = e % KB, i = e / KB;
-        Vs[k][i] = vload_unit(V, t.ldv, k0 + k, i, rows, nbc);
+#pragma unroll
+      for (int q = 0; q < PV; ++q) {
+        const int e = tid + 256 * q;
+        Vs[e % KB][e / KB] = vreg[q];
       }
-      for (int e = tid; e < KB * NC; e += 256) {
-        const int k = e % KB, j = e / KB;
-        Cs[k][j] = (k0 + k < rows && j < ncols) ? C[(k0 + k) + (int64_t)j * t.ldc] : s_zero(T());
+#pragma unroll
+      for (int q = 0; q < PC; ++q) {
+        const int e = tid + 256 * q;
+        Cs[e % KB][e / KB] = creg[q];
       }
+      if (k0 + KB < rows) load1(k0 + KB);
       __syncthreads();
 #pragma unroll
       for (int u = 0; u < TPW1; ++u) {
@@ -189,8 +210,21 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
       }
     }
     // ---- phase 3: C -= V W2 per KB-row block; tiles (KB/16 x NC/16) over 4 waves
+    // The block's C values are loaded before V, so both latencies overlap.
     constexpr int TPW3 = (KB / 16) * (NC / 16) / 4;
     for (int k0 = 0; k0 < rows; k0 += KB) {
+      T cv[TPW3][4];
+#pragma unroll
+      for (int u = 0; u < TPW3; ++u) {
+        const int id = w + 4 * u;
+        const int ti = id % (KB / 16), tj = id / (KB / 16);
+        const int row = k0 + ti * 16 + lr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int col = tj * 16 + mfma_f64_row(lane, r);
+          cv[u][r] = (row < rows && col < ncols) ? C[row + (int64_t)col * t.ldc] : s_zero(T());
+        }
+      }
       __syncthreads();
       for (int e = tid; e < KB * NB; e += 256) {
         const int k = e % KB, i = e / KB;
@@ -216,8 +250,8 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
           const int col = tj * 16 + mfma_f64_row(lane, r);
           if (row < rows && col < ncols) {
             T* cp = C + row + (int64_t)col * t.ldc;
-            if constexpr (CPLX) *cp = s_sub(*cp, mkz(dr[r], di[r]));
-            else *cp = *cp - dr[r];
+            if constexpr (CPLX) *cp = s_sub(cv[u][r], mkz(dr[r], di[r]));
+            else *cp = cv[u][r] - dr[r];
           }
         }
       }
