@@ -210,10 +210,11 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
       }
     }
     // ---- phase 3: C -= V W2 per KB-row block; tiles (KB/16 x NC/16) over 4 waves
-    // The block's C values are loaded before V, so both latencies overlap.
+    // The next block's V and C values are loaded into registers while the
+    // current block is updated.
     constexpr int TPW3 = (KB / 16) * (NC / 16) / 4;
-    for (int k0 = 0; k0 < rows; k0 += KB) {
-      T cv[TPW3][4];
+    T cv[TPW3][4], cn[TPW3][4];
+    auto loadC = [&](T (&dst)[TPW3][4], int k0) {
 #pragma unroll
       for (int u = 0; u < TPW3; ++u) {
         const int id = w + 4 * u;
@@ -222,13 +223,29 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int col = tj * 16 + mfma_f64_row(lane, r);
-          cv[u][r] = (row < rows && col < ncols) ? C[row + (int64_t)col * t.ldc] : s_zero(T());
+          dst[u][r] = (row < rows && col < ncols) ? C[row + (int64_t)col * t.ldc] : s_zero(T());
         }
       }
+    };
+    auto loadV = [&](int k0) {
+#pragma unroll
+      for (int q = 0; q < PV; ++q) {
+        const int e = tid + 256 * q;
+        vreg[q] = vload_unit(V, t.ldv, k0 + e % KB, e / KB, rows, nbc);
+      }
+    };
+    loadC(cv, 0);
+    loadV(0);
+    for (int k0 = 0; k0 < rows; k0 += KB) {
       __syncthreads();
-      for (int e = tid; e < KB * NB; e += 256) {
-        const int k = e % KB, i = e / KB;
-        Vs[k][i] = vload_unit(V, t.ldv, k0 + k, i, rows, nbc);
+#pragma unroll
+      for (int q = 0; q < PV; ++q) {
+        const int e = tid + 256 * q;
+        Vs[e % KB][e / KB] = vreg[q];
+      }
+      if (k0 + KB < rows) {
+        loadC(cn, k0 + KB);
+        loadV(k0 + KB);
       }
       __syncthreads();
 #pragma unroll
@@ -255,6 +272,10 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
           }
         }
       }
+#pragma unroll
+      for (int u = 0; u < TPW3; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cv[u][r] = cn[u][r];
     }
     __syncthreads();
   }
