@@ -92,8 +92,8 @@ __device__ __forceinline__ void order_after(zd& v, const zd& dep) {
 // S(j, c) / d_j, then S(r, c) -= conj(W(j, r)) W(j, c) for r > j (entries
 // below the diagonal only receive the mirrored update and are ignored).
 // `dinv` of lane j receives 1 / W(j, j).
-template <typename T, int J>
-__device__ __forceinline__ void chol32_step(T (&s)[32], int c0, int& bad, T& dinv) {
+template <typename T, int N, int J>
+__device__ __forceinline__ void cholN_step(T (&s)[N], int c0, int& bad, T& dinv) {
   const int c = step_lane(c0);
   const double d2 = s_re(rdl(s[J], J));
   if (!bad && !(d2 > 0.0)) bad = J + 1;
@@ -104,40 +104,39 @@ __device__ __forceinline__ void chol32_step(T (&s)[32], int c0, int& bad, T& din
   s[J] = w;
   T src = w;
 #pragma unroll
-  for (int r0 = J + 1; r0 < 32; r0 += kBc<T>) {
+  for (int r0 = J + 1; r0 < N; r0 += kBc<T>) {
     T b[kBc<T>];
 #pragma unroll
-    for (int u = 0; u < kBc<T>; ++u) b[u] = (r0 + u < 32) ? rdl(src, r0 + u) : s_zero(T());
+    for (int u = 0; u < kBc<T>; ++u) b[u] = (r0 + u < N) ? rdl(src, r0 + u) : s_zero(T());
 #pragma unroll
     for (int u = 0; u < kBc<T>; ++u)
-      if (r0 + u < 32) s[r0 + u] = s_sub(s[r0 + u], s_mul(s_conj(b[u]), w));
+      if (r0 + u < N) s[r0 + u] = s_sub(s[r0 + u], s_mul(s_conj(b[u]), w));
     order_after(src, s[r0]);
   }
 }
-template <typename T, int... Js>
-__device__ __forceinline__ void chol32_steps(T (&s)[32], int c, int& bad, T& dinv,
-                                             std::integer_sequence<int, Js...>) {
-  (chol32_step<T, Js>(s, c, bad, dinv), ...);
+template <typename T, int N, int... Js>
+__device__ __forceinline__ void cholN_steps(T (&s)[N], int c, int& bad, T& dinv, std::integer_sequence<int, Js...>) {
+  (cholN_step<T, N, Js>(s, c, bad, dinv), ...);
 }
 
-// Inverse Y of an upper triangular 32 x 32 block U held by lanes as above
+// Inverse Y of an upper triangular N x N block U held by lanes as above
 // (lane c: s[r] = U(r, c)), `dinv` of lane r = 1 / U(r, r).  Rows from the
 // bottom: Y(r, c) = (delta_rc - sum_{k > r} U(r, k) Y(k, c)) / U(r, r).
-template <typename T, int I>
-__device__ __forceinline__ void trinv32_row(T (&y)[32], const T (&s)[32], const T& dinv, int c0) {
-  constexpr int r = 31 - I;
+template <typename T, int N, int I>
+__device__ __forceinline__ void trinvN_row(T (&y)[N], const T (&s)[N], const T& dinv, int c0) {
+  constexpr int r = N - 1 - I;
   const int c = step_lane(c0);
   T acc0 = s_zero(T()), acc1 = s_zero(T());
   T src = s[r];
-  if constexpr (r < 31) order_after(src, y[r < 31 ? r + 1 : r]);
+  if constexpr (r < N - 1) order_after(src, y[r < N - 1 ? r + 1 : r]);
 #pragma unroll
-  for (int k0 = r + 1; k0 < 32; k0 += kBc<T>) {
+  for (int k0 = r + 1; k0 < N; k0 += kBc<T>) {
     T b[kBc<T>];
 #pragma unroll
-    for (int u = 0; u < kBc<T>; ++u) b[u] = (k0 + u < 32) ? rdl(src, k0 + u) : s_zero(T());   // U(r, k)
+    for (int u = 0; u < kBc<T>; ++u) b[u] = (k0 + u < N) ? rdl(src, k0 + u) : s_zero(T());   // U(r, k)
 #pragma unroll
     for (int u = 0; u < kBc<T>; ++u) {
-      if (k0 + u < 32) {
+      if (k0 + u < N) {
         if (u & 1) acc1 = s_fma(b[u], y[k0 + u], acc1);
         else acc0 = s_fma(b[u], y[k0 + u], acc0);
       }
@@ -147,10 +146,15 @@ __device__ __forceinline__ void trinv32_row(T (&y)[32], const T (&s)[32], const 
   const T inv = rdl(dinv, r);
   y[r] = sel(r < c, s_neg(s_mul(s_add(acc0, acc1), inv)), sel(r == c, inv, s_zero(T())));
 }
+template <typename T, int N, int... Is>
+__device__ __forceinline__ void trinvN_rows(T (&y)[N], const T (&s)[N], const T& dinv, int c,
+                                            std::integer_sequence<int, Is...>) {
+  (trinvN_row<T, N, Is>(y, s, dinv, c), ...);
+}
 template <typename T, int... Is>
 __device__ __forceinline__ void trinv32_rows(T (&y)[32], const T (&s)[32], const T& dinv, int c,
-                                             std::integer_sequence<int, Is...>) {
-  (trinv32_row<T, Is>(y, s, dinv, c), ...);
+                                             std::integer_sequence<int, Is...> seq) {
+  trinvN_rows<T, 32>(y, s, dinv, c, seq);
 }
 
 // Inverse Z of a unit lower triangular 32 x 32 block L (lane c: s[r] = L(r, c)
@@ -183,30 +187,102 @@ __device__ __forceinline__ void trilinv32_rows(T (&z)[32], const T (&s)[32], int
   (trilinv32_row<T, Rs>(z, s, c), ...);
 }
 
-template <typename T>
-__device__ __forceinline__ int wave_chol_inv32(T (*S)[33], T (*Y)[33]) {
+// 16 x 16 x 16 product by one wave on FP64 MFMA, operands and result in LDS
+// (row stride 33): D = alpha op(A) B (+ D when acc), op(A) = A^H when CA.
+// Every operand is read before D is written, so D may alias A or B.
+// MFMA f64 16x16x4 layout: A operand lane l = (row l % 16, k l / 16), B
+// operand lane l = (col l % 16, k l / 16), result register r of lane l =
+// (row 4 r + l / 16, col l % 16).
+template <typename T, bool CA>
+__device__ __forceinline__ void wave_mm16(T (*D)[33], const T (*A)[33], const T (*B)[33], double alpha, bool acc) {
   const int l = lane_id();
-  const int c = l & 31;
-  const bool writer = l < 32;
-  T s[32];
+  const int lr = l & 15, lk = l >> 4;
+  v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+  T a[4], b[4];
 #pragma unroll
-  for (int r = 0; r < 32; ++r) {
-    const T v = S[r][c];
+  for (int q = 0; q < 4; ++q) {
+    const int k = 4 * q + lk;
+    a[q] = CA ? s_conj(A[k][lr]) : A[lr][k];
+    b[q] = B[k][lr];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if constexpr (ScalarTraits<T>::is_complex) {
+      cr = mfma_f64(s_re(a[q]), s_re(b[q]), cr);
+      cr = mfma_f64(-s_im(a[q]), s_im(b[q]), cr);
+      ci = mfma_f64(s_re(a[q]), s_im(b[q]), ci);
+      ci = mfma_f64(s_im(a[q]), s_re(b[q]), ci);
+    } else {
+      cr = mfma_f64(a[q], b[q], cr);
+    }
+  }
+  T old[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) old[r] = acc ? D[4 * r + lk][lr] : s_zero(T());
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    T v;
+    if constexpr (ScalarTraits<T>::is_complex) v = mkz(alpha * cr[r], alpha * ci[r]);
+    else v = alpha * cr[r];
+    D[4 * r + lk][lr] = s_add(old[r], v);
+  }
+}
+
+// Cholesky with inverse of a 16 x 16 diagonal block (rows / columns r0..r0+15
+// of S and Y) by one wave, lane c (c = lane % 16) holding column c; W and
+// W^{-1} (upper, zero below) replace the block of S and fill the block of Y.
+template <typename T>
+__device__ __forceinline__ int wave_chol_inv16(T (*S)[33], T (*Y)[33], int r0) {
+  const int l = lane_id();
+  const int c = l & 15;
+  const bool writer = l < 16;
+  T s[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const T v = S[r0 + r][r0 + c];
     s[r] = sel(r <= c, v, s_zero(T()));
   }
   int bad = 0;
   T dinv = s_zero(T());
-  chol32_steps<T>(s, c, bad, dinv, std::make_integer_sequence<int, 32>{});
+  cholN_steps<T, 16>(s, c, bad, dinv, std::make_integer_sequence<int, 16>{});
 #pragma unroll
-  for (int r = 0; r < 32; ++r)
-    if (writer) S[r][c] = (r <= c) ? s[r] : s_zero(T());
-  T y[32];
+  for (int r = 0; r < 16; ++r)
+    if (writer) S[r0 + r][r0 + c] = (r <= c) ? s[r] : s_zero(T());
+  T y[16];
 #pragma unroll
-  for (int r = 0; r < 32; ++r) y[r] = s_zero(T());
-  trinv32_rows<T>(y, s, dinv, c, std::make_integer_sequence<int, 32>{});
+  for (int r = 0; r < 16; ++r) y[r] = s_zero(T());
+  trinvN_rows<T, 16>(y, s, dinv, c, std::make_integer_sequence<int, 16>{});
 #pragma unroll
-  for (int r = 0; r < 32; ++r)
-    if (writer) Y[r][c] = y[r];
+  for (int r = 0; r < 16; ++r)
+    if (writer) Y[r0 + r][r0 + c] = y[r];
+  return bad;
+}
+
+// Cholesky with inverse of a 32 x 32 Hermitian positive definite block, by
+// ONE wave (no workgroup barrier):
+//   S (LDS, 32 x 33) holds the upper triangle on entry (rows/columns beyond the
+//   active size padded with the identity by the caller); on exit S = W (upper,
+//   zero below) with S_in = W^H W, and Y = W^{-1} (upper, zero below).
+//   Returns 0, or j + 1 for the first pivot j that is not positive (uniform
+//   over the wave; S and Y are then unspecified).
+// 2 x 2 blocks of 16: W11, Y11 in-wave; W12 = Y11^H S12; S22 -= W12^H W12;
+// W22, Y22 in-wave; Y12 = -Y11 (W12 Y22) -- the off-diagonal products on MFMA.
+template <typename T>
+__device__ __forceinline__ int wave_chol_inv32(T (*S)[33], T (*Y)[33]) {
+  auto blk = [](T (*M)[33], int r0, int c0) { return reinterpret_cast<T (*)[33]>(&M[r0][c0]); };
+  int bad = wave_chol_inv16<T>(S, Y, 0);
+  wave_mm16<T, true>(blk(S, 0, 16), blk(Y, 0, 0), blk(S, 0, 16), 1.0, false);     // W12 = Y11^H S12
+  wave_mm16<T, true>(blk(S, 16, 16), blk(S, 0, 16), blk(S, 0, 16), -1.0, true);   // S22 -= W12^H W12
+  const int b2 = wave_chol_inv16<T>(S, Y, 16);
+  if (bad == 0 && b2 != 0) bad = 16 + b2;
+  wave_mm16<T, false>(blk(Y, 0, 16), blk(S, 0, 16), blk(Y, 16, 16), 1.0, false);  // P = W12 Y22
+  wave_mm16<T, false>(blk(Y, 0, 16), blk(Y, 0, 0), blk(Y, 0, 16), -1.0, false);   // Y12 = -Y11 P
+  const int l = lane_id();
+  for (int e = l; e < 256; e += 64) {
+    const int i = e & 15, j = e >> 4;
+    S[16 + i][j] = s_zero(T());
+    Y[16 + i][j] = s_zero(T());
+  }
   wave_lds_sync();
   return bad;
 }
