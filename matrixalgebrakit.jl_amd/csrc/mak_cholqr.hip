@@ -267,17 +267,17 @@ __global__ void __launch_bounds__(256) gram_chol_kernel(const T* X, int64_t ldx,
   }
 }
 
-// In-wave LU of a 32 x 32 block without pivoting, with the sign choice
+// In-wave LU of an N x N block without pivoting, with the sign choice
 // d_j = -sign(M(j, j)) subtracted from each pivot first (|pivot| >= 1 for the
 // top of an orthonormal Q).  Lane c keeps column c: U(r, c) for r <= c and,
 // below the diagonal, the eliminated entry M(r, c) (scaled into L(r, c) =
-// M(r, c) / U(c, c) by lu32_scale_l afterwards); other lanes' entries come
+// M(r, c) / U(c, c) by luN_scale_l afterwards); other lanes' entries come
 // from lane broadcasts.  Step j: lanes c > j update S(r, c) -= M(r, j) /
 // U(j, j) * S(j, c) for r > j (lanes c <= j multiply by zero, which leaves
 // their finite entries exact).  dv receives d (when dv is not null), `pinv`
 // of lane j receives 1 / U(j, j).
-template <typename T, int J>
-__device__ __forceinline__ void lu32_step(T (&s)[32], int c0, T* dv, T& pinv) {
+template <typename T, int N, int J>
+__device__ __forceinline__ void luN_step(T (&s)[N], int c0, T* dv, T& pinv) {
   const int c = step_lane(c0);
   const T piv = rdl(s[J], J);
   const T d = s_neg(sign_safe(piv));
@@ -290,28 +290,104 @@ __device__ __forceinline__ void lu32_step(T (&s)[32], int c0, T* dv, T& pinv) {
   const T f = s_neg(s_mul(inv, mjc));            // -U(J, c) / U(J, J)
   T dep = p;
 #pragma unroll
-  for (int r0 = J + 1; r0 < 32; r0 += kBc<T>) {
+  for (int r0 = J + 1; r0 < N; r0 += kBc<T>) {
     T b[kBc<T>];
 #pragma unroll
     for (int u = 0; u < kBc<T>; ++u) {
-      if (r0 + u < 32) order_after(s[r0 + u], dep);
-      b[u] = (r0 + u < 32) ? rdl(s[r0 + u], J) : s_zero(T());   // M(r, J)
+      if (r0 + u < N) order_after(s[r0 + u], dep);
+      b[u] = (r0 + u < N) ? rdl(s[r0 + u], J) : s_zero(T());   // M(r, J)
     }
 #pragma unroll
     for (int u = 0; u < kBc<T>; ++u)
-      if (r0 + u < 32) s[r0 + u] = s_fma(b[u], f, s[r0 + u]);
+      if (r0 + u < N) s[r0 + u] = s_fma(b[u], f, s[r0 + u]);
     dep = s[r0];
   }
 }
-template <typename T, int... Js>
-__device__ __forceinline__ void lu32_steps(T (&s)[32], int c, T* dv, T& pinv, std::integer_sequence<int, Js...>) {
-  (lu32_step<T, Js>(s, c, dv, pinv), ...);
+template <typename T, int N, int... Js>
+__device__ __forceinline__ void luN_steps(T (&s)[N], int c, T* dv, T& pinv, std::integer_sequence<int, Js...>) {
+  (luN_step<T, N, Js>(s, c, dv, pinv), ...);
 }
 // L(r, c) = M(r, c) / U(c, c) below the diagonal (lane c holds 1 / U(c, c) in pinv).
-template <typename T>
-__device__ __forceinline__ void lu32_scale_l(T (&s)[32], int c, const T& pinv) {
+template <typename T, int N>
+__device__ __forceinline__ void luN_scale_l(T (&s)[N], int c, const T& pinv) {
 #pragma unroll
-  for (int r = 0; r < 32; ++r) s[r] = sel(r > c, s_mul(s[r], pinv), s[r]);
+  for (int r = 0; r < N; ++r) s[r] = sel(r > c, s_mul(s[r], pinv), s[r]);
+}
+
+// LU of the 16 x 16 diagonal block (r0, r0) of In by the calling wave (lanes
+// c = lane % 16 hold column c): L \ U and d go to Out and dv (when dv is not
+// null), U^{-1} to Ui (when not null), L^{-1} to Li (when not null), all at
+// the same block position.  In and Out must differ.
+template <typename T>
+__device__ __forceinline__ void wave_lu16(const T (*In)[33], T (*Out)[33], int r0, T* dv, T (*Ui)[33],
+                                          T (*Li)[33]) {
+  const int l = lane_id();
+  const int c = l & 15;
+  const bool writer = l < 16;
+  T s[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) s[r] = In[r0 + r][r0 + c];
+  T pinv = s_zero(T());
+  luN_steps<T, 16>(s, c, dv != nullptr ? dv + r0 : (T*)nullptr, pinv, std::make_integer_sequence<int, 16>{});
+  luN_scale_l<T, 16>(s, c, pinv);
+  if (dv != nullptr) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      if (writer) Out[r0 + r][r0 + c] = s[r];
+  }
+  T y[16];
+  if (Ui != nullptr) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) y[r] = s_zero(T());
+    trinvN_rows<T, 16>(y, s, pinv, c, std::make_integer_sequence<int, 16>{});
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      if (writer) Ui[r0 + r][r0 + c] = y[r];
+  }
+  if (Li != nullptr) {
+    trilinvN_rows<T, 16>(y, s, c, std::make_integer_sequence<int, 16>{});
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      if (writer) Li[r0 + r][r0 + c] = y[r];
+  }
+}
+
+// LU of the 32 x 32 block Qt (padded on entry; its lower-right block is used
+// as scratch) with the sign choice d, as 2 x 2 blocks of 16 by waves 0 and 1
+// of the workgroup (every thread must call it): Ls = L \ U, dv = d,
+// Ui = U^{-1}, Li = L^{-1}.  Both waves factor each diagonal block; wave 0
+// writes L \ U and U^{-1}, wave 1 L^{-1}; the off-diagonal blocks are MFMA
+// products: U12 = L11^{-1} A12, L21 = A21 U11^{-1}, A22 - L21 U12,
+// (U^{-1})12 = -U11^{-1} U12 U22^{-1}, (L^{-1})21 = -L22^{-1} L21 L11^{-1}.
+template <typename T>
+__device__ __forceinline__ void lu32_recon(T (*Qt)[33], T (*Ls)[33], T* dv, T (*Ui)[33], T (*Li)[33]) {
+  auto blk = [](T (*M)[33], int r0, int c0) { return reinterpret_cast<T (*)[33]>(&M[r0][c0]); };
+  const int tid = threadIdx.x;
+  const int w = tid >> 6;
+  for (int e = tid; e < NB * NB; e += 256) Ls[e % NB][e / NB] = Qt[e % NB][e / NB];
+  __syncthreads();
+  if (w == 0) wave_lu16<T>(Qt, Ls, 0, dv, Ui, nullptr);
+  else if (w == 1) wave_lu16<T>(Qt, Ls, 0, nullptr, nullptr, Li);
+  __syncthreads();
+  if (w == 0) wave_mm16<T, false>(blk(Ls, 0, 16), blk(Li, 0, 0), blk(Ls, 0, 16), 1.0, false);        // U12
+  else if (w == 1) wave_mm16<T, false>(blk(Ls, 16, 0), blk(Ls, 16, 0), blk(Ui, 0, 0), 1.0, false);  // L21
+  __syncthreads();
+  if (w == 0) wave_mm16<T, false>(blk(Qt, 16, 16), blk(Ls, 16, 0), blk(Ls, 0, 16), -1.0, true);  // A22 - L21 U12
+  __syncthreads();
+  if (w == 0) wave_lu16<T>(Qt, Ls, 16, dv, Ui, nullptr);
+  else if (w == 1) wave_lu16<T>(Qt, Ls, 16, nullptr, nullptr, Li);
+  __syncthreads();
+  const int l = tid & 63;
+  if (w == 0) {
+    wave_mm16<T, false>(blk(Ui, 0, 16), blk(Ls, 0, 16), blk(Ui, 16, 16), 1.0, false);   // U12 U22^{-1}
+    wave_mm16<T, false>(blk(Ui, 0, 16), blk(Ui, 0, 0), blk(Ui, 0, 16), -1.0, false);    // -U11^{-1} (..)
+    for (int e = l; e < 256; e += 64) Ui[16 + (e & 15)][e >> 4] = s_zero(T());
+  } else if (w == 1) {
+    wave_mm16<T, false>(blk(Li, 16, 0), blk(Ls, 16, 0), blk(Li, 0, 0), 1.0, false);    // L21 L11^{-1}
+    wave_mm16<T, false>(blk(Li, 16, 0), blk(Li, 16, 16), blk(Li, 16, 0), -1.0, false);  // -L22^{-1} (..)
+    for (int e = l; e < 256; e += 64) Li[e & 15][16 + (e >> 4)] = s_zero(T());
+  }
+  __syncthreads();
 }
 
 // Pass 2 and the reconstruction (runs only if the flag is clear after pass 1):
@@ -374,31 +450,11 @@ __global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t
   }
   mm32<T>(Rl, G, false, Rl, false, 1.0, false);   // R = W2 R1
   mm32<T>(Qt, Qt, false, Yl, false, 1.0, false);  // top of Q = Q1 W2^{-1}
-  if (w < 2) {
-    // both waves factor the same block; wave 0 then inverts U, wave 1 L
-    T s[32];
-#pragma unroll
-    for (int r = 0; r < 32; ++r) s[r] = sel(r < nbc && c < nbc, Qt[r][c], sel(r == c, s_one(T()), s_zero(T())));
-    T pinv = s_zero(T());
-    lu32_steps<T>(s, c, w == 0 ? dv : (T*)nullptr, pinv, std::make_integer_sequence<int, 32>{});
-    lu32_scale_l<T>(s, c, pinv);
-    T y[32];
-    if (w == 0) {
-#pragma unroll
-      for (int r = 0; r < 32; ++r)
-        if (lane < 32) Ls[r][c] = s[r];
-      trinv32_rows<T>(y, s, pinv, c, std::make_integer_sequence<int, 32>{});
-#pragma unroll
-      for (int r = 0; r < 32; ++r)
-        if (lane < 32) Ui[r][c] = y[r];
-    } else {
-      trilinv32_rows<T>(y, s, c, std::make_integer_sequence<int, 32>{});
-#pragma unroll
-      for (int r = 0; r < 32; ++r)
-        if (lane < 32) Li[r][c] = y[r];
-    }
+  for (int e = tid; e < NB * NB; e += 256) {  // pad the top of Q with the identity outside nbc x nbc
+    const int r = e % NB, cc = e / NB;
+    Qt[r][cc] = sel(r < nbc && cc < nbc, Qt[r][cc], sel(r == cc, s_one(T()), s_zero(T())));
   }
-  __syncthreads();
+  lu32_recon<T>(Qt, Ls, dv, Ui, Li);
   for (int e = tid; e < NB * NB; e += 256) {
     const int r = e % NB, cc = e / NB;
     Ms[r][cc] = (r <= cc) ? s_mul(Ls[r][cc], s_conj(dv[cc])) : s_zero(T());

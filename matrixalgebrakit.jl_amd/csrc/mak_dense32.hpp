@@ -157,10 +157,10 @@ __device__ __forceinline__ void trinv32_rows(T (&y)[32], const T (&s)[32], const
   trinvN_rows<T, 32>(y, s, dinv, c, seq);
 }
 
-// Inverse Z of a unit lower triangular 32 x 32 block L (lane c: s[r] = L(r, c)
+// Inverse Z of a unit lower triangular N x N block L (lane c: s[r] = L(r, c)
 // for r > c), rows from the top: Z(r, c) = delta_rc - sum_{c <= k < r} L(r, k) Z(k, c).
-template <typename T, int R>
-__device__ __forceinline__ void trilinv32_row(T (&z)[32], const T (&s)[32], int c0) {
+template <typename T, int N, int R>
+__device__ __forceinline__ void trilinvN_row(T (&z)[N], const T (&s)[N], int c0) {
   const int c = step_lane(c0);
   T acc0 = s_zero(T()), acc1 = s_zero(T());
   T src = s[R];
@@ -181,10 +181,9 @@ __device__ __forceinline__ void trilinv32_row(T (&z)[32], const T (&s)[32], int 
   }
   z[R] = sel(R == c, s_one(T()), sel(R > c, s_neg(s_add(acc0, acc1)), s_zero(T())));
 }
-template <typename T, int... Rs>
-__device__ __forceinline__ void trilinv32_rows(T (&z)[32], const T (&s)[32], int c,
-                                               std::integer_sequence<int, Rs...>) {
-  (trilinv32_row<T, Rs>(z, s, c), ...);
+template <typename T, int N, int... Rs>
+__device__ __forceinline__ void trilinvN_rows(T (&z)[N], const T (&s)[N], int c, std::integer_sequence<int, Rs...>) {
+  (trilinvN_row<T, N, Rs>(z, s, c), ...);
 }
 
 // 16 x 16 x 16 product by one wave on FP64 MFMA, operands and result in LDS
