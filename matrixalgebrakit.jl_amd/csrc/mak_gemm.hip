@@ -327,6 +327,9 @@ static void gemm_shape(const GemmArgs& g, hipStream_t s) {
   auto wgs = [&](int bm, int bn) { return (int64_t)cdiv(g.M, bm) * cdiv(g.N, bn) * slabs; };
   if (g.M <= 32) {
     launch_gemm_tile<32, 64, 16, 2, 2, OPA, OPB, T>(g, s);
+  } else if (!cplx && g.K <= 128 && wgs(64, 64) >= 1024) {
+    // short-K updates stream C: smaller tiles keep more loads in flight
+    launch_gemm_tile<64, 64, 16, 2, 2, OPA, OPB, T>(g, s);
   } else if (!cplx && wgs(128, 128) >= 192) {
     launch_gemm_tile<128, 128, 16, 2, 2, OPA, OPB, T>(g, s);
   } else if (wgs(64, 64) >= 128) {
