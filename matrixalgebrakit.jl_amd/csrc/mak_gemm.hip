@@ -374,7 +374,7 @@ void splitk_plan(int N, int K, int bn, int& ksplit, int& kchunk) {
     return;
   }
   const int tiles = cdiv(N, bn);
-  int ks = cdiv(256, tiles);
+  int ks = cdiv(kSplitTargetWgs, tiles);
   ks = std::min(ks, std::max(1, K / 256));
   ks = std::max(ks, 1);
   kchunk = cdiv(cdiv(K, ks), 16) * 16;

@@ -132,7 +132,10 @@ struct GemmArgs {
 // C = alpha op(A) op(B) + beta C on FP64 MFMA (real) or 4 real MFMAs (complex).
 template <typename T> void gemm(int opA, int opB, GemmArgs g, hipStream_t s);
 
-// Split-K partial count and chunk for an (M x N x K) product with small M.
+// Split-K partial count and chunk for an (M x N x K) product with small M:
+// about kSplitTargetWgs workgroups, slices of at least 256 rows.  ksplit * N
+// stays below kSplitTargetWgs * bn + N.
+constexpr int kSplitTargetWgs = 512;
 void splitk_plan(int N, int K, int bn, int& ksplit, int& kchunk);
 
 // W2 = op(Tm) * sum_z Wp[z]  (nbc x ncols, ld nbc), op = conj-transpose if transT;

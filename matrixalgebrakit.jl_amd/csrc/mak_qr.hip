@@ -124,7 +124,7 @@ template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
   splitk_plan(ncmax, m, 64, ks, kc);
   L.wpElems = (size_t)NB * ncmax * std::max(ks, 1);
   // ksplit grows as N shrinks: bound ksplit * N over every N <= ncmax
-  L.wpElems = std::max(L.wpElems, (size_t)NB * (size_t)(256 * 64 + ncmax));
+  L.wpElems = std::max(L.wpElems, (size_t)NB * (size_t)(kSplitTargetWgs * 64 + ncmax));
   if (minmn > NB) L.wpElems = std::max(L.wpElems, wp_block_elems(m, ncmax));
   L.offWp = off;
   off += align256(L.wpElems * sizeof(T));
