@@ -119,12 +119,17 @@ template <typename T> void copy_mat(int m, int n, const T* X, int64_t ldx, T* Y,
   hipLaunchKernelGGL((copy_kernel<T>), dim3((unsigned)cdiv64(tot, 256)), dim3(256), 0, s, m, n, X, ldx, Y, ldy);
 }
 
-// Recursive Cholesky with inverse on an n x n block (n a multiple of DB except
-// possibly the last diagonal block):
+// Recursive Cholesky with inverse on an n x n block (splits at multiples of
+// DB, so the leaves are the DB x DB diagonal blocks):
 //   Z = [Z11 Z12; . Z22] = W^H W,  W = [W11 W12; 0 W22],  Y = W^{-1}:
 //   W11, Y11 <- recursion;  W12 = Y11^H Z12;  Z22 -= W12^H W12;
-//   W22, Y22 <- recursion;  Y12 = -(Y11 W12) Y22;  Y21 = 0.
-// Every product is one MFMA GEMM; the 64 x 64 leaves run potrf_inv64_kernel.
+//   W22, Y22 <- recursion;  Y12 = -(Y11 W12) Y22.
+// Every product is one MFMA GEMM; the DB x DB leaves run potrf_inv64_kernel.
+// W12 is written conjugate-transposed into the free lower block Z21 (the
+// product that forms it cannot run in place), and potrf_finish_kernel moves
+// every such block to the upper part at the end.  Y's strict lower part is
+// zeroed once before the recursion (the triangular-hint GEMMs read the zeros
+// inside their diagonal tiles); nothing below Y's diagonal is written after.
 template <typename T>
 static void potrf_inv_rec(T* Z, int64_t ldz, T* Y, int64_t ldy, int n, int offset, T* tmp, int* dinfo,
                           hipStream_t s) {
@@ -135,27 +140,52 @@ static void potrf_inv_rec(T* Z, int64_t ldz, T* Y, int64_t ldy, int n, int offse
   const int n1 = ((n / 2 + DB - 1) / DB) * DB;
   const int n2 = n - n1;
   T* Z12 = Z + (int64_t)n1 * ldz;
+  T* Z21 = Z + n1;
   T* Z22 = Z + n1 + (int64_t)n1 * ldz;
   T* Y12 = Y + (int64_t)n1 * ldy;
   T* Y22 = Y + n1 + (int64_t)n1 * ldy;
   potrf_inv_rec<T>(Z, ldz, Y, ldy, n1, offset, tmp, dinfo, s);
-  // Y11, Y22 are upper triangular with zeros below; Z22 is read only above
-  // its diagonal by the recursion
-  GemmArgs g = gargs<T>(n1, n2, n1, Y, ldy, Z12, ldz, tmp, n1, 1.0, 0.0);
-  g.tri = TRI_A_LOWER;  // op(A) = Y11^H
+  // Z21 = W12^H = Z12^H Y11  (Y11 upper)
+  GemmArgs g = gargs<T>(n2, n1, n1, Z12, ldz, Y, ldy, Z21, ldz, 1.0, 0.0);
+  g.tri = TRI_B_UPPER;
   gemm<T>(OP_C, OP_N, g, s);
-  copy_mat<T>(n1, n2, tmp, n1, Z12, ldz, s);
-  g = gargs<T>(n2, n2, n1, Z12, ldz, Z12, ldz, Z22, ldz, -1.0, 1.0);
+  // Z22 -= W12^H W12 = Z21 Z21^H  (upper triangle only: the recursion reads nothing else)
+  g = gargs<T>(n2, n2, n1, Z21, ldz, Z21, ldz, Z22, ldz, -1.0, 1.0);
   g.upperC = 1;
-  gemm<T>(OP_C, OP_N, g, s);
+  gemm<T>(OP_N, OP_C, g, s);
   potrf_inv_rec<T>(Z22, ldz, Y22, ldy, n2, offset + n1, tmp, dinfo, s);
-  g = gargs<T>(n1, n2, n1, Y, ldy, Z12, ldz, tmp, n1, 1.0, 0.0);
-  g.tri = TRI_A_UPPER;  // Y11
-  gemm<T>(OP_N, OP_N, g, s);
+  // tmp = Y11 W12 = Y11 Z21^H;  Y12 = -tmp Y22
+  g = gargs<T>(n1, n2, n1, Y, ldy, Z21, ldz, tmp, n1, 1.0, 0.0);
+  g.tri = TRI_A_UPPER;
+  gemm<T>(OP_N, OP_C, g, s);
   g = gargs<T>(n1, n2, n2, tmp, n1, Y22, ldy, Y12, ldy, -1.0, 0.0);
-  g.tri = TRI_B_UPPER;  // Y22
+  g.tri = TRI_B_UPPER;
   gemm<T>(OP_N, OP_N, g, s);
-  fill_zero<T>(Y + n1, ldy, n2, n1, s);
+}
+
+// After the recursion: every 32 x 32 block strictly below the diagonal that
+// lies outside the DB x DB leaves holds W^H there; move it (conjugate
+// transposed) to the mirrored upper block and zero it.  Blocks inside a leaf
+// were written by the leaf kernel.  One workgroup per lower block pair, the
+// transpose staged through LDS.
+template <typename T>
+__global__ void __launch_bounds__(256) potrf_finish_kernel(T* Z, int64_t ldz, int n) {
+  __shared__ T tile[32][33];
+  const int bc = blockIdx.x, br = blockIdx.y;  // lower block (rows br, cols bc), br > bc
+  if (br <= bc || (br * 32) / DB == (bc * 32) / DB) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {
+    const int r = br * 32 + tx, c = bc * 32 + k;
+    if (r < n && c < n) {
+      tile[k][tx] = Z[r + (int64_t)c * ldz];
+      Z[r + (int64_t)c * ldz] = s_zero(T());
+    }
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int r = bc * 32 + tx, c = br * 32 + k;  // upper element (r, c) = conj(lower (c, r))
+    if (r < n && c < n) Z[r + (int64_t)c * ldz] = s_conj(tile[tx][k]);
+  }
 }
 
 // Z (n x n, upper part) -> W (upper, in place, strict lower zeroed) with
@@ -166,8 +196,13 @@ template <typename T>
 void potrf_upper(T* Z, int64_t ldz, int n, T* Winv, int64_t ldw, T* tmp, int* dinfo, hipStream_t s) {
   hipMemsetAsync(dinfo, 0, sizeof(int), s);
   if (n <= 0) return;
+  hipLaunchKernelGGL((zero_lower_kernel<T>), dim3((unsigned)cdiv64((int64_t)n * n, 256)), dim3(256), 0, s, Winv, ldw,
+                     n);
   potrf_inv_rec<T>(Z, ldz, Winv, ldw, n, 0, tmp, dinfo, s);
-  hipLaunchKernelGGL((zero_lower_kernel<T>), dim3((unsigned)cdiv64((int64_t)n * n, 256)), dim3(256), 0, s, Z, ldz, n);
+  if (n > DB) {
+    const int nb = cdiv(n, 32);
+    hipLaunchKernelGGL((potrf_finish_kernel<T>), dim3(nb, nb), dim3(256), 0, s, Z, ldz, n);
+  }
 }
 
 // ---------------------------------------------------------------------------

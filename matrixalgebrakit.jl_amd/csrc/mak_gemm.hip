@@ -47,19 +47,20 @@ __device__ inline zd load_b(const zd* B, int64_t ldb, int gk, int gj) {
 
 // Tile order: linear workgroup ids are dealt round-robin to the 8 XCDs; regroup
 // them so every XCD walks a contiguous range of tiles (shared A / B panels stay
-// in that XCD's L2).
-// With triangular hints whose K range grows with the tile's row or column
-// index (TRI_A_LOWER: rows, TRI_B_UPPER: columns), that index is reversed so
-// that the longest tiles are dispatched first.
-__device__ inline void tile_coords(int& tm, int& tn, int tri = 0) {
+// in that XCD's L2).  Under structure hints the work per tile varies with its
+// row or column, and contiguous ranges would leave one XCD with all the long
+// tiles; those launches keep the round-robin deal (every XCD gets every
+// column), and the index whose K range grows (TRI_A_LOWER: rows, TRI_B_UPPER:
+// columns; upperC: columns) is reversed so that the longest tiles go first.
+__device__ inline void tile_coords(int& tm, int& tn, int tri, int upperC) {
   const int gx = gridDim.x, gy = gridDim.y;
   const int total = gx * gy;
   int L = blockIdx.x + blockIdx.y * gx;
-  if ((total & 7) == 0) L = (L & 7) * (total >> 3) + (L >> 3);
+  if ((total & 7) == 0 && tri == 0 && upperC == 0) L = (L & 7) * (total >> 3) + (L >> 3);
   tm = L % gx;
   tn = L / gx;
   if (tri & TRI_A_LOWER) tm = gx - 1 - tm;
-  if (tri & TRI_B_UPPER) tn = gy - 1 - tn;
+  if ((tri & TRI_B_UPPER) || upperC) tn = gy - 1 - tn;
 }
 
 // Clip [kbeg, kend) to the terms a tile (rows m0.., cols n0..) can have
@@ -87,7 +88,7 @@ __global__ void __launch_bounds__(256, 2) gemm_f64_kernel(GemmArgs g) {
   const int w = tid >> 6;
   const int wm = w % WM, wn = w / WM;
   int tmi, tni;
-  tile_coords(tmi, tni, g.tri);
+  tile_coords(tmi, tni, g.tri, g.upperC);
   const int bz = blockIdx.z;
   const int bat = bz / g.ksplit;
   const int ks = bz % g.ksplit;
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
   const int w = tid >> 6;
   const int wm = w % WM, wn = w / WM;
   int tmi, tni;
-  tile_coords(tmi, tni, g.tri);
+  tile_coords(tmi, tni, g.tri, g.upperC);
   const int bz = blockIdx.z;
   const int bat = bz / g.ksplit;
   const int ks = bz % g.ksplit;
@@ -325,8 +326,21 @@ static void gemm_shape(const GemmArgs& g, hipStream_t s) {
   constexpr bool cplx = ScalarTraits<T>::is_complex;
   const int64_t slabs = (int64_t)std::max(g.batch, 1) * std::max(g.ksplit, 1);
   auto wgs = [&](int bm, int bn) { return (int64_t)cdiv(g.M, bm) * cdiv(g.N, bn) * slabs; };
+  // tiles that do work: upperC drops those strictly below the diagonal
+  auto busy = [&](int b) {
+    if (!g.upperC) return wgs(b, b);
+    const int64_t tm = cdiv(g.M, b), tn = cdiv(g.N, b);
+    int64_t t = 0;
+    for (int64_t j = 0; j < tn; ++j) t += std::min<int64_t>(tm, (j * b + b + b - 1) / b);
+    return t * slabs;
+  };
+  const bool hinted = g.tri != 0 || g.upperC != 0;
   if (g.M <= 32) {
     launch_gemm_tile<32, 64, 16, 2, 2, OPA, OPB, T>(g, s);
+  } else if (!cplx && hinted && busy(128) < 1024 && busy(64) >= 512) {
+    // hinted products spread uneven work over the tiles: more, smaller tiles
+    // balance the CUs (and avoid a second round of a few 128x128 tiles)
+    launch_gemm_tile<64, 64, 16, 2, 2, OPA, OPB, T>(g, s);
   } else if (!cplx && g.K <= 128 && wgs(64, 64) >= 1024) {
     // short-K updates stream C: smaller tiles keep more loads in flight
     launch_gemm_tile<64, 64, 16, 2, 2, OPA, OPB, T>(g, s);

@@ -1,13 +1,24 @@
 // Reduced-precision GEMM on FP64 data for the mixed-precision polar bulk:
 //   C = alpha op(A) op(B) + beta C with A, B, C in FP64 memory, the product
 //   computed on FP32 MFMA (v_mfma_f32_16x16x4_f32) or BF16 MFMA
-//   (v_mfma_f32_16x16x32_bf16), both accumulating in FP32.  Operands are
-//   rounded to the compute type when they are staged into LDS; alpha, beta and
-//   the accumulation with C happen in FP64.
+//   (v_mfma_f32_16x16x32_bf16), both accumulating in FP32; alpha, beta and the
+//   accumulation with C happen in FP64.
 //
-// Result layout of the FP32-accumulating 16x16 MFMAs: lane l, register r holds
-// row 4 (l / 16) + r, column l % 16.  As in the FP64 kernel the operands are
-// swapped so that lane l holds ROW l % 16 of C (128-byte column stores).
+// Two steps:
+//   1. pack_lo_kernel rounds op(A) (M x K) and op(B)^T (N x K) once into
+//      row-major, k-contiguous copies in the compute type (rows padded to a
+//      multiple of 128, k to a multiple of 64, padding zero), so the GEMM reads
+//      each operand row as 128-byte runs with 16-byte loads and needs no bounds
+//      checks in its main loop;
+//   2. gemm_lo_kernel: 128 x 128 tile, 4 waves of 64 x 64, one 128-byte k-slab
+//      per step (32 FP32 or 64 BF16 values), the next slab loaded into
+//      registers while the MFMAs of the current one run.
+//
+// A lane of either MFMA holds 8 consecutive k values of one row per step:
+// the BF16 MFMA takes k = 8 (l / 16) .. + 7 natively; for FP32 the sum over k
+// is order-free, so MFMA e of a 32-wide slab uses k = 8 (l / 16) + e for both
+// operands and every fragment is two 16-byte LDS reads.  The operands are
+// swapped in the MFMA so lane l holds ROW l % 16 of C (128-byte column stores).
 #include <algorithm>
 
 #include "mak_common.hpp"
@@ -15,87 +26,178 @@
 
 namespace mak {
 
-template <int OPA>
-__device__ inline double mload_a(const double* A, int64_t lda, int gi, int gk) {
-  return (OPA == OP_N) ? A[gi + (int64_t)gk * lda] : A[gk + (int64_t)gi * lda];
-}
-template <int OPB>
-__device__ inline double mload_b(const double* B, int64_t ldb, int gk, int gj) {
-  return (OPB == OP_N) ? B[gk + (int64_t)gj * ldb] : B[gj + (int64_t)gk * ldb];
+namespace {
+
+template <int PREC> struct LoType;
+template <> struct LoType<32> { using T = float; };
+template <> struct LoType<16> { using T = __bf16; };
+
+constexpr int kLoRows = 128;  // tile rows / row padding of the packed copies
+constexpr int kLoK = 64;      // k padding of the packed copies
+
+inline int64_t lo_rows(int r) { return cdiv64(std::max(r, 0), kLoRows) * kLoRows; }
+inline int64_t lo_kpad(int k) { return cdiv64(std::max(k, 0), kLoK) * kLoK; }
+
+// D (Rp x Kp, row-major, ld Kp) = round(src(r, k)) for r < R, k < K, zero
+// elsewhere; src(r, k) = S[k + r ld] (kcontig) or S[r + k ld].  One 64 x 64
+// block per workgroup, read along the source's contiguous index and written
+// along k through an LDS transpose.
+template <typename TL>
+__global__ void __launch_bounds__(256) pack_lo_kernel(const double* S, int64_t ld, int kcontig, int R, int K, TL* D,
+                                                      int64_t Kp) {
+  __shared__ float tile[64][65];
+  const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  if (kcontig) {
+    for (int q = ty; q < 64; q += 4) {
+      const int r = r0 + q, k = k0 + tx;
+      tile[q][tx] = (r < R && k < K) ? (float)S[k + (int64_t)r * ld] : 0.f;
+    }
+  } else {
+    for (int q = ty; q < 64; q += 4) {
+      const int r = r0 + tx, k = k0 + q;
+      tile[tx][q] = (r < R && k < K) ? (float)S[r + (int64_t)k * ld] : 0.f;
+    }
+  }
+  __syncthreads();
+  for (int q = ty; q < 64; q += 4) D[(int64_t)(r0 + q) * Kp + k0 + tx] = (TL)tile[q][tx];
 }
 
-__device__ inline void mixed_tile_coords(int& tm, int& tn) {
+template <typename TL>
+void pack_lo(const double* S, int64_t ld, int kcontig, int R, int K, TL* D, int64_t Rp, int64_t Kp,
+             hipStream_t s) {
+  if (Rp <= 0 || Kp <= 0) return;
+  dim3 grid((unsigned)(Kp / 64), (unsigned)(Rp / 64));
+  hipLaunchKernelGGL((pack_lo_kernel<TL>), grid, dim3(256), 0, s, S, ld, kcontig, R, K, D, Kp);
+}
+
+// native 16-byte vector (HIP's uint4 is a struct, which keeps arrays of it in
+// scratch memory)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct LoGemmArgs {
+  int M, N;
+  int64_t Kp;            // packed k extent (multiple of 64)
+  const void* Ap;        // op(A), Mp x Kp, k contiguous
+  const void* Bp;        // op(B)^T, Np x Kp, k contiguous
+  double* C;
+  int64_t ldc;
+  double alpha, beta;
+  int tri;
+};
+
+// Tile order: XCD-contiguous ranges of tiles, except under triangular hints,
+// where the work per tile varies and the round-robin deal over the XCDs keeps
+// them balanced; the index whose K range grows is reversed (longest first).
+__device__ inline void lo_tile_coords(int& tm, int& tn, int tri) {
   const int gx = gridDim.x, gy = gridDim.y;
   const int total = gx * gy;
   int L = blockIdx.x + blockIdx.y * gx;
-  if ((total & 7) == 0) L = (L & 7) * (total >> 3) + (L >> 3);
+  if ((total & 7) == 0 && tri == 0) L = (L & 7) * (total >> 3) + (L >> 3);
   tm = L % gx;
   tn = L / gx;
+  if (tri & TRI_A_LOWER) tm = gx - 1 - tm;
+  if (tri & TRI_B_UPPER) tn = gy - 1 - tn;
 }
 
-// FP32 compute.  128 x 128 tile, BK = 16, 4 waves of 64 x 64.
-template <int OPA, int OPB>
-__global__ void __launch_bounds__(256, 2) gemm_mixed_f32_kernel(GemmArgs g) {
-  constexpr int BM = 128, BN = 128, BK = 16, WTM = 64, WTN = 64, TM = 4, TN = 4;
-  __shared__ float As[BK][BM + 4];
-  __shared__ float Bs[BK][BN + 4];
+template <int PREC>
+__global__ void __launch_bounds__(256, 2) gemm_lo_kernel(LoGemmArgs g) {
+  using TL = typename LoType<PREC>::T;
+  constexpr int BK = 128 / (int)sizeof(TL);  // k values per 128-byte slab
+  constexpr int TM = 4, TN = 4;              // 16 x 16 blocks per wave (64 x 64)
+  __shared__ u32x4 As[kLoRows][9];           // 8 x 16 bytes per row + 16 bytes pad
+  __shared__ u32x4 Bs[kLoRows][9];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   int tmi, tni;
-  mixed_tile_coords(tmi, tni);
-  const int m0 = tmi * BM, n0 = tni * BN;
-  const double* A = reinterpret_cast<const double*>(g.A);
-  const double* B = reinterpret_cast<const double*>(g.B);
-  double* C = reinterpret_cast<double*>(g.C);
-  const int ia0 = (OPA == OP_N) ? (tid % BM) : (tid / BK);
-  const int ka0 = (OPA == OP_N) ? (tid / BM) : (tid % BK);
-  constexpr int dA = (OPA == OP_N) ? (256 / BM) : (256 / BK);
-  const int jb0 = (OPB == OP_N) ? (tid / BK) : (tid % BN);
-  const int kb0 = (OPB == OP_N) ? (tid % BK) : (tid / BN);
-  constexpr int dB = (OPB == OP_N) ? (256 / BK) : (256 / BN);
-  float pa[8], pb[8];
-  auto fetch = [&](int k0) {
+  lo_tile_coords(tmi, tni, g.tri);
+  const int m0 = tmi * kLoRows, n0 = tni * kLoRows;
+  // K range: the hints bound the nonzero terms of a tile; the slab-aligned
+  // range covers them (the packed copies are zero beyond K)
+  int64_t kbeg = 0, kend = g.Kp;
+  if (g.tri & TRI_A_LOWER) kend = std::min<int64_t>(kend, m0 + kLoRows);
+  if (g.tri & TRI_B_UPPER) kend = std::min<int64_t>(kend, n0 + kLoRows);
+  if (g.tri & TRI_A_UPPER) kbeg = std::max<int64_t>(kbeg, m0);
+  if (g.tri & TRI_B_LOWER) kbeg = std::max<int64_t>(kbeg, n0);
+  kbeg = kbeg / BK * BK;
+  kend = std::min<int64_t>(g.Kp, (kend + BK - 1) / BK * BK);
+
+  // loads: thread t moves 16-byte vector (t % 8) of rows t / 8 + 32 u
+  const int lr = tid >> 3, lv = tid & 7;
+  const char* Ab = reinterpret_cast<const char*>(g.Ap) + ((int64_t)(m0 + lr) * g.Kp) * (int64_t)sizeof(TL) + lv * 16;
+  const char* Bb = reinterpret_cast<const char*>(g.Bp) + ((int64_t)(n0 + lr) * g.Kp) * (int64_t)sizeof(TL) + lv * 16;
+  const int64_t rstride = 32 * g.Kp * (int64_t)sizeof(TL);
+  u32x4 pa[4], pb[4];
+  auto fetch = [&](int64_t k0) {
+    const int64_t off = k0 * (int64_t)sizeof(TL);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = (OPA == OP_N) ? ia0 : ia0 + u * dA, k = (OPA == OP_N) ? ka0 + u * dA : ka0;
-      const int gi = m0 + i, gk = k0 + k;
-      pa[u] = (gi < g.M && gk < g.K) ? (float)mload_a<OPA>(A, g.lda, gi, gk) : 0.f;
-      const int j = (OPB == OP_N) ? jb0 + u * dB : jb0, kk = (OPB == OP_N) ? kb0 : kb0 + u * dB;
-      const int gj = n0 + j, gk2 = k0 + kk;
-      pb[u] = (gj < g.N && gk2 < g.K) ? (float)mload_b<OPB>(B, g.ldb, gk2, gj) : 0.f;
+    for (int u = 0; u < 4; ++u) {
+      pa[u] = *reinterpret_cast<const u32x4*>(Ab + u * rstride + off);
+      pb[u] = *reinterpret_cast<const u32x4*>(Bb + u * rstride + off);
     }
   };
+
   v4f acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-  if (g.K > 0) fetch(0);
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
+
+  const int fr = lane & 15, fq = lane >> 4;
+  if (kbeg < kend) fetch(kbeg);
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = (OPA == OP_N) ? ia0 : ia0 + u * dA, k = (OPA == OP_N) ? ka0 + u * dA : ka0;
-      As[k][i] = pa[u];
-      const int j = (OPB == OP_N) ? jb0 + u * dB : jb0, kk = (OPB == OP_N) ? kb0 : kb0 + u * dB;
-      Bs[kk][j] = pb[u];
+    for (int u = 0; u < 4; ++u) {
+      As[lr + 32 * u][lv] = pa[u];
+      Bs[lr + 32 * u][lv] = pb[u];
     }
     __syncthreads();
-    if (k0 + BK < g.K) fetch(k0 + BK);
+    if (k0 + BK < kend) fetch(k0 + BK);
+    if constexpr (PREC == 16) {
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      float af[TM], bf[TN];
+      for (int st = 0; st < 2; ++st) {
+        v8bf af[TM], bf[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) af[i] = As[kk + (lane >> 4)][wm * WTM + i * 16 + (lane & 15)];
+        for (int i = 0; i < TM; ++i) {
+          const u32x4 v = As[wm * 64 + i * 16 + fr][4 * st + fq];
+          af[i] = __builtin_bit_cast(v8bf, v);
+        }
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bf[j] = Bs[kk + (lane >> 4)][wn * WTN + j * 16 + (lane & 15)];
+        for (int j = 0; j < TN; ++j) {
+          const u32x4 v = Bs[wn * 64 + j * 16 + fr][4 * st + fq];
+          bf[j] = __builtin_bit_cast(v8bf, v);
+        }
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j) acc[i][j] = mfma_f32(bf[j], af[i], acc[i][j]);
+          for (int j = 0; j < TN; ++j) acc[i][j] = mfma_bf16(bf[j], af[i], acc[i][j]);
+      }
+    } else {
+      float af[TM][8], bf[TN][8];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const u32x4 v0 = As[wm * 64 + i * 16 + fr][2 * fq], v1 = As[wm * 64 + i * 16 + fr][2 * fq + 1];
+        const f32x4 f0 = __builtin_bit_cast(f32x4, v0), f1 = __builtin_bit_cast(f32x4, v1);
+        for (int e = 0; e < 4; ++e) { af[i][e] = f0[e]; af[i][4 + e] = f1[e]; }
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const u32x4 v0 = Bs[wn * 64 + j * 16 + fr][2 * fq], v1 = Bs[wn * 64 + j * 16 + fr][2 * fq + 1];
+        const f32x4 f0 = __builtin_bit_cast(f32x4, v0), f1 = __builtin_bit_cast(f32x4, v1);
+        for (int e = 0; e < 4; ++e) { bf[j][e] = f0[e]; bf[j][4 + e] = f1[e]; }
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) acc[i][j] = mfma_f32(bf[j][e], af[i][e], acc[i][j]);
     }
     __syncthreads();
   }
-  const int rowb = m0 + wm * WTM + (lane & 15), colb = n0 + wn * WTN;
+  const int rowb = m0 + wm * 64 + fr, colb = n0 + wn * 64;
+  const double alpha = g.alpha, beta = g.beta;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -104,117 +206,52 @@ __global__ void __launch_bounds__(256, 2) gemm_mixed_f32_kernel(GemmArgs g) {
       for (int r = 0; r < 4; ++r) {
         const int row = rowb + i * 16, col = colb + j * 16 + mfma_f32_row(lane, r);
         if (row < g.M && col < g.N) {
-          double* cp = C + row + (int64_t)col * g.ldc;
-          const double v = g.alpha_re * (double)acc[i][j][r];
-          *cp = (g.beta_re != 0.0) ? fma(g.beta_re, *cp, v) : v;
+          double* cp = g.C + row + (int64_t)col * g.ldc;
+          const double v = alpha * (double)acc[i][j][r];
+          *cp = (beta != 0.0) ? fma(beta, *cp, v) : v;
         }
       }
 }
 
-// BF16 compute.  128 x 128 tile, BK = 32 (one MFMA k-step), 4 waves of 64 x 64.
-// LDS holds rows of op(A) and columns of op(B) with k contiguous so a lane's
-// 8 consecutive k values are one 16-byte read.
-template <int OPA, int OPB>
-__global__ void __launch_bounds__(256, 2) gemm_mixed_bf16_kernel(GemmArgs g) {
-  constexpr int BM = 128, BN = 128, BK = 32, WTM = 64, WTN = 64, TM = 4, TN = 4;
-  __shared__ __attribute__((aligned(16))) __bf16 As[BM][BK + 8];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[BN][BK + 8];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wm = w & 1, wn = w >> 1;
-  int tmi, tni;
-  mixed_tile_coords(tmi, tni);
-  const int m0 = tmi * BM, n0 = tni * BN;
-  const double* A = reinterpret_cast<const double*>(g.A);
-  const double* B = reinterpret_cast<const double*>(g.B);
-  double* C = reinterpret_cast<double*>(g.C);
-  // 128 x 32 tile = 4096 elements = 16 per thread
-  const int ia0 = (OPA == OP_N) ? (tid % BM) : (tid / BK);
-  const int ka0 = (OPA == OP_N) ? (tid / BM) : (tid % BK);
-  constexpr int dA = (OPA == OP_N) ? (256 / BM) : (256 / BK);
-  const int jb0 = (OPB == OP_N) ? (tid / BK) : (tid % BN);
-  const int kb0 = (OPB == OP_N) ? (tid % BK) : (tid / BN);
-  constexpr int dB = (OPB == OP_N) ? (256 / BK) : (256 / BN);
-  float pa[16], pb[16];
-  auto fetch = [&](int k0) {
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int i = (OPA == OP_N) ? ia0 : ia0 + u * dA, k = (OPA == OP_N) ? ka0 + u * dA : ka0;
-      const int gi = m0 + i, gk = k0 + k;
-      pa[u] = (gi < g.M && gk < g.K) ? (float)mload_a<OPA>(A, g.lda, gi, gk) : 0.f;
-      const int j = (OPB == OP_N) ? jb0 + u * dB : jb0, kk = (OPB == OP_N) ? kb0 : kb0 + u * dB;
-      const int gj = n0 + j, gk2 = k0 + kk;
-      pb[u] = (gj < g.N && gk2 < g.K) ? (float)mload_b<OPB>(B, g.ldb, gk2, gj) : 0.f;
-    }
-  };
-  v4f acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-  if (g.K > 0) fetch(0);
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int i = (OPA == OP_N) ? ia0 : ia0 + u * dA, k = (OPA == OP_N) ? ka0 + u * dA : ka0;
-      As[i][k] = (__bf16)pa[u];
-      const int j = (OPB == OP_N) ? jb0 + u * dB : jb0, kk = (OPB == OP_N) ? kb0 : kb0 + u * dB;
-      Bs[j][kk] = (__bf16)pb[u];
-    }
-    __syncthreads();
-    if (k0 + BK < g.K) fetch(k0 + BK);
-    v8bf af[TM], bf[TN];
-    const int kg = 8 * (lane >> 4);
-#pragma unroll
-    for (int i = 0; i < TM; ++i) af[i] = *reinterpret_cast<const v8bf*>(&As[wm * WTM + i * 16 + (lane & 15)][kg]);
-#pragma unroll
-    for (int j = 0; j < TN; ++j) bf[j] = *reinterpret_cast<const v8bf*>(&Bs[wn * WTN + j * 16 + (lane & 15)][kg]);
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j) acc[i][j] = mfma_bf16(bf[j], af[i], acc[i][j]);
-    __syncthreads();
-  }
-  const int rowb = m0 + wm * WTM + (lane & 15), colb = n0 + wn * WTN;
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rowb + i * 16, col = colb + j * 16 + mfma_f32_row(lane, r);
-        if (row < g.M && col < g.N) {
-          double* cp = C + row + (int64_t)col * g.ldc;
-          const double v = g.alpha_re * (double)acc[i][j][r];
-          *cp = (g.beta_re != 0.0) ? fma(g.beta_re, *cp, v) : v;
-        }
-      }
+template <int PREC>
+void gemm_lo(int opA, int opB, const GemmArgs& g, void* ws, hipStream_t s) {
+  using TL = typename LoType<PREC>::T;
+  const int64_t Mp = lo_rows(g.M), Np = lo_rows(g.N), Kp = lo_kpad(g.K);
+  TL* Ap = reinterpret_cast<TL*>(ws);
+  TL* Bp = Ap + Mp * Kp;
+  // op(A)(i, k): A[i + k lda] (N) or A[k + i lda] (T / C, real data)
+  pack_lo<TL>(reinterpret_cast<const double*>(g.A), g.lda, opA != OP_N, g.M, g.K, Ap, Mp, Kp, s);
+  // op(B)^T(j, k) = op(B)(k, j): B[k + j ldb] (N) or B[j + k ldb] (T / C)
+  pack_lo<TL>(reinterpret_cast<const double*>(g.B), g.ldb, opB == OP_N, g.N, g.K, Bp, Np, Kp, s);
+  LoGemmArgs a{};
+  a.M = g.M; a.N = g.N; a.Kp = Kp;
+  a.Ap = Ap; a.Bp = Bp;
+  a.C = reinterpret_cast<double*>(g.C); a.ldc = g.ldc;
+  a.alpha = g.alpha_re; a.beta = g.beta_re;
+  a.tri = g.tri;
+  dim3 grid((unsigned)(Mp / kLoRows), (unsigned)(Np / kLoRows), 1);
+  hipLaunchKernelGGL((gemm_lo_kernel<PREC>), grid, dim3(256), 0, s, a);
 }
 
-template <int OPA, int OPB> static void launch_mixed(int prec, const GemmArgs& g, hipStream_t s) {
-  dim3 grid(cdiv(g.M, 128), cdiv(g.N, 128), 1);
-  if (prec == 16)
-    hipLaunchKernelGGL((gemm_mixed_bf16_kernel<OPA, OPB>), grid, dim3(256), 0, s, g);
-  else
-    hipLaunchKernelGGL((gemm_mixed_f32_kernel<OPA, OPB>), grid, dim3(256), 0, s, g);
+}  // namespace
+
+size_t gemm_prec_workspace_bytes(int M, int N, int K) {
+  return (size_t)(4 * (lo_rows(M) + lo_rows(N)) * lo_kpad(K));
 }
 
 // Precision-dispatched GEMM: 64 -> FP64 MFMA, 32 / 16 -> FP32 / BF16 MFMA
-// (real data only; complex data always runs in FP64).
-template <typename T> void gemm_prec(int prec, int opA, int opB, GemmArgs g, hipStream_t s) {
+// (real data only; complex data always runs in FP64).  ws holds
+// gemm_prec_workspace_bytes(M, N, K) bytes for the packed operand copies.
+template <typename T> void gemm_prec(int prec, int opA, int opB, GemmArgs g, void* ws, hipStream_t s) {
   if constexpr (ScalarTraits<T>::is_complex) {
     gemm<T>(opA, opB, g, s);
   } else {
     if (prec == 64 || g.M <= 0 || g.N <= 0) { gemm<T>(opA, opB, g, s); return; }
-    const int a = (opA == OP_N) ? 0 : 1, b = (opB == OP_N) ? 0 : 1;
-    switch (a * 2 + b) {
-      case 0: launch_mixed<OP_N, OP_N>(prec, g, s); break;
-      case 1: launch_mixed<OP_N, OP_T>(prec, g, s); break;
-      case 2: launch_mixed<OP_T, OP_N>(prec, g, s); break;
-      default: launch_mixed<OP_T, OP_T>(prec, g, s); break;
-    }
+    if (prec == 16) gemm_lo<16>(opA, opB, g, ws, s);
+    else gemm_lo<32>(opA, opB, g, ws, s);
   }
 }
-template void gemm_prec<double>(int, int, int, GemmArgs, hipStream_t);
-template void gemm_prec<zd>(int, int, int, GemmArgs, hipStream_t);
+template void gemm_prec<double>(int, int, int, GemmArgs, void*, hipStream_t);
+template void gemm_prec<zd>(int, int, int, GemmArgs, void*, hipStream_t);
 
 }  // namespace mak

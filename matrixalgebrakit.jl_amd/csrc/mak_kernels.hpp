@@ -116,8 +116,8 @@ struct GemmArgs {
   int64_t splitStride;
   int batch;
   int unitLowerA;         // stored A is unit lower trapezoidal (strict lower part valid)
-  // Structure hints (FP64 / complex kernels; the others ignore them, which is
-  // also correct): the K range of a tile is clipped to the terms that can be
+  // Structure hints (FP64, complex and FP32 / BF16 kernels; kernels that ignore
+  // them are also correct): the K range of a tile is clipped to the terms that can be
   // nonzero when op(A) is lower (tri & TRI_A_LOWER: k <= i) or upper
   // (TRI_A_UPPER: k >= i) triangular and op(B) is upper (TRI_B_UPPER: k <= j)
   // or lower (TRI_B_LOWER: k >= j) triangular; upperC skips tiles strictly
@@ -146,7 +146,10 @@ void wy_reduce(const void* Wp, int ksplit, int64_t slice, const void* Tm, int tr
 
 // Precision-dispatched GEMM on FP64 data: prec 64 (FP64 MFMA), 32 (FP32 MFMA),
 // 16 (BF16 MFMA), FP32 accumulation for 32 / 16.  Complex data runs in FP64.
-template <typename T> void gemm_prec(int prec, int opA, int opB, GemmArgs g, hipStream_t s);
+// ws: gemm_prec_workspace_bytes(M, N, K) bytes (the rounded operand copies;
+// unused for prec 64).
+size_t gemm_prec_workspace_bytes(int M, int N, int K);
+template <typename T> void gemm_prec(int prec, int opA, int opB, GemmArgs g, void* ws, hipStream_t s);
 
 // ---------------------------------------------------------------- dense blocks
 template <typename T>

@@ -32,7 +32,7 @@ constexpr double kMaxCholQrC = 1e14;  // sqrt(1 + c) <= 1e7: (1e7)^2 u << 1
 inline size_t al256(size_t b) { return ((b + 255) / 256) * 256; }
 
 struct PolarLayout {
-  size_t offX1, offX2, offQ1, offG, offY1, offY2, offTmp, offPart, offScal, offInfo, bytes;
+  size_t offX1, offX2, offQ1, offG, offY1, offY2, offTmp, offPart, offScal, offInfo, offLo, bytes;
 };
 
 template <typename T> PolarLayout polar_layout(int m, int n) {
@@ -50,6 +50,9 @@ template <typename T> PolarLayout polar_layout(int m, int n) {
   L.offPart = take(1024 * sizeof(double));
   L.offScal = take(16 * sizeof(double));
   L.offInfo = take(16 * sizeof(int));
+  // rounded operand copies of the FP32 / BF16 products (real data only): the
+  // largest is (m x n)(n x n)
+  L.offLo = take(ScalarTraits<T>::is_complex ? 0 : gemm_prec_workspace_bytes(m, n, n));
   L.bytes = off;
   return L;
 }
@@ -99,6 +102,38 @@ void conj_transpose_scale(int m, int n, double a, const T* A, int64_t lda, T* B,
   hipLaunchKernelGGL((conj_transpose_scale_kernel<T>), grid, dim3(256), 0, s, m, n, a, A, lda, B, ldb);
 }
 
+// G := a I + b Gu, Gu the Hermitian matrix whose upper triangle G holds (the
+// strict lower part of G is not read).  One workgroup per 32 x 32 block pair
+// (rows bx, cols by) with bx <= by: it reads only its upper block and writes
+// that block and its mirror, so no two workgroups touch the same element.
+template <typename T>
+__global__ void __launch_bounds__(256) shift_herm_kernel(T* G, int64_t ldg, int n, double a, double b) {
+  __shared__ T tile[32][33];
+  const int bx = blockIdx.x, by = blockIdx.y;
+  if (bx > by) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {
+    const int r = bx * 32 + tx, c = by * 32 + k;
+    if (r < n && c < n && (bx < by || tx <= k)) tile[tx][k] = G[r + (int64_t)c * ldg];
+  }
+  __syncthreads();
+  if (bx == by) {  // complete the diagonal block from its upper triangle
+    for (int k = ty; k < 32; k += 8)
+      if (tx > k) tile[tx][k] = s_conj(tile[k][tx]);
+    __syncthreads();
+  }
+  for (int k = ty; k < 32; k += 8) {
+    const int r = bx * 32 + tx, c = by * 32 + k;
+    if (r < n && c < n) {
+      T v = s_scale(tile[tx][k], b);
+      if (r == c) v = s_add(v, s_scale(s_one(T()), a));
+      G[r + (int64_t)c * ldg] = v;
+    }
+    const int r2 = by * 32 + tx, c2 = bx * 32 + k;  // mirror: (r2, c2) = conj of (c2, r2)
+    if (bx < by && r2 < n && c2 < n) G[r2 + (int64_t)c2 * ldg] = s_scale(s_conj(tile[k][tx]), b);
+  }
+}
+
 template <typename T> double read_scalar(const double* d, hipStream_t s) {
   double h = 0.0;
   hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, s);
@@ -120,6 +155,7 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
   double* part = reinterpret_cast<double*>(ws + L.offPart);
   double* scal = reinterpret_cast<double*>(ws + L.offScal);
   int* dinfo = reinterpret_cast<int*>(ws + L.offInfo);
+  void* lo = ws + L.offLo;
   const int64_t ldm = m, ldn = n;
   const int prec = (precision == 32 || precision == 16) ? precision : 64;
   // BF16 bulk only in the well-conditioned Cholesky iterations; the CholeskyQR2
@@ -156,26 +192,26 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
     int nxt;
     if (c > 100.0) {
       // pass 1: Q1 = sqrt(c) X W1^{-1} (buf[s2]), Q2 = W1^{-1} (Y1)
-      gemm_prec<T>(prec_qr, OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y1, ldn, buf[s2], ldb[s2], std::sqrt(c), 0.0, TRI_B_UPPER), s);
+      gemm_prec<T>(prec_qr, OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y1, ldn, buf[s2], ldb[s2], std::sqrt(c), 0.0, TRI_B_UPPER), lo, s);
       // pass 2: G = Q2^H Q2 + Q1^H Q1 = W2^H W2
       gemm<T>(OP_C, OP_N, ga<T>(n, n, n, Y1, ldn, Y1, ldn, G, ldn, 1.0, 0.0, TRI_A_LOWER | TRI_B_UPPER, 1), s);
       gemm<T>(OP_C, OP_N, ga<T>(n, n, m, buf[s2], ldb[s2], buf[s2], ldb[s2], G, ldn, 1.0, 1.0, 0, 1), s);
       potrf_upper<T>(G, ldn, n, Y2, ldn, tmp, dinfo, s);
       // Q1' = Q1 W2^{-1} (buf[s1]), Q2' = Q2 W2^{-1} (G)
-      gemm_prec<T>(prec_qr, OP_N, OP_N, ga<T>(m, n, n, buf[s2], ldb[s2], Y2, ldn, buf[s1], ldb[s1], 1.0, 0.0, TRI_B_UPPER), s);
+      gemm_prec<T>(prec_qr, OP_N, OP_N, ga<T>(m, n, n, buf[s2], ldb[s2], Y2, ldn, buf[s1], ldb[s1], 1.0, 0.0, TRI_B_UPPER), lo, s);
       gemm<T>(OP_N, OP_N, ga<T>(n, n, n, Y1, ldn, Y2, ldn, G, ldn, 1.0, 0.0, TRI_A_UPPER | TRI_B_UPPER), s);
       // X+ = (b/c) X + (a - b/c)/sqrt(c) Q1' Q2'^H  (buf[s2])
       copy_mat<T>(m, n, Xc, ldc, buf[s2], ldb[s2], s);
       gemm_prec<T>(prec_qr, OP_N, OP_C,
                    ga<T>(m, n, n, buf[s1], ldb[s1], G, ldn, buf[s2], ldb[s2], (a - b / c) / std::sqrt(c), b / c,
-                         TRI_B_LOWER), s);
+                         TRI_B_LOWER), lo, s);
       nxt = s2;
       ++nqr;
     } else {
       // Z^{-1} = W^{-1} W^{-H};  X+ = (b/c) X + (a - b/c) X Z^{-1}  (buf[s1])
-      gemm_prec<T>(prec, OP_N, OP_C, ga<T>(n, n, n, Y1, ldn, Y1, ldn, Y2, ldn, 1.0, 0.0, TRI_A_UPPER | TRI_B_LOWER), s);
+      gemm_prec<T>(prec, OP_N, OP_C, ga<T>(n, n, n, Y1, ldn, Y1, ldn, Y2, ldn, 1.0, 0.0, TRI_A_UPPER | TRI_B_LOWER), lo, s);
       copy_mat<T>(m, n, Xc, ldc, buf[s1], ldb[s1], s);
-      gemm_prec<T>(prec, OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y2, ldn, buf[s1], ldb[s1], a - b / c, b / c), s);
+      gemm_prec<T>(prec, OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y2, ldn, buf[s1], ldb[s1], a - b / c, b / c), lo, s);
       nxt = s1;
       ++nch;
     }
@@ -200,9 +236,10 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
     const int steps = (prec == 16) ? 3 : 2;
     for (int k = 0; k < steps; ++k) {
       const int nx = (cur + 1) % 3;
-      gemm<T>(OP_C, OP_N, ga<T>(n, n, m, buf[cur], ldb[cur], buf[cur], ldb[cur], G, ldn, 1.0, 0.0), s);
-      copy_mat<T>(m, n, buf[cur], ldb[cur], buf[nx], ldb[nx], s);
-      gemm<T>(OP_N, OP_N, ga<T>(m, n, n, buf[cur], ldb[cur], G, ldn, buf[nx], ldb[nx], -0.5, 1.5), s);
+      // G = (3 I - X^H X) / 2 from the upper triangle of X^H X;  X+ = X G
+      gemm<T>(OP_C, OP_N, ga<T>(n, n, m, buf[cur], ldb[cur], buf[cur], ldb[cur], G, ldn, 1.0, 0.0, 0, 1), s);
+      hipLaunchKernelGGL((shift_herm_kernel<T>), dim3(cdiv(n, 32), cdiv(n, 32)), dim3(256), 0, s, G, ldn, n, 1.5, -0.5);
+      gemm<T>(OP_N, OP_N, ga<T>(m, n, n, buf[cur], ldb[cur], G, ldn, buf[nx], ldb[nx], 1.0, 0.0), s);
       cur = nx;
       ++nns;
     }

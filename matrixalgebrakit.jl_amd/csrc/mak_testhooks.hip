@@ -21,6 +21,20 @@ int maktest_gemm(void* stream, int dtype, int opA, int opB, int M, int N, int K,
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+// FP32 / BF16 MFMA product of FP64 data (prec 32 / 16; 64 runs the FP64
+// kernel); ws: maktest_gemm_prec_ws(M, N, K) bytes.
+size_t maktest_gemm_prec_ws(int M, int N, int K) { return mak::gemm_prec_workspace_bytes(M, N, K); }
+int maktest_gemm_prec(void* stream, int prec, int opA, int opB, int M, int N, int K, const double* A, int lda,
+                      const double* B, int ldb, double* C, int ldc, double alpha, double beta, int tri, void* ws) {
+  mak::GemmArgs g{};
+  g.M = M; g.N = N; g.K = K;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.alpha_re = alpha; g.beta_re = beta;
+  g.ksplit = 1; g.kchunk = K; g.batch = 1; g.tri = tri;
+  mak::gemm_prec<double>(prec, opA, opB, g, ws, (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 int maktest_geqr2(void* stream, int dtype, int H, int nbc, void* X, int ldx, void* Tout) {
   const int cap = dtype == MAK_F64 ? mak::geqr2_node_capacity<double>() : mak::geqr2_node_capacity<mak::zd>();
   if (H > cap || nbc > 32) return -3;

@@ -125,3 +125,96 @@ def test_potrf_reports_first_nonpositive_minor(dev):
     Zh[:, 100:] = 0.0          # leading 100 x 100 is PD, minor 101 is singular
     _, _, info = run_potrf(Zh, dev)
     assert info == 101
+
+
+# ------------------------------------------- FP32 / BF16 MFMA products ----
+
+def gemm_prec_hook():
+    lib = _lib.load()
+    lib.maktest_gemm_prec_ws.restype = C.c_size_t
+    lib.maktest_gemm_prec_ws.argtypes = [C.c_int, C.c_int, C.c_int]
+    lib.maktest_gemm_prec.restype = C.c_int
+    lib.maktest_gemm_prec.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                      C.c_double, C.c_double, C.c_int, C.c_void_p]
+    return lib
+
+
+def round_to(x, prec):
+    """x rounded as the kernels round it: to FP32, then (BF16) to the nearest
+    BF16 with ties to even."""
+    f = np.asarray(x, dtype=np.float64).astype(np.float32)
+    if prec == 16:
+        b = f.view(np.uint32).astype(np.uint64)
+        b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+        f = b.astype(np.uint32).view(np.float32)
+    return f.astype(np.float64)
+
+
+def run_gemm_prec(prec, opA, opB, A, B, Cin, alpha, beta, tri, dev):
+    lib = gemm_prec_hook()
+    M, N = Cin.shape
+    K = A.shape[1] if opA == 0 else A.shape[0]
+    At = mak.colmajor(torch.from_numpy(np.asfortranarray(A)).to(dev))
+    Bt = mak.colmajor(torch.from_numpy(np.asfortranarray(B)).to(dev))
+    Ct = mak.colmajor(torch.from_numpy(np.asfortranarray(Cin)).to(dev))
+    ws = torch.full((lib.maktest_gemm_prec_ws(M, N, K) // 4 + 1,), float("nan"), dtype=torch.float32, device=dev)
+    code = lib.maktest_gemm_prec(mak.api._stream(dev), prec, opA, opB, M, N, K, At.data_ptr(), max(At.stride(1), 1),
+                                 Bt.data_ptr(), max(Bt.stride(1), 1), Ct.data_ptr(), max(Ct.stride(1), 1),
+                                 alpha, beta, tri, ws.data_ptr())
+    torch.cuda.synchronize()
+    assert code == 0
+    return Ct.cpu().numpy()
+
+
+PREC_SHAPES = [(1, 1, 1), (37, 129, 65), (130, 70, 200), (256, 384, 96), (300, 17, 3), (64, 64, 0)]
+
+
+@pytest.mark.parametrize("prec", [32, 16])
+@pytest.mark.parametrize("opA,opB", [(0, 0), (0, 1), (1, 0), (1, 1)], ids=["NN", "NT", "TN", "TT"])
+@pytest.mark.parametrize("M,N,K", PREC_SHAPES, ids=[f"{m}x{n}x{k}" for m, n, k in PREC_SHAPES])
+def test_gemm_prec_exact_on_small_integers(M, N, K, opA, opB, prec, dev):
+    """Small-integer operands are exact in BF16 and FP32 and every partial sum
+    stays below 2^24, so the FP32-accumulated product must equal the exact one
+    bit for bit: a wrong k pairing, row/column index or transposition fails."""
+    rng = np.random.default_rng(M * 1000 + N * 10 + K + opA * 3 + opB)
+    A = rng.integers(-4, 5, size=(M, K) if opA == 0 else (K, M)).astype(np.float64)
+    B = rng.integers(-4, 5, size=(K, N) if opB == 0 else (N, K)).astype(np.float64)
+    Cin = rng.integers(-8, 9, size=(M, N)).astype(np.float64)
+    opa = A if opA == 0 else A.T
+    opb = B if opB == 0 else B.T
+    want = -2.0 * (opa @ opb) + 0.5 * Cin
+    got = run_gemm_prec(prec, opA, opB, A, B, Cin, -2.0, 0.5, 0, dev)
+    assert np.array_equal(got, want), np.abs(got - want).max()
+
+
+@pytest.mark.parametrize("prec,tol", [(32, 1e-5), (16, 1e-5)])
+def test_gemm_prec_gaussian_vs_rounded_inputs(prec, tol, dev):
+    """Gaussian operands: the result matches the FP64 product of the rounded
+    inputs to FP32 accumulation accuracy (relative Frobenius error <= tol);
+    beta = 0 overwrites NaN in C."""
+    M, N, K = 520, 390, 700
+    A = O.gaussian(M, K, np.float64, seed=11)
+    B = O.gaussian(N, K, np.float64, seed=12)
+    Cin = np.full((M, N), np.nan)
+    want = 1.25 * (round_to(A, prec) @ round_to(B, prec).T)
+    got = run_gemm_prec(prec, 0, 1, A, B, Cin, 1.25, 0.0, 0, dev)
+    assert rel(got, want) < tol, rel(got, want)
+
+
+@pytest.mark.parametrize("prec", [32, 16])
+@pytest.mark.parametrize("tri", [2 | 8, 1 | 4], ids=["AupperBlower", "AlowerBupper"])
+def test_gemm_prec_triangular_hints(tri, prec, dev):
+    """With exact zeros on the hinted side of op(A) and op(B) the clipped K
+    ranges give the full product (small integers: exact)."""
+    n = 300
+    rng = np.random.default_rng(tri + prec)
+    A = rng.integers(-3, 4, size=(n, n)).astype(np.float64)
+    B = rng.integers(-3, 4, size=(n, n)).astype(np.float64)
+    if tri & 2:
+        A, B = np.triu(A), np.tril(B)
+    else:
+        A, B = np.tril(A), np.triu(B)
+    Cin = np.zeros((n, n))
+    got = run_gemm_prec(prec, 0, 0, A, B, Cin, 1.0, 0.0, tri, dev)
+    assert np.array_equal(got, A @ B)
