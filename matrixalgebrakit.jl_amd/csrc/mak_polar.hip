@@ -208,10 +208,13 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
       nxt = s2;
       ++nqr;
     } else {
-      // Z^{-1} = W^{-1} W^{-H};  X+ = (b/c) X + (a - b/c) X Z^{-1}  (buf[s1])
-      gemm_prec<T>(prec, OP_N, OP_C, ga<T>(n, n, n, Y1, ldn, Y1, ldn, Y2, ldn, 1.0, 0.0, TRI_A_UPPER | TRI_B_LOWER), lo, s);
+      // Z^{-1} = W^{-1} W^{-H}:  Tm = X W^{-1} (buf[s2]);
+      // X+ = (b/c) X + (a - b/c) Tm W^{-H}  (buf[s1]); two triangular products
+      // instead of W^{-1} W^{-H} followed by a full product
+      gemm_prec<T>(prec, OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y1, ldn, buf[s2], ldb[s2], 1.0, 0.0, TRI_B_UPPER), lo, s);
       copy_mat<T>(m, n, Xc, ldc, buf[s1], ldb[s1], s);
-      gemm_prec<T>(prec, OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y2, ldn, buf[s1], ldb[s1], a - b / c, b / c), lo, s);
+      gemm_prec<T>(prec, OP_N, OP_C,
+                   ga<T>(m, n, n, buf[s2], ldb[s2], Y1, ldn, buf[s1], ldb[s1], a - b / c, b / c, TRI_B_LOWER), lo, s);
       nxt = s1;
       ++nch;
     }
