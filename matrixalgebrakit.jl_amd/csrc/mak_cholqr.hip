@@ -18,6 +18,16 @@
 // A panel whose first Gram is not numerically positive definite or whose
 // second Gram fails the check (kappa(X) beyond ~1e7) is factored by the TSQR
 // path instead, so rank-deficient inputs keep Householder's stability.
+//
+// Carried first Gram (consecutive panels, host-checked mode): the finish
+// kernel of panel k also forms Gw = Xn^H Xn of the next panel's columns over
+// the same rows j0.. (before panel k's trailing update).  The update applies
+// the unitary H_k^H to those rows, which leaves Xn^H Xn unchanged, so the next
+// panel's first Gram (rows j0 + 32..) is Gw - Rt^H Rt, Rt the 32 rows of its
+// columns that become R (carried_chol_kernel: no pass over the panel).  The
+// difference loses accuracy when Xn lies mostly in the span of panel k; R1
+// then only preconditions worse, and the second pass's check still decides
+// whether the panel is accepted.
 #include <algorithm>
 
 #include "mak_common.hpp"
@@ -46,9 +56,12 @@ enum GramMode { kPlain = 0, kTimesB = 1 };
 // (H x nbc, columns nbc..31 read as zero), with Y formed as MODE says.
 // The next tile's loads are issued before the current tile is used.
 // Wave w accumulates the 16 x 16 block (w % 2, w / 2) of G in (cr, ci).
+// With Xn (not null), the same row tiles of Xn (H x nbcn, ld ldx) are staged
+// in Xns alongside and Gn += Xn^H Xn accumulates in (nr, ni).
 template <typename T, int MODE>
 __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*Xs)[NB + 1], const T (*Bs)[NB + 1],
-                                  T* Yout, v4d& cr, v4d& ci) {
+                                  T* Yout, v4d& cr, v4d& ci, const T* Xn = nullptr, int nbcn = 0,
+                                  T (*Xns)[NB + 1] = nullptr, v4d* nr = nullptr, v4d* ni = nullptr) {
   constexpr int TR = CqTile<T>::TR;
   constexpr int PER = TR * NB / 256;
   constexpr bool CPLX = ScalarTraits<T>::is_complex;
@@ -58,7 +71,8 @@ __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*X
   const int lr = lane & 15, lk = lane >> 4;
   const int ti = w & 1, tj = w >> 1;
   const int ntile = cdiv(H, TR);
-  T cur[PER];
+  const bool next = Xn != nullptr;
+  T cur[PER], curn[PER];
   auto load = [&](int t) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -66,6 +80,15 @@ __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*X
       const int r = e % TR, c = e / TR;
       const int row = t * TR + r;
       cur[q] = (row < H && c < nbc) ? X[row + (int64_t)c * ldx] : s_zero(T());
+    }
+    if (next) {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int e = tid + 256 * q;
+        const int r = e % TR, c = e / TR;
+        const int row = t * TR + r;
+        curn[q] = (row < H && c < nbcn) ? Xn[row + (int64_t)c * ldx] : s_zero(T());
+      }
     }
   };
   int t = blockIdx.x;
@@ -77,8 +100,30 @@ __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*X
       const int e = tid + 256 * q;
       Xs[e % TR][e / TR] = cur[q];
     }
+    if (next) {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int e = tid + 256 * q;
+        Xns[e % TR][e / TR] = curn[q];
+      }
+    }
     if (t + (int)gridDim.x < ntile) load(t + gridDim.x);
     __syncthreads();
+    if (next) {
+#pragma unroll 4
+      for (int kk = 0; kk < TR; kk += 4) {
+        const T a = Xns[kk + lk][ti * 16 + lr];
+        const T b = Xns[kk + lk][tj * 16 + lr];
+        if constexpr (CPLX) {
+          *nr = mfma_f64(s_re(a), s_re(b), *nr);
+          *nr = mfma_f64(s_im(a), s_im(b), *nr);
+          *ni = mfma_f64(s_re(a), s_im(b), *ni);
+          *ni = mfma_f64(-s_im(a), s_re(b), *ni);
+        } else {
+          *nr = mfma_f64(a, b, *nr);
+        }
+      }
+    }
     if constexpr (MODE == kTimesB) {
       // Y = X B as transposed 16 x 16 blocks (Y^T = B^T X^T): lane l, register
       // r holds Y(i0 + l % 16, j0 + 4 r + l / 16), so stores are row-contiguous.
@@ -149,14 +194,11 @@ __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*X
   }
 }
 
-// Stores this workgroup's partial Gram (32 x 32 at partial + blockIdx.x * 1024)
-// and takes a ticket; returns true (uniformly) only in the last workgroup to
-// finish, after an acquire fence, so every partial is visible to it.
+// Stores this workgroup's partial Gram (32 x 32 at partial + blockIdx.x * 1024).
 template <typename T>
-__device__ inline bool partial_is_last(T* partial, const v4d& cr, const v4d& ci, int* counter, int* last_s) {
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int w = tid >> 6;
+__device__ inline void store_partial(T* partial, const v4d& cr, const v4d& ci) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
   const int ti = w & 1, tj = w >> 1;
   T* P = partial + (int64_t)blockIdx.x * NB * NB;
 #pragma unroll
@@ -165,6 +207,15 @@ __device__ inline bool partial_is_last(T* partial, const v4d& cr, const v4d& ci,
     if constexpr (ScalarTraits<T>::is_complex) P[i + j * NB] = mkz(cr[r], ci[r]);
     else P[i + j * NB] = cr[r];
   }
+}
+
+// Stores this workgroup's partial Gram (32 x 32 at partial + blockIdx.x * 1024)
+// and takes a ticket; returns true (uniformly) only in the last workgroup to
+// finish, after an acquire fence, so every partial is visible to it.
+template <typename T>
+__device__ inline bool partial_is_last(T* partial, const v4d& cr, const v4d& ci, int* counter, int* last_s) {
+  const int tid = threadIdx.x;
+  store_partial<T>(partial, cr, ci);
   __threadfence();
   __syncthreads();
   if (tid == 0) *last_s = (atomicAdd(counter, 1) == (int)gridDim.x - 1);
@@ -261,6 +312,46 @@ __global__ void __launch_bounds__(256) gram_chol_kernel(const T* X, int64_t ldx,
     status[0] = (bad_s != 0 || (sticky && prev != 0)) ? 1 : 0;
   }
   if (bad_s != 0) return;
+  for (int e = tid; e < NB * NB; e += 256) {
+    R1[e] = G[e % NB][e / NB];
+    R1inv[e] = Yl[e % NB][e / NB];
+  }
+}
+
+// Pass 1 from a carried Gram: G1 = Gw - Rt^H Rt (Gw: 32 x 32, ld 32; Rt: the
+// 32 rows above the panel, ld lda), then as gram_chol_kernel in sticky mode.
+// One workgroup.
+template <typename T>
+__global__ void __launch_bounds__(256) carried_chol_kernel(const T* Gw, const T* Rt, int64_t lda, int nbc,
+                                                           int* status, int j0, T* R1, T* R1inv) {
+  __shared__ T G[NB][NB + 1], Rs[NB][NB + 1], Yl[NB][NB + 1];
+  __shared__ int bad_s;
+  if (status[0] != 0) return;  // an earlier panel was rejected: the flag stays set
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e % NB, j = e / NB;
+    G[i][j] = Gw[e];
+    Rs[i][j] = (j < nbc) ? Rt[i + (int64_t)j * lda] : s_zero(T());
+  }
+  __syncthreads();
+  mm32<T>(G, Rs, true, Rs, false, -1.0, true);  // G -= Rt^H Rt
+  for (int e = tid; e < NB * NB; e += 256) {    // identity outside nbc x nbc
+    const int i = e % NB, j = e / NB;
+    if (i >= nbc || j >= nbc) G[i][j] = (i == j) ? s_one(T()) : s_zero(T());
+  }
+  __syncthreads();
+  if ((tid >> 6) == 0) {
+    const int b = wave_chol_inv32<T>(G, Yl);
+    if (tid == 0) bad_s = b;
+  }
+  __syncthreads();
+  if (bad_s != 0) {
+    if (tid == 0) {
+      status[2] = j0;
+      status[0] = 1;
+    }
+    return;
+  }
   for (int e = tid; e < NB * NB; e += 256) {
     R1[e] = G[e % NB][e / NB];
     R1inv[e] = Yl[e % NB][e / NB];
@@ -397,14 +488,18 @@ __device__ __forceinline__ void lu32_recon(T (*Qt)[33], T (*Ls)[33], T* dv, T (*
 // Q = Q1 W2^{-1}, its LU with signs D, T = -U D^H L^{-H}, M = W2^{-1} U^{-1}
 // (to Mout, 32 x 32) and the top nbc rows of the panel Ap (D R on and above
 // the diagonal, L below).  T goes to Tout (32 x 32, zero outside nbc x nbc).
+// With Xn (not null) it also forms Gw = Xn^H Xn (H x nbcn, ld ldx; to Gw, 32 x 32,
+// zero outside nbcn x nbcn) through partialN, before the check.
 template <typename T>
 __global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t ldx, int H, int nbc, const T* R1,
                                                              const T* R1inv, T* Q1, T* partial, int* status,
-                                                             int j0, T* Tout, T* Mout, T* Ap, int64_t lda) {
+                                                             int j0, T* Tout, T* Mout, T* Ap, int64_t lda,
+                                                             const T* Xn, int nbcn, T* partialN, T* Gw) {
   constexpr int TR = CqTile<T>::TR;
   constexpr int XM = TR / NB;  // 32-row matrices spanned by the staged tile
-  static_assert(XM + 1 <= 8, "LDS pool too small for the staged tile");
-  __shared__ T pool[8][NB][NB + 1];
+  // the staged tiles of X and Xn, R1^{-1}; at least the 8 matrices of the end
+  constexpr int NPOOL = (2 * XM + 1 > 8) ? 2 * XM + 1 : 8;
+  __shared__ T pool[NPOOL][NB][NB + 1];
   __shared__ T dv[NB];
   __shared__ int last_s, bad_s;
   __shared__ double dev_s[4];
@@ -419,8 +514,16 @@ __global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t
     for (int e = tid; e < NB * NB; e += 256) Bs[e % NB][e / NB] = R1inv[e];
     // (gram_tiles begins each tile with a workgroup barrier)
     v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
-    gram_tiles<T, kTimesB>(X, ldx, H, nbc, pool[0], Bs, Q1, cr, ci);
+    v4d nr = {0.0, 0.0, 0.0, 0.0}, ni = {0.0, 0.0, 0.0, 0.0};
+    gram_tiles<T, kTimesB>(X, ldx, H, nbc, pool[0], Bs, Q1, cr, ci, Xn, nbcn, pool[XM + 1], &nr, &ni);
+    if (Xn != nullptr) store_partial<T>(partialN, nr, ni);
     if (!partial_is_last<T>(partial, cr, ci, counter, &last_s)) return;
+  }
+  if (Xn != nullptr) {  // Gw = sum of the partials (zero outside nbcn x nbcn)
+    T (*Gn)[NB + 1] = pool[1];
+    reduce_partials<T>(partialN, (int)gridDim.x, NB, Gn, counter, dev_s);
+    for (int e = tid; e < NB * NB; e += 256) Gw[e] = Gn[e % NB][e / NB];
+    __syncthreads();
   }
   T (*G)[NB + 1] = pool[0];   // G2, then W2
   T (*Yl)[NB + 1] = pool[1];  // W2^{-1}
@@ -493,27 +596,40 @@ void zero_ints(int* p, int n, hipStream_t s) {
 }
 
 size_t cholqr_workspace_elems(int m) {
-  // Q1 (m x 32), the partial Grams, R1, R1^{-1}, M
-  return (size_t)std::max(m, 1) * NB + (size_t)kMaxGramBlocks * NB * NB + (size_t)3 * NB * NB;
+  // the carried Gram and its partials, the partial Grams, R1, R1^{-1}, M,
+  // Q1 (m x 32)
+  return (size_t)std::max(m, 1) * NB + (size_t)kMaxGramBlocks * NB * NB + (size_t)3 * NB * NB +
+         (size_t)kMaxGramBlocks * NB * NB + (size_t)NB * NB;
 }
 
 template <typename T>
 void cholqr_panel(T* Ap, int64_t lda, int H, int nbc, T* Tpan, T* cw, int* status, int j0, int sticky,
-                  hipStream_t s) {
-  T* Q1 = cw;
-  T* part = Q1 + (size_t)H * NB;
+                  int carried, int nbc_next, hipStream_t s) {
+  // fixed offsets first (the carried Gram must sit at the same address for
+  // the next panel, whose H is smaller), Q1 last
+  T* Gw = cw;
+  T* partN = Gw + NB * NB;
+  T* part = partN + (size_t)kMaxGramBlocks * NB * NB;
   T* R1 = part + (size_t)kMaxGramBlocks * NB * NB;
   T* R1i = R1 + NB * NB;
   T* M = R1i + NB * NB;
+  T* Q1 = M + NB * NB;
   const int nblk = std::max(1, std::min(kMaxGramBlocks, cdiv(H, 256)));
-  hipLaunchKernelGGL((gram_chol_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc, part, status,
-                     j0, sticky, R1, R1i);
+  if (carried) {
+    hipLaunchKernelGGL((carried_chol_kernel<T>), dim3(1), dim3(256), 0, s, (const T*)Gw, (const T*)(Ap - NB), lda,
+                       nbc, status, j0, R1, R1i);
+  } else {
+    hipLaunchKernelGGL((gram_chol_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc, part,
+                       status, j0, sticky, R1, R1i);
+  }
+  const T* Xn = nbc_next > 0 ? (const T*)(Ap + (int64_t)nbc * lda) : nullptr;
   hipLaunchKernelGGL((cholqr2_finish_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc,
-                     (const T*)R1, (const T*)R1i, Q1, part, status, j0, Tpan, M, Ap, lda);
+                     (const T*)R1, (const T*)R1i, Q1, part, status, j0, Tpan, M, Ap, lda, Xn, nbc_next, partN, Gw);
   if (H > nbc) gemm<T>(OP_N, OP_N, gg<T>(H - nbc, nbc, nbc, Q1 + nbc, H, M, NB, Ap + nbc, lda, status), s);
 }
 
-template void cholqr_panel<double>(double*, int64_t, int, int, double*, double*, int*, int, int, hipStream_t);
-template void cholqr_panel<zd>(zd*, int64_t, int, int, zd*, zd*, int*, int, int, hipStream_t);
+template void cholqr_panel<double>(double*, int64_t, int, int, double*, double*, int*, int, int, int, int,
+                                   hipStream_t);
+template void cholqr_panel<zd>(zd*, int64_t, int, int, zd*, zd*, int*, int, int, int, int, hipStream_t);
 
 }  // namespace mak

@@ -27,11 +27,17 @@ int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, in
 // the caller's flag-gated Householder launches factor exactly the rejected
 // panels; with sticky = 1 a set flag stays set (and skips every later panel),
 // for a caller that checks it on the host.  cw: cholqr_workspace_elems(H).
+// nbc_next > 0 (sticky mode only): also form the Gram of the next panel's
+// nbc_next columns (over the same rows, before this panel's trailing update);
+// carried = 1: take this panel's first Gram from the previous panel's call
+// (which had nbc_next > 0) instead of a pass over the panel, once the previous
+// trailing update has run.
 size_t cholqr_workspace_elems(int m);
 template <typename T>
 void cholqr_panel(T* Ap, int64_t lda, int H, int nbc, T* Tpan, T* cw, int* status, int j0, int sticky,
-                  hipStream_t s);
+                  int carried, int nbc_next, hipStream_t s);
 void zero_ints(int* p, int n, hipStream_t s);
+extern int g_last_cholqr_reject;
 
 template <typename T> size_t polar_workspace_bytes(int m, int n);
 template <typename T>

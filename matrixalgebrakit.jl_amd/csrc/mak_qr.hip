@@ -157,9 +157,11 @@ template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
 //                     and refactors from the first rejected panel.
 enum PanelMode { kHouseholderOnly = 0, kCholGated = 1, kCholHostChecked = 2 };
 
+// carried / nbc_next: the carried first Gram (cholqr_panel), host-checked
+// CholeskyQR2 panels only.
 template <typename T>
 void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrLayout<T>& L, int mode,
-                  hipStream_t s) {
+                  hipStream_t s, int carried = 0, int nbc_next = 0) {
   T* Ap = A + j0 + (int64_t)j0 * lda;
   const int H = m - j0;
   T* Tpan = reinterpret_cast<T*>(ws + L.offTpanels) + (size_t)(j0 / NB) * NB * NB;
@@ -168,8 +170,9 @@ void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrL
   const int* gate = nullptr;
   int want = 1;
   if (mode != kHouseholderOnly && status != nullptr && H >= kCholQrMinRows) {
-    cholqr_panel<T>(Ap, lda, H, nbc, Tpan, reinterpret_cast<T*>(ws + L.offCq), status, j0,
-                    mode == kCholHostChecked ? 1 : 0, s);
+    const bool hc = mode == kCholHostChecked;
+    cholqr_panel<T>(Ap, lda, H, nbc, Tpan, reinterpret_cast<T*>(ws + L.offCq), status, j0, hc ? 1 : 0,
+                    hc ? carried : 0, hc ? nbc_next : 0, s);
     if (mode == kCholHostChecked) return;
     gate = status;  // the Householder launches below run only for a rejected panel
   } else if (mode == kCholHostChecked && status != nullptr) {
@@ -283,9 +286,15 @@ template <typename T>
 void geqrf_panels(T* A, int64_t lda, int m, int n, char* ws, const QrLayout<T>& L, int from, int mode,
                   hipStream_t s) {
   const int minmn = std::min(m, n);
+  const bool chol = mode == kCholHostChecked && L.offFlags != 0;
+  int carried = 0;
   for (int j0 = from; j0 < minmn; j0 += NB) {
     const int nbc = std::min(NB, minmn - j0);
-    factor_panel<T>(A, lda, j0, m, nbc, ws, L, mode, s);
+    // consecutive CholeskyQR2 panels pass the next panel's first Gram along
+    const int nbc_next = (chol && m - j0 >= kCholQrMinRows && j0 + nbc < minmn && m - j0 - nbc >= kCholQrMinRows)
+                             ? std::min(NB, minmn - j0 - nbc) : 0;
+    factor_panel<T>(A, lda, j0, m, nbc, ws, L, mode, s, carried, nbc_next);
+    carried = nbc_next > 0 ? 1 : 0;
     const int nc = n - j0 - nbc;
     if (nc > 0) {
       const T* Tpan = reinterpret_cast<const T*>(ws + L.offTpanels) + (size_t)(j0 / NB) * NB * NB;
@@ -328,15 +337,22 @@ bool geqrf_enqueue(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipS
 // final and the trailing matrix is updated up to it; every later panel was
 // skipped, so the factorisation resumes there with Householder TSQR.  Returns
 // true when it refactored (work enqueued after geqrf_enqueue must be redone).
+// First column of the first panel that the last host-checked factorisation
+// refactored with Householder TSQR (-1: every CholeskyQR2 panel was accepted);
+// read by the test suite.
+int g_last_cholqr_reject = -1;
+
 template <typename T>
 bool geqrf_finish(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s) {
   const int minmn = std::min(m, n);
   QrLayout<T> L = qr_layout<T>(m, n, (int)qcols);
+  g_last_cholqr_reject = -1;
   if (!L.offFlags) return false;
   int h[3] = {0, 0, 0};
   if (hipMemcpyAsync(h, ws + L.offFlags, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess || h[0] == 0 || h[2] < 0 || h[2] >= minmn || h[2] % NB != 0)
     return false;
+  g_last_cholqr_reject = h[2];
   geqrf_panels<T>(A, lda, m, n, ws, L, h[2], kHouseholderOnly, s);
   return true;
 }

@@ -35,6 +35,10 @@ int maktest_gemm_prec(void* stream, int prec, int opA, int opB, int M, int N, in
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+// First column of the first CholeskyQR2 panel that the last qr call (outside
+// graph capture) handed to the Householder path, or -1 when none was.
+int maktest_last_cholqr_reject() { return mak::g_last_cholqr_reject; }
+
 int maktest_geqr2(void* stream, int dtype, int H, int nbc, void* X, int ldx, void* Tout) {
   const int cap = dtype == MAK_F64 ? mak::geqr2_node_capacity<double>() : mak::geqr2_node_capacity<mak::zd>();
   if (H > cap || nbc > 32) return -3;

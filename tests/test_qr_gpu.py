@@ -122,24 +122,60 @@ def test_qr_rank_deficient_and_special(dev):
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
-@pytest.mark.parametrize("kind", ["repeated_column", "graded_1e10"])
+@pytest.mark.parametrize("kind", ["repeated_column", "graded_1e10", "mixed_1e12"])
 def test_qr_cholqr_rejected_later_panel(kind, dt, dev):
-    """Panels 0-3 pass the CholeskyQR2 check; a later panel is rejected (an
-    exactly repeated column, or column scales falling to 1e-10), so the driver
-    refactors from that panel with Householder TSQR.  Q is not unique for a
-    rank-deficient A, so the checks are backward error, orthogonality and the
-    gauge of R."""
+    """Panels 0-3 are Gaussian; panel 4 (columns 128..159) is made hard for
+    CholeskyQR2.  mixed_1e12: its columns span singular values 1 .. 1e-12 in
+    mixed directions, so the first Gram is not numerically positive definite
+    or the second fails the check, and the driver must refactor from column
+    128 with Householder TSQR.  graded_1e10 (column scales 1 .. 1e-10) and an
+    exactly repeated column may legitimately pass: diagonal scaling does not
+    hurt the Cholesky factor, and a repeated column gives a tiny pivot whose
+    Q1 column is normalised rounding noise, orthogonal to the others.  Q is
+    not unique for a rank-deficient A, so the checks are backward error,
+    orthogonality and the gauge of R."""
     m, n = 2000, 200
     A = O.gaussian(m, n, dt, seed=21)
     if kind == "repeated_column":
         A[:, 150] = A[:, 10]
-    else:  # panel 4 (columns 128..159) spans column scales 1 .. 1e-10
+    elif kind == "graded_1e10":
         A = A * (10.0 ** (-10.0 * np.clip(np.arange(n) - 128, 0, 31) / 31))[None, :]
+    else:
+        U, _ = np.linalg.qr(O.gaussian(m, 32, dt, seed=22))
+        V, _ = np.linalg.qr(O.gaussian(32, 32, dt, seed=23))
+        A[:, 128:160] = (U * np.logspace(0, -12, 32)[None, :]) @ V.conj().T
     Q, R = run_qr(A, dev)
+    rejected = last_cholqr_reject()
     assert rel(Q @ R, A) < 1e-13
     assert np.linalg.norm(Q.conj().T @ Q - np.eye(n)) < 1e-12
     d = np.diagonal(R)
     assert np.all(d.imag == 0) and np.all(d.real >= 0)
+    if kind == "mixed_1e12":
+        assert rejected == 128
+    else:
+        assert rejected in (-1, 128)
+
+
+def last_cholqr_reject():
+    import ctypes as C
+    from mak_mi355x import _lib
+    lib = _lib.load()
+    lib.maktest_last_cholqr_reject.restype = C.c_int
+    return lib.maktest_last_cholqr_reject()
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+@pytest.mark.parametrize("m,n", [(3000, 400), (700, 650), (1200, 1200)])
+def test_qr_cholqr_carried_gram_accepts_gaussian_panels(m, n, dt, dev):
+    """Every panel after the first takes its first Gram from the previous
+    panel's pass (Gw - Rt^H Rt); for Gaussian inputs all of them must pass the
+    CholeskyQR2 check (none handed to the Householder path), and the factors
+    must match the oracle's unique positive-gauge QR."""
+    A = O.gaussian(m, n, dt, seed=m + n)
+    Q, R = run_qr(A, dev)
+    assert last_cholqr_reject() == -1
+    Qo, Ro = O.qr_compact(A, positive=True)
+    assert rel(Q, Qo) < 1e-12 and rel(R, Ro) < 1e-12
 
 
 def test_qr_input_untouched_by_allocating_form(dev):
