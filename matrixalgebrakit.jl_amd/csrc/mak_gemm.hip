@@ -521,8 +521,17 @@ template void sum_slices<zd>(const zd*, int, int64_t, int, int, zd*, int64_t, hi
 // by block on FP64 MFMA.  Float64 keeps every block of T in LDS; ComplexF64
 // (twice the bytes) re-reads the finished blocks from Tout.
 template <typename T>
-__global__ void __launch_bounds__(256) wy_merge_kernel(const T* S, const T* Tp, int np, int kg, T* Tout) {
+__global__ void __launch_bounds__(256) wy_merge_kernel(const T* S, const T* Tp, int np, int kg, T* Tout, int npan,
+                                                       int minmn) {
   constexpr bool LDS_T = !ScalarTraits<T>::is_complex;
+  if (npan > 0) {  // one group of four panels per workgroup (see wy_merge_groups)
+    const int g = blockIdx.x;
+    np = min(4, npan - 4 * g);
+    kg = min(minmn - 128 * g, 128);
+    S += (int64_t)g * 128 * 128;
+    Tp += (int64_t)g * 4 * 1024;
+    Tout += (int64_t)g * 128 * 128;
+  }
   __shared__ T Tt[LDS_T ? 10 : 1][32][33];  // block (a, c), a <= c, at c (c + 1) / 2 + a
   __shared__ T Y[3][32][33];
   __shared__ T Z[3][32][33];
@@ -596,10 +605,17 @@ __global__ void __launch_bounds__(256) wy_merge_kernel(const T* S, const T* Tp, 
 }
 template <typename T> void wy_merge(const T* S, const T* Tp, int np, int kg, T* Tout, hipStream_t s) {
   if (np <= 0 || np > 4 || kg <= 0) return;
-  hipLaunchKernelGGL((wy_merge_kernel<T>), dim3(1), dim3(256), 0, s, S, Tp, np, kg, Tout);
+  hipLaunchKernelGGL((wy_merge_kernel<T>), dim3(1), dim3(256), 0, s, S, Tp, np, kg, Tout, 0, 0);
+}
+template <typename T> void wy_merge_groups(const T* S, const T* Tp, int npan, int minmn, T* Tout, hipStream_t s) {
+  const int ngr = cdiv(npan, 4);
+  if (ngr <= 0) return;
+  hipLaunchKernelGGL((wy_merge_kernel<T>), dim3(ngr), dim3(256), 0, s, S, Tp, 0, 0, Tout, npan, minmn);
 }
 template void wy_merge<double>(const double*, const double*, int, int, double*, hipStream_t);
 template void wy_merge<zd>(const zd*, const zd*, int, int, zd*, hipStream_t);
+template void wy_merge_groups<double>(const double*, const double*, int, int, double*, hipStream_t);
+template void wy_merge_groups<zd>(const zd*, const zd*, int, int, zd*, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Elementwise helpers

@@ -177,6 +177,11 @@ template <typename T> void sum_slices(const T* P, int ks, int64_t slice, int M, 
 // T (kg x kg, ld kg) of the block reflector of np <= 4 consecutive 32-column
 // panels, from S = V^H V (kg x kg, ld kg) and the panels' 32 x 32 T blocks.
 template <typename T> void wy_merge(const T* S, const T* Tp, int np, int kg, T* Tout, hipStream_t s);
+// Every group of four panels at once (workgroup g: panels 4g .. 4g + 3 of
+// npan, kg = min(minmn - 128 g, 128) columns): S and Tout hold one 128 x 128
+// slot per group (ld kg), Tp the panels' 32 x 32 T blocks; groups of one
+// panel are copied through.
+template <typename T> void wy_merge_groups(const T* S, const T* Tp, int npan, int minmn, T* Tout, hipStream_t s);
 
 // ---------------------------------------------------------------- elementwise
 template <typename T> void set_identity(void* Q, int64_t ldq, int m, int n, hipStream_t s);

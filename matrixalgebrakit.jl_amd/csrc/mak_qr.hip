@@ -133,12 +133,15 @@ template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
   L.offW2 = off;
   off += align256(L.w2Elems * sizeof(T));
   if (minmn > NB) {
+    // V of every panel (unit lower), then per group of kGroup panels S = V^H V
+    // and the merged T, so all groups' T are merged in one launch
+    const size_t ngr = (size_t)cdiv(minmn, kGB);
     L.offVc = off;
-    off += align256((size_t)m * kGB * sizeof(T));
+    off += align256((size_t)m * ngr * kGB * sizeof(T));
     L.offS = off;
-    off += align256((size_t)kGB * kGB * sizeof(T));
+    off += align256(ngr * kGB * kGB * sizeof(T));
     L.offTg = off;
-    off += align256((size_t)kGB * kGB * sizeof(T));
+    off += align256(ngr * kGB * kGB * sizeof(T));
     L.offWg = off;
     off += align256((size_t)kGB * ncmax * sizeof(T));
   }
@@ -362,35 +365,33 @@ void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipS
   if (geqrf_enqueue<T>(A, lda, m, n, ws, qcols, s)) geqrf_finish<T>(A, lda, m, n, ws, qcols, s);
 }
 
-// C (rows x ncols) <- C - V op(Tg) (V^H C) for the block reflector of panels
-// p0 .. p0 + np - 1 (kg columns of compact WY factors at Vp, their T blocks at
-// Tp): V is copied out unit lower, S = V^H V, Tg from S and the T blocks,
-// then three GEMMs with K = rows, kg, kg.
+// out (kg x N, ld kg) = V^H X for V = Vc (rows x kg, ld ldv): split-K
+// product into the Wp slices, then their sum.
 template <typename T>
-void apply_block_wy(const T* Vp, int64_t ldv, int rows, int kg, int np, const T* Tp, int transT, T* C,
-                    int64_t ldc, int ncols, char* ws, const QrLayout<T>& L, hipStream_t s) {
-  if (ncols <= 0 || rows <= 0) return;
-  T* Vc = reinterpret_cast<T*>(ws + L.offVc);
-  T* S = reinterpret_cast<T*>(ws + L.offS);
-  T* Tg = reinterpret_cast<T*>(ws + L.offTg);
-  T* Wg = reinterpret_cast<T*>(ws + L.offWg);
+void vh_times(const T* Vc, int64_t ldv, int rows, int kg, const T* X, int64_t ldx, int N, T* out, char* ws,
+              const QrLayout<T>& L, hipStream_t s) {
   T* Wp = reinterpret_cast<T*>(ws + L.offWp);
+  int ks, kc;
+  splitk_plan(N, rows, 128, ks, kc);
+  GemmArgs g{};
+  g.M = kg; g.N = N; g.K = rows;
+  g.A = Vc; g.lda = ldv; g.B = X; g.ldb = ldx; g.C = Wp; g.ldc = kg;
+  g.alpha_re = 1.0; g.beta_re = 0.0;
+  g.ksplit = ks; g.kchunk = kc; g.splitStride = (int64_t)kg * N; g.batch = 1;
+  gemm<T>(OP_C, OP_N, g, s);
+  sum_slices<T>(Wp, ks, (int64_t)kg * N, kg, N, out, kg, s);
+}
+
+// C (rows x ncols) <- C - V op(Tg) (V^H C) for a block reflector with V = Vc
+// (unit lower, rows x kg, ld ldv) and its kg x kg T (ld kg): three GEMMs with
+// K = rows, kg, kg.
+template <typename T>
+void apply_block_wy(const T* Vc, int64_t ldv, int rows, int kg, const T* Tg, int transT, T* C, int64_t ldc,
+                    int ncols, char* ws, const QrLayout<T>& L, hipStream_t s) {
+  if (ncols <= 0 || rows <= 0) return;
+  T* Wg = reinterpret_cast<T*>(ws + L.offWg);
   T* W2 = reinterpret_cast<T*>(ws + L.offW2);
-  copy_unit_lower<T>(Vp, ldv, rows, kg, Vc, rows, s);
-  auto vhx = [&](const T* X, int64_t ldx, int N, T* out) {  // out (kg x N, ld kg) = V^H X
-    int ks, kc;
-    splitk_plan(N, rows, 128, ks, kc);
-    GemmArgs g{};
-    g.M = kg; g.N = N; g.K = rows;
-    g.A = Vc; g.lda = rows; g.B = X; g.ldb = ldx; g.C = Wp; g.ldc = kg;
-    g.alpha_re = 1.0; g.beta_re = 0.0;
-    g.ksplit = ks; g.kchunk = kc; g.splitStride = (int64_t)kg * N; g.batch = 1;
-    gemm<T>(OP_C, OP_N, g, s);
-    sum_slices<T>(Wp, ks, (int64_t)kg * N, kg, N, out, kg, s);
-  };
-  vhx(Vc, rows, kg, S);
-  wy_merge<T>(S, Tp, np, kg, Tg, s);
-  vhx(C, ldc, ncols, Wg);
+  vh_times<T>(Vc, ldv, rows, kg, C, ldc, ncols, Wg, ws, L, s);
   {
     GemmArgs g{};
     g.M = kg; g.N = ncols; g.K = kg;
@@ -402,31 +403,51 @@ void apply_block_wy(const T* Vp, int64_t ldv, int rows, int kg, int np, const T*
   {
     GemmArgs g{};
     g.M = rows; g.N = ncols; g.K = kg;
-    g.A = Vc; g.lda = rows; g.B = W2; g.ldb = kg; g.C = C; g.ldc = ldc;
+    g.A = Vc; g.lda = ldv; g.B = W2; g.ldb = kg; g.C = C; g.ldc = ldc;
     g.alpha_re = -1.0; g.beta_re = 1.0;
     g.ksplit = 1; g.kchunk = kg; g.batch = 1;
     gemm<T>(OP_N, OP_N, g, s);
   }
 }
 
+// Q = H_0 ... H_{npan-1} [I; 0], applied backwards in groups of kGroup panels
+// (block reflectors of kGB columns).  Before the backward sweep: V of every
+// panel is copied out unit lower in one launch, S = V^H V per group, and the
+// T of every group is merged in one launch.
 template <typename T>
 void orgqr_blocked(const T* A, int64_t lda, int m, int n, T* Q, int64_t ldq, int qcols, char* ws, hipStream_t s) {
   const int minmn = std::min(m, n);
   QrLayout<T> L = qr_layout<T>(m, n, qcols);
   set_identity<T>(Q, ldq, m, qcols, s);
   const int npan = cdiv(minmn, NB);
+  const int ngr = cdiv(npan, kGroup);
   const T* Tpanels = reinterpret_cast<const T*>(ws + L.offTpanels);
-  // Q = H_0 ... H_{npan-1} [I; 0], applied backwards in groups of kGroup panels
-  for (int g = cdiv(npan, kGroup) - 1; g >= 0; --g) {
+  T* Vc = reinterpret_cast<T*>(ws + L.offVc);
+  T* Sg = reinterpret_cast<T*>(ws + L.offS);
+  T* Tg = reinterpret_cast<T*>(ws + L.offTg);
+  if (npan > 1) {
+    copy_unit_lower<T>(A, lda, m, minmn, Vc, m, s);
+    for (int g = 0; g < ngr; ++g) {
+      const int p0 = g * kGroup, np = std::min(kGroup, npan - p0);
+      if (np < 2) continue;
+      const int j0 = p0 * NB;
+      const int kg = std::min(minmn, (p0 + np) * NB) - j0;
+      const T* Vg = Vc + j0 + (int64_t)j0 * m;
+      vh_times<T>(Vg, m, m - j0, kg, Vg, m, kg, Sg + (size_t)g * kGB * kGB, ws, L, s);
+    }
+    wy_merge_groups<T>(Sg, Tpanels, npan, minmn, Tg, s);
+  }
+  for (int g = ngr - 1; g >= 0; --g) {
     const int p0 = g * kGroup, np = std::min(kGroup, npan - p0);
     const int j0 = p0 * NB;
     const int kg = std::min(minmn, (p0 + np) * NB) - j0;
-    const T* Vp = A + j0 + (int64_t)j0 * lda;
     T* C = Q + j0 + (int64_t)j0 * ldq;
     if (np == 1)
-      apply_wy<T>(Vp, lda, m - j0, kg, Tpanels + (size_t)p0 * NB * NB, 0, C, ldq, qcols - j0, ws, L, s);
+      apply_wy<T>(A + j0 + (int64_t)j0 * lda, lda, m - j0, kg, Tpanels + (size_t)p0 * NB * NB, 0, C, ldq,
+                  qcols - j0, ws, L, s);
     else
-      apply_block_wy<T>(Vp, lda, m - j0, kg, np, Tpanels + (size_t)p0 * NB * NB, 0, C, ldq, qcols - j0, ws, L, s);
+      apply_block_wy<T>(Vc + j0 + (int64_t)j0 * m, m, m - j0, kg, Tg + (size_t)g * kGB * kGB, 0, C, ldq,
+                        qcols - j0, ws, L, s);
   }
 }
 
