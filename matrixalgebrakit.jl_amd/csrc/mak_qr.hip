@@ -427,7 +427,24 @@ void orgqr_blocked(const T* A, int64_t lda, int m, int n, T* Q, int64_t ldq, int
   T* Tg = reinterpret_cast<T*>(ws + L.offTg);
   if (npan > 1) {
     copy_unit_lower<T>(A, lda, m, minmn, Vc, m, s);
-    for (int g = 0; g < ngr; ++g) {
+    // the full groups' S in one batched split-K product over all m rows (the
+    // rows above a group's first column are zero in Vc)
+    const int nfull = minmn / kGB;
+    if (nfull > 0) {
+      int ks, kc;
+      splitk_plan(kGB * nfull, m, 128, ks, kc);
+      T* Wp = reinterpret_cast<T*>(ws + L.offWp);
+      GemmArgs g{};
+      g.M = kGB; g.N = kGB; g.K = m;
+      g.A = Vc; g.lda = m; g.strideA = (int64_t)kGB * m;
+      g.B = Vc; g.ldb = m; g.strideB = (int64_t)kGB * m;
+      g.C = Wp; g.ldc = kGB; g.strideC = (int64_t)kGB * kGB;
+      g.alpha_re = 1.0; g.beta_re = 0.0;
+      g.ksplit = ks; g.kchunk = kc; g.splitStride = (int64_t)kGB * kGB * nfull; g.batch = nfull;
+      gemm<T>(OP_C, OP_N, g, s);
+      sum_slices<T>(Wp, ks, (int64_t)kGB * kGB * nfull, kGB, kGB * nfull, Sg, kGB, s);
+    }
+    for (int g = nfull; g < ngr; ++g) {
       const int p0 = g * kGroup, np = std::min(kGroup, npan - p0);
       if (np < 2) continue;
       const int j0 = p0 * NB;
