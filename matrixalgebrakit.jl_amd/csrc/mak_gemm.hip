@@ -402,47 +402,44 @@ template <typename T>
 __global__ void __launch_bounds__(256) wy_reduce_kernel(const T* __restrict__ Wp, int ksplit, int64_t slice,
                                                         const T* __restrict__ Tm, int transT, int nbc,
                                                         int ncols, T* __restrict__ W2, const int* gate) {
-  // 16 columns per workgroup; thread (r, cc) sums the ksplit partials of
-  // two outputs with four independent accumulators each.
-  __shared__ T S[32][17];
+  // 8 columns per workgroup; thread (r, cc) sums the ksplit partials of one
+  // output with eight independent accumulators (the loads of eight slices in
+  // flight at once).
+  __shared__ T S[32][9];
   __shared__ T Tl[32][33];
   if (gate_closed(gate, 0)) return;
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * 16;
+  const int c0 = blockIdx.x * 8;
   for (int i = tid; i < 32 * 32; i += 256) {
     const int r = i % 32, c = i / 32;
     T v = s_zero(T());
     if (r < nbc && c < nbc) v = transT ? s_conj(Tm[c + r * 32]) : Tm[r + c * 32];
     Tl[r][c] = v;
   }
-  const int r = tid % 32;
+  const int r = tid % 32, cc = tid / 32;
+  const int c = c0 + cc;
+  {
+    T acc[8];
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int cc = tid / 32 + 8 * q;
-    const int c = c0 + cc;
-    T acc[4] = {s_zero(T()), s_zero(T()), s_zero(T()), s_zero(T())};
+    for (int u = 0; u < 8; ++u) acc[u] = s_zero(T());
     if (r < nbc && c < ncols) {
       const T* p = Wp + r + (int64_t)c * nbc;
       int z = 0;
-      for (; z + 4 <= ksplit; z += 4) {
+      for (; z + 8 <= ksplit; z += 8) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc[u] = s_add(acc[u], p[(int64_t)(z + u) * slice]);
+        for (int u = 0; u < 8; ++u) acc[u] = s_add(acc[u], p[(int64_t)(z + u) * slice]);
       }
-      for (; z < ksplit; ++z) acc[0] = s_add(acc[0], p[(int64_t)z * slice]);
+      for (; z < ksplit; ++z) acc[z & 7] = s_add(acc[z & 7], p[(int64_t)z * slice]);
     }
-    S[r][cc] = s_add(s_add(acc[0], acc[1]), s_add(acc[2], acc[3]));
+    S[r][cc] = s_add(s_add(s_add(acc[0], acc[1]), s_add(acc[2], acc[3])),
+                     s_add(s_add(acc[4], acc[5]), s_add(acc[6], acc[7])));
   }
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int cc = tid / 32 + 8 * q;
-    const int c = c0 + cc;
-    if (r < nbc && c < ncols) {
-      T s = s_zero(T());
+  if (r < nbc && c < ncols) {
+    T v = s_zero(T());
 #pragma unroll 8
-      for (int k = 0; k < 32; ++k) s = s_fma(Tl[r][k], S[k][cc], s);
-      W2[r + (int64_t)c * nbc] = s;
-    }
+    for (int k = 0; k < 32; ++k) v = s_fma(Tl[r][k], S[k][cc], v);
+    W2[r + (int64_t)c * nbc] = v;
   }
 }
 
@@ -450,7 +447,7 @@ template <typename T>
 void wy_reduce(const void* Wp, int ksplit, int64_t slice, const void* Tm, int transT, int nbc, int ncols,
                void* W2, hipStream_t s, const int* gate) {
   if (ncols <= 0) return;
-  hipLaunchKernelGGL((wy_reduce_kernel<T>), dim3(cdiv(ncols, 16)), dim3(256), 0, s,
+  hipLaunchKernelGGL((wy_reduce_kernel<T>), dim3(cdiv(ncols, 8)), dim3(256), 0, s,
                      reinterpret_cast<const T*>(Wp), ksplit, slice, reinterpret_cast<const T*>(Tm), transT,
                      nbc, ncols, reinterpret_cast<T*>(W2), gate);
 }
