@@ -24,7 +24,8 @@
 // the same rows j0.. (before panel k's trailing update).  The update applies
 // the unitary H_k^H to those rows, which leaves Xn^H Xn unchanged, so the next
 // panel's first Gram (rows j0 + 32..) is Gw - Rt^H Rt, Rt the 32 rows of its
-// columns that become R (carried_chol_kernel: no pass over the panel).  The
+// columns that become R (formed and factored at the start of the next finish
+// kernel: no pass over the panel and no launch of its own).  The
 // difference loses accuracy when Xn lies mostly in the span of panel k; R1
 // then only preconditions worse, and the second pass's check still decides
 // whether the panel is accepted.
@@ -58,10 +59,18 @@ enum GramMode { kPlain = 0, kTimesB = 1 };
 // Wave w accumulates the 16 x 16 block (w % 2, w / 2) of G in (cr, ci).
 // With Xn (not null), the same row tiles of Xn (H x nbcn, ld ldx) are staged
 // in Xns alongside and Gn += Xn^H Xn accumulates in (nr, ni).
-template <typename T, int MODE>
+struct NoPre {
+  __device__ void operator()() const {}
+};
+
+// pre() runs once, after the first tile's loads are issued and before the
+// tile is staged (it may use the LDS of Xs and Bs and must end with a
+// workgroup barrier).
+template <typename T, int MODE, typename Pre = NoPre>
 __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*Xs)[NB + 1], const T (*Bs)[NB + 1],
                                   T* Yout, v4d& cr, v4d& ci, const T* Xn = nullptr, int nbcn = 0,
-                                  T (*Xns)[NB + 1] = nullptr, v4d* nr = nullptr, v4d* ni = nullptr) {
+                                  T (*Xns)[NB + 1] = nullptr, v4d* nr = nullptr, v4d* ni = nullptr,
+                                  Pre pre = Pre()) {
   constexpr int TR = CqTile<T>::TR;
   constexpr int PER = TR * NB / 256;
   constexpr bool CPLX = ScalarTraits<T>::is_complex;
@@ -93,6 +102,7 @@ __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*X
   };
   int t = blockIdx.x;
   if (t < ntile) load(t);
+  pre();
   for (; t < ntile; t += gridDim.x) {
     __syncthreads();
 #pragma unroll
@@ -318,46 +328,6 @@ __global__ void __launch_bounds__(256) gram_chol_kernel(const T* X, int64_t ldx,
   }
 }
 
-// Pass 1 from a carried Gram: G1 = Gw - Rt^H Rt (Gw: 32 x 32, ld 32; Rt: the
-// 32 rows above the panel, ld lda), then as gram_chol_kernel in sticky mode.
-// One workgroup.
-template <typename T>
-__global__ void __launch_bounds__(256) carried_chol_kernel(const T* Gw, const T* Rt, int64_t lda, int nbc,
-                                                           int* status, int j0, T* R1, T* R1inv) {
-  __shared__ T G[NB][NB + 1], Rs[NB][NB + 1], Yl[NB][NB + 1];
-  __shared__ int bad_s;
-  if (status[0] != 0) return;  // an earlier panel was rejected: the flag stays set
-  const int tid = threadIdx.x;
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int i = e % NB, j = e / NB;
-    G[i][j] = Gw[e];
-    Rs[i][j] = (j < nbc) ? Rt[i + (int64_t)j * lda] : s_zero(T());
-  }
-  __syncthreads();
-  mm32<T>(G, Rs, true, Rs, false, -1.0, true);  // G -= Rt^H Rt
-  for (int e = tid; e < NB * NB; e += 256) {    // identity outside nbc x nbc
-    const int i = e % NB, j = e / NB;
-    if (i >= nbc || j >= nbc) G[i][j] = (i == j) ? s_one(T()) : s_zero(T());
-  }
-  __syncthreads();
-  if ((tid >> 6) == 0) {
-    const int b = wave_chol_inv32<T>(G, Yl);
-    if (tid == 0) bad_s = b;
-  }
-  __syncthreads();
-  if (bad_s != 0) {
-    if (tid == 0) {
-      status[2] = j0;
-      status[0] = 1;
-    }
-    return;
-  }
-  for (int e = tid; e < NB * NB; e += 256) {
-    R1[e] = G[e % NB][e / NB];
-    R1inv[e] = Yl[e % NB][e / NB];
-  }
-}
-
 // In-wave LU of an N x N block without pivoting, with the sign choice
 // d_j = -sign(M(j, j)) subtracted from each pivot first (|pivot| >= 1 for the
 // top of an orthonormal Q).  Lane c keeps column c: U(r, c) for r <= c and,
@@ -490,11 +460,18 @@ __device__ __forceinline__ void lu32_recon(T (*Qt)[33], T (*Ls)[33], T* dv, T (*
 // the diagonal, L below).  T goes to Tout (32 x 32, zero outside nbc x nbc).
 // With Xn (not null) it also forms Gw = Xn^H Xn (H x nbcn, ld ldx; to Gw, 32 x 32,
 // zero outside nbcn x nbcn) through partialN, before the check.
+// With carried, pass 1 runs here instead of in a kernel of its own: every
+// workgroup forms G1 = Gw - Rt^H Rt (Gw as the previous panel's call left it,
+// Rt the 32 rows above X) and factors it, redundantly and identically, while
+// its first tile's loads are in flight; workgroup 0 writes R1 for the last
+// workgroup (published by the ticket).  A G1 that is not positive definite
+// sets the flag (and status[2] = j0) and every workgroup returns.
 template <typename T>
-__global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t ldx, int H, int nbc, const T* R1,
+__global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t ldx, int H, int nbc, T* R1,
                                                              const T* R1inv, T* Q1, T* partial, int* status,
                                                              int j0, T* Tout, T* Mout, T* Ap, int64_t lda,
-                                                             const T* Xn, int nbcn, T* partialN, T* Gw) {
+                                                             const T* Xn, int nbcn, T* partialN, T* Gw,
+                                                             int carried) {
   constexpr int TR = CqTile<T>::TR;
   constexpr int XM = TR / NB;  // 32-row matrices spanned by the staged tile
   // the staged tiles of X and Xn, R1^{-1}; at least the 8 matrices of the end
@@ -511,11 +488,44 @@ __global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t
   const int c = lane & 31;
   {
     T (*Bs)[NB + 1] = pool[XM];
-    for (int e = tid; e < NB * NB; e += 256) Bs[e % NB][e / NB] = R1inv[e];
-    // (gram_tiles begins each tile with a workgroup barrier)
+    if (!carried)
+      for (int e = tid; e < NB * NB; e += 256) Bs[e % NB][e / NB] = R1inv[e];
+    auto chol1 = [&]() {
+      if (!carried) return;
+      T (*G1)[NB + 1] = pool[0];
+      T (*Rs)[NB + 1] = pool[1];
+      for (int e = tid; e < NB * NB; e += 256) {
+        const int i = e % NB, j = e / NB;
+        G1[i][j] = Gw[e];
+        Rs[i][j] = (j < nbc) ? X[i - NB + (int64_t)j * ldx] : s_zero(T());
+      }
+      __syncthreads();
+      mm32<T>(G1, Rs, true, Rs, false, -1.0, true);  // G1 = Gw - Rt^H Rt
+      for (int e = tid; e < NB * NB; e += 256) {    // identity outside nbc x nbc
+        const int i = e % NB, j = e / NB;
+        if (i >= nbc || j >= nbc) G1[i][j] = (i == j) ? s_one(T()) : s_zero(T());
+      }
+      __syncthreads();
+      if (w == 0) {
+        const int b = wave_chol_inv32<T>(G1, Bs);
+        if (lane == 0) bad_s = b;
+      }
+      __syncthreads();
+      if (bad_s == 0 && blockIdx.x == 0)
+        for (int e = tid; e < NB * NB; e += 256) R1[e] = G1[e % NB][e / NB];
+      __syncthreads();
+    };
     v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
     v4d nr = {0.0, 0.0, 0.0, 0.0}, ni = {0.0, 0.0, 0.0, 0.0};
-    gram_tiles<T, kTimesB>(X, ldx, H, nbc, pool[0], Bs, Q1, cr, ci, Xn, nbcn, pool[XM + 1], &nr, &ni);
+    // (gram_tiles begins each tile with a workgroup barrier)
+    gram_tiles<T, kTimesB>(X, ldx, H, nbc, pool[0], Bs, Q1, cr, ci, Xn, nbcn, pool[XM + 1], &nr, &ni, chol1);
+    if (carried && bad_s != 0) {
+      if (tid == 0 && blockIdx.x == 0) {
+        status[2] = j0;
+        status[0] = 1;
+      }
+      return;
+    }
     if (Xn != nullptr) store_partial<T>(partialN, nr, ni);
     if (!partial_is_last<T>(partial, cr, ci, counter, &last_s)) return;
   }
@@ -615,16 +625,12 @@ void cholqr_panel(T* Ap, int64_t lda, int H, int nbc, T* Tpan, T* cw, int* statu
   T* M = R1i + NB * NB;
   T* Q1 = M + NB * NB;
   const int nblk = std::max(1, std::min(kMaxGramBlocks, cdiv(H, 256)));
-  if (carried) {
-    hipLaunchKernelGGL((carried_chol_kernel<T>), dim3(1), dim3(256), 0, s, (const T*)Gw, (const T*)(Ap - NB), lda,
-                       nbc, status, j0, R1, R1i);
-  } else {
+  if (!carried)
     hipLaunchKernelGGL((gram_chol_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc, part,
                        status, j0, sticky, R1, R1i);
-  }
   const T* Xn = nbc_next > 0 ? (const T*)(Ap + (int64_t)nbc * lda) : nullptr;
-  hipLaunchKernelGGL((cholqr2_finish_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc,
-                     (const T*)R1, (const T*)R1i, Q1, part, status, j0, Tpan, M, Ap, lda, Xn, nbc_next, partN, Gw);
+  hipLaunchKernelGGL((cholqr2_finish_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc, R1,
+                     (const T*)R1i, Q1, part, status, j0, Tpan, M, Ap, lda, Xn, nbc_next, partN, Gw, carried);
   if (H > nbc) gemm<T>(OP_N, OP_N, gg<T>(H - nbc, nbc, nbc, Q1 + nbc, H, M, NB, Ap + nbc, lda, status), s);
 }
 
