@@ -659,14 +659,24 @@ void cholqr_panel(T* Ap, int64_t lda, int H, int nbc, T* Tpan, T* cw, int* statu
   T* M = R1i + NB * NB;
   T* Q1 = M + NB * NB;
   const int nblk = std::max(1, std::min(kMaxGramBlocks, cdiv(H, 256)));
-  if (!carried)
+  if (carried == 0)
     hipLaunchKernelGGL((gram_chol_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc, part,
                        status, j0, sticky, R1, R1i);
   const T* Xn = nbc_next > 0 ? (const T*)(Ap + (int64_t)nbc * lda) : nullptr;
   hipLaunchKernelGGL((cholqr2_finish_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc, R1,
-                     (const T*)R1i, Q1, part, status, j0, Tpan, M, Ap, lda, Xn, nbc_next, partN, Gw, carried);
+                     (const T*)R1i, Q1, part, status, j0, Tpan, M, Ap, lda, Xn, nbc_next, partN, Gw,
+                     carried == 1 ? 1 : 0);
   if (H > nbc) gemm<T>(OP_N, OP_N, gg<T>(H - nbc, nbc, nbc, Q1 + nbc, H, M, NB, Ap + nbc, lda, status), s);
 }
+
+template <typename T> void cholqr_carry_buffers(T* cw, T** Gw, T** R1, T** R1inv) {
+  *Gw = cw;
+  T* part = cw + NB * NB + (size_t)kMaxGramBlocks * NB * NB;
+  *R1 = part + (size_t)kMaxGramBlocks * NB * NB;
+  *R1inv = *R1 + NB * NB;
+}
+template void cholqr_carry_buffers<double>(double*, double**, double**, double**);
+template void cholqr_carry_buffers<zd>(zd*, zd**, zd**, zd**);
 
 template void cholqr_panel<double>(double*, int64_t, int, int, double*, double*, int*, int, int, int, int,
                                    hipStream_t);

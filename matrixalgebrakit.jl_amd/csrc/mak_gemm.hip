@@ -72,9 +72,56 @@ __device__ inline void tri_k_range(int tri, int m0, int bm, int n0, int bn, int&
   if (tri & TRI_B_LOWER) kbeg = max(kbeg, n0);
 }
 
+// The carried first Gram (GemmArgs::cGw) in the workgroup of tile (0, 0),
+// after its C stores: Rt and G1 in the LDS of the operand tiles (each at least
+// 32 x 33 doubles), W^{-1} over Rt once G1 is formed.
+__device__ inline void carry_chol(const GemmArgs& g, double* lds_a, double* lds_b) {
+  __syncthreads();  // this workgroup's C stores are complete and visible to it
+  const int* status = g.cStatus;
+  if (status[0] != 0) return;
+  const int tid = threadIdx.x;
+  const int nbc = g.cNbc;
+  double (*Rt)[33] = reinterpret_cast<double (*)[33]>(lds_a);
+  double (*G)[33] = reinterpret_cast<double (*)[33]>(lds_b);
+  const double* C = reinterpret_cast<const double*>(g.C);
+  const double* Gw = reinterpret_cast<const double*>(g.cGw);
+  for (int e = tid; e < 1024; e += 256) {
+    const int i = e % 32, j = e / 32;
+    Rt[i][j] = (j < nbc) ? C[i + (int64_t)j * g.ldc] : 0.0;
+  }
+  __syncthreads();
+  for (int e = tid; e < 1024; e += 256) {
+    const int i = e % 32, j = e / 32;
+    double v = Gw[e];
+#pragma unroll 8
+    for (int k = 0; k < 32; ++k) v = fma(-Rt[k][i], Rt[k][j], v);
+    G[i][j] = (i < nbc && j < nbc) ? v : (i == j ? 1.0 : 0.0);
+  }
+  __shared__ int bad_s;
+  __syncthreads();
+  if ((tid >> 6) == 0) {
+    const int b = wave_chol_inv32<double>(G, Rt);
+    if (tid == 0) bad_s = b;
+  }
+  __syncthreads();
+  if (bad_s != 0) {
+    if (tid == 0) {
+      g.cStatus[2] = g.cJ0;
+      g.cStatus[0] = 1;
+    }
+    return;
+  }
+  double* R1 = reinterpret_cast<double*>(g.cR1);
+  double* R1i = reinterpret_cast<double*>(g.cR1i);
+  for (int e = tid; e < 1024; e += 256) {
+    R1[e] = G[e % 32][e / 32];
+    R1i[e] = Rt[e % 32][e / 32];
+  }
+}
+
 // Real FP64 GEMM.  Each thread owns fixed load coordinates; the next K-tile is
 // fetched into registers while the MFMAs of the current tile run.
-template <int BM, int BN, int BK, int WM, int WN, int OPA, int OPB>
+template <int BM, int BN, int BK, int WM, int WN, int OPA, int OPB, bool CARRY = false>
 __global__ void __launch_bounds__(256, 2) gemm_f64_kernel(GemmArgs g) {
   constexpr int WTM = BM / WM, WTN = BN / WN, TM = WTM / 16, TN = WTN / 16;
   constexpr int NA = BM * BK / 256, NB_ = BN * BK / 256;
@@ -203,6 +250,11 @@ __global__ void __launch_bounds__(256, 2) gemm_f64_kernel(GemmArgs g) {
           if (row < g.M && col < g.N) C[row + (int64_t)col * g.ldc] = alpha * acc[i][j][r];
         }
   }
+  if constexpr (CARRY) {
+    static_assert(BK * (BM + 2) >= 32 * 33 && BK * (BN + 2) >= 32 * 33, "carry needs 32 x 33 of LDS per tile");
+    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && g.cGw != nullptr)
+      carry_chol(g, &As[0][0], &Bs[0][0]);
+  }
 }
 
 template <int BM, int BN, int BK, int WM, int WN, int OPA, int OPB>
@@ -316,6 +368,11 @@ static void launch_gemm_tile(const GemmArgs& g, hipStream_t s) {
   else
     hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, BK, WM, WN, OPA, OPB>), grid, dim3(256), 0, s, g);
 }
+template <int BM, int BN>
+static void launch_gemm_carry(const GemmArgs& g, hipStream_t s) {
+  dim3 grid(cdiv(g.M, BM), cdiv(g.N, BN), 1);
+  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, 16, 2, 2, OP_N, OP_N, true>), grid, dim3(256), 0, s, g);
+}
 
 // The largest tile that still gives the launch enough workgroups to cover the
 // chip (256 CUs): 128x128 (real) / 64x64 while that yields >= 192 / 128
@@ -335,6 +392,14 @@ static void gemm_shape(const GemmArgs& g, hipStream_t s) {
     return t * slabs;
   };
   const bool hinted = g.tri != 0 || g.upperC != 0;
+  if constexpr (!cplx && OPA == OP_N && OPB == OP_N) {
+    if (g.cGw != nullptr) {  // QR trailing update carrying the next panel's first Gram: tiles of >= 64
+      if (g.K <= 128 && wgs(64, 64) >= 1024) launch_gemm_carry<64, 64>(g, s);
+      else if (wgs(128, 128) >= 192) launch_gemm_carry<128, 128>(g, s);
+      else launch_gemm_carry<64, 64>(g, s);
+      return;
+    }
+  }
   if (g.M <= 32) {
     launch_gemm_tile<32, 64, 16, 2, 2, OPA, OPB, T>(g, s);
   } else if (!cplx && hinted && busy(128) < 1024 && busy(64) >= 512) {

@@ -127,6 +127,17 @@ struct GemmArgs {
   int upperC;
   const int* gate;        // run only if *gate == gate_want (when non-null)
   int gate_want;
+  // Carried first Gram of the next QR panel (real C = C - op(A) op(B) with
+  // op N, N): when cGw is not null, the workgroup of tile (0, 0) also forms
+  // G1 = cGw - Rt^H Rt from Rt = C(0:32, 0:cNbc) after its update (rows /
+  // columns beyond cNbc: identity), factors G1 = W^H W and writes W and W^{-1}
+  // (32 x 32, ld 32) to cR1 / cR1i -- or, when G1 is not positive definite,
+  // sets cStatus[2] = cJ0 and cStatus[0] = 1.  Skipped when cStatus[0] is set.
+  const void* cGw;
+  void* cR1;
+  void* cR1i;
+  int* cStatus;
+  int cJ0, cNbc;
 };
 
 // C = alpha op(A) op(B) + beta C on FP64 MFMA (real) or 4 real MFMAs (complex).

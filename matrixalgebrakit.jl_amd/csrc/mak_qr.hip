@@ -247,8 +247,10 @@ void factor_panel(T* A, int64_t lda, int j0, int m, int nbc, char* ws, const QrL
 
 // C (rows x ncols) <- C - V op(T) (V^H C); V is the unit-lower panel at Vp.
 template <typename T>
+// carry_nbc > 0 (real data, host-checked CholeskyQR2): the update also forms
+// the next panel's R1 from its carried Gram (GemmArgs::cGw).
 void apply_wy(const T* Vp, int64_t ldv, int rows, int nbc, const T* Tm, int transT, T* C, int64_t ldc,
-              int ncols, char* ws, const QrLayout<T>& L, hipStream_t s) {
+              int ncols, char* ws, const QrLayout<T>& L, hipStream_t s, int carry_nbc = 0, int carry_j0 = 0) {
   if (ncols <= 0 || rows <= 0) return;
   T* Wp = reinterpret_cast<T*>(ws + L.offWp);
   T* W2 = reinterpret_cast<T*>(ws + L.offW2);
@@ -270,6 +272,13 @@ void apply_wy(const T* Vp, int64_t ldv, int rows, int nbc, const T* Tm, int tran
     g.A = Vp; g.lda = ldv; g.B = W2; g.ldb = nbc; g.C = C; g.ldc = ldc;
     g.alpha_re = -1.0; g.beta_re = 1.0;
     g.ksplit = 1; g.kchunk = nbc; g.batch = 1; g.unitLowerA = 1;
+    if (carry_nbc > 0 && !ScalarTraits<T>::is_complex) {
+      T *Gw, *R1, *R1i;
+      cholqr_carry_buffers<T>(reinterpret_cast<T*>(ws + L.offCq), &Gw, &R1, &R1i);
+      g.cGw = Gw; g.cR1 = R1; g.cR1i = R1i;
+      g.cStatus = reinterpret_cast<int*>(ws + L.offFlags);
+      g.cJ0 = carry_j0; g.cNbc = carry_nbc;
+    }
     gemm<T>(OP_N, OP_N, g, s);
   }
 }
@@ -297,12 +306,14 @@ void geqrf_panels(T* A, int64_t lda, int m, int n, char* ws, const QrLayout<T>& 
     const int nbc_next = (chol && m - j0 >= kCholQrMinRows && j0 + nbc < minmn && m - j0 - nbc >= kCholQrMinRows)
                              ? std::min(NB, minmn - j0 - nbc) : 0;
     factor_panel<T>(A, lda, j0, m, nbc, ws, L, mode, s, carried, nbc_next);
-    carried = nbc_next > 0 ? 1 : 0;
+    // real data: the trailing update also factors the next panel's carried
+    // Gram (2); complex: the next finish kernel does (1)
+    carried = nbc_next > 0 ? (ScalarTraits<T>::is_complex ? 1 : 2) : 0;
     const int nc = n - j0 - nbc;
     if (nc > 0) {
       const T* Tpan = reinterpret_cast<const T*>(ws + L.offTpanels) + (size_t)(j0 / NB) * NB * NB;
       apply_wy<T>(A + j0 + (int64_t)j0 * lda, lda, m - j0, nbc, Tpan, 1, A + j0 + (int64_t)(j0 + nbc) * lda, lda,
-                  nc, ws, L, s);
+                  nc, ws, L, s, carried == 2 ? nbc_next : 0, j0 + nbc);
     }
   }
 }
