@@ -393,10 +393,8 @@ static void gemm_shape(const GemmArgs& g, hipStream_t s) {
   };
   const bool hinted = g.tri != 0 || g.upperC != 0;
   if constexpr (!cplx && OPA == OP_N && OPB == OP_N) {
-    if (g.cGw != nullptr) {  // QR trailing update carrying the next panel's first Gram: tiles of >= 64
-      if (g.K <= 128 && wgs(64, 64) >= 1024) launch_gemm_carry<64, 64>(g, s);
-      else if (wgs(128, 128) >= 192) launch_gemm_carry<128, 128>(g, s);
-      else launch_gemm_carry<64, 64>(g, s);
+    if (g.cGw != nullptr) {  // QR trailing update carrying the next panel's first Gram (K = 32)
+      launch_gemm_carry<64, 64>(g, s);
       return;
     }
   }
@@ -406,7 +404,7 @@ static void gemm_shape(const GemmArgs& g, hipStream_t s) {
     // hinted products spread uneven work over the tiles: more, smaller tiles
     // balance the CUs (and avoid a second round of a few 128x128 tiles)
     launch_gemm_tile<64, 64, 16, 2, 2, OPA, OPB, T>(g, s);
-  } else if (!cplx && g.K <= 128 && wgs(64, 64) >= 1024) {
+  } else if (!cplx && g.K <= 128 && wgs(64, 64) >= 512) {
     // short-K updates stream C: smaller tiles keep more loads in flight
     launch_gemm_tile<64, 64, 16, 2, 2, OPA, OPB, T>(g, s);
   } else if (!cplx && wgs(128, 128) >= 192) {
