@@ -134,6 +134,66 @@ __global__ void __launch_bounds__(256) shift_herm_kernel(T* G, int64_t ldg, int 
   }
 }
 
+// Second CholeskyQR pass with G = I + E close to the identity: the Cholesky
+// factor W = I + U and its inverse follow from E to within O(||E||^3):
+// U1 = up(E), U = up(E - U1^H U1), W^{-1} = I - U + U^2, where up() keeps the
+// strict upper part and half the (real) diagonal -- from (I + U)^H (I + U) =
+// I + E.  Only the upper triangle of G (and of Sub) is read.
+template <typename T>
+__global__ void near_identity_sumsq_kernel(const T* G, int64_t ldg, int n, double* partial) {
+  __shared__ double red[4];
+  const int64_t tot = (int64_t)n * n;
+  double acc = 0.0;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * 256) {
+    const int r = (int)(idx % n), c = (int)(idx / n);
+    if (r > c) continue;
+    T v = G[r + (int64_t)c * ldg];
+    if (r == c) v = s_sub(v, s_one(T()));
+    acc += (r < c ? 2.0 : 1.0) * s_abs2(v);
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+__global__ void sum_double_kernel(const double* partial, int nb, double* out) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) acc += partial[i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = red[0] + red[1] + red[2] + red[3];
+}
+// U = up(G - I - Sub) (Sub may be null), zero below the diagonal.
+template <typename T>
+__global__ void up_part_kernel(const T* G, int64_t ldg, const T* Sub, int64_t lds, int n, T* U, int64_t ldu) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * n) return;
+  const int r = (int)(idx % n), c = (int)(idx / n);
+  T u = s_zero(T());
+  if (r <= c) {
+    T v = G[r + (int64_t)c * ldg];
+    if (Sub) v = s_sub(v, Sub[r + (int64_t)c * lds]);
+    if (r == c) u = s_scale(s_add(s_sub(v, s_one(T())), s_conj(s_sub(v, s_one(T())))), 0.25);
+    else u = v;
+  }
+  U[r + (int64_t)c * ldu] = u;
+}
+// Y = I - U + U2 (upper), zero below the diagonal; Y may alias U.
+template <typename T>
+__global__ void inv_near_identity_kernel(T* Y, int64_t ldy, const T* U2, int64_t ldu2, int n) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * n) return;
+  const int r = (int)(idx % n), c = (int)(idx / n);
+  T v = s_zero(T());
+  if (r <= c) {
+    v = s_add(s_neg(Y[r + (int64_t)c * ldy]), U2[r + (int64_t)c * ldu2]);
+    if (r == c) v = s_add(v, s_one(T()));
+  }
+  Y[r + (int64_t)c * ldy] = v;
+}
+
 template <typename T> double read_scalar(const double* d, hipStream_t s) {
   double h = 0.0;
   hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, s);
@@ -196,7 +256,22 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
       // pass 2: G = Q2^H Q2 + Q1^H Q1 = W2^H W2
       gemm<T>(OP_C, OP_N, ga<T>(n, n, n, Y1, ldn, Y1, ldn, G, ldn, 1.0, 0.0, TRI_A_LOWER | TRI_B_UPPER, 1), s);
       gemm<T>(OP_C, OP_N, ga<T>(n, n, m, buf[s2], ldb[s2], buf[s2], ldb[s2], G, ldn, 1.0, 1.0, 0, 1), s);
-      potrf_upper<T>(G, ldn, n, Y2, ldn, tmp, dinfo, s);
+      // W2^{-1} (Y2): in closed form when ||G - I||_F <= 1e-6 (||G - I|| is about
+      // (1 + c) u after the first pass), else by the Cholesky factorisation
+      hipLaunchKernelGGL((near_identity_sumsq_kernel<T>), dim3(256), dim3(256), 0, s, G, ldn, n, part);
+      hipLaunchKernelGGL(sum_double_kernel, dim3(1), dim3(256), 0, s, part, 256, scal + 2);
+      const double e2 = read_scalar<T>(scal + 2, s);
+      if (e2 <= 1e-12) {
+        T* Tm = buf[s1];  // free until Q1' is formed; n x n, ld n
+        const unsigned nb = (unsigned)cdiv64((int64_t)n * n, 256);
+        hipLaunchKernelGGL((up_part_kernel<T>), dim3(nb), dim3(256), 0, s, G, ldn, (const T*)nullptr, ldn, n, Y2, ldn);
+        gemm<T>(OP_C, OP_N, ga<T>(n, n, n, Y2, ldn, Y2, ldn, Tm, ldn, 1.0, 0.0, TRI_A_LOWER | TRI_B_UPPER, 1), s);
+        hipLaunchKernelGGL((up_part_kernel<T>), dim3(nb), dim3(256), 0, s, G, ldn, (const T*)Tm, ldn, n, Y2, ldn);
+        gemm<T>(OP_N, OP_N, ga<T>(n, n, n, Y2, ldn, Y2, ldn, Tm, ldn, 1.0, 0.0, TRI_A_UPPER | TRI_B_UPPER), s);
+        hipLaunchKernelGGL((inv_near_identity_kernel<T>), dim3(nb), dim3(256), 0, s, Y2, ldn, (const T*)Tm, ldn, n);
+      } else {
+        potrf_upper<T>(G, ldn, n, Y2, ldn, tmp, dinfo, s);
+      }
       // Q1' = Q1 W2^{-1} (buf[s1]), Q2' = Q2 W2^{-1} (G)
       gemm_prec<T>(prec_qr, OP_N, OP_N, ga<T>(m, n, n, buf[s2], ldb[s2], Y2, ldn, buf[s1], ldb[s1], 1.0, 0.0, TRI_B_UPPER), lo, s);
       gemm<T>(OP_N, OP_N, ga<T>(n, n, n, Y1, ldn, Y2, ldn, G, ldn, 1.0, 0.0, TRI_A_UPPER | TRI_B_UPPER), s);
