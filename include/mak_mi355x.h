@@ -93,8 +93,9 @@ int mak_polar_workspace_size(int dtype, int m, int n, size_t* bytes);
  * (P not computed).  A is left untouched.
  *   precision: 64 (FP64 throughout), 32 (QDWH bulk GEMMs in FP32) or 16 (BF16
  *   in the Cholesky iterations, FP32 in the QR iterations), all with FP32
- *   accumulation; Gram matrices, Cholesky and weights stay FP64, followed by
- *   FP64 Newton-Schulz refinement.  Complex data only supports 64.
+ *   accumulation; Gram matrices, Cholesky, weights and the first CholeskyQR
+ *   pass of the QR iterations stay FP64, followed by FP64 Newton-Schulz
+ *   refinement.  Complex data only supports 64.
  *   maxiter <= 0 selects 20; l0 outside (0, 1) selects 1e-10.
  * Replaces left_polar!(A, WP, ::PolarViaSVD / ::PolarNewton)
  * (src/implementations/polar.jl:59-70, :99-111). */

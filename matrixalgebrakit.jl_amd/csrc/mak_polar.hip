@@ -251,8 +251,10 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
     potrf_upper<T>(G, ldn, n, Y1, ldn, tmp, dinfo, s);
     int nxt;
     if (c > 100.0) {
-      // pass 1: Q1 = sqrt(c) X W1^{-1} (buf[s2]), Q2 = W1^{-1} (Y1)
-      gemm_prec<T>(prec_qr, OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y1, ldn, buf[s2], ldb[s2], std::sqrt(c), 0.0, TRI_B_UPPER), lo, s);
+      // pass 1: Q1 = sqrt(c) X W1^{-1} (buf[s2]), Q2 = W1^{-1} (Y1).  FP64 in
+      // every precision: it feeds the second Gram, which is then I + E with
+      // ||E|| ~ (1 + c) u and factors in closed form (below)
+      gemm<T>(OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y1, ldn, buf[s2], ldb[s2], std::sqrt(c), 0.0, TRI_B_UPPER), s);
       // pass 2: G = Q2^H Q2 + Q1^H Q1 = W2^H W2
       gemm<T>(OP_C, OP_N, ga<T>(n, n, n, Y1, ldn, Y1, ldn, G, ldn, 1.0, 0.0, TRI_A_LOWER | TRI_B_UPPER, 1), s);
       gemm<T>(OP_C, OP_N, ga<T>(n, n, m, buf[s2], ldb[s2], buf[s2], ldb[s2], G, ldn, 1.0, 1.0, 0, 1), s);

@@ -236,8 +236,8 @@ class MI355XPolar:
 
     ``precision`` 64 runs every product in FP64; 32 / 16 run the bulk products
     of the iteration in FP32 / BF16 (FP32 accumulation) with FP64 Gram
-    matrices, Cholesky factorisations and weights, followed by FP64
-    Newton-Schulz refinement.  ``maxiter`` (default 20) and ``l0`` (lower bound
+    matrices, Cholesky factorisations, weights and first CholeskyQR pass of
+    the QR iterations, followed by FP64 Newton-Schulz refinement.  ``maxiter`` (default 20) and ``l0`` (lower bound
     of sigma_min(A) / ||A||_F, default 1e-10) tune the iteration; like
     ``PolarNewton`` (src/interface/decompositions.jl:305) non-convergence only
     warns.
