@@ -316,10 +316,14 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
     const int steps = (prec == 16) ? 3 : 2;
     for (int k = 0; k < steps; ++k) {
       const int nx = (cur + 1) % 3;
-      // G = (3 I - X^H X) / 2 from the upper triangle of X^H X;  X+ = X G
+      // E = I - X^H X (FP64 Gram, upper triangle, completed by the shift);
+      // X+ = X (3I - X^H X) / 2 = X + X E / 2 with the correction X E on FP32
+      // MFMA: E is small (its FP32 rounding costs |E| 6e-8 absolute), and the
+      // sum with X is formed in FP64
       gemm<T>(OP_C, OP_N, ga<T>(n, n, m, buf[cur], ldb[cur], buf[cur], ldb[cur], G, ldn, 1.0, 0.0, 0, 1), s);
-      hipLaunchKernelGGL((shift_herm_kernel<T>), dim3(cdiv(n, 32), cdiv(n, 32)), dim3(256), 0, s, G, ldn, n, 1.5, -0.5);
-      gemm<T>(OP_N, OP_N, ga<T>(m, n, n, buf[cur], ldb[cur], G, ldn, buf[nx], ldb[nx], 1.0, 0.0), s);
+      hipLaunchKernelGGL((shift_herm_kernel<T>), dim3(cdiv(n, 32), cdiv(n, 32)), dim3(256), 0, s, G, ldn, n, 1.0, -1.0);
+      copy_mat<T>(m, n, buf[cur], ldb[cur], buf[nx], ldb[nx], s);
+      gemm_prec<T>(32, OP_N, OP_N, ga<T>(m, n, n, buf[cur], ldb[cur], G, ldn, buf[nx], ldb[nx], 0.5, 1.0), lo, s);
       cur = nx;
       ++nns;
     }
