@@ -138,6 +138,21 @@ def test_polar_mixed_precision(prec, bwd_tol, dev):
     assert info["ns_steps"] >= 2
 
 
+@pytest.mark.parametrize("prec,tol", [(32, 1e-4), (16, 5e-2)])
+@pytest.mark.parametrize("m,n", [(54, 37), (300, 170), (1000, 130)], ids=["54x37", "300x170", "1000x130"])
+def test_polar_mixed_precision_ragged(m, n, prec, tol, dev):
+    """Sizes off every tile multiple (packed FP32 / BF16 operand copies, the
+    Hermitian shift of the Newton-Schulz steps): W within tol of the SVD
+    oracle's (Gaussian inputs, modest condition numbers), orthogonal to FP64
+    accuracy after the FP64 Newton-Schulz steps."""
+    A = O.gaussian(m, n, np.float64, seed=m + n)
+    W, P = run_left(A, dev, mak.MI355XPolar(precision=prec))
+    Wo, _ = O.left_polar_svd(A)
+    assert rel(W, Wo) < tol, rel(W, Wo)
+    assert np.linalg.norm(W.T @ W - np.eye(n)) < 1e-12
+    assert np.linalg.norm(A - W @ P) / np.linalg.norm(A) < tol
+
+
 def test_polar_full_size_gaussian_properties(dev):
     n = 4096
     A = O.gaussian(n, n, np.float64, seed=7)
