@@ -458,8 +458,10 @@ __device__ __forceinline__ void lu32_recon(T (*Qt)[33], T (*Ls)[33], T* dv, T (*
 // Q = Q1 W2^{-1}, its LU with signs D, T = -U D^H L^{-H}, M = W2^{-1} U^{-1}
 // (to Mout, 32 x 32) and the top nbc rows of the panel Ap (D R on and above
 // the diagonal, L below).  T goes to Tout (32 x 32, zero outside nbc x nbc).
-// With Xn (not null) it also forms Gw = Xn^H Xn (H x nbcn, ld ldx; to Gw, 32 x 32,
-// zero outside nbcn x nbcn) through partialN, before the check.
+// With Xn (not null) it also forms Gw = Xn^H Xn (H x nbcn, ld ldx; 32 x 32, zero
+// outside nbcn x nbcn) as one 32 x 32 partial per workgroup in partialN, summed
+// into Gw here when reduce_next is set (otherwise the next trailing update's
+// split-K product sums them on the side, GemmArgs::rPart).
 // With carried, pass 1 runs here instead of in a kernel of its own: every
 // workgroup forms G1 = Gw - Rt^H Rt (Gw as the previous panel's call left it,
 // Rt the 32 rows above X) and factors it, redundantly and identically, while
@@ -471,7 +473,7 @@ __global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t
                                                              const T* R1inv, T* Q1, T* partial, int* status,
                                                              int j0, T* Tout, T* Mout, T* Ap, int64_t lda,
                                                              const T* Xn, int nbcn, T* partialN, T* Gw,
-                                                             int carried) {
+                                                             int carried, int reduce_next) {
   constexpr int TR = CqTile<T>::TR;
   constexpr int XM = TR / NB;  // 32-row matrices spanned by the staged tile
   // the staged tiles of X and Xn, R1^{-1}; at least the 8 matrices of the end
@@ -529,7 +531,7 @@ __global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t
     if (Xn != nullptr) store_partial<T>(partialN, nr, ni);
     if (!partial_is_last<T>(partial, cr, ci, counter, &last_s)) return;
   }
-  if (Xn != nullptr) {  // Gw = sum of the partials (zero outside nbcn x nbcn)
+  if (Xn != nullptr && reduce_next) {  // Gw = sum of the partials (zero outside nbcn x nbcn)
     T (*Gn)[NB + 1] = pool[1];
     reduce_partials<T>(partialN, (int)gridDim.x, NB, Gn, counter, dev_s);
     for (int e = tid; e < NB * NB; e += 256) Gw[e] = Gn[e % NB][e / NB];
@@ -639,6 +641,8 @@ void zero_ints(int* p, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(zero_ints_kernel, dim3(cdiv(n, 64)), dim3(64), 0, s, p, n);
 }
 
+int cholqr_gram_blocks(int H) { return std::max(1, std::min(kMaxGramBlocks, cdiv(H, 256))); }
+
 size_t cholqr_workspace_elems(int m) {
   // the carried Gram and its partials, the partial Grams, R1, R1^{-1}, M,
   // Q1 (m x 32)
@@ -658,25 +662,26 @@ void cholqr_panel(T* Ap, int64_t lda, int H, int nbc, T* Tpan, T* cw, int* statu
   T* R1i = R1 + NB * NB;
   T* M = R1i + NB * NB;
   T* Q1 = M + NB * NB;
-  const int nblk = std::max(1, std::min(kMaxGramBlocks, cdiv(H, 256)));
+  const int nblk = cholqr_gram_blocks(H);
   if (carried == 0)
     hipLaunchKernelGGL((gram_chol_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc, part,
                        status, j0, sticky, R1, R1i);
   const T* Xn = nbc_next > 0 ? (const T*)(Ap + (int64_t)nbc * lda) : nullptr;
   hipLaunchKernelGGL((cholqr2_finish_kernel<T>), dim3(nblk), dim3(256), 0, s, (const T*)Ap, lda, H, nbc, R1,
                      (const T*)R1i, Q1, part, status, j0, Tpan, M, Ap, lda, Xn, nbc_next, partN, Gw,
-                     carried == 1 ? 1 : 0);
+                     carried == 1 ? 1 : 0, ScalarTraits<T>::is_complex ? 1 : 0);
   if (H > nbc) gemm<T>(OP_N, OP_N, gg<T>(H - nbc, nbc, nbc, Q1 + nbc, H, M, NB, Ap + nbc, lda, status), s);
 }
 
-template <typename T> void cholqr_carry_buffers(T* cw, T** Gw, T** R1, T** R1inv) {
+template <typename T> void cholqr_carry_buffers(T* cw, T** Gw, T** partN, T** R1, T** R1inv) {
   *Gw = cw;
+  *partN = cw + NB * NB;
   T* part = cw + NB * NB + (size_t)kMaxGramBlocks * NB * NB;
   *R1 = part + (size_t)kMaxGramBlocks * NB * NB;
   *R1inv = *R1 + NB * NB;
 }
-template void cholqr_carry_buffers<double>(double*, double**, double**, double**);
-template void cholqr_carry_buffers<zd>(zd*, zd**, zd**, zd**);
+template void cholqr_carry_buffers<double>(double*, double**, double**, double**, double**);
+template void cholqr_carry_buffers<zd>(zd*, zd**, zd**, zd**, zd**);
 
 template void cholqr_panel<double>(double*, int64_t, int, int, double*, double*, int*, int, int, int, int,
                                    hipStream_t);

@@ -121,7 +121,10 @@ __device__ inline void carry_chol(const GemmArgs& g, double* lds_a, double* lds_
 
 // Real FP64 GEMM.  Each thread owns fixed load coordinates; the next K-tile is
 // fetched into registers while the MFMAs of the current tile run.
-template <int BM, int BN, int BK, int WM, int WN, int OPA, int OPB, bool CARRY = false>
+// SIDE: the QR trailing update's side jobs -- with BM >= 64 the carried
+// Cholesky in the workgroup of tile (0, 0) (GemmArgs::cGw), with BM == 32 the
+// side reduction in an extra z slice (GemmArgs::rPart).
+template <int BM, int BN, int BK, int WM, int WN, int OPA, int OPB, bool SIDE = false>
 __global__ void __launch_bounds__(256, 2) gemm_f64_kernel(GemmArgs g) {
   constexpr int WTM = BM / WM, WTN = BN / WN, TM = WTM / 16, TN = WTN / 16;
   constexpr int NA = BM * BK / 256, NB_ = BN * BK / 256;
@@ -129,6 +132,25 @@ __global__ void __launch_bounds__(256, 2) gemm_f64_kernel(GemmArgs g) {
   static_assert(NA * 256 == BM * BK && NB_ * 256 == BN * BK, "tile / thread count");
   __shared__ double As[BK][BM + 2];
   __shared__ double Bs[BK][BN + 2];
+  if constexpr (SIDE && BM == 32) {
+    // the side reduction (GemmArgs::rPart): the first workgroup of the extra z slice
+    if (blockIdx.z == (unsigned)(g.batch * g.ksplit)) {
+      if (blockIdx.x == 0 && blockIdx.y == 0) {
+        const double* P = reinterpret_cast<const double*>(g.rPart);
+        double* out = reinterpret_cast<double*>(g.rOut);
+        for (int e = threadIdx.x; e < 1024; e += 256) {
+          double acc[4] = {0.0, 0.0, 0.0, 0.0};
+          int q = 0;
+          for (; q + 4 <= g.rN; q += 4)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[u] += P[(int64_t)(q + u) * 1024 + e];
+          for (; q < g.rN; ++q) acc[0] += P[(int64_t)q * 1024 + e];
+          out[e] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        }
+      }
+      return;
+    }
+  }
   if (gate_closed(g.gate, g.gate_want)) return;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -250,7 +272,7 @@ __global__ void __launch_bounds__(256, 2) gemm_f64_kernel(GemmArgs g) {
           if (row < g.M && col < g.N) C[row + (int64_t)col * g.ldc] = alpha * acc[i][j][r];
         }
   }
-  if constexpr (CARRY) {
+  if constexpr (SIDE && BM >= 64) {
     static_assert(BK * (BM + 2) >= 32 * 33 && BK * (BN + 2) >= 32 * 33, "carry needs 32 x 33 of LDS per tile");
     if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && g.cGw != nullptr)
       carry_chol(g, &As[0][0], &Bs[0][0]);
@@ -373,6 +395,12 @@ static void launch_gemm_carry(const GemmArgs& g, hipStream_t s) {
   dim3 grid(cdiv(g.M, BM), cdiv(g.N, BN), 1);
   hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, 16, 2, 2, OP_N, OP_N, true>), grid, dim3(256), 0, s, g);
 }
+// skinny split-K product with the side reduction (one extra z slice)
+template <int OPA, int OPB>
+static void launch_gemm_sidered(const GemmArgs& g, hipStream_t s) {
+  dim3 grid(cdiv(g.M, 32), cdiv(g.N, 64), g.batch * g.ksplit + 1);
+  hipLaunchKernelGGL((gemm_f64_kernel<32, 64, 16, 2, 2, OPA, OPB, true>), grid, dim3(256), 0, s, g);
+}
 
 // The largest tile that still gives the launch enough workgroups to cover the
 // chip (256 CUs): 128x128 (real) / 64x64 while that yields >= 192 / 128
@@ -399,6 +427,12 @@ static void gemm_shape(const GemmArgs& g, hipStream_t s) {
     }
   }
   if (g.M <= 32) {
+    if constexpr (!cplx && OPA == OP_T && OPB == OP_N) {
+      if (g.rPart != nullptr) {
+        launch_gemm_sidered<OPA, OPB>(g, s);
+        return;
+      }
+    }
     launch_gemm_tile<32, 64, 16, 2, 2, OPA, OPB, T>(g, s);
   } else if (!cplx && hinted && busy(128) < 1024 && busy(64) >= 512) {
     // hinted products spread uneven work over the tiles: more, smaller tiles

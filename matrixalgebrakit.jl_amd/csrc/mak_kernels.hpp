@@ -138,6 +138,12 @@ struct GemmArgs {
   void* cR1i;
   int* cStatus;
   int cJ0, cNbc;
+  // Side job of a real split-K product (M <= 32 tiles): when rPart is not
+  // null, one extra workgroup sums the rN 32 x 32 partials at rPart (in order)
+  // into rOut (32 x 32), concurrently with the product's tiles.
+  const void* rPart;
+  void* rOut;
+  int rN;
 };
 
 // C = alpha op(A) op(B) + beta C on FP64 MFMA (real) or 4 real MFMAs (complex).

@@ -33,12 +33,16 @@ int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, in
 // (which had nbc_next > 0) instead of a pass over the panel, once the previous
 // trailing update has run; carried = 2: as 1, with R1 and R1^{-1} already
 // formed from it by that trailing update (GemmArgs::cGw, at the addresses
-// cholqr_carry_buffers gives).
+// cholqr_carry_buffers gives).  For real data the next panel's Gram is left as
+// cholqr_gram_blocks(H) partials, which that trailing update's split-K product
+// sums into Gw (GemmArgs::rPart).
 size_t cholqr_workspace_elems(int m);
 template <typename T>
 void cholqr_panel(T* Ap, int64_t lda, int H, int nbc, T* Tpan, T* cw, int* status, int j0, int sticky,
                   int carried, int nbc_next, hipStream_t s);
-template <typename T> void cholqr_carry_buffers(T* cw, T** Gw, T** R1, T** R1inv);
+template <typename T> void cholqr_carry_buffers(T* cw, T** Gw, T** partN, T** R1, T** R1inv);
+// Workgroups (and Gram partials) of the panel kernels for an H-row panel.
+int cholqr_gram_blocks(int H);
 void zero_ints(int* p, int n, hipStream_t s);
 extern int g_last_cholqr_reject;
 

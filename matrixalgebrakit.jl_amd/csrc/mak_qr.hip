@@ -263,6 +263,11 @@ void apply_wy(const T* Vp, int64_t ldv, int rows, int nbc, const T* Tm, int tran
     g.A = Vp; g.lda = ldv; g.B = C; g.ldb = ldc; g.C = Wp; g.ldc = nbc;
     g.alpha_re = 1.0; g.beta_re = 0.0;
     g.ksplit = ks; g.kchunk = kc; g.splitStride = slice; g.batch = 1; g.unitLowerA = 1;
+    if (carry_nbc > 0 && !ScalarTraits<T>::is_complex) {  // sum the next panel's Gram partials on the side
+      T *Gw, *partN, *R1, *R1i;
+      cholqr_carry_buffers<T>(reinterpret_cast<T*>(ws + L.offCq), &Gw, &partN, &R1, &R1i);
+      g.rPart = partN; g.rOut = Gw; g.rN = cholqr_gram_blocks(rows);
+    }
     gemm<T>(OP_C, OP_N, g, s);
   }
   wy_reduce<T>(Wp, ks, slice, Tm, transT, nbc, ncols, W2, s);
@@ -273,8 +278,8 @@ void apply_wy(const T* Vp, int64_t ldv, int rows, int nbc, const T* Tm, int tran
     g.alpha_re = -1.0; g.beta_re = 1.0;
     g.ksplit = 1; g.kchunk = nbc; g.batch = 1; g.unitLowerA = 1;
     if (carry_nbc > 0 && !ScalarTraits<T>::is_complex) {
-      T *Gw, *R1, *R1i;
-      cholqr_carry_buffers<T>(reinterpret_cast<T*>(ws + L.offCq), &Gw, &R1, &R1i);
+      T *Gw, *partN, *R1, *R1i;
+      cholqr_carry_buffers<T>(reinterpret_cast<T*>(ws + L.offCq), &Gw, &partN, &R1, &R1i);
       g.cGw = Gw; g.cR1 = R1; g.cR1i = R1i;
       g.cStatus = reinterpret_cast<int*>(ws + L.offFlags);
       g.cJ0 = carry_j0; g.cNbc = carry_nbc;
