@@ -9,7 +9,21 @@
  *   - Complex matrices are ComplexF64, interleaved (re, im): the layout of
  *     Julia's ComplexF64 and numpy's complex128.  They are passed as void*.
  *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
- *     Calls are asynchronous on that stream; nothing synchronises the device.
+ *     Work is enqueued on that stream only; no call synchronises the device
+ *     or other streams.  Some calls wait for their own stream to read a small
+ *     status word back to the host:
+ *       mak_qr_*           once per call (the CholeskyQR2 acceptance check),
+ *                          twice when the whole-matrix CholeskyQR2 result is
+ *                          rejected; under stream capture never (the gated
+ *                          form is enqueued instead);
+ *       mak_*_polar_*      once for ||A||_F, then once per QDWH iteration
+ *                          (convergence and the Cholesky status) and once
+ *                          more per QR-based iteration (the closed-form test);
+ *       mak_*_pullback_*   once (the rank / convergence counters);
+ *       mak_qr_batched_execute is fully asynchronous (one graph launch).
+ *   - Calls are reentrant: all state lives in the caller's workspace (the
+ *     batched plans own their graph and stream); concurrent calls need
+ *     distinct workspaces.
  *   - `work` is a caller-owned device workspace of at least the size returned
  *     by the matching *_workspace_size query (the library allocates nothing;
  *     this mirrors LAPACK's lwork query, src/yalapack.jl:180-197).
@@ -83,6 +97,8 @@ typedef struct mak_polar_info {
   double l0;        /* lower bound used for sigma_min(X_0) */
   double l_final;   /* lower bound after the last iteration */
   double last_diff; /* ||X_k - X_{k-1}||_F / ||X_k||_F of the last iteration */
+  int completed;    /* 1: A was rank deficient and the partial isometry of the
+                       iteration was completed to an isometry (a second run) */
 } mak_polar_info;
 
 /* Device workspace bytes for the polar entry points on an m x n input. */
@@ -97,6 +113,9 @@ int mak_polar_workspace_size(int dtype, int m, int n, size_t* bytes);
  *   pass of the QR iterations stay FP64, followed by FP64 Newton-Schulz
  *   refinement.  Complex data only supports 64.
  *   maxiter <= 0 selects 20; l0 outside (0, 1) selects 1e-10.
+ *   W is an isometry also for rank-deficient A (as the SVD route's U V^H):
+ *   the iteration's partial isometry is completed (info->completed), and
+ *   A = 0 gives W = I (m x n), P = 0.
  * Replaces left_polar!(A, WP, ::PolarViaSVD / ::PolarNewton)
  * (src/implementations/polar.jl:59-70, :99-111). */
 int mak_left_polar_d(void* stream, int m, int n, const double* A, int lda, double* W, int ldw, double* P, int ldp,

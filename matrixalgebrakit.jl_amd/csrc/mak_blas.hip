@@ -1,7 +1,7 @@
 // Dense building blocks for the polar iterations (FP64 / ComplexF64):
-//   * potrf_upper: recursive Cholesky Z = W^H W (W upper) that also returns
-//     W^{-1}; 64 x 64 leaves are factored and inverted in registers by one
-//     workgroup, every off-diagonal product is an MFMA GEMM;
+//   * potrf_upper: Cholesky Z = W^H W (W upper) that also returns W^{-1}:
+//     the persistent tile kernel (mak_chol.hip), inside a blocked loop of MFMA
+//     GEMMs for large n;
 //   * Frobenius norms (deterministic two-pass reductions);
 //   * elementwise helpers (identity fill, axpby, Hermitian part).
 #include <algorithm>
@@ -13,76 +13,7 @@
 
 namespace mak {
 
-constexpr int DB = 64;  // diagonal block of the blocked Cholesky
-
-template <typename T> void fill_zero(T* A, int64_t lda, int m, int n, hipStream_t s);
-
-// Factor the kb x kb (kb <= 64) diagonal block Z (upper part read) into W
-// (upper, written over Z; strict lower part set to zero) and write W^{-1}
-// (upper, zero below) to Y.  info[0] receives offset + j + 1 when the leading
-// (j+1) x (j+1) minor is not positive definite.
-//
-// 2 x 2 blocks of 32, one workgroup of 4 waves, everything in LDS:
-//   W11, Y11 = chol_inv(Z11)                 (wave 0, no workgroup barrier)
-//   W12 = Y11^H Z12 ; Z22 -= W12^H W12       (MFMA, 4 waves)
-//   W22, Y22 = chol_inv(Z22)                 (wave 0)
-//   Y12 = -(Y11 W12) Y22                     (MFMA)
-// Rows/columns >= kb are padded with the identity.
-template <typename T>
-__global__ void __launch_bounds__(256) potrf_inv64_kernel(T* Z, int64_t ldz, int kb, T* Y, int64_t ldy,
-                                                          int* info, int offset) {
-  __shared__ T Z11[32][33], Z12[32][33], Z22[32][33];
-  __shared__ T Y11[32][33], Y12[32][33], Y22[32][33];
-  __shared__ int bad_s;
-  const int tid = threadIdx.x;
-  const int w = tid >> 6;
-  for (int e = tid; e < 32 * 32; e += 256) {
-    const int r = e % 32, c = e / 32;
-    auto ld = [&](int rr, int cc) {
-      return (rr <= cc && cc < kb) ? Z[rr + (int64_t)cc * ldz] : ((rr == cc) ? s_one(T()) : s_zero(T()));
-    };
-    Z11[r][c] = ld(r, c);
-    Z12[r][c] = ld(r, c + 32);
-    Z22[r][c] = ld(r + 32, c + 32);
-  }
-  if (tid == 0) bad_s = 0;
-  __syncthreads();
-  if (w == 0) {
-    const int b = wave_chol_inv32<T>(Z11, Y11);
-    if (lane_id() == 0) bad_s = b;
-  }
-  __syncthreads();
-  if (bad_s) {
-    if (tid == 0 && info && info[0] == 0) info[0] = offset + bad_s;
-    return;
-  }
-  mm32<T>(Z12, Y11, true, Z12, false, 1.0, false);    // W12 = Y11^H Z12
-  mm32<T>(Z22, Z12, true, Z12, false, -1.0, true);    // Z22 -= W12^H W12
-  if (w == 0) {
-    const int b = wave_chol_inv32<T>(Z22, Y22);
-    if (lane_id() == 0) bad_s = b;
-  }
-  __syncthreads();
-  if (bad_s) {
-    if (tid == 0 && info && info[0] == 0) info[0] = offset + 32 + bad_s;
-    return;
-  }
-  mm32<T>(Y12, Y11, false, Z12, false, 1.0, false);   // Y11 W12
-  mm32<T>(Y12, Y12, false, Y22, false, -1.0, false);  // -(Y11 W12) Y22
-  for (int e = tid; e < 64 * 64; e += 256) {
-    const int r = e % 64, c = e / 64;
-    if (r < kb && c < kb) {
-      T wv = s_zero(T()), yv = s_zero(T());
-      if (r <= c) {
-        if (c < 32) { wv = Z11[r][c]; yv = Y11[r][c]; }
-        else if (r < 32) { wv = Z12[r][c - 32]; yv = Y12[r][c - 32]; }
-        else { wv = Z22[r - 32][c - 32]; yv = Y22[r - 32][c - 32]; }
-      }
-      Z[r + (int64_t)c * ldz] = wv;
-      Y[r + (int64_t)c * ldy] = yv;
-    }
-  }
-}
+constexpr int DB = 64;  // row block of the in-place triangular multiply
 
 template <typename T> static GemmArgs gargs(int M, int N, int K, const T* A, int64_t lda, const T* B, int64_t ldb,
                                             T* C, int64_t ldc, double alpha, double beta) {
@@ -119,60 +50,16 @@ template <typename T> void copy_mat(int m, int n, const T* X, int64_t ldx, T* Y,
   hipLaunchKernelGGL((copy_kernel<T>), dim3((unsigned)cdiv64(tot, 256)), dim3(256), 0, s, m, n, X, ldx, Y, ldy);
 }
 
-// Recursive Cholesky with inverse on an n x n block (splits at multiples of
-// DB, so the leaves are the DB x DB diagonal blocks):
-//   Z = [Z11 Z12; . Z22] = W^H W,  W = [W11 W12; 0 W22],  Y = W^{-1}:
-//   W11, Y11 <- recursion;  W12 = Y11^H Z12;  Z22 -= W12^H W12;
-//   W22, Y22 <- recursion;  Y12 = -(Y11 W12) Y22.
-// Every product is one MFMA GEMM; the DB x DB leaves run potrf_inv64_kernel.
-// W12 is written conjugate-transposed into the free lower block Z21 (the
-// product that forms it cannot run in place), and potrf_finish_kernel moves
-// every such block to the upper part at the end.  Y's strict lower part is
-// zeroed once before the recursion (the triangular-hint GEMMs read the zeros
-// inside their diagonal tiles); nothing below Y's diagonal is written after.
+// After the blocked loop: every 32 x 32 block strictly below the diagonal
+// that lies outside the diagonal blocks of the loop holds W^H there; move it
+// (conjugate transposed) to the mirrored upper block and zero it.  Blocks
+// inside a diagonal block were written by the tile kernel.  One workgroup per
+// lower block pair, the transpose staged through LDS.
 template <typename T>
-static void potrf_inv_rec(T* Z, int64_t ldz, T* Y, int64_t ldy, int n, int offset, T* tmp, int* dinfo,
-                          hipStream_t s) {
-  if (n <= DB) {
-    hipLaunchKernelGGL((potrf_inv64_kernel<T>), dim3(1), dim3(256), 0, s, Z, ldz, n, Y, ldy, dinfo, offset);
-    return;
-  }
-  const int n1 = ((n / 2 + DB - 1) / DB) * DB;
-  const int n2 = n - n1;
-  T* Z12 = Z + (int64_t)n1 * ldz;
-  T* Z21 = Z + n1;
-  T* Z22 = Z + n1 + (int64_t)n1 * ldz;
-  T* Y12 = Y + (int64_t)n1 * ldy;
-  T* Y22 = Y + n1 + (int64_t)n1 * ldy;
-  potrf_inv_rec<T>(Z, ldz, Y, ldy, n1, offset, tmp, dinfo, s);
-  // Z21 = W12^H = Z12^H Y11  (Y11 upper)
-  GemmArgs g = gargs<T>(n2, n1, n1, Z12, ldz, Y, ldy, Z21, ldz, 1.0, 0.0);
-  g.tri = TRI_B_UPPER;
-  gemm<T>(OP_C, OP_N, g, s);
-  // Z22 -= W12^H W12 = Z21 Z21^H  (upper triangle only: the recursion reads nothing else)
-  g = gargs<T>(n2, n2, n1, Z21, ldz, Z21, ldz, Z22, ldz, -1.0, 1.0);
-  g.upperC = 1;
-  gemm<T>(OP_N, OP_C, g, s);
-  potrf_inv_rec<T>(Z22, ldz, Y22, ldy, n2, offset + n1, tmp, dinfo, s);
-  // tmp = Y11 W12 = Y11 Z21^H;  Y12 = -tmp Y22
-  g = gargs<T>(n1, n2, n1, Y, ldy, Z21, ldz, tmp, n1, 1.0, 0.0);
-  g.tri = TRI_A_UPPER;
-  gemm<T>(OP_N, OP_C, g, s);
-  g = gargs<T>(n1, n2, n2, tmp, n1, Y22, ldy, Y12, ldy, -1.0, 0.0);
-  g.tri = TRI_B_UPPER;
-  gemm<T>(OP_N, OP_N, g, s);
-}
-
-// After the recursion: every 32 x 32 block strictly below the diagonal that
-// lies outside the DB x DB leaves holds W^H there; move it (conjugate
-// transposed) to the mirrored upper block and zero it.  Blocks inside a leaf
-// were written by the leaf kernel.  One workgroup per lower block pair, the
-// transpose staged through LDS.
-template <typename T>
-__global__ void __launch_bounds__(256) potrf_finish_kernel(T* Z, int64_t ldz, int n) {
+__global__ void __launch_bounds__(256) potrf_finish_kernel(T* Z, int64_t ldz, int n, int nb) {
   __shared__ T tile[32][33];
   const int bc = blockIdx.x, br = blockIdx.y;  // lower block (rows br, cols bc), br > bc
-  if (br <= bc || (br * 32) / DB == (bc * 32) / DB) return;
+  if (br <= bc || (br * 32) / nb == (bc * 32) / nb) return;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   for (int k = ty; k < 32; k += 8) {
     const int r = br * 32 + tx, c = bc * 32 + k;
@@ -188,21 +75,71 @@ __global__ void __launch_bounds__(256) potrf_finish_kernel(T* Z, int64_t ldz, in
   }
 }
 
+// Diagonal block width of the blocked loop (n > kCholTileMaxN).
+constexpr int kCholBlock = 1024;
+
+size_t potrf_tmp_elems(int n) { return (size_t)64 * std::max(n, 1) + (size_t)n * n / 4; }
+
 // Z (n x n, upper part) -> W (upper, in place, strict lower zeroed) with
 // Z = W^H W, and Winv = W^{-1} (upper, n x n, fully overwritten).  tmp holds
 // potrf_tmp_elems(n) elements.  dinfo (device) receives j > 0 when the leading
 // j x j minor is not positive definite.
+//
+// n <= kCholTileMaxN: one launch of the persistent tile kernel.  Larger n:
+// diagonal blocks of kCholBlock columns, each factored by the tile kernel;
+// with Wkk, Xkk = W_kk, W_kk^{-1}:
+//   panel     W_k,r = Xkk^H Z_k,r  (written conjugate-transposed into the free
+//             lower block Z_r,k: the product cannot run in place)
+//   trailing  Z_r,r -= W_k,r^H W_k,r   (upper triangle only)
+//   inverse   X_0:k,k = -(X_0:k,0:k W_0:k,k) Xkk   (left-looking; W_0:k,k
+//             read from the lower blocks)
+// and potrf_finish_kernel moves the W blocks to the upper part at the end.
 template <typename T>
 void potrf_upper(T* Z, int64_t ldz, int n, T* Winv, int64_t ldw, T* tmp, int* dinfo, hipStream_t s) {
   hipMemsetAsync(dinfo, 0, sizeof(int), s);
   if (n <= 0) return;
+  // the tile kernel's flags live at the end of tmp
+  const size_t tmpn = potrf_tmp_elems(n);
+  const size_t fl_elems = (chol_tile_flag_ints(std::min(n, kCholTileMaxN)) * sizeof(int) + sizeof(T) - 1) / sizeof(T);
+  int* flags = reinterpret_cast<int*>(tmp + (tmpn - fl_elems));
+  if (n <= kCholTileMaxN) {
+    chol_tile_factor<T>(Z, ldz, n, Winv, ldw, flags, dinfo, 0, s);
+    return;
+  }
   hipLaunchKernelGGL((zero_lower_kernel<T>), dim3((unsigned)cdiv64((int64_t)n * n, 256)), dim3(256), 0, s, Winv, ldw,
                      n);
-  potrf_inv_rec<T>(Z, ldz, Winv, ldw, n, 0, tmp, dinfo, s);
-  if (n > DB) {
-    const int nb = cdiv(n, 32);
-    hipLaunchKernelGGL((potrf_finish_kernel<T>), dim3(nb, nb), dim3(256), 0, s, Z, ldz, n);
+  for (int k0 = 0; k0 < n; k0 += kCholBlock) {
+    const int kb = std::min(kCholBlock, n - k0);
+    const int rest = n - k0 - kb;
+    T* Zkk = Z + k0 + (int64_t)k0 * ldz;
+    T* Xkk = Winv + k0 + (int64_t)k0 * ldw;
+    chol_tile_factor<T>(Zkk, ldz, kb, Xkk, ldw, flags, dinfo, k0, s);
+    if (rest > 0) {
+      T* Zkr = Z + k0 + (int64_t)(k0 + kb) * ldz;
+      T* Zrk = Z + (k0 + kb) + (int64_t)k0 * ldz;
+      T* Zrr = Z + (k0 + kb) + (int64_t)(k0 + kb) * ldz;
+      // Z_r,k = W_k,r^H = Z_k,r^H Xkk
+      GemmArgs g = gargs<T>(rest, kb, kb, Zkr, ldz, Xkk, ldw, Zrk, ldz, 1.0, 0.0);
+      g.tri = TRI_B_UPPER;
+      gemm<T>(OP_C, OP_N, g, s);
+      // Z_r,r -= Z_r,k Z_r,k^H (upper triangle)
+      g = gargs<T>(rest, rest, kb, Zrk, ldz, Zrk, ldz, Zrr, ldz, -1.0, 1.0);
+      g.upperC = 1;
+      gemm<T>(OP_N, OP_C, g, s);
+    }
+    if (k0 > 0) {
+      // tmp (k0 x kb) = X_0:k,0:k W_0:k,k = X_0:k,0:k Z_k,0:k^H ; X_0:k,k = -tmp Xkk
+      const T* Zk0 = Z + k0;  // rows k0.., columns 0..k0-1: W_0:k,k^H
+      GemmArgs g = gargs<T>(k0, kb, k0, Winv, ldw, Zk0, ldz, tmp, k0, 1.0, 0.0);
+      g.tri = TRI_A_UPPER;
+      gemm<T>(OP_N, OP_C, g, s);
+      g = gargs<T>(k0, kb, kb, tmp, k0, Xkk, ldw, Winv + (int64_t)k0 * ldw, ldw, -1.0, 0.0);
+      g.tri = TRI_B_UPPER;
+      gemm<T>(OP_N, OP_N, g, s);
+    }
   }
+  const int nb32 = cdiv(n, 32);
+  hipLaunchKernelGGL((potrf_finish_kernel<T>), dim3(nb32, nb32), dim3(256), 0, s, Z, ldz, n, kCholBlock);
 }
 
 // ---------------------------------------------------------------------------

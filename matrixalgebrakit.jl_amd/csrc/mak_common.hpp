@@ -23,11 +23,20 @@ struct zd {
 __host__ __device__ inline zd mkz(double a, double b) { zd r; r.x = a; r.y = b; return r; }
 
 template <typename T> struct ScalarTraits;
+struct zd2 {
+  zd a, b;
+};
 template <> struct ScalarTraits<double> {
   static constexpr bool is_complex = false;
+  using vec2 = double2;  // two consecutive elements (16-byte loads)
+  __host__ __device__ static double lo(const double2& v) { return v.x; }
+  __host__ __device__ static double hi(const double2& v) { return v.y; }
 };
 template <> struct ScalarTraits<zd> {
   static constexpr bool is_complex = true;
+  using vec2 = zd2;
+  __host__ __device__ static zd lo(const zd2& v) { return v.a; }
+  __host__ __device__ static zd hi(const zd2& v) { return v.b; }
 };
 
 // real arithmetic

@@ -78,6 +78,12 @@ __device__ __forceinline__ int step_lane(int c) {
 __device__ __forceinline__ void order_after(zd& v, const zd& dep) {
   asm volatile("" : "+v"(v.x), "+v"(v.y) : "v"(dep.x));
 }
+// An index (zero) that appears to depend on `dep`: a load through it cannot
+// be issued before `dep` is computed.
+__device__ __forceinline__ void index_after(int& z, double dep) { asm volatile("" : "+v"(z) : "v"(dep)); }
+__device__ __forceinline__ void index_after(int& z, const zd& dep) {
+  asm volatile("" : "+v"(z) : "v"(dep.x), "v"(dep.y));
+}
 
 // Cholesky with inverse of a 32 x 32 Hermitian positive definite block, by
 // ONE wave (no workgroup barrier, no LDS traffic inside the factorisation):
@@ -283,6 +289,158 @@ __device__ __forceinline__ int wave_chol_inv32(T (*S)[33], T (*Y)[33]) {
     Y[16 + i][j] = s_zero(T());
   }
   wave_lds_sync();
+  return bad;
+}
+
+// Cholesky factorisation with inverse of a 32 x 32 Hermitian positive definite
+// tile by ONE wave, eliminating [G | I] in a single pass:
+//   S (LDS, 32 x 33): upper triangle of G on entry (strict lower part not
+//   read); on exit R (upper, zero below) with G = R^H R.
+//   Y (LDS, 32 x 33): on exit X = R^{-1} (upper, zero below).
+//   row: LDS scratch of 64 elements, 16-byte aligned (two pivot-row buffers).
+// Lane c < 32 keeps column c of G, lane 32 + c column c of the identity (that
+// becomes M = R^{-H}, lower).  Step k, with the pivot d = G'(k, k) read from
+// lane k and the raw pivot row G'(k, r) in LDS:
+//   x[r] -= conj(G'(k, r)) x[k] / d  (r > k),   x[k] <- x[k] / sqrt(d)
+// is the same instruction stream for both halves: the Cholesky update
+// G'(r, c) -= conj(R(k, r)) R(k, c) and the forward elimination of the
+// identity Z(r, c) -= conj(R(k, r)) M(k, c).
+// Software pipelined: the update of row k + 1 and its publication as the next
+// pivot row (the dependency chain: one LDS round trip, the pivot reciprocal
+// and two FMAs per column) are issued before the bulk of the step's updates,
+// which then overlap the next round trip; the pivot rows alternate between
+// two buffers so the bulk reads of step k never race the write of step k + 1.
+// Returns 0, or k + 1 for the first pivot k that is not positive (S and Y are
+// then unspecified).
+template <typename T>
+__device__ __forceinline__ int wave_chol_aug32(T (*S)[33], T (*Y)[33], T* row) {
+  const int l = lane_id();
+  const int c = l & 31;
+  const bool lo = l < 32;
+  T x[32];
+#pragma unroll
+  for (int r = 0; r < 32; ++r) {
+    const T v = S[r][c];
+    x[r] = lo ? (r <= c ? v : s_zero(T())) : (r == c ? s_one(T()) : s_zero(T()));
+  }
+  // `row` is read at wave-uniform addresses; an index offset the compiler
+  // cannot see is zero keeps those values in vector registers (as uniform
+  // values they would be moved to scalar registers and hoisted, and spill).
+  int zoff = 0;
+  asm volatile("" : "+v"(zoff));
+  if (lo) row[c] = x[0];
+  int bad = 0;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    T* cur = row + (k & 1) * 32;
+    T* nxt = row + ((k + 1) & 1) * 32;
+    wave_lds_sync();  // the pivot row of step k is visible
+    // the whole pivot row in one burst of 16-byte reads (issued together, so
+    // their latency is paid once per step; read one at a time, as the
+    // scheduler otherwise does, they cost most of the step)
+    T rv[32];
+    {
+      using V2 = typename ScalarTraits<T>::vec2;
+      const V2* cv = reinterpret_cast<const V2*>(cur + zoff);
+#pragma unroll
+      for (int r = (k + 1) & ~1; r < 32; r += 2) {
+        const V2 v = cv[r / 2];
+        rv[r] = ScalarTraits<T>::lo(v);
+        rv[r + 1] = ScalarTraits<T>::hi(v);
+      }
+    }
+    const T raw = x[k];
+    const double d = s_re(rdl(raw, k));
+    bad = bad ? bad : (d > 0.0 ? 0 : k + 1);
+    const double dd = (d > 0.0) ? d : 1.0;
+    const double inv = recip_nr(dd);
+    const T w = s_scale(raw, inv);
+    if (k < 31) {
+      // the chain: row k + 1, published as the next pivot row
+      x[k + 1] = s_sub(x[k + 1], s_mul(s_conj(rv[k + 1]), w));
+      if (lo) nxt[c] = x[k + 1];
+    }
+#pragma unroll
+    for (int r = k + 2; r < 32; ++r) x[r] = s_sub(x[r], s_mul(s_conj(rv[r]), w));
+    // one step of overlap at most: step k + 1's chain may run ahead of this
+    // bulk, but row k + 3 (the chain of step k + 2) waits for all of it
+#pragma unroll
+    for (int r = k + 4; r < 32; ++r) order_after(x[k + 3], x[r]);
+    x[k] = s_scale(raw, rsqrt_nr(dd));
+  }
+#pragma unroll
+  for (int r = 0; r < 32; ++r) {
+    if (lo) S[r][c] = (r <= c) ? x[r] : s_zero(T());
+    else Y[c][r] = (r >= c) ? s_conj(x[r]) : s_zero(T());
+  }
+  wave_lds_sync();
+  return bad;
+}
+
+// Cholesky factorisation with inverse of a 32 x 32 Hermitian positive definite
+// tile by a whole 256-thread workgroup (every thread must call it):
+//   S (LDS, 32 x 33): upper triangle of G on entry (strict lower part not
+//   read); on exit R (upper, zero below) with G = R^H R.
+//   Y (LDS, 32 x 33): on exit X = R^{-1} (upper, zero below).
+//   rows: LDS scratch of 4 * 32 elements (two double-buffered row pairs).
+// Thread (c, q) = (t % 32, t / 32) keeps rows q, q + 8, q + 16, q + 24 of
+// column c of G and of M = R^{-H} (M starts as I) in registers.  Step k: the
+// owners of row k publish its raw values (G'(k, :), M'(k, :)) in LDS, one
+// workgroup barrier, then every thread updates its rows r > k,
+//   G'(r, c) -= conj(G'(k, r)) G'(k, c) / d,  M'(r, c) -= conj(G'(k, r)) M'(k, c) / d,
+// with d = G'(k, k), and the owners scale row k by 1/sqrt(d).  The row
+// buffers alternate between steps, so one barrier per column suffices.
+// Returns 0, or k + 1 for the first pivot k that is not positive.
+template <typename T>
+__device__ __forceinline__ int wg_chol_aug32(T (*S)[33], T (*Y)[33], T* rows) {
+  const int t = threadIdx.x;
+  const int c = t & 31, q = t >> 5;
+  T g[4], mm[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = q + 8 * i;
+    g[i] = (r <= c) ? S[r][c] : s_zero(T());
+    mm[i] = (r == c) ? s_one(T()) : s_zero(T());
+  }
+  int bad = 0;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    T* rg = rows + (k & 1) * 64;
+    T* rm = rg + 32;
+    const int qk = k & 7, ik = k >> 3;
+    if (q == qk) {
+      rg[c] = g[ik];
+      rm[c] = mm[ik];
+    }
+    __syncthreads();
+    const double d = s_re(rg[k]);
+    if (bad == 0 && !(d > 0.0)) bad = k + 1;
+    const double dd = (d > 0.0) ? d : 1.0;
+    const double inv = recip_nr(dd);
+    const T a = s_scale(rg[c], inv), b = s_scale(rm[c], inv);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = q + 8 * i;
+      if (r > k) {
+        const T cf = s_conj(rg[r]);
+        g[i] = s_sub(g[i], s_mul(cf, a));
+        mm[i] = s_sub(mm[i], s_mul(cf, b));
+      }
+    }
+    if (q == qk) {
+      const double rs = rsqrt_nr(dd);
+      g[ik] = s_scale(rg[c], rs);
+      mm[ik] = s_scale(rm[c], rs);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = q + 8 * i;
+    S[r][c] = (r <= c) ? g[i] : s_zero(T());
+    Y[c][r] = (r >= c) ? s_conj(mm[i]) : s_zero(T());  // X = M^H
+  }
+  __syncthreads();
   return bad;
 }
 

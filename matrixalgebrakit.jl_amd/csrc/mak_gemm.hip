@@ -52,6 +52,12 @@ __device__ inline zd load_b(const zd* B, int64_t ldb, int gk, int gj) {
 // tiles; those launches keep the round-robin deal (every XCD gets every
 // column), and the index whose K range grows (TRI_A_LOWER: rows, TRI_B_UPPER:
 // columns; upperC: columns) is reversed so that the longest tiles go first.
+//
+// upperC launches use a 1-D grid over the tiles on or above the diagonal only
+// (times the z slabs; see upper_tiles): with a 2-D grid every tile of tile
+// row r would land on XCD r % 8, and the upper triangle's first rows hold
+// most of the tiles.  Consecutive ids walk the triangle column by column from
+// the last column (the longest K ranges under triangular hints come first).
 __device__ inline void tile_coords(int& tm, int& tn, int tri, int upperC) {
   const int gx = gridDim.x, gy = gridDim.y;
   const int total = gx * gy;
@@ -61,6 +67,28 @@ __device__ inline void tile_coords(int& tm, int& tn, int tri, int upperC) {
   tn = L / gx;
   if (tri & TRI_A_LOWER) tm = gx - 1 - tm;
   if ((tri & TRI_B_UPPER) || upperC) tn = gy - 1 - tn;
+}
+
+// Tiles (tm <= tn) of a gx x gy tile grid with square tiles.
+__host__ __device__ inline int upper_tiles(int gx, int gy) {
+  int t = 0;
+  for (int j = 0; j < gy; ++j) t += (j + 1 < gx ? j + 1 : gx);
+  return t;
+}
+// Workgroup -> (tile row, tile column, z slab) of a compact upperC launch.
+__device__ inline void upper_tile_coords(int gx, int gy, int& tm, int& tn, int& bz) {
+  const int nup = upper_tiles(gx, gy);
+  int L = blockIdx.x;
+  bz = L / nup;
+  L %= nup;
+  int j = gy - 1;
+  for (;; --j) {
+    const int c = (j + 1 < gx ? j + 1 : gx);
+    if (L < c || j == 0) break;
+    L -= c;
+  }
+  tn = j;
+  tm = L;
 }
 
 // Clip [kbeg, kend) to the terms a tile (rows m0.., cols n0..) can have
@@ -156,9 +184,13 @@ __global__ void __launch_bounds__(256, 2) gemm_f64_kernel(GemmArgs g) {
   const int lane = tid & 63;
   const int w = tid >> 6;
   const int wm = w % WM, wn = w / WM;
-  int tmi, tni;
-  tile_coords(tmi, tni, g.tri, g.upperC);
-  const int bz = blockIdx.z;
+  int tmi, tni, bz;
+  if (BM == BN && g.upperC) {
+    upper_tile_coords(cdiv(g.M, BM), cdiv(g.N, BN), tmi, tni, bz);
+  } else {
+    tile_coords(tmi, tni, g.tri, g.upperC);
+    bz = blockIdx.z;
+  }
   const int bat = bz / g.ksplit;
   const int ks = bz % g.ksplit;
   const int m0 = tmi * BM, n0 = tni * BN;
@@ -290,9 +322,13 @@ __global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
   const int lane = tid & 63;
   const int w = tid >> 6;
   const int wm = w % WM, wn = w / WM;
-  int tmi, tni;
-  tile_coords(tmi, tni, g.tri, g.upperC);
-  const int bz = blockIdx.z;
+  int tmi, tni, bz;
+  if (BM == BN && g.upperC) {
+    upper_tile_coords(cdiv(g.M, BM), cdiv(g.N, BN), tmi, tni, bz);
+  } else {
+    tile_coords(tmi, tni, g.tri, g.upperC);
+    bz = blockIdx.z;
+  }
   const int bat = bz / g.ksplit;
   const int ks = bz % g.ksplit;
   const int m0 = tmi * BM, n0 = tni * BN;
@@ -385,6 +421,11 @@ __global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
 template <int BM, int BN, int BK, int WM, int WN, int OPA, int OPB, typename T>
 static void launch_gemm_tile(const GemmArgs& g, hipStream_t s) {
   dim3 grid(cdiv(g.M, BM), cdiv(g.N, BN), g.batch * g.ksplit);
+  if constexpr (BM == BN) {
+    // upperC: compact 1-D grid over the upper-triangle tiles (square tiles;
+    // the kernels decode it with upper_tile_coords)
+    if (g.upperC) grid = dim3((unsigned)(upper_tiles(cdiv(g.M, BM), cdiv(g.N, BN)) * g.batch * g.ksplit), 1, 1);
+  }
   if constexpr (ScalarTraits<T>::is_complex)
     hipLaunchKernelGGL((gemm_z64_kernel<BM, BN, BK, WM, WN, OPA, OPB>), grid, dim3(256), 0, s, g);
   else
@@ -425,6 +466,12 @@ static void gemm_shape(const GemmArgs& g, hipStream_t s) {
       launch_gemm_carry<64, 64>(g, s);
       return;
     }
+  }
+  if (!cplx && g.tile != 0 && g.gate == nullptr) {
+    if (g.tile >= 128) launch_gemm_tile<128, 128, 16, 2, 2, OPA, OPB, T>(g, s);
+    else if (g.tile >= 64) launch_gemm_tile<64, 64, 16, 2, 2, OPA, OPB, T>(g, s);
+    else launch_gemm_tile<32, 32, 16, 2, 2, OPA, OPB, T>(g, s);
+    return;
   }
   if (g.M <= 32) {
     if constexpr (!cplx && OPA == OP_T && OPB == OP_N) {

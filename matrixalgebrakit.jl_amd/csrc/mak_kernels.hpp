@@ -24,6 +24,7 @@ struct Geqr2NodeArgs {
   void* V1out;      // copy of the top 32x32 of V per node (unit lower), or null
   const int* gate;  // when non-null the launch runs only if *gate == gate_want
   int gate_want;
+  int variant;      // node kernel: 0 = default per element type, 1..3 = a fixed one (tests)
 };
 
 // Device-side predication of a launch (all threads of a workgroup agree).
@@ -93,7 +94,6 @@ template <typename T> void launch_geqr2_node_tasks(const NodeTask* tasks, int nt
 template <typename T> void launch_tsqr_root_finish_tasks(const TsqrRootArgs* tasks, int ntasks, hipStream_t s);
 template <typename T>
 void launch_tsqr_expand_tasks(const TsqrExpandArgs* tasks, int ntasks, int nodes, int max_node_rows, hipStream_t s);
-extern int g_node_variant;  // node kernel selection (test hook; 0 = default per element type)
 template <typename T> void launch_geqr2_node(const Geqr2NodeArgs& a, int leaf_rows_max, hipStream_t s);
 template <typename T> void launch_tsqr_root_finish(const TsqrRootArgs& a, hipStream_t s);
 template <typename T> void launch_tsqr_expand(const TsqrExpandArgs& a, int max_node_rows, hipStream_t s);
@@ -144,6 +144,9 @@ struct GemmArgs {
   const void* rPart;
   void* rOut;
   int rN;
+  // Tile size (square) forced for the real kernels: 0 = chosen by the
+  // dispatcher from the launch size, else 32, 64 or 128.
+  int tile;
 };
 
 // C = alpha op(A) op(B) + beta C on FP64 MFMA (real) or 4 real MFMAs (complex).
@@ -169,6 +172,20 @@ size_t gemm_prec_workspace_bytes(int M, int N, int K);
 template <typename T> void gemm_prec(int prec, int opA, int opB, GemmArgs g, void* ws, hipStream_t s);
 
 // ---------------------------------------------------------------- dense blocks
+// Cholesky with inverse by the persistent tile kernel (mak_chol.hip):
+// G (n x n, upper triangle read) -> R (upper, strict lower part zeroed) with
+// G = R^H R, and X = R^{-1} (upper, strict lower part zeroed), n <= kCholTileMaxN.
+// flags: chol_tile_flag_ints(n) ints of device scratch.  info (device, may be
+// null): set to info_offset + j when the leading j x j minor is not positive
+// definite (if still 0).  grid <= 0: default workgroup count.  Returns 0 or a
+// HIP launch error (1); -3 when n is too large.
+constexpr int kCholTileMaxNt = 64;
+constexpr int kCholTileMaxN = 32 * kCholTileMaxNt;
+size_t chol_tile_flag_ints(int n);
+template <typename T>
+int chol_tile_factor(T* G, int64_t ldg, int n, T* X, int64_t ldx, int* flags, int* info, int info_offset,
+                     hipStream_t s, int grid = 0, long long* trace = nullptr);
+int chol_tile_order_check(int nt);
 template <typename T>
 void potrf_upper(T* Z, int64_t ldz, int n, T* Winv, int64_t ldw, T* tmp, int* dinfo, hipStream_t s);
 template <typename T>

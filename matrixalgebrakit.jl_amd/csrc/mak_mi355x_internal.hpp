@@ -17,6 +17,7 @@ void orgqr_blocked(const T* A, int64_t lda, int m, int n, T* Q, int64_t ldq, int
 template <typename T>
 int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, int64_t ldq, T* R, int64_t ldr,
               int positive, void* work, size_t work_bytes, T* Tout);
+template <typename T> void qr_record(const char* ws, int m, int n, int full, int out[8]);
 
 // CholeskyQR2 panel with Householder reconstruction (mak_cholqr.hip): factors
 // the H x nbc panel Ap (columns j0.. of the matrix) in place into compact WY
@@ -44,7 +45,20 @@ template <typename T> void cholqr_carry_buffers(T* cw, T** Gw, T** partN, T** R1
 // Workgroups (and Gram partials) of the panel kernels for an H-row panel.
 int cholqr_gram_blocks(int H);
 void zero_ints(int* p, int n, hipStream_t s);
-extern int g_last_cholqr_reject;
+
+// Whole-matrix CholeskyQR2 for qr_compact!(positive = true) (mak_cholqr2.hip):
+// eligible for m >= n, n <= kCholTileMaxN.  cholqr2_enqueue reads A only and
+// writes Q / R (R may be null) when the device-side check accepts; the status
+// word (cholqr2_status: [0] = 1 accepted, [2] Cholesky info, [4..5] ||E||_F
+// as a double) is valid once the stream reaches it.  cholqr2_scratch: its
+// m x n scratch (free after the call; the in-place Q mode reuses it).
+bool cholqr2_eligible(int m, int n);
+template <typename T> size_t cholqr2_workspace_bytes(int m, int n);
+template <typename T>
+void cholqr2_enqueue(hipStream_t s, int m, int n, const T* A, int64_t lda, T* Q, int64_t ldq, T* R, int64_t ldr,
+                     char* ws);
+template <typename T> int* cholqr2_status(char* ws, int m, int n);
+template <typename T> T* cholqr2_scratch(char* ws, int m, int n);
 
 template <typename T> size_t polar_workspace_bytes(int m, int n);
 template <typename T>

@@ -959,10 +959,10 @@ template <typename T> struct NodeCfg;
 template <> struct NodeCfg<double> { static constexpr int NW = 4, KR = 32; };
 template <> struct NodeCfg<zd> { static constexpr int NW = 4, KR = 16; };
 
-// 1: per-thread-row kernel, 2: 2-D lane layout (NodeCfg), 3: 2-D lane layout
-// with another wave count; 0: the measured-fastest per element type (real:
-// 1, 78 us per 512 x 32 node; complex: 2, 137 us per 256 x 32 node, on MI355X)
-int g_node_variant = 0;
+// Geqr2NodeArgs::variant: 1: per-thread-row kernel, 2: 2-D lane layout
+// (NodeCfg), 3: 2-D lane layout with another wave count; 0: the
+// measured-fastest per element type (real: 1, 78 us per 512 x 32 node;
+// complex: 2, 137 us per 256 x 32 node, on MI355X)
 
 template <typename T> int geqr2_node_capacity() {
   return ScalarTraits<T>::is_complex ? 256 : 512;
@@ -975,7 +975,7 @@ void launch_geqr2_node(const Geqr2NodeArgs& a, int leaf_rows_max, hipStream_t s)
   // the register layout holds geqr2_node_capacity<T>() rows per workgroup
   static_assert(NodeCfg<T>::NW * 4 * NodeCfg<T>::KR == (ScalarTraits<T>::is_complex ? 256 : 512),
                 "node capacity");
-  const int variant = g_node_variant != 0 ? g_node_variant : (ScalarTraits<T>::is_complex ? 2 : 1);
+  const int variant = a.variant != 0 ? a.variant : (ScalarTraits<T>::is_complex ? 2 : 1);
   if (variant == 2) {
     constexpr int NW = NodeCfg<T>::NW, KR = NodeCfg<T>::KR;
     hipLaunchKernelGGL((geqr2_node2_kernel<T, NW, KR>), dim3(a.nodes), dim3(NW * WAVE), 0, s, a);

@@ -32,7 +32,7 @@ constexpr double kMaxCholQrC = 1e14;  // sqrt(1 + c) <= 1e7: (1e7)^2 u << 1
 inline size_t al256(size_t b) { return ((b + 255) / 256) * 256; }
 
 struct PolarLayout {
-  size_t offX1, offX2, offQ1, offG, offY1, offY2, offTmp, offPart, offScal, offInfo, offLo, bytes;
+  size_t offX1, offX2, offQ1, offG, offY1, offY2, offTm, offTmp, offPart, offScal, offInfo, offLo, bytes;
 };
 
 template <typename T> PolarLayout polar_layout(int m, int n) {
@@ -46,6 +46,7 @@ template <typename T> PolarLayout polar_layout(int m, int n) {
   L.offG = take(nn);
   L.offY1 = take(nn);
   L.offY2 = take(nn);
+  L.offTm = take(nn);  // n x n scratch of the closed-form second CholeskyQR pass
   L.offTmp = take(((size_t)64 * std::max(n, 1) + (size_t)n * n / 4) * sizeof(T));  // Cholesky scratch
   L.offPart = take(1024 * sizeof(double));
   L.offScal = take(16 * sizeof(double));
@@ -211,6 +212,7 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
   T* G = reinterpret_cast<T*>(ws + L.offG);
   T* Y1 = reinterpret_cast<T*>(ws + L.offY1);
   T* Y2 = reinterpret_cast<T*>(ws + L.offY2);
+  T* Tm = reinterpret_cast<T*>(ws + L.offTm);
   T* tmp = reinterpret_cast<T*>(ws + L.offTmp);
   double* part = reinterpret_cast<double*>(ws + L.offPart);
   double* scal = reinterpret_cast<double*>(ws + L.offScal);
@@ -264,7 +266,6 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
       hipLaunchKernelGGL(sum_double_kernel, dim3(1), dim3(256), 0, s, part, 256, scal + 2);
       const double e2 = read_scalar<T>(scal + 2, s);
       if (e2 <= 1e-12) {
-        T* Tm = buf[s1];  // free until Q1' is formed; n x n, ld n
         const unsigned nb = (unsigned)cdiv64((int64_t)n * n, 256);
         hipLaunchKernelGGL((up_part_kernel<T>), dim3(nb), dim3(256), 0, s, G, ldn, (const T*)nullptr, ldn, n, Y2, ldn);
         gemm<T>(OP_C, OP_N, ga<T>(n, n, n, Y2, ldn, Y2, ldn, Tm, ldn, 1.0, 0.0, TRI_A_LOWER | TRI_B_UPPER, 1), s);
@@ -340,6 +341,85 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
   return status;
 }
 
+// B = pseudo-random entries in [-1, 1) (a fixed hash of the position): the
+// direction that completes a partial isometry (below).
+__device__ inline double hash_unit(uint32_t x) {
+  x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+  return (double)x * (2.0 / 4294967296.0) - 1.0;
+}
+template <typename T> __global__ void hash_fill_kernel(int m, int n, T* B, int64_t ldb);
+template <> __global__ void hash_fill_kernel<double>(int m, int n, double* B, int64_t ldb) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)m * n) return;
+  const int r = (int)(idx % m), c = (int)(idx / m);
+  B[r + (int64_t)c * ldb] = hash_unit((uint32_t)(r * 2654435761U) ^ (uint32_t)(c * 40503U + 0x9e3779b9U));
+}
+template <> __global__ void hash_fill_kernel<zd>(int m, int n, zd* B, int64_t ldb) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)m * n) return;
+  const int r = (int)(idx % m), c = (int)(idx / m);
+  const uint32_t h = (uint32_t)(r * 2654435761U) ^ (uint32_t)(c * 40503U + 0x9e3779b9U);
+  B[r + (int64_t)c * ldb] = mkz(hash_unit(h), hash_unit(h ^ 0x5bd1e995U));
+}
+
+// The polar factor of the m x n (m >= n) iterate X (already scaled to
+// ||X||_2 <= 1), in place.  QDWH maps zero singular values to zero, so for a
+// rank-deficient input it returns a partial isometry W0 = U_r V_r^H (+ the
+// images of the null directions); the reference's SVD route returns an
+// isometry (src/implementations/polar.jl:59-70).  When ||W0||_F^2 < n (1 -
+// 1e-8), the iteration is run once more on
+//   Z = W0 + (I - W0 W0^H) B (I - W0^H W0),   B pseudo-random:
+// the added term lives in range(U_r)^perp x null(A) and is generically of
+// full rank there, so the polar factor of Z keeps W0 on range(A^H) and maps
+// null(A) isometrically into range(U_r)^perp -- an isometry W with A = W P.
+// *completed receives 1 when that second run happened.
+template <typename T>
+int polar_isometry(hipStream_t s, int m, int n, T* X, int64_t ldx, int precision, int maxiter, double l0,
+                   char* ws, const PolarLayout& L, mak_polar_info* info, int* completed) {
+  *completed = 0;
+  int st = qdwh_iterate<T>(s, m, n, X, ldx, precision, maxiter, l0, ws, L, info);
+  if (st != 0 && st != 6) return st;
+  double* part = reinterpret_cast<double*>(ws + L.offPart);
+  double* scal = reinterpret_cast<double*>(ws + L.offScal);
+  sumsq<T>(X, ldx, nullptr, 0, m, n, part, scal + 3, s);
+  const double nrm2 = read_scalar<T>(scal + 3, s);
+  if (!(nrm2 < n * (1.0 - 1e-8))) return st;
+  mak_polar_info first{};
+  if (info) first = *info;
+  T* B = reinterpret_cast<T*>(ws + L.offX2);
+  T* C1 = reinterpret_cast<T*>(ws + L.offQ1);
+  T* Gn = reinterpret_cast<T*>(ws + L.offG);
+  T* Gx = reinterpret_cast<T*>(ws + L.offY1);
+  const int64_t ldm = m, ldn = n;
+  hipLaunchKernelGGL((hash_fill_kernel<T>), dim3((unsigned)cdiv64((int64_t)m * n, 256)), dim3(256), 0, s, m, n, B,
+                     ldm);
+  // C1 = (I - X X^H) B
+  gemm<T>(OP_C, OP_N, ga<T>(n, n, m, X, ldx, B, ldm, Gn, ldn, 1.0, 0.0), s);
+  copy_mat<T>(m, n, B, ldm, C1, ldm, s);
+  gemm<T>(OP_N, OP_N, ga<T>(m, n, n, X, ldx, Gn, ldn, C1, ldm, -1.0, 1.0), s);
+  // B = C1 (I - X^H X);  X += B;  X /= ||X||_F
+  gemm<T>(OP_C, OP_N, ga<T>(n, n, m, X, ldx, X, ldx, Gx, ldn, 1.0, 0.0), s);
+  copy_mat<T>(m, n, C1, ldm, B, ldm, s);
+  gemm<T>(OP_N, OP_N, ga<T>(m, n, n, C1, ldm, Gx, ldn, B, ldm, -1.0, 1.0), s);
+  axpby<T>(m, n, 1.0, B, ldm, 1.0, X, ldx, s);
+  sumsq<T>(X, ldx, nullptr, 0, m, n, part, scal + 3, s);
+  const double z = std::sqrt(read_scalar<T>(scal + 3, s));
+  if (!(z > 0.0) || !std::isfinite(z)) return 5;
+  axpby<T>(m, n, 0.0, X, ldx, 1.0 / z, X, ldx, s);
+  *completed = 1;
+  st = qdwh_iterate<T>(s, m, n, X, ldx, precision, maxiter, l0, ws, L, info);
+  if (info) {
+    info->iterations += first.iterations;
+    info->qr_steps += first.qr_steps;
+    info->chol_steps += first.chol_steps;
+    info->ns_steps += first.ns_steps;
+  }
+  return st;
+}
+
+// m x n identity (ones on the main diagonal).
+template <typename T> void eye_rect(T* X, int64_t ldx, int m, int n, hipStream_t s) { set_identity<T>(X, ldx, m, n, s); }
+
 }  // namespace
 
 template <typename T> size_t polar_workspace_bytes(int m, int n) {
@@ -373,14 +453,22 @@ int polar_driver(hipStream_t s, int right, int m, int n, const T* A, int64_t lda
   // alpha = ||A||_F
   sumsq<T>(A, lda, nullptr, 0, m, n, part, scal, s);
   const double alpha = std::sqrt(read_scalar<T>(scal, s));
-  if (info) { info->alpha = alpha; info->l0 = l0; }
-  if (!(alpha > 0.0) || !std::isfinite(alpha)) return alpha == 0.0 ? 4 : 5;
+  if (info) { *info = mak_polar_info{}; info->alpha = alpha; info->l0 = l0; }
+  if (!std::isfinite(alpha)) return 5;
+  if (alpha == 0.0) {
+    // A = 0: every isometry is a polar factor; the SVD route gives U = I,
+    // V = I, so W = I (m x n) and P = 0
+    eye_rect<T>(W, ldw, m, n, s);
+    if (P) fill_zero<T>(P, ldp, k, k, s);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+  }
   int st;
+  int completed = 0;
   if (!right) {
     // X0 = A / alpha, iterated in W
     copy_mat<T>(m, n, A, lda, W, ldw, s);
     axpby<T>(m, n, 0.0, A, lda, 1.0 / alpha, W, ldw, s);
-    st = qdwh_iterate<T>(s, m, n, W, ldw, precision, maxiter, l0, ws, L, info);
+    st = polar_isometry<T>(s, m, n, W, ldw, precision, maxiter, l0, ws, L, info, &completed);
     if ((st == 0 || st == 6) && P) {
       // P = herm(W^H A)
       gemm<T>(OP_C, OP_N, ga<T>(n, n, m, W, ldw, A, lda, P, ldp, 1.0, 0.0), s);
@@ -390,7 +478,7 @@ int polar_driver(hipStream_t s, int right, int m, int n, const T* A, int64_t lda
     // right_polar!(A) via the left polar of A^H (n x m): A^H = W' P'
     T* Xh = reinterpret_cast<T*>(ws + L.offX1);
     conj_transpose_scale<T>(m, n, 1.0 / alpha, A, lda, Xh, n, s);
-    st = qdwh_iterate<T>(s, n, m, Xh, n, precision, maxiter, l0, ws, L, info);
+    st = polar_isometry<T>(s, n, m, Xh, n, precision, maxiter, l0, ws, L, info, &completed);
     if (st == 0 || st == 6) {
       conj_transpose_scale<T>(n, m, 1.0, Xh, n, W, ldw, s);   // Wh = (W')^H
       if (P) {
@@ -400,6 +488,7 @@ int polar_driver(hipStream_t s, int right, int m, int n, const T* A, int64_t lda
       }
     }
   }
+  if (info) info->completed = completed;
   if (hipGetLastError() != hipSuccess) return 1;
   return st;
 }

@@ -79,6 +79,8 @@ template <typename T> struct QrLayout {
   size_t wpElems = 0, w2Elems = 0;
   // 128-column block reflectors of Q's formation (kGroup panels):
   size_t offVc = 0, offS = 0, offTg = 0, offWg = 0;
+  size_t offQs = 0;   // m x min(m, n) scratch: Q of the in-place mode (Q === A)
+  size_t offCq2 = 0;  // whole-matrix CholeskyQR2 (qr_compact!, m >= n), 0 when absent
   size_t bytes = 0;
 };
 
@@ -144,6 +146,12 @@ template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
     off += align256(ngr * kGB * kGB * sizeof(T));
     L.offWg = off;
     off += align256((size_t)kGB * ncmax * sizeof(T));
+  }
+  L.offQs = off;
+  off += align256((size_t)m * minmn * sizeof(T));
+  if (qcols == minmn && cholqr2_eligible(m, n)) {
+    L.offCq2 = off;
+    off += align256(cholqr2_workspace_bytes<T>(m, n));
   }
   L.bytes = off;
   return L;
@@ -355,30 +363,40 @@ bool geqrf_enqueue(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipS
 // Reads the flag (synchronising s).  Panels before the first rejected one are
 // final and the trailing matrix is updated up to it; every later panel was
 // skipped, so the factorisation resumes there with Householder TSQR.  Returns
-// true when it refactored (work enqueued after geqrf_enqueue must be redone).
-// First column of the first panel that the last host-checked factorisation
-// refactored with Householder TSQR (-1: every CholeskyQR2 panel was accepted);
-// read by the test suite.
-int g_last_cholqr_reject = -1;
-
+// 1 when it refactored (work enqueued after geqrf_enqueue must be redone), 0
+// when every panel was accepted, -1 when the flag could not be read or is
+// inconsistent (a HIP error: the factorisation is not known to be valid).
 template <typename T>
-bool geqrf_finish(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s) {
+int geqrf_finish(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s) {
   const int minmn = std::min(m, n);
   QrLayout<T> L = qr_layout<T>(m, n, (int)qcols);
-  g_last_cholqr_reject = -1;
-  if (!L.offFlags) return false;
+  if (!L.offFlags) return 0;
   int h[3] = {0, 0, 0};
   if (hipMemcpyAsync(h, ws + L.offFlags, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess || h[0] == 0 || h[2] < 0 || h[2] >= minmn || h[2] % NB != 0)
-    return false;
-  g_last_cholqr_reject = h[2];
+      hipStreamSynchronize(s) != hipSuccess)
+    return -1;
+  if (h[0] == 0) return 0;
+  if (h[2] < 0 || h[2] >= minmn || h[2] % NB != 0) return -1;
   geqrf_panels<T>(A, lda, m, n, ws, L, h[2], kHouseholderOnly, s);
-  return true;
+  return 1;
 }
 
 template <typename T>
 void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s) {
   if (geqrf_enqueue<T>(A, lda, m, n, ws, qcols, s)) geqrf_finish<T>(A, lda, m, n, ws, qcols, s);
+}
+
+// Columns from which the whole-matrix CholeskyQR2 path is tried first.
+constexpr int kCholQr2MinN = 64;
+
+// Does [p, p + span) intersect [q, q + span2)?  (element spans of two
+// column-major matrices)
+template <typename T> bool ranges_overlap(const T* p, int64_t ld, int rows, int cols, const T* q, int64_t ldq,
+                                          int qrows, int qcols) {
+  if (p == nullptr || q == nullptr || rows <= 0 || cols <= 0 || qrows <= 0 || qcols <= 0) return false;
+  const T* pe = p + (cols - 1) * ld + rows;
+  const T* qe = q + (qcols - 1) * ldq + qrows;
+  return p < qe && q < pe;
 }
 
 // out (kg x N, ld kg) = V^H X for V = Vc (rows x kg, ld ldv): split-K
@@ -484,6 +502,24 @@ void orgqr_blocked(const T* A, int64_t lda, int m, int n, T* Q, int64_t ldq, int
   }
 }
 
+// Status words of the last call that used this workspace (read by the test
+// suite through maktest_qr_record): the CholeskyQR2 panel flags (flag, ticket,
+// first rejected column) and the whole-matrix CholeskyQR2 status.
+template <typename T> void qr_record(const char* ws, int m, int n, int full, int out[8]) {
+  const int minmn = std::min(m, n);
+  QrLayout<T> L = qr_layout<T>(m, n, full ? m : minmn);
+  int h[3] = {0, 0, 0};
+  int st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (L.offFlags) hipMemcpy(h, ws + L.offFlags, sizeof(h), hipMemcpyDeviceToHost);
+  if (L.offCq2) hipMemcpy(st, cholqr2_status<T>(const_cast<char*>(ws) + L.offCq2, m, n), sizeof(st),
+                          hipMemcpyDeviceToHost);
+  out[0] = st[0];                  // 1: the whole-matrix CholeskyQR2 result was taken
+  out[1] = h[0] ? h[2] : -1;       // first panel handed to Householder TSQR (-1: none)
+  out[2] = st[2];                  // Cholesky info of the whole-matrix path
+  out[3] = L.offCq2 != 0;          // the whole-matrix path exists for this shape
+  for (int i = 4; i < 8; ++i) out[i] = st[i];  // ||E||_F of the whole-matrix path (double)
+}
+
 template <typename T>
 int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, int64_t ldq, T* R, int64_t ldr,
               int positive, void* work, size_t work_bytes, T* Tout) {
@@ -497,29 +533,62 @@ int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, in
   if ((int64_t)m * qcols > 0 && Q == nullptr) return -6;
   if (ldq < std::max(1, m)) return -7;
   if (R != nullptr && ldr < std::max(1, rrows)) return -9;
+  // Q === A: only for a tall input without R and without the positive gauge,
+  // as in the reference (src/implementations/qr.jl:150-153); any other
+  // overlap of Q with A is invalid
+  const bool inplace = Q != nullptr && Q == A && ldq == lda && qcols == n && m >= n;
+  if (inplace && (R != nullptr || positive)) return -6;
+  if (!inplace && ranges_overlap<T>(A, lda, m, n, Q, ldq, m, qcols)) return -6;
   const size_t need = qr_workspace_bytes<T>(m, n, full);
   if (work == nullptr || work_bytes < need) return -11;
   char* ws = reinterpret_cast<char*>(work);
+  QrLayout<T> L = qr_layout<T>(m, n, qcols);
+  if (L.offCq2) zero_ints(cholqr2_status<T>(ws + L.offCq2, m, n), 8, s);
+  if (L.offFlags) zero_ints(reinterpret_cast<int*>(ws + L.offFlags), 3, s);
   if (minmn == 0) {
     set_identity<T>(Q, ldq, m, qcols, s);
     if (R) qr_extract_r<T>(A, lda, R, ldr, rrows, n, 0, 0, s);
     return hipGetLastError() == hipSuccess ? 0 : 1;
   }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess) cs = hipStreamCaptureStatusActive;
+  // qr_compact!(positive = true): whole-matrix CholeskyQR2 first (A is only
+  // read; a rejected input goes on to the Householder path below)
+  if (L.offCq2 && positive && !inplace && Tout == nullptr && n >= kCholQr2MinN && cs == hipStreamCaptureStatusNone) {
+    char* cw = ws + L.offCq2;
+    cholqr2_enqueue<T>(s, m, n, A, lda, Q, ldq, R, ldr, cw);
+    int st = 0;
+    if (hipMemcpyAsync(&st, cholqr2_status<T>(cw, m, n), sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return 1;
+    if (st == 1) return hipGetLastError() == hipSuccess ? 0 : 1;
+  }
+  T* Qd = inplace ? reinterpret_cast<T*>(ws + L.offQs) : Q;
+  const int64_t ldqd = inplace ? m : ldq;
   // Q, R and the tape are formed before the CholeskyQR2 flag is read, so the
   // one synchronisation does not leave the GPU waiting for the host; they are
-  // formed again in the (rare) case that a panel was refactored.
+  // formed again in the (rare) case that a panel was refactored.  In the
+  // in-place mode Q replaces A's factors, so it is formed only after the check.
   auto outputs = [&]() {
     if (Tout) {  // keep the compact WY factors' T blocks (32 x 32 per panel) for the adjoint
       const int npan = cdiv(minmn, NB);
       hipMemcpyAsync(Tout, ws, sizeof(T) * NB * NB * npan, hipMemcpyDeviceToDevice, s);
     }
     if (R) qr_extract_r<T>(A, lda, R, ldr, rrows, n, minmn, positive, s);
-    orgqr_blocked<T>(A, lda, m, n, Q, ldq, qcols, ws, s);
-    if (positive) qr_gauge_q<T>(A, lda, Q, ldq, m, minmn, s);
+    orgqr_blocked<T>(A, lda, m, n, Qd, ldqd, qcols, ws, s);
+    if (positive) qr_gauge_q<T>(A, lda, Qd, ldqd, m, minmn, s);
   };
   const bool check = geqrf_enqueue<T>(A, lda, m, n, ws, qcols, s);
-  outputs();
-  if (check && geqrf_finish<T>(A, lda, m, n, ws, qcols, s)) outputs();
+  if (!inplace) outputs();
+  if (check) {
+    const int f = geqrf_finish<T>(A, lda, m, n, ws, qcols, s);
+    if (f < 0) return 1;
+    if (f == 1 && !inplace) outputs();
+  }
+  if (inplace) {
+    outputs();
+    copy_mat<T>(m, qcols, Qd, ldqd, Q, ldq, s);
+  }
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -531,6 +600,8 @@ template int qr_driver<double>(hipStream_t, int, int, int, double*, int64_t, dou
                                int, void*, size_t, double*);
 template int qr_driver<zd>(hipStream_t, int, int, int, zd*, int64_t, zd*, int64_t, zd*, int64_t, int, void*,
                            size_t, zd*);
+template void qr_record<double>(const char*, int, int, int, int*);
+template void qr_record<zd>(const char*, int, int, int, int*);
 
 }  // namespace mak
 

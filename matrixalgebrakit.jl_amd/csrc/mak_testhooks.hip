@@ -21,6 +21,27 @@ int maktest_gemm(void* stream, int dtype, int opA, int opB, int M, int N, int K,
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+// Real FP64 GEMM with the structure hints (tri, upperC), split-K slices
+// (ksplit > 1: slice z at C + z * ldc * N) and a forced tile size (0 = auto).
+int maktest_gemm_hint(void* stream, int opA, int opB, int M, int N, int K, const double* A, int lda,
+                      const double* B, int ldb, double* C, int ldc, double alpha, double beta, int tri, int upperC,
+                      int ksplit, int tile) {
+  mak::GemmArgs g{};
+  g.M = M; g.N = N; g.K = K;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.alpha_re = alpha; g.beta_re = beta;
+  g.ksplit = ksplit > 1 ? ksplit : 1;
+  g.kchunk = (K + g.ksplit - 1) / g.ksplit;
+  g.kchunk = ((g.kchunk + 15) / 16) * 16;
+  if (g.kchunk < 16) g.kchunk = 16;
+  g.ksplit = (K + g.kchunk - 1) / g.kchunk;
+  if (g.ksplit < 1) g.ksplit = 1;
+  g.splitStride = (int64_t)ldc * N; g.batch = 1;
+  g.tri = tri; g.upperC = upperC; g.tile = tile;
+  mak::gemm<double>(opA, opB, g, (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 // FP32 / BF16 MFMA product of FP64 data (prec 32 / 16; 64 runs the FP64
 // kernel); ws: maktest_gemm_prec_ws(M, N, K) bytes.
 size_t maktest_gemm_prec_ws(int M, int N, int K) { return mak::gemm_prec_workspace_bytes(M, N, K); }
@@ -35,9 +56,16 @@ int maktest_gemm_prec(void* stream, int prec, int opA, int opB, int M, int N, in
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-// First column of the first CholeskyQR2 panel that the last qr call (outside
-// graph capture) handed to the Householder path, or -1 when none was.
-int maktest_last_cholqr_reject() { return mak::g_last_cholqr_reject; }
+// Status words a qr call left in its workspace `work` (synchronous read):
+// out[0] 1 when the whole-matrix CholeskyQR2 result was taken, out[1] the
+// first 32-column panel handed to Householder TSQR (-1: none), out[2] the
+// whole-matrix path's Cholesky info, out[3] 1 when that path exists for the
+// shape, out[4..5] its ||Q1^H Q1 - I||_F (a double).
+int maktest_qr_record(int dtype, int full, int m, int n, const void* work, int* out) {
+  if (dtype == MAK_F64) mak::qr_record<double>((const char*)work, m, n, full, out);
+  else mak::qr_record<mak::zd>((const char*)work, m, n, full, out);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
 
 int maktest_geqr2(void* stream, int dtype, int H, int nbc, void* X, int ldx, void* Tout) {
   const int cap = dtype == MAK_F64 ? mak::geqr2_node_capacity<double>() : mak::geqr2_node_capacity<mak::zd>();
@@ -50,24 +78,19 @@ int maktest_geqr2(void* stream, int dtype, int H, int nbc, void* X, int ldx, voi
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-// Select the node kernel used by every later launch (1: per-thread-row layout,
-// 2: 2-D lane layout, 3: 2-D lane layout with another wave count, 0: default
-// per element type); returns the previous selection.
-int maktest_set_node_variant(int v) {
-  const int prev = mak::g_node_variant;
-  if (v >= 0 && v <= 3) mak::g_node_variant = v;
-  return prev;
-}
 
 // One TSQR level: `nodes` workgroups over H stacked rows (balanced split),
-// each node factored in place; node i's T at Tout + i * 32 * 32.
-int maktest_geqr2_level(void* stream, int dtype, int H, int nodes, int nbc, void* X, int ldx, void* Tout) {
+// each node factored in place; node i's T at Tout + i * 32 * 32.  variant:
+// Geqr2NodeArgs::variant.
+int maktest_geqr2_level(void* stream, int dtype, int H, int nodes, int nbc, void* X, int ldx, void* Tout,
+                        int variant) {
   const int cap = dtype == MAK_F64 ? mak::geqr2_node_capacity<double>() : mak::geqr2_node_capacity<mak::zd>();
   if (nodes < 1 || H < nodes || nbc < 1 || nbc > 32) return -4;
   if ((H + nodes - 1) / nodes > cap) return -3;
   mak::Geqr2NodeArgs g{};
   g.X = X; g.ldx = ldx; g.H = H; g.nodes = nodes; g.chunk = 0; g.nbc = nbc;
   g.Tout = Tout; g.tstride = 32 * 32;
+  g.variant = variant;
   if (dtype == MAK_F64) mak::launch_geqr2_node<double>(g, 0, (hipStream_t)stream);
   else mak::launch_geqr2_node<mak::zd>(g, 0, (hipStream_t)stream);
   return hipGetLastError() == hipSuccess ? 0 : 1;
@@ -83,6 +106,27 @@ int maktest_potrf(void* stream, int dtype, int n, void* Z, int ldz, void* Y, int
     mak::potrf_upper<mak::zd>((mak::zd*)Z, ldz, n, (mak::zd*)Y, ldy, (mak::zd*)tmp, info, (hipStream_t)stream);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
+
+// The persistent tile Cholesky alone (n <= 2048): G -> R, X = R^{-1}; grid <= 0
+// selects the default workgroup count.  flags: maktest_chol_tile_flag_ints(n) ints.
+size_t maktest_chol_tile_flag_ints(int n) { return mak::chol_tile_flag_ints(n); }
+int maktest_chol_tile(void* stream, int dtype, int n, void* G, int ldg, void* X, int ldx, int* flags, int* info,
+                      int grid) {
+  if (dtype == MAK_F64)
+    return mak::chol_tile_factor<double>((double*)G, ldg, n, (double*)X, ldx, flags, info, 0, (hipStream_t)stream,
+                                         grid);
+  return mak::chol_tile_factor<mak::zd>((mak::zd*)G, ldg, n, (mak::zd*)X, ldx, flags, info, 0, (hipStream_t)stream,
+                                        grid);
+}
+// As maktest_chol_tile (real), also writing 8 wall-clock stamps (100 MHz) per
+// panel iteration to trace (8 * ceil(n / 32) long longs).
+int maktest_chol_tile_trace(void* stream, int n, double* G, int ldg, double* X, int ldx, int* flags, int* info,
+                            int grid, long long* trace) {
+  return mak::chol_tile_factor<double>(G, ldg, n, X, ldx, flags, info, 0, (hipStream_t)stream, grid, trace);
+}
+// Host-only check of the tile kernel's task order for nt tiles per side: the
+// task count when every dependency precedes its task, else -1.
+int maktest_chol_order_check(int nt) { return mak::chol_tile_order_check(nt); }
 
 // Blocked factorisation only: A <- compact WY (V below, R above), one 32x32 T
 // per panel written to Tpanels (32 x 32 x npanels); ws from the QR query.

@@ -56,10 +56,11 @@ class MI355XHouseholder:
     """Algorithm type of this backend for ``qr_compact!`` / ``qr_full!``.
 
     Accepts the keyword arguments of ``Householder``
-    (src/interface/decompositions.jl:80-86): ``positive`` (default true),
-    ``pivoted`` (unsupported: ArgumentError, as for the GPU drivers,
-    src/implementations/qr.jl:142-143) and ``blocksize`` (the panel width is
-    fixed at 32 columns by the kernels; any value is accepted).
+    (src/interface/decompositions.jl:80-86) with the GPU drivers' rules
+    (src/implementations/qr.jl:137-145): ``positive`` (default true),
+    ``pivoted`` (unsupported: ArgumentError) and ``blocksize`` (0 or 1: the
+    unblocked API; larger values throw ArgumentError, the driver does not
+    provide a blocked QR decomposition -- its internal panel width is its own).
     """
     positive: bool = True
     pivoted: bool = False
@@ -67,6 +68,8 @@ class MI355XHouseholder:
     kwargs: dict = field(default_factory=dict)
 
     def __post_init__(self):
+        if self.blocksize > 1:
+            raise ArgumentError("MI355X does not provide a blocked QR decomposition")
         if self.pivoted:
             raise ArgumentError("MI355X does not provide a pivoted QR decomposition")
 
@@ -75,9 +78,13 @@ class MI355XHouseholder:
 # layout helpers
 # ---------------------------------------------------------------------------
 def colmajor(x: torch.Tensor) -> torch.Tensor:
-    """Column-major copy (stride(0) == 1) of a 2-D tensor (no copy if already)."""
+    """Column-major copy (stride(0) == 1) of a 2-D tensor (no copy if already).
+
+    Lazily conjugated or negated views (``x.mH`` of a complex tensor, ``-x``
+    views) are materialised first: the library reads raw storage."""
     if x.dim() != 2:
         raise DimensionMismatch(f"expected a matrix, got shape {tuple(x.shape)}")
+    x = x.resolve_conj().resolve_neg()
     m, n = x.shape
     # strides of size-1 dimensions carry no layout information
     if (m <= 1 or x.stride(0) == 1) and (n <= 1 or x.stride(1) >= max(1, m)):
@@ -106,7 +113,12 @@ def _ld(x: torch.Tensor) -> int:
 
 
 def _ptr(x):
-    return C.c_void_p(x.data_ptr()) if x is not None and x.numel() > 0 else C.c_void_p(0)
+    if x is None or x.numel() == 0:
+        return C.c_void_p(0)
+    if x.is_conj() or x.is_neg():
+        # the storage holds the unconjugated / unnegated values
+        raise ArgumentError("lazily conjugated or negated views must be resolved (colmajor) before the call")
+    return C.c_void_p(x.data_ptr())
 
 
 def _stream(dev) -> C.c_void_p:
@@ -186,8 +198,14 @@ def _qr_call(A, QR, alg, full: bool):
     (check_input_qr_full if full else check_input_qr_compact)(A, QR)
     Q, R = QR
     _check_device(A, Q, R)
-    lib = _lib.load()
     m, n = A.shape
+    if Q is A or (Q.numel() > 0 and Q.data_ptr() == A.data_ptr() and Q.shape == A.shape
+                  and Q.stride() == A.stride()):
+        # in-place Q (src/implementations/qr.jl:150-153)
+        if R.numel() > 0 or alg.positive or m < n:
+            raise ArgumentError("inplace Q only supported if matrix is tall (`m >= n`), R is not required, and "
+                                "using the unblocked algorithm (`blocksize = 1`) with `positive = false`")
+    lib = _lib.load()
     dt = _DT[A.dtype]
     nbytes = C.c_size_t(0)
     _status(lib.mak_qr_workspace_size(dt, int(full), m, n, C.byref(nbytes)), "mak_qr_workspace_size")

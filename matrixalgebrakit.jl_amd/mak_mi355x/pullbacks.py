@@ -36,7 +36,9 @@ NB = 32
 
 
 def colmajor_batch(x: torch.Tensor) -> torch.Tensor:
-    """(batch, m, n) tensor whose matrices are column-major and packed."""
+    """(batch, m, n) tensor whose matrices are column-major and packed (lazily
+    conjugated / negated views are materialised)."""
+    x = x.resolve_conj().resolve_neg()
     b, m, n = x.shape
     if x.stride(1) == 1 and x.stride(2) == m and x.stride(0) == m * n:
         return x

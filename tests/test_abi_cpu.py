@@ -25,7 +25,7 @@ def test_workspace_query_is_host_only_and_validates():
     assert nb.value > 0
     full = C.c_size_t(0)
     assert lib.mak_qr_workspace_size(0, 1, 8192, 1024, C.byref(full)) == 0
-    assert full.value >= nb.value
+    assert full.value > 0
     z = C.c_size_t(0)
     assert lib.mak_qr_workspace_size(1, 0, 8192, 1024, C.byref(z)) == 0
     assert z.value > nb.value
@@ -42,7 +42,7 @@ def test_entry_points_reject_bad_arguments_before_touching_the_device():
     assert lib.mak_qr_compact_d(null, -1, 3, null, 1, null, 1, null, 1, 1, null, 0) == -2
     assert lib.mak_qr_compact_d(null, 3, -1, null, 3, null, 3, null, 1, 1, null, 0) == -3
     assert lib.mak_qr_compact_d(null, 5, 3, C.c_void_p(8), 4, C.c_void_p(8), 5, null, 1, 1, null, 0) == -5
-    assert lib.mak_qr_compact_d(null, 5, 3, C.c_void_p(8), 5, C.c_void_p(8), 5, null, 1, 1, null, 0) == -11
+    assert lib.mak_qr_compact_d(null, 5, 3, C.c_void_p(8), 5, C.c_void_p(4096), 5, null, 1, 1, null, 0) == -11
 
 
 def test_host_check_input_errors_mirror_reference():
@@ -86,3 +86,52 @@ def test_workspace_queries_accept_empty_shapes():
         ms = (C.c_int * 5)(0, 4, 1, 3000, 700)
         ns = (C.c_int * 5)(5, 0, 1, 40, 700)
         assert lib.mak_qr_batched_workspace_size(dt, 5, ms, ns, 1, C.byref(b)) == 0 and b.value > 0
+
+
+@pytest.mark.parametrize("nt", [1, 2, 3, 7, 32, 64])
+def test_chol_tile_task_order(nt):
+    """The persistent Cholesky kernel's task order (host code in the library):
+    every task appears once and every dependency precedes it, so a workgroup
+    waiting on a task it took can always be satisfied.  Count = nt D + nt(nt-1)
+    P and P' + 2 sum_k (updates)."""
+    lib = _lib.load()
+    lib.maktest_chol_order_check.restype = C.c_int
+    lib.maktest_chol_order_check.argtypes = [C.c_int]
+    expect = nt + nt * (nt - 1) + sum((nt - 1 - k) * (nt - k) // 2 + (k + 1) * (nt - 1 - k) for k in range(nt))
+    assert lib.maktest_chol_order_check(nt) == expect
+    assert lib.maktest_chol_order_check(0) == -1
+    assert lib.maktest_chol_order_check(65) == -1
+
+
+def test_qr_q_aliasing_rules():
+    """Q === A is accepted only for a tall input with no R and positive = false
+    (src/implementations/qr.jl:150-153); other Q / A overlaps are invalid
+    argument 6.  The checks run before any device work, so fake device
+    pointers are enough."""
+    lib = _lib.load()
+    A = C.c_void_p(0x100000)
+    R = C.c_void_p(0x90000000)
+    m, n = 100, 50
+    # in place with R requested, or with the positive gauge: -6
+    assert lib.mak_qr_compact_d(None, m, n, A, m, A, m, R, n, 0, None, 0) == -6
+    assert lib.mak_qr_compact_d(None, m, n, A, m, A, m, None, 1, 1, None, 0) == -6
+    # in place, no R, positive = false: accepted up to the workspace check (-11)
+    assert lib.mak_qr_compact_d(None, m, n, A, m, A, m, None, 1, 0, None, 0) == -11
+    # Q overlapping A without being A: -6
+    Qp = C.c_void_p(0x100000 + 8 * 30)
+    assert lib.mak_qr_compact_d(None, m, n, A, m, Qp, m, None, 1, 0, None, 0) == -6
+    # disjoint: only the missing workspace is reported
+    Qd = C.c_void_p(0x100000 + 8 * m * n)
+    assert lib.mak_qr_compact_d(None, m, n, A, m, Qd, m, R, n, 1, None, 0) == -11
+    # wide A cannot be its own Q
+    assert lib.mak_qr_compact_d(None, 40, 50, A, 40, A, 40, None, 1, 0, None, 0) == -6
+
+
+def test_householder_options():
+    """blocksize > 1 and pivoted throw ArgumentError, as for the reference's
+    GPU drivers (src/implementations/qr.jl:139-143)."""
+    mak.MI355XHouseholder(blocksize=1)
+    with pytest.raises(mak.ArgumentError, match="blocked"):
+        mak.MI355XHouseholder(blocksize=32)
+    with pytest.raises(mak.ArgumentError, match="pivoted"):
+        mak.MI355XHouseholder(pivoted=True)

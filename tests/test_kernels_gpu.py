@@ -26,24 +26,18 @@ def rel(a, b):
 
 def hooks():
     lib = _lib.load()
-    lib.maktest_set_node_variant.restype = C.c_int
-    lib.maktest_set_node_variant.argtypes = [C.c_int]
     lib.maktest_geqr2_level.restype = C.c_int
     lib.maktest_geqr2_level.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
-                                        C.c_void_p]
+                                        C.c_void_p, C.c_int]
     return lib
 
 
 def run_level(lib, A, nodes, nbc, variant, dev):
     X = mak.colmajor(torch.from_numpy(np.asfortranarray(A)).to(dev))
     T = torch.zeros((nodes, 32, 32), dtype=X.dtype, device=dev)
-    prev = lib.maktest_set_node_variant(variant)
-    try:
-        code = lib.maktest_geqr2_level(mak.api._stream(dev), 1 if X.is_complex() else 0, A.shape[0], nodes, nbc,
-                                       X.data_ptr(), X.stride(1), T.data_ptr())
-        torch.cuda.synchronize()
-    finally:
-        lib.maktest_set_node_variant(prev)
+    code = lib.maktest_geqr2_level(mak.api._stream(dev), 1 if X.is_complex() else 0, A.shape[0], nodes, nbc,
+                                   X.data_ptr(), X.stride(1), T.data_ptr(), variant)
+    torch.cuda.synchronize()
     assert code == 0
     return X.cpu().numpy(), T.cpu().numpy().transpose(0, 2, 1)   # T[i] column-major
 
@@ -103,7 +97,7 @@ def run_potrf(Zh, dev):
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
-@pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 130, 200, 1000])
+@pytest.mark.parametrize("n", [1, 5, 31, 32, 33, 63, 64, 65, 130, 200, 1000, 1024, 2048, 2100, 3000])
 def test_potrf_with_inverse(n, dt, dev):
     """W upper with W^H W = Z (numpy's Cholesky factor, conjugate-transposed)
     and Y = W^{-1}; 1e-12 relative for these well-conditioned Z."""
@@ -117,14 +111,55 @@ def test_potrf_with_inverse(n, dt, dev):
     assert np.linalg.norm(Y @ W - np.eye(n)) / np.sqrt(n) < 1e-12
 
 
-def test_potrf_reports_first_nonpositive_minor(dev):
-    n = 150
+@pytest.mark.parametrize("n,bad", [(150, 100), (2300, 2090)])
+def test_potrf_reports_first_nonpositive_minor(n, bad, dev):
+    """info = j + 1 for the first minor that is singular (tile kernel alone and
+    inside the blocked loop)."""
     G = O.gaussian(n, n, np.float64, seed=2)
     Zh = G.T @ G
-    Zh[100:, :] = 0.0
-    Zh[:, 100:] = 0.0          # leading 100 x 100 is PD, minor 101 is singular
+    Zh[bad:, :] = 0.0
+    Zh[:, bad:] = 0.0          # leading bad x bad is PD, minor bad + 1 is singular
     _, _, info = run_potrf(Zh, dev)
-    assert info == 101
+    assert info == bad + 1
+
+
+def chol_tile_hook():
+    lib = _lib.load()
+    lib.maktest_chol_tile_flag_ints.restype = C.c_size_t
+    lib.maktest_chol_tile_flag_ints.argtypes = [C.c_int]
+    lib.maktest_chol_tile.restype = C.c_int
+    lib.maktest_chol_tile.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                      C.c_void_p, C.c_void_p, C.c_int]
+    return lib
+
+
+@pytest.mark.parametrize("grid", [2, 3, 64, 0])
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+def test_chol_tile_any_grid(grid, dt, dev):
+    """The persistent tile kernel gives the same factors with any number of
+    workgroups (one workgroup runs every task in order; waits never block it),
+    and its strict lower parts are zero."""
+    n = 300
+    lib = chol_tile_hook()
+    G = O.gaussian(2 * n, n, dt, seed=7)
+    Zh = G.conj().T @ G
+    Wref = np.linalg.cholesky(Zh).conj().T
+    Z = mak.colmajor(torch.from_numpy(np.asfortranarray(Zh)).to(dev))
+    Y = mak.colmajor(torch.full((n, n), 5.0, dtype=Z.dtype, device=dev))
+    flags = torch.full((lib.maktest_chol_tile_flag_ints(n),), -9, dtype=torch.int32, device=dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    code = lib.maktest_chol_tile(mak.api._stream(dev), 1 if Z.is_complex() else 0, n, Z.data_ptr(), Z.stride(1),
+                                 Y.data_ptr(), Y.stride(1), flags.data_ptr(), info.data_ptr(), grid)
+    torch.cuda.synchronize()
+    assert code == 0 and int(info.item()) == 0
+    W, X = Z.cpu().numpy(), Y.cpu().numpy()
+    assert rel(W, Wref) < 1e-12
+    assert np.all(np.tril(W, -1) == 0) and np.all(np.tril(X, -1) == 0)
+    assert np.linalg.norm(X @ W - np.eye(n)) / np.sqrt(n) < 1e-12
+    # abort flag clear, every task taken
+    fl = flags.cpu().numpy()
+    nt = -(-n // 32)
+    assert fl[2 * nt * nt + 1] == 0
 
 
 # ------------------------------------------- FP32 / BF16 MFMA products ----

@@ -163,3 +163,49 @@ def test_polar_full_size_gaussian_properties(dev):
     lmin = np.linalg.eigvalsh(P).min()
     print(f"4096^2 Gaussian FP64: bwd {bwd:.2e} orth/sqrt(n) {orth:.2e} min eig P {lmin:.2e} info {info}", flush=True)
     assert bwd < 1e-13 and orth < 1e-13 and lmin > 0
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+def test_left_polar_padded_w_leaves_parent_storage(dt, dev):
+    """W given as a view W = big[:m, :n] of a larger column-major buffer (ld >
+    m): the result matches the packed call and the parent's other rows are
+    untouched (every scratch product lives in the workspace)."""
+    m, n = 300, 200
+    A = O.gaussian(m, n, dt, seed=77)
+    big = mak.colmajor(torch.full((m + 37, n), 3.25, dtype=torch.float64 if dt == np.float64 else torch.complex128,
+                                  device=dev))
+    W = big[:m, :]
+    assert W.stride(0) == 1 and W.stride(1) == m + 37
+    At = to_dev(A, dev)
+    P = mak.initialize_output_polar(At, False)[1]
+    mak.left_polar_(At, (W, P))
+    torch.cuda.synchronize()
+    Wo, Po = O.left_polar_svd(A)
+    assert rel(W.cpu().numpy(), Wo) < 1e-11 and rel(P.cpu().numpy(), Po) < 1e-11
+    assert torch.all(big[m:, :] == 3.25)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+@pytest.mark.parametrize("m,n,r", [(300, 100, 80), (120, 120, 100), (200, 64, 1)])
+def test_left_polar_rank_deficient_is_isometric(m, n, r, dt, dev):
+    """Rank-deficient A: the SVD route (src/implementations/polar.jl:59-70)
+    returns an isometry W = U V^H; QDWH's partial isometry is completed, so
+    W^H W = I, A = W P, and P matches the oracle's (unique) P."""
+    A = O.gaussian(m, r, dt, seed=m + r) @ O.gaussian(r, n, dt, seed=n + r)
+    W, P = run_left(A, dev)
+    assert O.isisometric(W)
+    assert np.linalg.norm(A - W @ P) / np.linalg.norm(A) < 1e-12
+    _, Po = O.left_polar_svd(A)
+    assert rel(P, Po) < 1e-10
+    assert mak.last_polar_info()["completed"] == 1
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+def test_polar_zero_matrix(dt, dev):
+    """A = 0: W = I (m x n), P = 0 -- what the SVD route returns (U = I, V = I)."""
+    W, P = run_left(np.zeros((50, 30), dtype=dt), dev)
+    assert np.array_equal(W, np.eye(50, 30)) and np.all(P == 0)
+    At = to_dev(np.zeros((30, 50), dtype=dt), dev)
+    P, Wh = mak.right_polar_(At)
+    torch.cuda.synchronize()
+    assert np.array_equal(Wh.cpu().numpy(), np.eye(30, 50)) and np.all(P.cpu().numpy() == 0)

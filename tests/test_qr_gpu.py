@@ -27,6 +27,40 @@ def to_dev(A, dev):
     return mak.colmajor(torch.from_numpy(np.asfortranarray(A)).to(dev))
 
 
+def run_qr_rec(A, dev, full=False, positive=True, wy=False):
+    """qr through the C ABI with a workspace kept for the status record
+    (maktest_qr_record): returns Q, R and the record
+    [whole-matrix CholeskyQR2 taken, first rejected panel column, its
+    Cholesky info, whole-matrix path available, ...]."""
+    import ctypes as C
+    from mak_mi355x import _lib
+    At = to_dev(A, dev)
+    m, n = At.shape
+    Q, R = mak.initialize_output_qr(At, full)
+    lib = _lib.load()
+    lib.maktest_qr_record.restype = C.c_int
+    lib.maktest_qr_record.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    dt = 1 if At.is_complex() else 0
+    nb = C.c_size_t(0)
+    assert lib.mak_qr_workspace_size(dt, int(full), m, n, C.byref(nb)) == 0
+    ws = torch.empty(max(nb.value, 1), dtype=torch.uint8, device=dev)
+    sfx = "z" if dt else "d"
+    args = [mak.api._stream(dev), m, n, mak.api._ptr(At), mak.api._ld(At), mak.api._ptr(Q), mak.api._ld(Q),
+            mak.api._ptr(R), mak.api._ld(R)]
+    if wy:
+        T = torch.zeros(max(1, -(-min(m, n) // 32)) * 32 * 32, dtype=At.dtype, device=dev)
+        code = getattr(lib, f"mak_qr_compact_wy_{sfx}")(*args, mak.api._ptr(T), int(positive), mak.api._ptr(ws),
+                                                         nb.value)
+    else:
+        code = getattr(lib, f"mak_qr_{'full' if full else 'compact'}_{sfx}")(*args, int(positive),
+                                                                            mak.api._ptr(ws), nb.value)
+    torch.cuda.synchronize()
+    assert code == 0
+    rec = (C.c_int * 8)()
+    assert lib.maktest_qr_record(dt, int(full), m, n, C.c_void_p(ws.data_ptr()), rec) == 0
+    return Q.cpu().numpy(), R.cpu().numpy(), list(rec)
+
+
 def run_qr(A, dev, full=False, positive=True, with_r=True):
     At = to_dev(A, dev)
     Q, R = mak.initialize_output_qr(At, full)
@@ -144,8 +178,9 @@ def test_qr_cholqr_rejected_later_panel(kind, dt, dev):
         U, _ = np.linalg.qr(O.gaussian(m, 32, dt, seed=22))
         V, _ = np.linalg.qr(O.gaussian(32, 32, dt, seed=23))
         A[:, 128:160] = (U * np.logspace(0, -12, 32)[None, :]) @ V.conj().T
-    Q, R = run_qr(A, dev)
-    rejected = last_cholqr_reject()
+    # the rule's primal (compact WY tape) always takes the panel path
+    Q, R, rec = run_qr_rec(A, dev, wy=True)
+    rejected = rec[1]
     assert rel(Q @ R, A) < 1e-13
     assert np.linalg.norm(Q.conj().T @ Q - np.eye(n)) < 1e-12
     d = np.diagonal(R)
@@ -154,14 +189,16 @@ def test_qr_cholqr_rejected_later_panel(kind, dt, dev):
         assert rejected == 128
     else:
         assert rejected in (-1, 128)
-
-
-def last_cholqr_reject():
-    import ctypes as C
-    from mak_mi355x import _lib
-    lib = _lib.load()
-    lib.maktest_last_cholqr_reject.restype = C.c_int
-    return lib.maktest_last_cholqr_reject()
+    # qr_compact!: the whole-matrix CholeskyQR2 check rejects the mixed case
+    # (kappa ~ 1e12) and the panel path takes over; either way the same checks
+    Q, R, rec = run_qr_rec(A, dev)
+    assert rec[3] == 1
+    if kind == "mixed_1e12":
+        assert rec[0] == 0 and rec[1] == 128
+    assert rel(Q @ R, A) < 1e-13
+    assert np.linalg.norm(Q.conj().T @ Q - np.eye(n)) < 1e-12
+    d = np.diagonal(R)
+    assert np.all(d.imag == 0) and np.all(d.real >= 0)
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
@@ -172,10 +209,50 @@ def test_qr_cholqr_carried_gram_accepts_gaussian_panels(m, n, dt, dev):
     CholeskyQR2 check (none handed to the Householder path), and the factors
     must match the oracle's unique positive-gauge QR."""
     A = O.gaussian(m, n, dt, seed=m + n)
-    Q, R = run_qr(A, dev)
-    assert last_cholqr_reject() == -1
+    Q, R, rec = run_qr_rec(A, dev, wy=True)     # the panel path (compact WY tape)
+    assert rec[1] == -1
     Qo, Ro = O.qr_compact(A, positive=True)
     assert rel(Q, Qo) < 1e-12 and rel(R, Ro) < 1e-12
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+@pytest.mark.parametrize("m,n", [(64, 64), (100, 64), (1025, 65), (3000, 400), (700, 650), (1200, 1200),
+                                 (5000, 1000), (2100, 2048)])
+def test_qr_whole_matrix_cholqr2_matches_oracle(m, n, dt, dev):
+    """qr_compact!(positive = true) with 64 <= n <= 2048 <= m: the whole-matrix
+    CholeskyQR2 path is taken for Gaussian inputs (record[0] == 1) and gives
+    the unique positive-gauge QR of the oracle."""
+    A = O.gaussian(m, n, dt, seed=7 * m + n)
+    Q, R, rec = run_qr_rec(A, dev)
+    assert rec[3] == 1 and rec[0] == 1 and rec[2] == 0, rec
+    Qo, Ro = O.qr_compact(A, positive=True)
+    assert rel(Q, Qo) < 1e-12 and rel(R, Ro) < 1e-12
+    assert np.all(np.tril(R, -1) == 0)
+    d = np.diagonal(R)
+    assert np.all(d.imag == 0) and np.all(d.real > 0)
+
+
+@pytest.mark.parametrize("cond", [1e3, 1e5, 1e8, 1e13])
+def test_qr_whole_matrix_cholqr2_conditioning(cond, dev):
+    """Graded singular values 1 .. 1/cond in random directions: the closed-form
+    second pass is accepted while ||Q1^H Q1 - I|| <= 1e-6 (kappa up to about
+    1e5) and the input goes on to the Householder path beyond; either way
+    the backward error and orthogonality are those of Householder QR."""
+    m, n = 3000, 256
+    U, _ = np.linalg.qr(O.gaussian(m, n, np.float64, seed=31))
+    V, _ = np.linalg.qr(O.gaussian(n, n, np.float64, seed=32))
+    A = (U * np.logspace(0, -np.log10(cond), n)[None, :]) @ V.T
+    Q, R, rec = run_qr_rec(A, dev)
+    if cond <= 1e3:
+        assert rec[0] == 1
+    if cond >= 1e8:
+        assert rec[0] == 0
+    assert rel(Q @ R, A) < 1e-14
+    assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 1e-13
+    assert np.all(np.diagonal(R) >= 0)
+    if cond <= 1e5:
+        Qo, Ro = O.qr_compact(A, positive=True)
+        assert rel(R, Ro) < 1e-15 * cond * 100
 
 
 def test_qr_input_untouched_by_allocating_form(dev):
@@ -201,3 +278,27 @@ def test_qr_full_size_within_10x_lapack(dt, dev):
           f"|Q^HQ-I| {orth:.2e} (LAPACK {orth_o:.2e}), |R-Rref|/|Rref| {rel(R, Ro):.2e}", flush=True)
     assert res <= 10 * res_o and orth <= 10 * orth_o
     assert rel(R, Ro) < 1e-12 and rel(Q, Qo) < 1e-12
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+@pytest.mark.parametrize("m,n", [(200, 80), (90, 90), (3000, 300)])
+def test_qr_inplace_q(m, n, dt, dev):
+    """qr_compact!(A, (A, empty R); positive = false) writes Q over A
+    (src/implementations/qr.jl:150-153, 170-171): the same Q as the
+    out-of-place call, and the positive gauge or an R in place throw."""
+    A = O.gaussian(m, n, dt, seed=m * n)
+    alg = mak.MI355XHouseholder(positive=False)
+    At = to_dev(A, dev)
+    E = torch.empty((0, 0), dtype=At.dtype, device=dev)
+    Q2, R2 = mak.qr_compact_(At, (At, E), alg)
+    assert Q2 is At and R2 is E
+    Ab = to_dev(A, dev)
+    Qref, _ = mak.qr_compact_(Ab, (mak.initialize_output_qr(Ab, False)[0], E), alg)
+    torch.cuda.synchronize()
+    assert torch.equal(At, Qref)
+    assert O.isisometric(At.cpu().numpy())
+    with pytest.raises(mak.ArgumentError, match="inplace"):
+        mak.qr_compact_(At, (At, E), mak.MI355XHouseholder(positive=True))
+    with pytest.raises(mak.ArgumentError, match="inplace"):
+        R = mak.initialize_output_qr(At, False)[1]
+        mak.qr_compact_(At, (At, R), alg)
