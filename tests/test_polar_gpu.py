@@ -209,3 +209,49 @@ def test_polar_zero_matrix(dt, dev):
     P, Wh = mak.right_polar_(At)
     torch.cuda.synchronize()
     assert np.array_equal(Wh.cpu().numpy(), np.eye(30, 50)) and np.all(P.cpu().numpy() == 0)
+
+
+@pytest.fixture(scope="module")
+def cond1e6_4096():
+    """A = U diag(s) V^T, 4096 x 4096, U and V Haar-orthogonal (QR of a
+    Gaussian with the sign fix), s log-spaced from 1 to 1e-6: the mixed-
+    precision polar configuration.  Returns A and the exact polar factor U V^T."""
+    n = 4096
+    rng = np.random.default_rng(20261203)
+
+    def haar(k):
+        Q, R = np.linalg.qr(rng.standard_normal((k, k)))
+        return Q * np.sign(np.diagonal(R))[None, :]
+
+    U, V = haar(n), haar(n)
+    s = np.logspace(0, -6, n)
+    return (U * s[None, :]) @ V.T, U @ V.T
+
+
+@pytest.mark.parametrize("prec,bwd", [(64, 1e-13), (32, 1e-5), (16, 1e-2)])
+def test_left_polar_4096_cond1e6_mixed(prec, bwd, cond1e6_4096, dev):
+    """left_polar! on the 4096^2, cond 1e6 configuration with FP64 / FP32 /
+    BF16 bulk products + FP64 Newton-Schulz: orthogonality ||W^T W - I||_F
+    <= 1e-12 in every precision, backward error ||A - W P||_F / ||A||_F <=
+    1e-13 / 1e-5 / 1e-2, P symmetric.  The distance of W to the exact U V^T
+    is bounded below by cond * u_bulk (the polar factor's condition number
+    is ~1e6 here); it is reported, and for FP64 asserted <= 1e-9."""
+    A, Wex = cond1e6_4096
+    At = to_dev(A, dev)
+    W, P = mak.initialize_output_polar(At, False)
+    mak.left_polar_(At, (W, P), mak.MI355XPolar(precision=prec))
+    torch.cuda.synchronize()
+    info = mak.last_polar_info()
+    eye = torch.eye(At.shape[1], dtype=At.dtype, device=dev)
+    orth = torch.linalg.norm(W.T @ W - eye).item()
+    bwd_err = (torch.linalg.norm(At - W @ P) / torch.linalg.norm(At)).item()
+    relW = (torch.linalg.norm(W - torch.from_numpy(Wex).to(dev)) / np.sqrt(At.shape[1])).item()
+    sym = (torch.linalg.norm(P - P.T) / torch.linalg.norm(P)).item()
+    print(f"left_polar 4096 cond1e6 prec {prec}: |W^TW-I|_F {orth:.2e}, |A-WP|/|A| {bwd_err:.2e}, "
+          f"|W-UV^T|/sqrt(n) {relW:.2e}, iterations {info['iterations']} (QR {info['qr_steps']}, "
+          f"Chol {info['chol_steps']}, NS {info['ns_steps']})", flush=True)
+    assert orth <= 1e-12
+    assert bwd_err <= bwd
+    assert sym <= 1e-14
+    if prec == 64:
+        assert relW <= 1e-9
