@@ -90,15 +90,20 @@ int mak_qr_compact_wy_z(void* stream, int m, int n, void* A, int lda, void* Q, i
 /* Diagnostics of one polar call (optional output, may be NULL). */
 typedef struct mak_polar_info {
   int iterations;   /* QDWH iterations run */
-  int qr_steps;     /* iterations with c > 100 (QR of [sqrt(c) X; I]) */
-  int chol_steps;   /* iterations with c <= 100 (Cholesky form) */
+  int qr_steps;     /* iterations in QR form (QR of [sqrt(c) X; I]): c > 100,
+                       or c > 1e5 at precision 32 / 16 */
+  int chol_steps;   /* iterations in Cholesky form */
   int ns_steps;     /* FP64 Newton-Schulz refinement steps (precision 32 / 16) */
   double alpha;     /* ||A||_F, the scaling of X_0 */
   double l0;        /* lower bound used for sigma_min(X_0) */
   double l_final;   /* lower bound after the last iteration */
-  double last_diff; /* ||X_k - X_{k-1}||_F / ||X_k||_F of the last iteration */
+  double last_diff; /* ||X_k - X_{k-1}||_F / ||X_k||_F of the last iteration;
+                       precision 32 / 16: ||I - X^H X||_F before the last
+                       Newton-Schulz step */
   int completed;    /* 1: A was rank deficient and the partial isometry of the
                        iteration was completed to an isometry (a second run) */
+  int poly_steps;   /* precision 32 / 16: steps of the polynomial phase that
+                       replaces the Cholesky-form iterations (c <= 100) */
 } mak_polar_info;
 
 /* Device workspace bytes for the polar entry points on an m x n input. */
@@ -107,11 +112,14 @@ int mak_polar_workspace_size(int dtype, int m, int n, size_t* bytes);
 /* left_polar!(A, (W, P), alg) for m >= n: A = W P, W m x n with orthonormal
  * columns, P = herm(W^H A) n x n Hermitian positive semidefinite, or NULL
  * (P not computed).  A is left untouched.
- *   precision: 64 (FP64 throughout), 32 (QDWH bulk GEMMs in FP32) or 16 (BF16
- *   in the Cholesky iterations, FP32 in the QR iterations), all with FP32
- *   accumulation; Gram matrices, Cholesky, weights and the first CholeskyQR
- *   pass of the QR iterations stay FP64, followed by FP64 Newton-Schulz
- *   refinement.  Complex data only supports 64.
+ *   precision: 64 (FP64 throughout), 32 or 16 (reduced-precision bulk): the
+ *   ill-conditioned QR-form iterations keep their Gram, Cholesky and first
+ *   CholeskyQR pass in FP64 (second pass on FP32 MFMA); Cholesky-form
+ *   iterations run in FP64 while c > 100; from c <= 100 on, cubic polynomial
+ *   steps X (alpha I - beta X^H X) on FP32 MFMA (BF16 for the small late
+ *   corrections at 16) take over, followed by FP64 Newton-Schulz refinement
+ *   (FP64 Gram).  P = herm(W^H A) is then an FP32 product.  Complex data
+ *   only supports 64.
  *   maxiter <= 0 selects 20; l0 outside (0, 1) selects 1e-10.
  *   W is an isometry also for rank-deficient A (as the SVD route's U V^H):
  *   the iteration's partial isometry is completed (info->completed), and

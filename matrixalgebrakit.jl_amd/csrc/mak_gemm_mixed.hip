@@ -85,6 +85,7 @@ struct LoGemmArgs {
   int64_t ldc;
   double alpha, beta;
   int tri;
+  int upper;             // only the tiles on or above the diagonal (Hermitian results)
 };
 
 // Tile order: XCD-contiguous ranges of tiles, except under triangular hints,
@@ -112,6 +113,7 @@ __global__ void __launch_bounds__(256, 2) gemm_lo_kernel(LoGemmArgs g) {
   const int wm = w & 1, wn = w >> 1;
   int tmi, tni;
   lo_tile_coords(tmi, tni, g.tri);
+  if (g.upper && tmi > tni) return;  // strictly below the diagonal: not wanted
   const int m0 = tmi * kLoRows, n0 = tni * kLoRows;
   // K range: the hints bound the nonzero terms of a tile; the slab-aligned
   // range covers them (the packed copies are zero beyond K)
@@ -205,7 +207,7 @@ __global__ void __launch_bounds__(256, 2) gemm_lo_kernel(LoGemmArgs g) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = rowb + i * 16, col = colb + j * 16 + mfma_f32_row(lane, r);
-        if (row < g.M && col < g.N) {
+        if (row < g.M && col < g.N && (!g.upper || row <= col)) {
           double* cp = g.C + row + (int64_t)col * g.ldc;
           const double v = alpha * (double)acc[i][j][r];
           *cp = (beta != 0.0) ? fma(beta, *cp, v) : v;
@@ -229,6 +231,7 @@ void gemm_lo(int opA, int opB, const GemmArgs& g, void* ws, hipStream_t s) {
   a.C = reinterpret_cast<double*>(g.C); a.ldc = g.ldc;
   a.alpha = g.alpha_re; a.beta = g.beta_re;
   a.tri = g.tri;
+  a.upper = g.upperC;
   dim3 grid((unsigned)(Mp / kLoRows), (unsigned)(Np / kLoRows), 1);
   hipLaunchKernelGGL((gemm_lo_kernel<PREC>), grid, dim3(256), 0, s, a);
 }

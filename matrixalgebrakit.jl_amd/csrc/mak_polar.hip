@@ -28,6 +28,18 @@ namespace mak {
 namespace {
 
 constexpr double kMaxCholQrC = 1e14;  // sqrt(1 + c) <= 1e7: (1e7)^2 u << 1
+// Reduced-precision bulk (precision 32 / 16, real data):
+//   c > kMixedCholMaxC      QR-based steps as in FP64 (the iterate is ill
+//                           conditioned; its first CholeskyQR pass stays FP64)
+//   100 < c <= kMixedCholMaxC  Cholesky-based steps in FP64: their error
+//                           ~ c u_64 is far below the FP32 level of the result
+//   c <= 100                the polynomial phase takes over (below)
+constexpr double kMixedCholMaxC = 1e5;
+constexpr double kPolyTarget = 1e-4;  // polynomial phase ends once 1 - l <= this
+constexpr int kMaxPoly = 16;
+constexpr int kMaxNs = 6;             // FP64-Gram Newton-Schulz steps, at most
+constexpr double kNsLast = 1e-6;      // ||I - X^H X||_F below this: one more step, then stop
+constexpr double kNsDone = 1e-14;     // ... below this: no step needed
 
 inline size_t al256(size_t b) { return ((b + 255) / 256) * 256; }
 
@@ -53,7 +65,9 @@ template <typename T> PolarLayout polar_layout(int m, int n) {
   L.offInfo = take(16 * sizeof(int));
   // rounded operand copies of the FP32 / BF16 products (real data only): the
   // largest is (m x n)(n x n)
-  L.offLo = take(ScalarTraits<T>::is_complex ? 0 : gemm_prec_workspace_bytes(m, n, n));
+  L.offLo = take(ScalarTraits<T>::is_complex
+                     ? 0
+                     : std::max(gemm_prec_workspace_bytes(m, n, n), gemm_prec_workspace_bytes(n, n, m)));
   L.bytes = off;
   return L;
 }
@@ -135,6 +149,26 @@ __global__ void __launch_bounds__(256) shift_herm_kernel(T* G, int64_t ldg, int 
   }
 }
 
+template <typename T> void shift_herm(T* G, int64_t ldg, int n, double a, double b, hipStream_t s) {
+  hipLaunchKernelGGL((shift_herm_kernel<T>), dim3(cdiv(n, 32), cdiv(n, 32)), dim3(256), 0, s, G, ldg, n, a, b);
+}
+
+// The odd cubic p(x) = alpha x - beta x^3 closest to 1 in the max norm on
+// [l, 1]: it equioscillates at l, at its interior maximum x* = sqrt(s / 3)
+// and at 1 (s = 1 + l + l^2, p(l) = p(1) gives alpha = beta s), then is
+// scaled so that its maximum on [l, 1] is 1.  Returns min p / max p, the
+// lower bound of the next iterate's singular values.  l = 1 gives the
+// Newton-Schulz step (alpha, beta) = (3/2, 1/2).
+double poly_cubic(double l, double& alpha, double& beta) {
+  const double s = 1.0 + l + l * l;
+  const double xs = std::sqrt(s / 3.0);
+  const double b = 2.0 / ((2.0 / 3.0) * s * xs + s - 1.0);
+  const double e = 1.0 - b * (s - 1.0);  // 1 - p(l) = 1 - p(1) = p(x*) - 1
+  alpha = b * s / (1.0 + e);
+  beta = b / (1.0 + e);
+  return (1.0 - e) / (1.0 + e);
+}
+
 // Second CholeskyQR pass with G = I + E close to the identity: the Cholesky
 // factor W = I + U and its inverse follow from E to within O(||E||^3):
 // U1 = up(E), U = up(E - U1^H U1), W^{-1} = I - U + U^2, where up() keeps the
@@ -204,6 +238,16 @@ template <typename T> double read_scalar(const double* d, hipStream_t s) {
 
 // The iteration on an m x n (m >= n) matrix held in Xw (ld ldx), already
 // scaled to ||X||_2 <= 1.  On return Xw holds the unitary polar factor.
+//
+// precision 32 / 16 (real data): the QDWH steps run while c > 100 (see
+// kMixedCholMaxC), then a polynomial phase on FP32 MFMA replaces the
+// Cholesky-based steps: with l the lower bound of sigma_min(X),
+//   X+ = X (alpha I - beta X^H X),  (alpha, beta) = poly_cubic(l),
+// formed as X + X ((alpha - 1) I - beta G) with G = X^H X (FP32 Gram, upper
+// triangle) and the correction product in FP32 (BF16 for precision 16 once
+// 1 - l <= 0.1, where the correction is small), the sum in FP64; until
+// 1 - l <= kPolyTarget.  FP64 Newton-Schulz steps X+ = X + X (I - X^H X) / 2
+// (FP64 Gram, FP32 correction product) then restore FP64 orthogonality.
 template <typename T>
 int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision, int maxiter, double l0,
                  char* ws, const PolarLayout& L, mak_polar_info* info) {
@@ -220,9 +264,10 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
   void* lo = ws + L.offLo;
   const int64_t ldm = m, ldn = n;
   const int prec = (precision == 32 || precision == 16) ? precision : 64;
-  // BF16 bulk only in the well-conditioned Cholesky iterations; the CholeskyQR2
-  // iterations (c > 100, ill-conditioned iterate) use FP32 products.
-  const int prec_qr = (prec == 16) ? 32 : prec;
+  const bool mixed = prec != 64 && !ScalarTraits<T>::is_complex;
+  // the second CholeskyQR pass of a QR-based step uses FP32 products in both
+  // reduced precisions (the iterate is still ill conditioned there)
+  const int prec_qr = mixed ? 32 : 64;
 
   // three m x n buffers rotate roles: the current iterate and two spares
   T* buf[3] = {Xw, X2, Q1};
@@ -234,6 +279,7 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
   int it = 0, nqr = 0, nch = 0;
   double diff = 1.0;
   int status = 0;
+  bool converged = false;
   for (it = 0; it < maxiter; ++it) {
     double a, b, c;
     if (1.0 - l < 1e-15) { a = 3.0; b = 1.0; c = 3.0; }  // Halley's limit (l = 1)
@@ -244,6 +290,8 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
       l *= 2.0;
       qdwh_params(l, a, b, c);
     }
+    if (mixed && c <= 100.0) break;  // the polynomial phase takes over
+    const bool qr_step = c > (mixed ? kMixedCholMaxC : 100.0);
     const int s1 = (cur + 1) % 3, s2 = (cur + 2) % 3;
     T* Xc = buf[cur];
     const int64_t ldc = ldb[cur];
@@ -252,7 +300,7 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
     gemm<T>(OP_C, OP_N, ga<T>(n, n, m, Xc, ldc, Xc, ldc, G, ldn, c, 1.0, 0, 1), s);
     potrf_upper<T>(G, ldn, n, Y1, ldn, tmp, dinfo, s);
     int nxt;
-    if (c > 100.0) {
+    if (qr_step) {
       // pass 1: Q1 = sqrt(c) X W1^{-1} (buf[s2]), Q2 = W1^{-1} (Y1).  FP64 in
       // every precision: it feeds the second Gram, which is then I + E with
       // ||E|| ~ (1 + c) u and factors in closed form (below)
@@ -288,11 +336,12 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
     } else {
       // Z^{-1} = W^{-1} W^{-H}:  Tm = X W^{-1} (buf[s2]);
       // X+ = (b/c) X + (a - b/c) Tm W^{-H}  (buf[s1]); two triangular products
-      // instead of W^{-1} W^{-H} followed by a full product
-      gemm_prec<T>(prec, OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y1, ldn, buf[s2], ldb[s2], 1.0, 0.0, TRI_B_UPPER), lo, s);
+      // instead of W^{-1} W^{-H} followed by a full product (FP64: a
+      // reduced-precision run only gets here with c > 100)
+      gemm<T>(OP_N, OP_N, ga<T>(m, n, n, Xc, ldc, Y1, ldn, buf[s2], ldb[s2], 1.0, 0.0, TRI_B_UPPER), s);
       copy_mat<T>(m, n, Xc, ldc, buf[s1], ldb[s1], s);
-      gemm_prec<T>(prec, OP_N, OP_C,
-                   ga<T>(m, n, n, buf[s2], ldb[s2], Y1, ldn, buf[s1], ldb[s1], a - b / c, b / c, TRI_B_LOWER), lo, s);
+      gemm<T>(OP_N, OP_C, ga<T>(m, n, n, buf[s2], ldb[s2], Y1, ldn, buf[s1], ldb[s1], a - b / c, b / c, TRI_B_LOWER),
+              s);
       nxt = s1;
       ++nch;
     }
@@ -308,25 +357,47 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
     if (pinfo != 0) { status = 2; ++it; break; }
     diff = std::sqrt(hs[0]) / std::max(std::sqrt(hs[1]), 1e-300);
     l = l * (a + b * l * l) / (1.0 + c * l * l);
-    if (1.0 - l < 10.0 * u && diff <= (prec == 64 ? tol : std::sqrt(tol))) { ++it; break; }
+    if (1.0 - l < 10.0 * u && diff <= (prec == 64 ? tol : std::sqrt(tol))) { ++it; converged = true; break; }
     if (it + 1 == maxiter) status = 6;  // not converged within maxiter (a warning)
   }
-  // low-precision bulk: FP64 Newton-Schulz refinement X <- X (3I - X^H X) / 2
-  int nns = 0;
-  if ((status == 0 || status == 6) && prec != 64) {
-    const int steps = (prec == 16) ? 3 : 2;
-    for (int k = 0; k < steps; ++k) {
+  int npoly = 0, nns = 0;
+  if (mixed && status == 0) {
+    // polynomial phase (no host synchronisation: the coefficients follow from l)
+    while (!converged && 1.0 - l > kPolyTarget && npoly < kMaxPoly) {
+      double al, be;
+      const double lnext = poly_cubic(l, al, be);
       const int nx = (cur + 1) % 3;
-      // E = I - X^H X (FP64 Gram, upper triangle, completed by the shift);
-      // X+ = X (3I - X^H X) / 2 = X + X E / 2 with the correction X E on FP32
-      // MFMA: E is small (its FP32 rounding costs |E| 6e-8 absolute), and the
-      // sum with X is formed in FP64
+      gemm_prec<T>(32, OP_C, OP_N, ga<T>(n, n, m, buf[cur], ldb[cur], buf[cur], ldb[cur], G, ldn, 1.0, 0.0, 0, 1), lo,
+                   s);
+      shift_herm<T>(G, ldn, n, al - 1.0, -be, s);
+      copy_mat<T>(m, n, buf[cur], ldb[cur], buf[nx], ldb[nx], s);
+      const int pp = (prec == 16 && 1.0 - l <= 0.1) ? 16 : 32;
+      gemm_prec<T>(pp, OP_N, OP_N, ga<T>(m, n, n, buf[cur], ldb[cur], G, ldn, buf[nx], ldb[nx], 1.0, 1.0), lo, s);
+      cur = nx;
+      l = lnext;
+      ++npoly;
+    }
+    // FP64 Newton-Schulz refinement X <- X (3I - X^H X) / 2 = X + X E / 2,
+    // E = I - X^H X from an FP64 Gram; E is small, so its FP32 product costs
+    // about |E| 6e-8 absolute, and the sum with X is formed in FP64.  Each
+    // step reads ||E||_F: the step after ||E||_F <= kNsLast is the last
+    // (the error then falls to about 0.75 ||E||^2).
+    for (int k = 0; k < kMaxNs; ++k) {
+      const int nx = (cur + 1) % 3;
       gemm<T>(OP_C, OP_N, ga<T>(n, n, m, buf[cur], ldb[cur], buf[cur], ldb[cur], G, ldn, 1.0, 0.0, 0, 1), s);
-      hipLaunchKernelGGL((shift_herm_kernel<T>), dim3(cdiv(n, 32), cdiv(n, 32)), dim3(256), 0, s, G, ldn, n, 1.0, -1.0);
+      hipLaunchKernelGGL((near_identity_sumsq_kernel<T>), dim3(256), dim3(256), 0, s, G, ldn, n, part);
+      hipLaunchKernelGGL(sum_double_kernel, dim3(1), dim3(256), 0, s, part, 256, scal + 4);
+      const double e = std::sqrt(read_scalar<T>(scal + 4, s));
+      diff = e;
+      if (!std::isfinite(e)) { status = 5; break; }
+      if (e <= kNsDone) break;
+      shift_herm<T>(G, ldn, n, 1.0, -1.0, s);
       copy_mat<T>(m, n, buf[cur], ldb[cur], buf[nx], ldb[nx], s);
       gemm_prec<T>(32, OP_N, OP_N, ga<T>(m, n, n, buf[cur], ldb[cur], G, ldn, buf[nx], ldb[nx], 0.5, 1.0), lo, s);
       cur = nx;
       ++nns;
+      if (e <= kNsLast) break;
+      if (k + 1 == kMaxNs) status = 6;
     }
   }
   if (cur != 0) copy_mat<T>(m, n, buf[cur], ldb[cur], Xw, ldx, s);
@@ -334,6 +405,7 @@ int qdwh_iterate(hipStream_t s, int m, int n, T* Xw, int64_t ldx, int precision,
     info->iterations = it;
     info->qr_steps = nqr;
     info->chol_steps = nch;
+    info->poly_steps = npoly;
     info->ns_steps = nns;
     info->last_diff = diff;
     info->l_final = l;
@@ -413,6 +485,7 @@ int polar_isometry(hipStream_t s, int m, int n, T* X, int64_t ldx, int precision
     info->qr_steps += first.qr_steps;
     info->chol_steps += first.chol_steps;
     info->ns_steps += first.ns_steps;
+    info->poly_steps += first.poly_steps;
   }
   return st;
 }
@@ -464,6 +537,9 @@ int polar_driver(hipStream_t s, int right, int m, int n, const T* A, int64_t lda
   }
   int st;
   int completed = 0;
+  // P in FP32 at reduced precision: its rounding (~1e-7 relative) is below
+  // the polar factor's own error there
+  const int pprec = precision == 64 ? 64 : 32;
   if (!right) {
     // X0 = A / alpha, iterated in W
     copy_mat<T>(m, n, A, lda, W, ldw, s);
@@ -471,7 +547,7 @@ int polar_driver(hipStream_t s, int right, int m, int n, const T* A, int64_t lda
     st = polar_isometry<T>(s, m, n, W, ldw, precision, maxiter, l0, ws, L, info, &completed);
     if ((st == 0 || st == 6) && P) {
       // P = herm(W^H A)
-      gemm<T>(OP_C, OP_N, ga<T>(n, n, m, W, ldw, A, lda, P, ldp, 1.0, 0.0), s);
+      gemm_prec<T>(pprec, OP_C, OP_N, ga<T>(n, n, m, W, ldw, A, lda, P, ldp, 1.0, 0.0), ws + L.offLo, s);
       herm_part<T>(P, ldp, n, s);
     }
   } else {
@@ -483,7 +559,7 @@ int polar_driver(hipStream_t s, int right, int m, int n, const T* A, int64_t lda
       conj_transpose_scale<T>(n, m, 1.0, Xh, n, W, ldw, s);   // Wh = (W')^H
       if (P) {
         // P = herm(A Wh^H)  (src/implementations/polar.jl:122)
-        gemm<T>(OP_N, OP_C, ga<T>(m, m, n, A, lda, W, ldw, P, ldp, 1.0, 0.0), s);
+        gemm_prec<T>(pprec, OP_N, OP_C, ga<T>(m, m, n, A, lda, W, ldw, P, ldp, 1.0, 0.0), ws + L.offLo, s);
         herm_part<T>(P, ldp, m, s);
       }
     }

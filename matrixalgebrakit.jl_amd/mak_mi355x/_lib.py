@@ -56,7 +56,8 @@ class PolarInfo(C.Structure):
     """mak_polar_info (include/mak_mi355x.h)."""
     _fields_ = [("iterations", C.c_int), ("qr_steps", C.c_int), ("chol_steps", C.c_int),
                 ("ns_steps", C.c_int), ("alpha", C.c_double), ("l0", C.c_double),
-                ("l_final", C.c_double), ("last_diff", C.c_double), ("completed", C.c_int)]
+                ("l_final", C.c_double), ("last_diff", C.c_double), ("completed", C.c_int),
+                ("poly_steps", C.c_int)]
 
     def asdict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
