@@ -17,17 +17,22 @@ in with --traffic-bytes or read from profiles/qr_compact_8192x1024_f64_hbm.json
 `extras` holds the other measured workloads of the same path, each timed with
 HIP events on the stream the library launches on:
   qr_full! 8192x1024 Float64, qr_compact!/qr_full! 8192x1024 ComplexF64,
-  left_polar! 4096x4096 Float64 Gaussian (FP64 QDWH),
+  left_polar! 4096x4096 Float64 Gaussian (FP64 QDWH), with the host's
+  dgesdd-route polar (the reference's PolarViaSVD) timed once beside it,
   left_polar! 4096x4096 with sigma log-spaced 1 .. 1e-6 in FP64 / FP32 / BF16
   bulk (speedups over FP64, W error against the FP64 GPU result and against
-  the exact U V^T of the construction),
+  the exact U V^T of the construction); polar TFLOP/s count 26n^3/3 per
+  QR-form step, 10n^3/3 per Cholesky-form step, 3n^3 per Newton-Schulz or
+  polynomial step and 2n^3 for P, from the reported iteration counts,
   qr and left-polar pullbacks on 256 x (2048 x 256) Float64,
   batched qr_compact! on 4096 blocks (m log-uniform in [16, 1024], n uniform in
   [8, m]) in Float64 and ComplexF64, one captured-graph execution per step.
 
 cpu_baseline runs the reference's LAPACK sequence (dgeqrt nb=36 -> dgemqrt on
 I -> triu + sign gauge, restated in oracle/mak_oracle.py through scipy's
-LAPACK) on the host cores for a bounded sample of the headline workload.
+LAPACK) on the host cores for a bounded sample of the headline workload, with
+OPENBLAS_NUM_THREADS set to the physical cores of the CPU affinity mask
+(capped by the host's OMP_NUM_THREADS share); `cores` is that thread count.
 """
 from __future__ import annotations
 
@@ -72,15 +77,39 @@ def qr_pullback_flops(m, n):
     return orgqr_flops(m, n, n) + 6.0 * m * n * n + 2.0 * n ** 3
 
 
+def host_cores():
+    """(physical cores in this process's CPU affinity mask, threads to use):
+    logical CPUs are mapped to (package, core) through sysfs; the thread count
+    is that, capped by OMP_NUM_THREADS when the host sets a share."""
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count()))
+    cores = set()
+    for c in cpus:
+        topo = Path(f"/sys/devices/system/cpu/cpu{c}/topology")
+        try:
+            cores.add((topo.joinpath("physical_package_id").read_text().strip(),
+                       topo.joinpath("core_id").read_text().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    phys = max(1, len(cores))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(phys, int(share)) if share.isdigit() and int(share) > 0 else phys
+    return phys, threads
+
+
+def blas_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        return max((d.get("num_threads", 1) for d in threadpool_info() if d.get("user_api") == "blas"),
+                   default=None)
+    except Exception:  # pragma: no cover
+        return None
+
+
 def cpu_baseline(m, n, budget_s=12.0):
     import numpy as np
     from oracle import mak_oracle as O
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max((d.get("num_threads", 1) for d in threadpool_info()
-                       if d.get("user_api") == "blas"), default=os.cpu_count())
-    except Exception:  # pragma: no cover
-        threads = os.cpu_count()
+    phys, _ = host_cores()
+    threads = blas_threads() or int(os.environ.get("OPENBLAS_NUM_THREADS", "1"))
     A = O.gaussian(m, n, np.float64, seed=0)
     O.qr_compact(A)                       # warm-up
     times = []
@@ -91,7 +120,7 @@ def cpu_baseline(m, n, budget_s=12.0):
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     return {"value": round(qr_flops(m, n) / t / 1e12, 4), "unit": "TFLOP/s", "cores": int(threads),
-            "kind": "port",
+            "physical_cores_in_mask": phys, "kind": "port",
             "sample": f"{len(times)} x qr_compact 8192x1024 Float64 (median {t * 1e3:.1f} ms), "
                       "dgeqrt(nb=36)+dgemqrt on I via scipy LAPACK"}
 
@@ -137,7 +166,25 @@ def bench_qr_extra(mak, dev, stream, m, n, dtype, full, steps):
     return {"ms": round(ms, 4), "tflops": round(fl / ms / 1e9, 3), "rel_residual": res, "orth_per_sqrt_n": orth}
 
 
-def bench_polar(mak, dev, stream, steps):
+def polar_flops(n, info):
+    """Algorithmic flops of one square left_polar! call from its iteration
+    counts: QR-form step 26n^3/3, Cholesky-form step 10n^3/3, Newton-Schulz
+    and cubic polynomial steps 3n^3 (Gram n^3 + product 2n^3), P = W^H A 2n^3.
+    Returns (total, {precision: flops}) with the share each precision runs in
+    (reduced precision: the QR-form step's second-pass products (2n^3), the
+    polynomial steps, the Newton-Schulz products (2n^3) and P run on FP32
+    MFMA, BF16 for the late polynomial products at precision 16)."""
+    n3 = float(n) ** 3
+    q, c, ns, po = info["qr_steps"], info["chol_steps"], info["ns_steps"], info.get("poly_steps", 0)
+    total = q * 26 * n3 / 3 + c * 10 * n3 / 3 + (ns + po) * 3 * n3 + 2 * n3
+    if ns == 0 and po == 0:
+        return total, {"f64": total}
+    lo = q * 2 * n3 + po * 3 * n3 + ns * 2 * n3 + 2 * n3
+    return total, {"f64": total - lo, "f32_or_bf16": lo}
+
+
+def bench_polar(mak, dev, stream, steps, cpu=True):
+    import numpy as np
     import torch
     n = 4096
     out = {}
@@ -150,7 +197,21 @@ def bench_polar(mak, dev, stream, steps):
     A = mak.colmajor(gaussian_dev(n, n, torch.float64, 21, dev))
     W, P = mak.initialize_output_polar(A, False)
     ms = timed(lambda i: mak.left_polar_(A, (W, P)), steps, 1, stream)
-    out["left_polar_f64_gaussian_4096"] = {"ms": round(ms, 3), **props(A, W, P), "info": mak.last_polar_info()}
+    info = mak.last_polar_info()
+    fl, _ = polar_flops(n, info)
+    rec = {"ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 3),
+           "frac_fp64_peak": round(fl / ms / 1e9 / FP64_PEAK_TFLOPS, 4), **props(A, W, P), "info": info}
+    if cpu:
+        # the reference's polar route on the host cores: dgesdd 'S' -> U V^H -> P
+        from oracle import mak_oracle as O
+        Ah = A.cpu().numpy()
+        t0 = time.perf_counter()
+        O.left_polar_svd(Ah)
+        t_cpu = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"ms": round(t_cpu * 1e3, 1), "cores": blas_threads(), "kind": "port",
+                               "sample": "1 x left_polar! 4096^2 Float64 via dgesdd (oracle left_polar_svd)",
+                               "gpu_speedup": round(t_cpu * 1e3 / ms, 1)}
+    out["left_polar_f64_gaussian_4096"] = rec
 
     # A = U diag(sigma) V^T, U, V Haar (positive-gauge QR of Gaussians), sigma 1 .. 1e-6
     U, _ = mak.qr_compact(gaussian_dev(n, n, torch.float64, 22, dev))
@@ -168,11 +229,14 @@ def bench_polar(mak, dev, stream, steps):
     nrm = torch.linalg.norm(W64).item()
     for prec, (ms, W, pr, info) in res.items():
         key = {64: "f64", 32: "f32_bulk", 16: "bf16_bulk"}[prec]
+        fl, split = polar_flops(n, info)
         out[f"left_polar_{key}_cond1e6_4096"] = {
             "ms": round(ms, 3), "speedup_vs_f64": round(res[64][0] / ms, 3),
+            "tflops": round(fl / ms / 1e9, 3), "gflop_by_precision": {k: round(v / 1e9, 1) for k, v in split.items()},
             "relW_vs_f64_gpu": (torch.linalg.norm(W - W64) / nrm).item(),
             "relW_vs_exact_UVt": (torch.linalg.norm(W - Wx) / nrm).item(), **pr,
-            "iterations": info["iterations"], "ns_steps": info["ns_steps"]}
+            "iterations": info["iterations"], "qr_steps": info["qr_steps"], "chol_steps": info["chol_steps"],
+            "poly_steps": info.get("poly_steps", 0), "ns_steps": info["ns_steps"]}
     return out
 
 
@@ -256,7 +320,7 @@ def bench_batched(mak, dev, stream, steps, dtype, count=4096):
             "blocks": count, "plan_s": round(t_plan, 3), "plan": info, "rel_residual_block0": res}
 
 
-def run_extras(mak, dev, stream, groups, xs):
+def run_extras(mak, dev, stream, groups, xs, cpu=True):
     import torch
     m, n = 8192, 1024
     ex = {}
@@ -266,7 +330,7 @@ def run_extras(mak, dev, stream, groups, xs):
         ex["qr_full_c64_8192x1024"] = bench_qr_extra(mak, dev, stream, m, n, torch.complex128, True, xs)
         torch.cuda.empty_cache()
     if "polar" in groups:
-        ex.update(bench_polar(mak, dev, stream, xs))
+        ex.update(bench_polar(mak, dev, stream, xs, cpu))
         torch.cuda.empty_cache()
     if "pullback" in groups:
         ex.update(bench_pullbacks(mak, dev, stream, xs))
@@ -304,6 +368,10 @@ def main():
               "runs on a single MI355X in this build", file=sys.stderr)
         sys.exit(2)
 
+    # host BLAS threads for the CPU legs: the physical cores of the affinity
+    # mask (capped by the host's OMP_NUM_THREADS share), set before numpy loads
+    _, threads = host_cores()
+    os.environ.setdefault("OPENBLAS_NUM_THREADS", str(threads))
     import numpy as np
     import torch
     import mak_mi355x as mak
@@ -322,7 +390,7 @@ def main():
     m, n = 8192, 1024
     groups = set() if args.no_extras else {g for g in args.extras.split(",") if g}
     if args.skip_headline:
-        ex = run_extras(mak, dev, stream, groups, max(2, args.steps // 3))
+        ex = run_extras(mak, dev, stream, groups, max(2, args.steps // 3), not args.no_cpu)
         print(json.dumps({"metric": METRIC, "value": None, "extras": ex}), flush=True)
         return
     rng = np.random.default_rng(0)
@@ -376,7 +444,7 @@ def main():
         "checks": {"rel_residual": res, "orth": orth},
     }
     if groups:
-        out["extras"] = run_extras(mak, dev, stream, groups, max(2, args.steps // 3))
+        out["extras"] = run_extras(mak, dev, stream, groups, max(2, args.steps // 3), not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, n)
     print(json.dumps(out), flush=True)

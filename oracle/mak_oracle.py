@@ -281,9 +281,18 @@ def _avgdiff(A, B):
     return (A + B) / 2, B - A
 
 
-def left_polar_newton(A, tol=None, maxiter=10, compute_p=True):
+def _norm_inf(X):
+    """Julia's ``norm(X, Inf)`` of an array: the largest entry in absolute value
+    (LinearAlgebra's ``norm`` treats a matrix as a vector; ``opnorm`` would be
+    the row-sum norm)."""
+    return float(np.max(np.abs(X))) if X.size else 0.0
+
+
+def left_polar_newton(A, tol=None, maxiter=10, compute_p=True, info=None):
     """``left_polar!(A, WP, PolarNewton)`` + ``left_polar_newton!``
-    (src/implementations/polar.jl:99-111,128-166)."""
+    (src/implementations/polar.jl:99-111,128-166).  The convergence measure is
+    ``norm(Rᴴinv, Inf)`` (:148,156), the max-abs entry of the last difference.
+    ``info`` (a dict, optional) receives the iteration count and that measure."""
     A = np.array(A, copy=True)
     m, n = A.shape
     if tol is None:
@@ -306,12 +315,15 @@ def left_polar_newton(A, tol=None, maxiter=10, compute_p=True):
 
     R, Rhinv = step(R, Rhinv)
     i = 1
-    conv = np.linalg.norm(Rhinv, np.inf)
+    conv = _norm_inf(Rhinv)
     while i < maxiter and conv > tol:
         Rhinv = np.linalg.inv(R).conj().T
         R, Rhinv = step(R, Rhinv)
-        conv = np.linalg.norm(Rhinv, np.inf)
+        conv = _norm_inf(Rhinv)
         i += 1
+    if info is not None:
+        info["iterations"] = i
+        info["conv"] = conv
     if conv > tol:
         warnings.warn("`left_polar!` via Newton iteration did not converge within "
                       f"{maxiter} iterations (final residual: {conv})")

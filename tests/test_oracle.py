@@ -154,6 +154,46 @@ def test_polar_newton_matches_svd(m, n, dt):
     assert rel(Wh, Ws.conj().T) < 1e-10
 
 
+def _newton_orthodiag_conv(Q, d, nsteps):
+    """Convergence measures of the scaled Newton iteration on A = Q diag(d)
+    (Q orthogonal): every iterate is Q diag(d_k), A^{-H} = Q diag(1/d), the
+    Frobenius scaling is gamma = sqrt(||1/d|| / ||d||), so the step is the
+    scalar recursion d <- (gamma d + 1/(gamma d)) / 2 with the difference
+    Q diag(1/(gamma d) - gamma d).  Returns, per step, the max-abs entry and
+    the row-sum norm of that difference."""
+    out = []
+    for _ in range(nsteps):
+        g = np.sqrt(np.linalg.norm(1.0 / d) / np.linalg.norm(d))
+        delta = 1.0 / (g * d) - g * d
+        D = Q * delta[None, :]
+        out.append((np.max(np.abs(D)), np.max(np.sum(np.abs(D), axis=1))))
+        d = (g * d + 1.0 / (g * d)) / 2.0
+    return out
+
+
+def test_polar_newton_stops_on_max_abs_entry():
+    """The stopping test is Julia's norm(R^{-H} diff, Inf) = max-abs entry
+    (src/implementations/polar.jl:148,156), not the row-sum norm: with tol
+    between the two measures the iteration must stop on the max-abs one.
+    The expected counts come from the scalar recursion on Q diag(d)."""
+    n = 40
+    Q = O.haar_orthogonal(n, seed=3)
+    d = np.logspace(0, -3, n)
+    convs = _newton_orthodiag_conv(Q, d, 8)
+    # an iteration where the two measures straddle tol
+    k = next(i for i, (a, r) in enumerate(convs) if r > 1.5 * a and a < 1e-2)
+    tol = np.sqrt(convs[k][0] * convs[k][1])
+    expect_maxabs = next(i for i, (a, _) in enumerate(convs) if a <= tol) + 1
+    expect_rowsum = next(i for i, (_, r) in enumerate(convs) if r <= tol) + 1
+    assert expect_rowsum > expect_maxabs
+    info = {}
+    W, P = O.left_polar_newton(Q * d[None, :], tol=tol, maxiter=20, info=info)
+    assert info["iterations"] == expect_maxabs
+    assert abs(info["conv"] - convs[expect_maxabs - 1][0]) <= 1e-6 * convs[expect_maxabs - 1][0] + 1e-15
+    # a planted row-sum norm would run longer
+    assert info["iterations"] != expect_rowsum
+
+
 def test_polar_newton_warns_on_maxiter():
     A = O.gaussian(30, 30, np.float64, seed=4)
     with pytest.warns(UserWarning, match="did not converge"):
