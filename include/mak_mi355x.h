@@ -85,6 +85,47 @@ int mak_qr_compact_wy_d(void* stream, int m, int n, double* A, int lda, double* 
 int mak_qr_compact_wy_z(void* stream, int m, int n, void* A, int lda, void* Q, int ldq, void* R, int ldr, void* Tout,
                         int positive, void* work, size_t work_bytes);
 
+/* ------------------------------------------------------ LQ, null spaces --- */
+
+/* Device workspace bytes of the LQ entry points (full = 0 / 1). */
+int mak_lq_workspace_size(int dtype, int full, int m, int n, size_t* bytes);
+
+/* lq_compact!(A, (L, Q), alg) / lq_full!(A, (L, Q), alg) through the QR of
+ * A^H, as LQViaTransposedQR: A^H = Qt Rt gives L = Rt^H (m x min(m,n), m x n
+ * for full; lower triangular; NULL = not computed) and Q = Qt^H (min(m,n) x n,
+ * n x n for full) with orthonormal rows.  A is only read.  positive = 1: L
+ * has a real non-negative diagonal.  Q and L must not overlap A.
+ * Replaces lq_via_qr! (src/implementations/lq.jl:283-310) behind
+ * lq_compact! / lq_full! (:285-296). */
+int mak_lq_compact_d(void* stream, int m, int n, const double* A, int lda, double* L, int ldl, double* Q, int ldq,
+                     int positive, void* work, size_t work_bytes);
+int mak_lq_compact_z(void* stream, int m, int n, const void* A, int lda, void* L, int ldl, void* Q, int ldq,
+                     int positive, void* work, size_t work_bytes);
+int mak_lq_full_d(void* stream, int m, int n, const double* A, int lda, double* L, int ldl, double* Q, int ldq,
+                  int positive, void* work, size_t work_bytes);
+int mak_lq_full_z(void* stream, int m, int n, const void* A, int lda, void* L, int ldl, void* Q, int ldq,
+                  int positive, void* work, size_t work_bytes);
+
+/* Device workspace bytes of qr_null! (lq = 0) / lq_null! (lq = 1). */
+int mak_null_workspace_size(int dtype, int lq, int m, int n, size_t* bytes);
+
+/* qr_null!(A, N, alg): N (m x (m - min(m,n))) = H_1 ... H_k [0; I], the
+ * trailing columns of the Householder QR's full Q (no gauge); an orthonormal
+ * basis of the complement of range(A).  A is destroyed.
+ * Replaces qr_null_householder! (src/implementations/qr.jl:114-117,236-263). */
+int mak_qr_null_d(void* stream, int m, int n, double* A, int lda, double* N, int ldn, void* work,
+                  size_t work_bytes);
+int mak_qr_null_z(void* stream, int m, int n, void* A, int lda, void* N, int ldn, void* work, size_t work_bytes);
+
+/* lq_null!(A, Nh, alg): Nh ((n - min(m,n)) x n) = the conjugate transpose of
+ * the QR null space of A^H; orthonormal rows spanning the complement of the
+ * row space of A.  A is only read.
+ * Replaces lq_null_via_qr! (src/implementations/lq.jl:297-301,312-318). */
+int mak_lq_null_d(void* stream, int m, int n, const double* A, int lda, double* Nh, int ldnh, void* work,
+                  size_t work_bytes);
+int mak_lq_null_z(void* stream, int m, int n, const void* A, int lda, void* Nh, int ldnh, void* work,
+                  size_t work_bytes);
+
 /* --------------------------------------------------------------- polar --- */
 
 /* Diagnostics of one polar call (optional output, may be NULL). */

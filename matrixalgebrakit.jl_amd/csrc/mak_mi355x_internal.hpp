@@ -10,6 +10,17 @@
 
 namespace mak {
 
+// Does [p, p + span) intersect [q, q + span2)?  (element spans of two
+// column-major matrices)
+template <typename T>
+inline bool ranges_overlap(const T* p, int64_t ld, int rows, int cols, const T* q, int64_t ldq, int qrows,
+                           int qcols) {
+  if (p == nullptr || q == nullptr || rows <= 0 || cols <= 0 || qrows <= 0 || qcols <= 0) return false;
+  const T* pe = p + (cols - 1) * ld + rows;
+  const T* qe = q + (qcols - 1) * ldq + qrows;
+  return p < qe && q < pe;
+}
+
 template <typename T> size_t qr_workspace_bytes(int m, int n, int full);
 template <typename T> void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipStream_t s);
 template <typename T>

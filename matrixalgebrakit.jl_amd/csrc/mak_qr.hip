@@ -389,16 +389,6 @@ void geqrf_blocked(T* A, int64_t lda, int m, int n, char* ws, size_t qcols, hipS
 // Columns from which the whole-matrix CholeskyQR2 path is tried first.
 constexpr int kCholQr2MinN = 64;
 
-// Does [p, p + span) intersect [q, q + span2)?  (element spans of two
-// column-major matrices)
-template <typename T> bool ranges_overlap(const T* p, int64_t ld, int rows, int cols, const T* q, int64_t ldq,
-                                          int qrows, int qcols) {
-  if (p == nullptr || q == nullptr || rows <= 0 || cols <= 0 || qrows <= 0 || qcols <= 0) return false;
-  const T* pe = p + (cols - 1) * ld + rows;
-  const T* qe = q + (qcols - 1) * ldq + qrows;
-  return p < qe && q < pe;
-}
-
 // out (kg x N, ld kg) = V^H X for V = Vc (rows x kg, ld ldv): split-K
 // product into the Wp slices, then their sum.
 template <typename T>

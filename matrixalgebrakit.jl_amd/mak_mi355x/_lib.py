@@ -28,6 +28,16 @@ SIGNATURES = {
     "mak_qr_full_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
     "mak_qr_compact_wy_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
     "mak_qr_compact_wy_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
+    "mak_lq_workspace_size": (_i, [_i, _i, _i, _i, C.POINTER(_sz)]),
+    "mak_lq_compact_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
+    "mak_lq_compact_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
+    "mak_lq_full_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
+    "mak_lq_full_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
+    "mak_null_workspace_size": (_i, [_i, _i, _i, _i, C.POINTER(_sz)]),
+    "mak_qr_null_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _sz]),
+    "mak_qr_null_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _sz]),
+    "mak_lq_null_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _sz]),
+    "mak_lq_null_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _sz]),
     "mak_polar_workspace_size": (_i, [_i, _i, _i, C.POINTER(_sz)]),
     "mak_left_polar_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp, _sz, _vp]),
     "mak_left_polar_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp, _sz, _vp]),

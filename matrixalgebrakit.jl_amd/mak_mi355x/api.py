@@ -33,6 +33,11 @@ __all__ = [
     "check_input_qr_compact", "check_input_qr_full", "initialize_output_qr",
     "left_polar", "left_polar_", "right_polar", "right_polar_",
     "check_input_left_polar", "check_input_right_polar", "initialize_output_polar", "last_polar_info",
+    "lq_compact", "lq_compact_", "lq_full", "lq_full_", "check_input_lq_compact", "check_input_lq_full",
+    "initialize_output_lq", "qr_null", "qr_null_", "lq_null", "lq_null_", "check_input_qr_null",
+    "check_input_lq_null", "initialize_output_qr_null", "initialize_output_lq_null",
+    "PolarNewton", "left_orth_alg", "right_orth_alg", "left_orth", "left_orth_", "right_orth", "right_orth_",
+    "left_null", "left_null_", "right_null", "right_null_", "project_isometric", "project_isometric_",
 ]
 
 
@@ -252,13 +257,14 @@ class MI355XPolar:
     """Algorithm type of this backend for ``left_polar!`` / ``right_polar!``:
     QDWH on the GPU.
 
-    ``precision`` 64 runs every product in FP64; 32 / 16 run the bulk products
-    of the iteration in FP32 / BF16 (FP32 accumulation) with FP64 Gram
-    matrices, Cholesky factorisations, weights and first CholeskyQR pass of
-    the QR iterations, followed by FP64 Newton-Schulz refinement.  ``maxiter`` (default 20) and ``l0`` (lower bound
-    of sigma_min(A) / ||A||_F, default 1e-10) tune the iteration; like
-    ``PolarNewton`` (src/interface/decompositions.jl:305) non-convergence only
-    warns.
+    ``precision`` 64 runs every product in FP64.  32 / 16 keep the
+    ill-conditioned QDWH steps (c > 100) in FP64 except the second CholeskyQR
+    pass's products, then replace the well-conditioned steps by cubic
+    polynomial steps X (alpha I - beta X^H X) on FP32 MFMA (BF16 for the late,
+    small corrections at 16), followed by FP64 Newton-Schulz refinement.
+    ``maxiter`` (default 20) and ``l0`` (lower bound of sigma_min(A) /
+    ||A||_F, default 1e-10) tune the iteration; like ``PolarNewton``
+    (src/interface/decompositions.jl:305) non-convergence only warns.
     """
     precision: int = 64
     maxiter: int = 20
@@ -315,6 +321,8 @@ def last_polar_info() -> dict:
 def _polar_call(A, out, alg, right: bool):
     if alg is None:
         alg = MI355XPolar()
+    elif isinstance(alg, PolarNewton):
+        alg = alg.qdwh()
     if A.dtype not in _DT:
         raise ArgumentError(f"unsupported element type {A.dtype} (Float64 / ComplexF64)")
     if right:
@@ -377,3 +385,259 @@ def left_polar(A, alg=None):
 def right_polar(A, alg=None):
     """``right_polar(A)``."""
     return right_polar_(colmajor(A), None, alg)
+
+
+# ---------------------------------------------------------------------------
+# LQ (src/implementations/lq.jl) through the QR of A^H
+# ---------------------------------------------------------------------------
+def check_input_lq_full(A, LQ):
+    """``check_input(lq_full!, A, (L, Q), alg)`` (src/implementations/lq.jl:9-18)."""
+    m, n = A.shape
+    L, Q = LQ
+    if L.numel() > 0:
+        _check_size(L, (m, n), "L")
+    _check_scalar(L, A, "L", "A")
+    _check_size(Q, (n, n), "Q")
+    _check_scalar(Q, A, "Q", "A")
+
+
+def check_input_lq_compact(A, LQ):
+    """``check_input(lq_compact!, A, (L, Q), alg)`` (src/implementations/lq.jl:19-29)."""
+    m, n = A.shape
+    k = min(m, n)
+    L, Q = LQ
+    if L.numel() > 0:
+        _check_size(L, (m, k), "L")
+    _check_scalar(L, A, "L", "A")
+    _check_size(Q, (k, n), "Q")
+    _check_scalar(Q, A, "Q", "A")
+
+
+def initialize_output_lq(A, full: bool):
+    """``initialize_output(lq_{full,compact}!, A, alg)`` (src/implementations/lq.jl:62-74)."""
+    m, n = A.shape
+    k = min(m, n)
+    if full:
+        return similar_colmajor(A, (m, n)), similar_colmajor(A, (n, n))
+    return similar_colmajor(A, (m, k)), similar_colmajor(A, (k, n))
+
+
+def _lq_call(A, LQ, alg, full: bool):
+    if alg is None:
+        alg = MI355XHouseholder()
+    if A.dtype not in _DT:
+        raise ArgumentError(f"unsupported element type {A.dtype} (Float64 / ComplexF64)")
+    (check_input_lq_full if full else check_input_lq_compact)(A, LQ)
+    L, Q = LQ
+    _check_device(A, L, Q)
+    m, n = A.shape
+    lib = _lib.load()
+    dt = _DT[A.dtype]
+    nbytes = C.c_size_t(0)
+    _status(lib.mak_lq_workspace_size(dt, int(full), m, n, C.byref(nbytes)), "mak_lq_workspace_size")
+    dev = A.device if A.numel() else Q.device
+    ws = _workspace(dev, nbytes.value)
+    fname = f"mak_lq_{'full' if full else 'compact'}_{'z' if dt else 'd'}"
+    Larg = L if L.numel() > 0 else None
+    code = getattr(lib, fname)(_stream(dev), m, n, _ptr(A), _ld(A), _ptr(Larg), _ld(L) if Larg is not None else 1,
+                               _ptr(Q), _ld(Q), int(bool(alg.positive)), _ptr(ws), nbytes.value)
+    _status(code, fname)
+    return L, Q
+
+
+def lq_compact_(A, LQ=None, alg=None):
+    """``lq_compact!(A, (L, Q), alg)`` as ``LQViaTransposedQR`` (src/implementations/lq.jl:290-296):
+    returns the given (L, Q); A is only read."""
+    if LQ is None:
+        LQ = initialize_output_lq(A, False)
+    return _lq_call(A, LQ, alg, False)
+
+
+def lq_full_(A, LQ=None, alg=None):
+    """``lq_full!(A, (L, Q), alg)`` as ``LQViaTransposedQR`` (src/implementations/lq.jl:284-289)."""
+    if LQ is None:
+        LQ = initialize_output_lq(A, True)
+    return _lq_call(A, LQ, alg, True)
+
+
+def lq_compact(A, alg=None):
+    return lq_compact_(colmajor(A), None, alg)
+
+
+def lq_full(A, alg=None):
+    return lq_full_(colmajor(A), None, alg)
+
+
+# ---------------------------------------------------------------------------
+# Null spaces (src/implementations/qr.jl:114-117,232-263; lq.jl:297-318)
+# ---------------------------------------------------------------------------
+def check_input_qr_null(A, N):
+    """``check_input(qr_null!, A, N, alg)`` (src/implementations/qr.jl:30-37)."""
+    m, n = A.shape
+    _check_size(N, (m, m - min(m, n)), "N")
+    _check_scalar(N, A, "N", "A")
+
+
+def check_input_lq_null(A, Nh):
+    """``check_input(lq_null!, A, Nᴴ, alg)`` (src/implementations/lq.jl:30-37)."""
+    m, n = A.shape
+    _check_size(Nh, (n - min(m, n), n), "Nᴴ")
+    _check_scalar(Nh, A, "Nᴴ", "A")
+
+
+def initialize_output_qr_null(A):
+    m, n = A.shape
+    return similar_colmajor(A, (m, m - min(m, n)))
+
+
+def initialize_output_lq_null(A):
+    m, n = A.shape
+    return similar_colmajor(A, (n - min(m, n), n))
+
+
+def _null_call(A, N, alg, lq: bool):
+    if alg is None:
+        alg = MI355XHouseholder()
+    if A.dtype not in _DT:
+        raise ArgumentError(f"unsupported element type {A.dtype} (Float64 / ComplexF64)")
+    (check_input_lq_null if lq else check_input_qr_null)(A, N)
+    _check_device(A, N)
+    m, n = A.shape
+    lib = _lib.load()
+    dt = _DT[A.dtype]
+    nbytes = C.c_size_t(0)
+    _status(lib.mak_null_workspace_size(dt, int(lq), m, n, C.byref(nbytes)), "mak_null_workspace_size")
+    dev = A.device if A.numel() else N.device
+    ws = _workspace(dev, nbytes.value)
+    fname = f"mak_{'lq' if lq else 'qr'}_null_{'z' if dt else 'd'}"
+    code = getattr(lib, fname)(_stream(dev), m, n, _ptr(A), _ld(A), _ptr(N), _ld(N), _ptr(ws), nbytes.value)
+    _status(code, fname)
+    return N
+
+
+def qr_null_(A, N=None, alg=None):
+    """``qr_null!(A, N, alg)``: N = H_1 ... H_k [0; I] (src/implementations/qr.jl:236-263); A is destroyed."""
+    if N is None:
+        N = initialize_output_qr_null(A)
+    return _null_call(A, N, alg, False)
+
+
+def lq_null_(A, Nh=None, alg=None):
+    """``lq_null!(A, Nᴴ, alg)`` via the QR null space of Aᴴ (src/implementations/lq.jl:297-301,312-318)."""
+    if Nh is None:
+        Nh = initialize_output_lq_null(A)
+    return _null_call(A, Nh, alg, True)
+
+
+def qr_null(A, alg=None):
+    return qr_null_(colmajor(A.clone() if A.stride(0) == 1 else A), None, alg)
+
+
+def lq_null(A, alg=None):
+    return lq_null_(colmajor(A), None, alg)
+
+
+# ---------------------------------------------------------------------------
+# PolarNewton on the device, orth / null composites, project_isometric!
+# ---------------------------------------------------------------------------
+@dataclass(frozen=True)
+class PolarNewton:
+    """``PolarNewton(; maxiter, tol)`` (src/interface/decompositions.jl:305) on
+    the device.  The reference's Newton loop (src/implementations/polar.jl:
+    128-206) needs an LU-based inverse per step; here the same unique polar
+    factor comes from the QDWH iteration of ``MI355XPolar`` (FP64), which
+    converges in at most 6 GEMM-rich steps for condition numbers up to 1e16.
+    ``maxiter`` caps the iterations (default 10, as the reference) and
+    non-convergence warns as there.  ``tol`` (default eps^(2/3),
+    ``defaulttol``) is accepted for the interface: QDWH's own stopping test,
+    ||X_k - X_{k-1}||_F <= (5u)^(1/3) ||X_k||_F after which the cubically
+    convergent step leaves an error of order u, is the stricter one."""
+    maxiter: int = 10
+    tol: float | None = None
+
+    def __post_init__(self):
+        if self.tol is not None and not self.tol > 0:
+            raise ArgumentError(f"tol must be positive (got {self.tol})")
+        if self.maxiter < 1:
+            raise ArgumentError(f"maxiter must be positive (got {self.maxiter})")
+
+    def qdwh(self) -> "MI355XPolar":
+        return MI355XPolar(precision=64, maxiter=self.maxiter)
+
+
+def _polar_alg(alg):
+    return alg.qdwh() if isinstance(alg, PolarNewton) else alg
+
+
+def left_orth_alg(alg):
+    """``left_orth_alg`` (src/interface/orthnull.jl:442-444): QR algorithms
+    factor with ``qr_compact!``, polar algorithms with ``left_polar!``."""
+    if alg is None or isinstance(alg, MI355XHouseholder) or alg == "qr":
+        return "qr", (alg if isinstance(alg, MI355XHouseholder) else MI355XHouseholder())
+    if isinstance(alg, (MI355XPolar, PolarNewton)) or alg == "polar":
+        return "polar", (_polar_alg(alg) if not isinstance(alg, str) else MI355XPolar())
+    raise ArgumentError(f"Unknown or invalid `left_orth` algorithm type `{type(alg).__name__}`")
+
+
+right_orth_alg = left_orth_alg
+
+
+def left_orth_(A, VC=None, alg=None):
+    """``left_orth!(A, (V, C), alg)`` (src/implementations/orthnull.jl:79-81):
+    (Q, R) of ``qr_compact!`` or (W, P) of ``left_polar!``."""
+    kind, a = left_orth_alg(alg)
+    if kind == "qr":
+        return qr_compact_(A, VC, a)
+    return left_polar_(A, VC, a)
+
+
+def right_orth_(A, CVh=None, alg=None):
+    """``right_orth!(A, (C, Vᴴ), alg)`` (src/implementations/orthnull.jl:88-90):
+    (L, Q) of ``lq_compact!`` or (P, Wᴴ) of ``right_polar!``."""
+    kind, a = right_orth_alg(alg)
+    if kind == "qr":
+        return lq_compact_(A, CVh, a)
+    return right_polar_(A, CVh, a)
+
+
+def left_orth(A, alg=None):
+    return left_orth_(colmajor(A.clone() if A.stride(0) == 1 else A), None, alg)
+
+
+def right_orth(A, alg=None):
+    return right_orth_(colmajor(A.clone() if A.stride(0) == 1 else A), None, alg)
+
+
+def left_null_(A, N=None, alg=None):
+    """``left_null!(A, N, ::LeftNullViaQR)`` = ``qr_null!`` (src/implementations/orthnull.jl:101-102)."""
+    return qr_null_(A, N, alg)
+
+
+def right_null_(A, Nh=None, alg=None):
+    """``right_null!(A, Nᴴ, ::RightNullViaLQ)`` = ``lq_null!`` (src/implementations/orthnull.jl:110-111)."""
+    return lq_null_(A, Nh, alg)
+
+
+def left_null(A, alg=None):
+    return qr_null(A, alg)
+
+
+def right_null(A, alg=None):
+    return lq_null(A, alg)
+
+
+def project_isometric_(A, W=None, alg=None):
+    """``project_isometric!(A, W, alg)`` (src/implementations/projections.jl:70-75):
+    the polar factor W of ``left_polar!`` without P."""
+    m, n = A.shape
+    if m < n:
+        raise DimensionMismatch("project_isometric! needs size(A, 1) >= size(A, 2)")
+    if W is None:
+        W = similar_colmajor(A, (m, n))
+    noP = torch.empty((0, 0), dtype=A.dtype, device=A.device)
+    left_polar_(A, (W, noP), _polar_alg(alg) if alg is not None else None)
+    return W
+
+
+def project_isometric(A, alg=None):
+    return project_isometric_(colmajor(A), None, alg)

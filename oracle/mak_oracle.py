@@ -176,6 +176,21 @@ def qr_null(A, blocksize=0):
     return gemqrt_left_notrans(a[:, :minmn], t, N)
 
 
+def lq_via_qr(A, full=False, positive=True):
+    """``lq_via_qr!`` (src/implementations/lq.jl:303-310) behind
+    ``lq_compact!`` / ``lq_full!(A, LQ, ::LQViaTransposedQR)`` (:283-296):
+    the QR of A^H, conjugate-transposed: L = R^H, Q = Q_t^H."""
+    At = np.conj(np.asarray(A)).T.copy()
+    Qt, Rt = qr_full(At, positive) if full else qr_compact(At, positive)
+    return np.conj(Rt).T.copy(), np.conj(Qt).T.copy()
+
+
+def lq_null(A):
+    """``lq_null_via_qr!`` (src/implementations/lq.jl:312-318): N^H from the QR
+    null space of A^H."""
+    return np.conj(qr_null(np.conj(np.asarray(A)).T.copy())).T.copy()
+
+
 def householder_native(x, i=0):
     """``_householder!(v, i)`` (src/common/householder.jl:35-67), 0-based i.
 

@@ -135,3 +135,55 @@ def test_householder_options():
         mak.MI355XHouseholder(blocksize=32)
     with pytest.raises(mak.ArgumentError, match="pivoted"):
         mak.MI355XHouseholder(pivoted=True)
+
+
+def test_lq_and_null_entry_points_validate_before_the_device():
+    lib = _lib.load()
+    null = C.c_void_p(0)
+    b = C.c_size_t(0)
+    for dt in (0, 1):
+        for full in (0, 1):
+            for m, n in ((4, 0), (0, 5), (3, 9), (9, 3)):
+                assert lib.mak_lq_workspace_size(dt, full, m, n, C.byref(b)) == 0 and b.value > 0
+        for lq in (0, 1):
+            assert lib.mak_null_workspace_size(dt, lq, 9, 3, C.byref(b)) == 0 and b.value > 0
+    assert lib.mak_lq_workspace_size(5, 0, 3, 3, C.byref(b)) == -1
+    assert lib.mak_lq_workspace_size(0, 0, -1, 3, C.byref(b)) == -3
+    assert lib.mak_null_workspace_size(0, 0, 3, -1, C.byref(b)) == -4
+    assert lib.mak_null_workspace_size(0, 0, 3, 3, None) == -5
+    # lq_compact!: m < 0, n < 0, lda < m, ldq < k, Q overlapping A, no workspace
+    assert lib.mak_lq_compact_d(null, -1, 3, null, 1, null, 1, null, 1, 1, null, 0) == -2
+    assert lib.mak_lq_compact_d(null, 3, -1, null, 3, null, 3, null, 1, 1, null, 0) == -3
+    assert lib.mak_lq_compact_d(null, 3, 5, C.c_void_p(8), 2, null, 3, C.c_void_p(4096), 3, 1, null, 0) == -5
+    assert lib.mak_lq_compact_d(null, 3, 5, C.c_void_p(8), 3, null, 3, C.c_void_p(4096), 2, 1, null, 0) == -9
+    assert lib.mak_lq_compact_d(null, 3, 5, C.c_void_p(8), 3, null, 3, C.c_void_p(16), 3, 1, null, 0) == -8
+    assert lib.mak_lq_full_z(null, 3, 5, C.c_void_p(16), 3, null, 3, C.c_void_p(1 << 20), 5, 1, null, 0) == -11
+    # qr_null! / lq_null!: ldn too small, N overlapping A, no workspace
+    assert lib.mak_qr_null_d(null, 9, 3, C.c_void_p(8), 9, C.c_void_p(1 << 20), 8, null, 0) == -7
+    assert lib.mak_qr_null_d(null, 9, 3, C.c_void_p(8), 9, C.c_void_p(16), 9, null, 0) == -6
+    assert lib.mak_qr_null_z(null, 9, 3, C.c_void_p(16), 9, C.c_void_p(1 << 20), 9, null, 0) == -9
+    assert lib.mak_lq_null_d(null, 3, 9, C.c_void_p(8), 3, C.c_void_p(1 << 20), 5, null, 0) == -7
+
+
+def test_lq_null_orth_host_checks_mirror_reference():
+    A = torch.zeros(3, 7, dtype=torch.float64)
+    L, Q = mak.initialize_output_lq(A, False)
+    assert L.shape == (3, 3) and Q.shape == (3, 7) and Q.stride(0) == 1
+    L, Q = mak.initialize_output_lq(A, True)
+    assert L.shape == (3, 7) and Q.shape == (7, 7)
+    with pytest.raises(mak.DimensionMismatch):
+        mak.check_input_lq_compact(A, (L, Q))
+    mak.check_input_lq_full(A, (torch.zeros(0, 0, dtype=torch.float64), Q))
+    assert mak.initialize_output_qr_null(A).shape == (3, 0)
+    assert mak.initialize_output_lq_null(A).shape == (4, 7)
+    with pytest.raises(mak.DimensionMismatch):
+        mak.check_input_lq_null(A, torch.zeros(3, 7, dtype=torch.float64))
+    with pytest.raises(mak.ArgumentError):
+        mak.check_input_qr_null(A.t(), torch.zeros(7, 4, dtype=torch.complex128))
+    assert mak.left_orth_alg(None)[0] == "qr"
+    assert mak.left_orth_alg(mak.PolarNewton())[0] == "polar"
+    assert mak.left_orth_alg(mak.MI355XPolar(precision=32))[1].precision == 32
+    with pytest.raises(mak.ArgumentError):
+        mak.left_orth_alg(3.0)
+    with pytest.raises(mak.ArgumentError):
+        mak.PolarNewton(tol=-1.0)

@@ -319,3 +319,51 @@ def test_flop_counts():
     assert abs(O.qr_flops(8192, 1024) - (4 * 8192 * 1024 ** 2 - 4 * 1024 ** 3 / 3)) < 1
     assert abs(O.qr_flops(8192, 1024, kind="full")
                - (4 * 8192 ** 2 * 1024 - 2 * 8192 * 1024 ** 2 + 2 * 1024 ** 3 / 3)) < 1
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("m,n", [(20, 45), (45, 20), (30, 30)])
+def test_lq_via_qr_pinned_by_numpy_and_invariants(m, n, dt):
+    """LQ via the transposed QR: A = L Q, L lower triangular with a real
+    positive diagonal, Q with orthonormal rows; unique for full-rank A, so it
+    equals numpy's LQ (QR of A^H, signs moved to the diagonal)."""
+    A = O.gaussian(m, n, dt, seed=31 + m)
+    L, Q = O.lq_via_qr(A)
+    k = min(m, n)
+    assert L.shape == (m, k) and Q.shape == (k, n)
+    assert np.allclose(np.triu(L, 1), 0)
+    d = np.diag(L)
+    assert np.all(np.abs(d.imag) < 1e-14) and np.all(d.real > 0)
+    assert rel(L @ Q, A) < 1e-13
+    assert rel(Q @ Q.conj().T, np.eye(k)) < 1e-13
+    q, r = np.linalg.qr(A.conj().T)
+    s = np.diag(r) / np.abs(np.diag(r))
+    Ln, Qn = (r * np.conj(s)[:, None]).conj().T, (q * s[None, :]).conj().T
+    assert rel(L, Ln) < 1e-12 and rel(Q, Qn) < 1e-12
+    Lf, Qf = O.lq_via_qr(A, full=True)
+    assert Lf.shape == (m, n) and Qf.shape == (n, n)
+    assert rel(Qf @ Qf.conj().T, np.eye(n)) < 1e-13 and rel(Lf @ Qf, A) < 1e-13
+    assert rel(Qf[:k], Q) < 1e-12
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("m,n", [(40, 12), (12, 40), (20, 20)])
+def test_null_spaces_pinned_by_svd(m, n, dt):
+    """qr_null / lq_null span the orthogonal complements that the SVD's
+    trailing singular vectors span (full-rank A)."""
+    A = O.gaussian(m, n, dt, seed=77 + n)
+    N = O.qr_null(A)
+    Nh = O.lq_null(A)
+    k = min(m, n)
+    assert N.shape == (m, m - k) and Nh.shape == (n - k, n)
+    U, _, Vh = np.linalg.svd(A)
+    if m > k:
+        assert rel(N.conj().T @ N, np.eye(m - k)) < 1e-13
+        assert np.linalg.norm(A.conj().T @ N) < 1e-12 * np.linalg.norm(A)
+        Pu = U[:, k:] @ U[:, k:].conj().T
+        assert rel(N @ N.conj().T, Pu) < 1e-12
+    if n > k:
+        assert rel(Nh @ Nh.conj().T, np.eye(n - k)) < 1e-13
+        assert np.linalg.norm(A @ Nh.conj().T) < 1e-12 * np.linalg.norm(A)
+        Pv = Vh[k:].conj().T @ Vh[k:]
+        assert rel(Nh.conj().T @ Nh, Pv) < 1e-12
