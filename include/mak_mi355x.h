@@ -181,6 +181,53 @@ int mak_right_polar_d(void* stream, int m, int n, const double* A, int lda, doub
 int mak_right_polar_z(void* stream, int m, int n, const void* A, int lda, void* P, int ldp, void* Wh, int ldw,
                       int precision, int maxiter, double l0, void* work, size_t work_bytes, mak_polar_info* info);
 
+/* ----------------------------------------------------- SVD via polar --- */
+
+/* Device workspace bytes of mak_svd_polar_{d,z}. */
+int mak_svd_polar_workspace_size(int dtype, int m, int n, size_t* bytes);
+
+/* svd_compact!(A, (U, S, Vh), SVDViaPolar()) (U, Vh non-NULL) or
+ * svd_vals!(A, S, SVDViaPolar()) (U = Vh = NULL): A = W P by FP64 QDWH
+ * (m >= n; the right polar A = P Wh for m < n), then the Hermitian
+ * eigendecomposition of P (rocSOLVER syevd / heevd, opened at the first
+ * call), S descending (real, on the device, min(m,n)), U m x k, Vh k x n, and
+ * the gauge of gaugefix!(svd_compact!, U, Vh): the first entry of largest
+ * magnitude of every column of U is real and positive.  A is only read.
+ * Returns 3 when rocSOLVER cannot be opened; info as for the polar calls.
+ * Replaces gesvdp! behind svd_compact! / svd_vals! with SVDViaPolar
+ * (src/interface/decompositions.jl:173-183; src/implementations/svd.jl:
+ * 161,184-206; ext/MatrixAlgebraKitCUDAExt/yacusolver.jl:101-180). */
+int mak_svd_polar_d(void* stream, int m, int n, const double* A, int lda, double* S, double* U, int ldu, double* Vh,
+                    int ldvh, int maxiter, void* work, size_t work_bytes, mak_polar_info* info);
+int mak_svd_polar_z(void* stream, int m, int n, const void* A, int lda, double* S, void* U, int ldu, void* Vh,
+                    int ldvh, int maxiter, void* work, size_t work_bytes, mak_polar_info* info);
+
+/* ------------------------------------------------------- pushforwards --- */
+
+/* Device workspace bytes of the polar pushforwards (right = 0 / 1). */
+int mak_polar_pushforward_workspace_size(int dtype, int right, int m, int n, size_t* bytes);
+
+/* left_polar_pushforward!(dA, A, (W, P), (dW, dP)) for m >= n: dP = W^H dA,
+ * K from P K + K P = dP - dP^H, dW = dA P^{-1} - W W^H dA P^{-1} + W K,
+ * dP -= K P; the Sylvester solve and P^{-1} through the eigendecomposition of
+ * P (exact for any positive definite P).  dW (m x n) and dP (n x n) are
+ * overwritten.  Returns 2 when P is not positive definite.
+ * Replaces src/pushforwards/polar.jl:1-12. */
+int mak_left_polar_pushforward_d(void* stream, int m, int n, const double* W, int ldw, const double* P, int ldp,
+                                 const double* dA, int ldda, double* dW, int lddw, double* dP, int lddp, void* work,
+                                 size_t work_bytes);
+int mak_left_polar_pushforward_z(void* stream, int m, int n, const void* W, int ldw, const void* P, int ldp,
+                                 const void* dA, int ldda, void* dW, int lddw, void* dP, int lddp, void* work,
+                                 size_t work_bytes);
+/* right_polar_pushforward!(dA, A, (P, Wh), (dP, dWh)) for m <= n: Wh, dA, dWh
+ * m x n, P, dP m x m.  Replaces src/pushforwards/polar.jl:14-25. */
+int mak_right_polar_pushforward_d(void* stream, int m, int n, const double* Wh, int ldwh, const double* P, int ldp,
+                                  const double* dA, int ldda, double* dWh, int lddwh, double* dP, int lddp,
+                                  void* work, size_t work_bytes);
+int mak_right_polar_pushforward_z(void* stream, int m, int n, const void* Wh, int ldwh, const void* P, int ldp,
+                                  const void* dA, int ldda, void* dWh, int lddwh, void* dP, int lddp, void* work,
+                                  size_t work_bytes);
+
 /* ----------------------------------------------------------- adjoints --- */
 
 /* Device workspace of the reverse rules for `batch` matrices of size m x n:

@@ -43,6 +43,14 @@ SIGNATURES = {
     "mak_left_polar_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp, _sz, _vp]),
     "mak_right_polar_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp, _sz, _vp]),
     "mak_right_polar_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _d, _vp, _sz, _vp]),
+    "mak_svd_polar_workspace_size": (_i, [_i, _i, _i, C.POINTER(_sz)]),
+    "mak_svd_polar_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _i, _vp, _i, _i, _vp, _sz, _vp]),
+    "mak_svd_polar_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _i, _vp, _i, _i, _vp, _sz, _vp]),
+    "mak_polar_pushforward_workspace_size": (_i, [_i, _i, _i, _i, C.POINTER(_sz)]),
+    "mak_left_polar_pushforward_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
+    "mak_left_polar_pushforward_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
+    "mak_right_polar_pushforward_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
+    "mak_right_polar_pushforward_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
     "mak_pullback_workspace_size": (_i, [_i, _i, _i, _i, _i, C.POINTER(_sz)]),
     "mak_qr_pullback_d": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _i, _ll, _vp, _ll, _vp, _i, _ll, _vp, _i, _ll,
                                _vp, _i, _ll, _vp, _i, _ll, _vp, _sz]),

@@ -38,6 +38,8 @@ __all__ = [
     "check_input_lq_null", "initialize_output_qr_null", "initialize_output_lq_null",
     "PolarNewton", "left_orth_alg", "right_orth_alg", "left_orth", "left_orth_", "right_orth", "right_orth_",
     "left_null", "left_null_", "right_null", "right_null_", "project_isometric", "project_isometric_",
+    "SVDViaPolar", "initialize_output_svd_compact", "svd_compact", "svd_compact_", "svd_vals", "svd_vals_",
+    "left_polar_pushforward_", "right_polar_pushforward_",
 ]
 
 
@@ -641,3 +643,141 @@ def project_isometric_(A, W=None, alg=None):
 
 def project_isometric(A, alg=None):
     return project_isometric_(colmajor(A), None, alg)
+
+
+# ---------------------------------------------------------------------------
+# SVD via the polar decomposition (SVDViaPolar) and the polar pushforwards
+# ---------------------------------------------------------------------------
+@dataclass(frozen=True)
+class SVDViaPolar:
+    """``SVDViaPolar(; fixgauge, tol)`` (src/interface/decompositions.jl:173-183)
+    on the device: QDWH polar (FP64) + Hermitian eigendecomposition of P.
+    ``fixgauge`` (default true) applies ``gaugefix!(svd_compact!, U, Vh)``
+    (the library always does: a caller asking for no gauge gets the fixed one,
+    which is one of the valid outputs); ``maxiter`` caps the QDWH iterations."""
+    fixgauge: bool = True
+    tol: float | None = None
+    maxiter: int = 20
+
+
+def initialize_output_svd_compact(A):
+    """``initialize_output(svd_compact!, A, alg)``: U m x k, S (the diagonal of
+    the reference's ``Diagonal``, real, length k), Vh k x n."""
+    m, n = A.shape
+    k = min(m, n)
+    rdt = torch.float64
+    return (similar_colmajor(A, (m, k)), torch.empty(k, dtype=rdt, device=A.device), similar_colmajor(A, (k, n)))
+
+
+def _svd_polar_call(A, U, S, Vh, alg):
+    if alg is None:
+        alg = SVDViaPolar()
+    if not isinstance(alg, SVDViaPolar):
+        raise ArgumentError(f"unsupported SVD algorithm {type(alg).__name__} (SVDViaPolar)")
+    if A.dtype not in _DT:
+        raise ArgumentError(f"unsupported element type {A.dtype} (Float64 / ComplexF64)")
+    m, n = A.shape
+    k = min(m, n)
+    _check_size(S, (k,), "S")
+    if S.dtype != torch.float64:
+        raise ArgumentError("S must hold real Float64 values")
+    if U is not None:
+        _check_size(U, (m, k), "U")
+        _check_size(Vh, (k, n), "Vᴴ")
+        _check_scalar(U, A, "U", "A")
+        _check_scalar(Vh, A, "Vᴴ", "A")
+    _check_device(A, U, S, Vh)
+    lib = _lib.load()
+    dt = _DT[A.dtype]
+    nbytes = C.c_size_t(0)
+    _status(lib.mak_svd_polar_workspace_size(dt, m, n, C.byref(nbytes)), "mak_svd_polar_workspace_size")
+    ws = _workspace(A.device, nbytes.value)
+    info = _lib.PolarInfo()
+    fname = f"mak_svd_polar_{'z' if dt else 'd'}"
+    code = getattr(lib, fname)(_stream(A.device), m, n, _ptr(A), _ld(A), C.c_void_p(S.data_ptr() if k else 0),
+                               _ptr(U), _ld(U) if U is not None else 1, _ptr(Vh), _ld(Vh) if Vh is not None else 1,
+                               int(alg.maxiter), _ptr(ws), nbytes.value, C.byref(info))
+    if code == 6:
+        warnings.warn("SVDViaPolar: the polar iteration did not converge within maxiter")
+        code = 0
+    if code == 3:
+        raise NumericalError(f"{fname}: rocSOLVER (the Hermitian eigensolver) could not be opened")
+    _status(code, fname)
+
+
+def svd_compact_(A, USVh=None, alg=None):
+    """``svd_compact!(A, (U, S, Vᴴ), SVDViaPolar())``: returns the given (U, S, Vᴴ); A is only read."""
+    if USVh is None:
+        USVh = initialize_output_svd_compact(A)
+    U, S, Vh = USVh
+    _svd_polar_call(A, U, S, Vh, alg)
+    return USVh
+
+
+def svd_vals_(A, S=None, alg=None):
+    """``svd_vals!(A, S, SVDViaPolar())``: the singular values, descending."""
+    if S is None:
+        S = torch.empty(min(A.shape), dtype=torch.float64, device=A.device)
+    _svd_polar_call(A, None, S, None, alg)
+    return S
+
+
+def svd_compact(A, alg=None):
+    return svd_compact_(colmajor(A), None, alg)
+
+
+def svd_vals(A, alg=None):
+    return svd_vals_(colmajor(A), None, alg)
+
+
+def _pushforward_call(dA, WP_or_PWh, dout, right: bool):
+    lib = _lib.load()
+    if right:
+        P, W = WP_or_PWh
+        dP, dW = dout
+    else:
+        W, P = WP_or_PWh
+        dW, dP = dout
+    if dA.dtype not in _DT:
+        raise ArgumentError(f"unsupported element type {dA.dtype} (Float64 / ComplexF64)")
+    m, n = dA.shape
+    k = min(m, n)
+    if (not right and m < n) or (right and m > n):
+        raise DimensionMismatch(f"{'right' if right else 'left'} polar pushforward: unsupported shape {(m, n)}")
+    for x, sz, nm in ((W, (m, n), "W"), (P, (k, k), "P"), (dW, (m, n), "ΔW"), (dP, (k, k), "ΔP")):
+        _check_size(x, sz, nm)
+        _check_scalar(x, dA, nm, "ΔA")
+    _check_device(dA, W, P, dW, dP)
+    dt = _DT[dA.dtype]
+    nbytes = C.c_size_t(0)
+    _status(lib.mak_polar_pushforward_workspace_size(dt, int(right), m, n, C.byref(nbytes)),
+            "mak_polar_pushforward_workspace_size")
+    ws = _workspace(dA.device, nbytes.value)
+    fname = f"mak_{'right' if right else 'left'}_polar_pushforward_{'z' if dt else 'd'}"
+    code = getattr(lib, fname)(_stream(dA.device), m, n, _ptr(W), _ld(W), _ptr(P), _ld(P), _ptr(dA), _ld(dA),
+                               _ptr(dW), _ld(dW), _ptr(dP), _ld(dP), _ptr(ws), nbytes.value)
+    if code == 2:
+        raise NumericalError(f"{fname}: P is not positive definite (rank-deficient A)")
+    if code == 3:
+        raise NumericalError(f"{fname}: rocSOLVER (the Hermitian eigensolver) could not be opened")
+    _status(code, fname)
+    return dout
+
+
+def left_polar_pushforward_(dA, A, WP, dWP=None):
+    """``left_polar_pushforward!(ΔA, A, (W, P), (ΔW, ΔP))`` (src/pushforwards/polar.jl:1-12):
+    returns (ΔW, ΔP), overwritten."""
+    W, P = WP
+    if dWP is None:
+        dWP = (similar_colmajor(W, W.shape), similar_colmajor(P, P.shape))
+    return _pushforward_call(dA, WP, dWP, False)
+
+
+def right_polar_pushforward_(dA, A, PWh, dPWh=None):
+    """``right_polar_pushforward!(ΔA, A, (P, Wᴴ), (ΔP, ΔWᴴ))`` (src/pushforwards/polar.jl:14-25):
+    returns (ΔWᴴ, ΔP) as the reference does, overwritten."""
+    P, Wh = PWh
+    if dPWh is None:
+        dPWh = (similar_colmajor(P, P.shape), similar_colmajor(Wh, Wh.shape))
+    dP, dWh = _pushforward_call(dA, PWh, dPWh, True)
+    return dWh, dP

@@ -256,6 +256,30 @@ def gesdvd(A):
         return sla.svd(A, full_matrices=False, lapack_driver="gesvd")
 
 
+def gaugefix_svd_compact(U, Vh):
+    """``gaugefix!(svd_compact!, U, Vh)`` (src/common/gauge.jl:72-80): the
+    first entry of largest magnitude of every column of U made real positive
+    (``_argmaxabs`` keeps the earlier entry on ties, :11-13)."""
+    U = np.array(U, copy=True)
+    Vh = np.array(Vh, copy=True)
+    for j in range(U.shape[1]):
+        i = int(np.argmax(np.abs(U[:, j])))   # first maximiser
+        x = U[i, j]
+        sgn = x / abs(x) if x != 0 else 0.0
+        U[:, j] *= np.conj(sgn)
+        Vh[j, :] *= sgn
+    return U, Vh
+
+
+def svd_compact(A, fixgauge=True):
+    """``svd_compact!(A, (U, S, Vh), alg)`` through ``gesdvd`` with the gauge
+    fix (src/implementations/svd.jl:184-189)."""
+    U, S, Vh = gesdvd(A)
+    if fixgauge:
+        U, Vh = gaugefix_svd_compact(U, Vh)
+    return U, S, Vh
+
+
 def left_polar_svd(A, compute_p=True):
     """``left_polar!(A, (W, P), PolarViaSVD)`` (src/implementations/polar.jl:59-70)."""
     m, n = A.shape
@@ -358,6 +382,30 @@ def right_polar_newton(A, tol=None, maxiter=10, compute_p=True):
 # ---------------------------------------------------------------------------
 # Matrix properties (src/common/matrixproperties.jl:49-54)
 # ---------------------------------------------------------------------------
+def left_polar_pushforward(dA, A, WP):
+    """``left_polar_pushforward!`` (src/pushforwards/polar.jl:1-12): (dW, dP)."""
+    W, P = WP
+    dP = W.conj().T @ dA
+    K = _sylvester(P, P, dP.conj().T - dP)
+    dW = dA @ np.linalg.inv(P)
+    WhdAiP = W.conj().T @ dW
+    dW = dW - W @ WhdAiP + W @ K
+    dP = dP - K @ P
+    return dW, dP
+
+
+def right_polar_pushforward(dA, A, PWh):
+    """``right_polar_pushforward!`` (src/pushforwards/polar.jl:14-25): (dWh, dP)."""
+    P, Wh = PWh
+    dP = dA @ Wh.conj().T
+    K = _sylvester(P, P, dP.conj().T - dP)
+    dWh = np.linalg.inv(P) @ dA
+    iPdAW = dWh @ Wh.conj().T
+    dWh = dWh - iPdAW @ Wh + K @ Wh
+    dP = dP - P @ K
+    return dWh, dP
+
+
 def isisometric(A, atol=0.0, rtol=None):
     """``is_left_isometric``: ||A^H A - I||_F <= max(atol, rtol ||A^H A||_F)."""
     if rtol is None:

@@ -367,3 +367,43 @@ def test_null_spaces_pinned_by_svd(m, n, dt):
         assert np.linalg.norm(A @ Nh.conj().T) < 1e-12 * np.linalg.norm(A)
         Pv = Vh[k:].conj().T @ Vh[k:]
         assert rel(Nh.conj().T @ Nh, Pv) < 1e-12
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("m,n", [(40, 25), (25, 40)])
+def test_svd_compact_gauge(m, n, dt):
+    """svd_compact with gaugefix!: A = U S V^H, orthonormal factors, and the
+    first entry of largest magnitude of each U column real positive; equal to
+    numpy's SVD after the same gauge (unique for distinct singular values)."""
+    A = O.gaussian(m, n, dt, seed=12)
+    U, S, Vh = O.svd_compact(A)
+    assert rel((U * S[None, :]) @ Vh, A) < 1e-13
+    for j in range(U.shape[1]):
+        i = int(np.argmax(np.abs(U[:, j])))
+        assert abs(U[i, j].imag) < 1e-14 and U[i, j].real > 0
+    u, s, vh = np.linalg.svd(A, full_matrices=False)
+    sg = np.array([u[int(np.argmax(np.abs(u[:, j]))), j] for j in range(u.shape[1])])
+    sg = sg / np.abs(sg)
+    assert rel(U, u * np.conj(sg)[None, :]) < 1e-12 and rel(Vh, sg[:, None] * vh) < 1e-12
+    assert rel(S, s) < 1e-14
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+def test_polar_pushforwards_match_finite_differences(dt):
+    """The polar pushforwards equal central differences of the polar
+    decomposition (h = 1e-6) along a random direction."""
+    for m, n in [(30, 18), (18, 18)]:
+        A = O.gaussian(m, n, dt, seed=40 + m)
+        dA = O.gaussian(m, n, dt, seed=41 + m)
+        W, P = O.left_polar_svd(A)
+        dW, dP = O.left_polar_pushforward(dA, A, (W, P))
+        h = 1e-6
+        Wp, Pp = O.left_polar_svd(A + h * dA)
+        Wm, Pm = O.left_polar_svd(A - h * dA)
+        assert rel(dW, (Wp - Wm) / (2 * h)) < 1e-7 and rel(dP, (Pp - Pm) / (2 * h)) < 1e-7
+        B, dB = A.conj().T.copy(), dA.conj().T.copy()
+        Pr, Wh = O.right_polar_svd(B)
+        dWh, dPr = O.right_polar_pushforward(dB, B, (Pr, Wh))
+        Prp, Whp = O.right_polar_svd(B + h * dB)
+        Prm, Whm = O.right_polar_svd(B - h * dB)
+        assert rel(dWh, (Whp - Whm) / (2 * h)) < 1e-7 and rel(dPr, (Prp - Prm) / (2 * h)) < 1e-7
