@@ -254,8 +254,12 @@ int mak_qr_pullback_z(void* stream, int batch, int m, int n, int p, int positive
 /* left_polar_pullback!(dA, A, (W, P), (dW, dP)) for `batch` equally sized
  * matrices (m >= n), accumulating into dA.  The Sylvester equation
  * P C + C P = M - M^H is solved by preconditioned conjugate gradients
- * (maxiter <= 0: 200, tol <= 0: 1e-14 relative residual); iters receives the
- * iteration count (may be NULL).  Returns 6 when CG did not reach tol.
+ * (maxiter <= 0: 200, tol <= 0: 1e-14 relative residual); when CG has not
+ * converged after 48 steps (ill-conditioned P), the batch is solved exactly
+ * through the eigendecomposition of P (rocSOLVER, opened at first use),
+ * which also gives P^{-1}.  iters receives the CG step count (may be NULL).
+ * Returns 6 when CG did not reach tol and rocSOLVER is not available, 2 when
+ * P is not positive definite.
  * Replaces left_polar_pullback! (src/pullbacks/polar.jl:7-29). */
 int mak_left_polar_pullback_d(void* stream, int batch, int m, int n, const double* W, int ldw, long long strideW,
                               const double* P, int ldp, long long strideP, const double* dW, int lddw,

@@ -15,6 +15,11 @@
 // src/common/pullbacks.jl:15) is solved by preconditioned conjugate gradients
 // on the SPD operator C -> P C + C P (diagonal preconditioner
 // 1 / (P_ii + P_jj)); P^{-1} comes from a batched Cholesky factorisation.
+// CG needs about sqrt(cond(P)) steps: when it has not converged after
+// kPcgFirst steps, the batch is solved exactly through the eigendecomposition
+// P = Z diag(lambda) Z^H (rocSOLVER syevd / heevd, batched):
+//   C = Z [(Z^H F Z)_ij / (lambda_i + lambda_j)] Z^H,  P^{-1} = Z diag(1/lambda) Z^H,
+// whose accuracy does not depend on cond(P).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -25,10 +30,17 @@
 
 namespace mak {
 
+bool eigh_available();
+size_t eigh_workspace_bytes(int n, int batch);
+template <typename T>
+int eigh_batched(hipStream_t s, int n, T* A, int64_t lda, int64_t strideA, double* D, int64_t strideD, int batch,
+                 char* ws);
+
 namespace {
 
 constexpr int NB = 32;
 constexpr int DBK = 64;
+constexpr int kPcgFirst = 48;  // CG steps before the eigendecomposition takes over
 
 inline size_t al256(size_t b) { return ((b + 255) / 256) * 256; }
 
@@ -364,9 +376,28 @@ template <typename T> size_t qr_pullback_ws(int batch, int m, int n) {
   b += al256((size_t)batch * k * k * sizeof(T));      // triu(R11)
   return b;
 }
+template <typename T>
+__global__ void bsylv_divide_kernel(int n, T* X, int64_t ldx, int64_t sx, const double* lam, int64_t sl) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * n) return;
+  const int r = (int)(idx % n), c = (int)(idx / n);
+  const double* l = lam + (int64_t)blockIdx.z * sl;
+  T* x = X + (int64_t)blockIdx.z * sx + r + (int64_t)c * ldx;
+  *x = s_scale(*x, 1.0 / (l[r] + l[c]));
+}
+template <typename T>
+__global__ void bscale_cols_inv_kernel(int m, int n, T* X, int64_t ldx, int64_t sx, const double* lam, int64_t sl) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)m * n) return;
+  const int r = (int)(idx % m), c = (int)(idx / m);
+  T* x = X + (int64_t)blockIdx.z * sx + r + (int64_t)c * ldx;
+  *x = s_scale(*x, 1.0 / lam[(int64_t)blockIdx.z * sl + c]);
+}
+
 template <typename T> size_t polar_pullback_ws(int batch, int m, int n) {
   size_t b = 0;
   const size_t nn = al256((size_t)batch * n * n * sizeof(T));
+  b += al256((size_t)batch * n * sizeof(double)) + al256(eigh_workspace_bytes(n, std::max(batch, 1)));  // lambda
   b += nn * 10;                                        // dPh, M, F, X, R, Z, D, Qm, Pc, Pinv
   b += al256((size_t)batch * m * n * sizeof(T));      // Zw = dW - W W^H dW
   b += al256((size_t)batch * DBK * n * sizeof(T));    // Cholesky scratch
@@ -461,6 +492,8 @@ int polar_pullback(hipStream_t s, int batch, int m, int n, const T* W, int64_t l
   auto take = [&](size_t bytes) { char* q = w; w += al256(bytes); return q; };
   const int64_t snn = (int64_t)n * n, smn = (int64_t)m * n;
   const size_t nnb = (size_t)batch * snn * sizeof(T);
+  double* lam = (double*)take((size_t)batch * n * sizeof(double));
+  char* eigws = take(eigh_workspace_bytes(n, batch));
   T* dPh = (T*)take(nnb);
   T* M = (T*)take(nnb);
   T* F = (T*)take(nnb);
@@ -501,7 +534,9 @@ int polar_pullback(hipStream_t s, int batch, int m, int n, const T* W, int64_t l
   int it = 0;
   bool done = true;
   for (int b = 0; b < batch; ++b) done = done && (hff[b] == 0.0);
-  for (it = 0; it < maxiter && !done; ++it) {
+  const bool can_eig = eigh_available();
+  const int pcg_steps = can_eig ? std::min(maxiter, kPcgFirst) : maxiter;
+  for (it = 0; it < pcg_steps && !done; ++it) {
     // Qm = P D + D P
     gemm<T>(OP_N, OP_N, bga<T>(batch, n, n, n, P, ldp, sp, D, n, snn, Qm, n, snn, 1.0, 0.0), s);
     gemm<T>(OP_N, OP_N, bga<T>(batch, n, n, n, D, n, snn, P, ldp, sp, Qm, n, snn, 1.0, 1.0), s);
@@ -511,7 +546,7 @@ int polar_pullback(hipStream_t s, int batch, int m, int n, const T* W, int64_t l
     bdot<T>(batch, n, n, R, n, snn, Z, n, snn, partial, rznew, s);
     hipLaunchKernelGGL((bcg_dir_kernel<T>), gnn, dim3(256), 0, s, (int)snn, rznew, rz, Z, D);
     std::swap(rz, rznew);
-    if ((it + 1) % 4 == 0 || it + 1 == maxiter) {
+    if ((it + 1) % 4 == 0 || it + 1 == pcg_steps) {
       bdot<T>(batch, n, n, R, n, snn, R, n, snn, partial, dq, s);
       hipMemcpyAsync(hrr.data(), dq, batch * sizeof(double), hipMemcpyDeviceToHost, s);
       hipStreamSynchronize(s);
@@ -520,6 +555,26 @@ int polar_pullback(hipStream_t s, int batch, int m, int n, const T* W, int64_t l
     }
   }
   if (iters_out) *iters_out = it;
+  // CG has not converged: solve exactly through P = Z diag(lambda) Z^H (Z in Pc)
+  bool eig = false;
+  if (!done && can_eig) {
+    hipLaunchKernelGGL((bcopy_kernel<T>), gnn, dim3(256), 0, s, n, n, P, ldp, sp, Pc, (int64_t)n, snn);
+    const int est = eigh_batched<T>(s, n, Pc, n, snn, lam, n, batch, eigws);
+    if (est != 0) return est;
+    std::vector<double> hl((size_t)batch * n);
+    hipMemcpyAsync(hl.data(), lam, hl.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+    hipStreamSynchronize(s);
+    for (int b = 0; b < batch; ++b)
+      if (!(hl[(size_t)b * n] > 0.0)) return 2;  // P not positive definite
+    gemm<T>(OP_N, OP_N, bga<T>(batch, n, n, n, F, n, snn, Pc, n, snn, Qm, n, snn, 1.0, 0.0), s);   // F Z
+    gemm<T>(OP_C, OP_N, bga<T>(batch, n, n, n, Pc, n, snn, Qm, n, snn, R, n, snn, 1.0, 0.0), s);   // Z^H F Z
+    hipLaunchKernelGGL((bsylv_divide_kernel<T>), gnn, dim3(256), 0, s, n, R, (int64_t)n, snn, (const double*)lam,
+                       (int64_t)n);
+    gemm<T>(OP_N, OP_N, bga<T>(batch, n, n, n, Pc, n, snn, R, n, snn, Qm, n, snn, 1.0, 0.0), s);   // Z Y
+    gemm<T>(OP_N, OP_C, bga<T>(batch, n, n, n, Qm, n, snn, Pc, n, snn, X, n, snn, 1.0, 0.0), s);   // Z Y Z^H
+    done = true;
+    eig = true;
+  }
   // C = X + dPh ; dA += W C
   if (dP) hipLaunchKernelGGL((baxpy_kernel<T>), gnn, dim3(256), 0, s, n, n, 1.0, dPh, (int64_t)n, snn, X, (int64_t)n, snn);
   gemm<T>(OP_N, OP_N, bga<T>(batch, m, n, n, W, ldw, sw, X, n, snn, dA, ldda, sda, 1.0, 1.0), s);
@@ -528,20 +583,29 @@ int polar_pullback(hipStream_t s, int batch, int m, int n, const T* W, int64_t l
     gemm<T>(OP_C, OP_N, bga<T>(batch, n, n, m, W, ldw, sw, dW, lddw, sdw, M, n, snn, 1.0, 0.0), s);
     hipLaunchKernelGGL((bcopy_kernel<T>), egrid(smn, batch), dim3(256), 0, s, m, n, dW, lddw, sdw, Zw, (int64_t)m, smn);
     gemm<T>(OP_N, OP_N, bga<T>(batch, m, n, n, W, ldw, sw, M, n, snn, Zw, m, smn, -1.0, 1.0), s);
-    // P = Wp^H Wp ; P^{-1} = Wp^{-1} Wp^{-H}
-    hipLaunchKernelGGL((bcopy_kernel<T>), gnn, dim3(256), 0, s, n, n, P, ldp, sp, Pc, (int64_t)n, snn);
-    hipMemsetAsync(dinfo, 0, sizeof(int), s);
-    bpotrf_upper<T>(batch, n, Pc, n, snn, Yd, DBK, (int64_t)DBK * DBK, tmp, (int64_t)DBK * n, dinfo, s);
-    btrtri_upper<T>(batch, n, Pc, n, snn, Qm, n, snn, s);
-    gemm<T>(OP_N, OP_C, bga<T>(batch, n, n, n, Qm, n, snn, Qm, n, snn, Pinv, n, snn, 1.0, 0.0), s);
-    gemm<T>(OP_N, OP_N, bga<T>(batch, m, n, n, Zw, m, smn, Pinv, n, snn, dA, ldda, sda, 1.0, 1.0), s);
-    int hinfo = 0;
-    hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, s);
-    hipStreamSynchronize(s);
-    if (hinfo != 0) return 2;
+    if (eig) {
+      // P^{-1} = (Z diag(1/lambda)) Z^H
+      hipLaunchKernelGGL((bcopy_kernel<T>), gnn, dim3(256), 0, s, n, n, Pc, (int64_t)n, snn, Qm, (int64_t)n, snn);
+      hipLaunchKernelGGL((bscale_cols_inv_kernel<T>), gnn, dim3(256), 0, s, n, n, Qm, (int64_t)n, snn,
+                         (const double*)lam, (int64_t)n);
+      gemm<T>(OP_N, OP_C, bga<T>(batch, n, n, n, Qm, n, snn, Pc, n, snn, Pinv, n, snn, 1.0, 0.0), s);
+      gemm<T>(OP_N, OP_N, bga<T>(batch, m, n, n, Zw, m, smn, Pinv, n, snn, dA, ldda, sda, 1.0, 1.0), s);
+    } else {
+      // P = Wp^H Wp ; P^{-1} = Wp^{-1} Wp^{-H}
+      hipLaunchKernelGGL((bcopy_kernel<T>), gnn, dim3(256), 0, s, n, n, P, ldp, sp, Pc, (int64_t)n, snn);
+      hipMemsetAsync(dinfo, 0, sizeof(int), s);
+      bpotrf_upper<T>(batch, n, Pc, n, snn, Yd, DBK, (int64_t)DBK * DBK, tmp, (int64_t)DBK * n, dinfo, s);
+      btrtri_upper<T>(batch, n, Pc, n, snn, Qm, n, snn, s);
+      gemm<T>(OP_N, OP_C, bga<T>(batch, n, n, n, Qm, n, snn, Qm, n, snn, Pinv, n, snn, 1.0, 0.0), s);
+      gemm<T>(OP_N, OP_N, bga<T>(batch, m, n, n, Zw, m, smn, Pinv, n, snn, dA, ldda, sda, 1.0, 1.0), s);
+      int hinfo = 0;
+      hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, s);
+      hipStreamSynchronize(s);
+      if (hinfo != 0) return 2;
+    }
   }
   if (hipGetLastError() != hipSuccess) return 1;
-  return (it >= maxiter && !done) ? 6 : 0;
+  return done ? 0 : 6;
 }
 
 #define MAK_INST_PB(T)                                                                                            \

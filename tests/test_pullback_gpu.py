@@ -246,3 +246,24 @@ def test_left_polar_pullback_batched(dev):
     for b in (0, 7, batch - 1):
         ref = O.left_polar_pullback(np.zeros((m, n)), As[b], (host(Ws[b]), host(Ps[b])), (dWs[b], dPs[b]))
         assert rel(out[b], ref) < 1e-10, (b, rel(out[b], ref))
+
+
+@pytest.mark.parametrize("cond,tol", [(1e4, 1e-10), (1e6, 1e-8)])
+def test_left_polar_pullback_ill_conditioned(cond, tol, dev):
+    """P with condition number 1e4 / 1e6: CG with the diagonal preconditioner
+    needs ~sqrt(cond) steps, so past 48 steps the batch is solved through the
+    eigendecomposition of P; compared with the oracle's rule (Bartels-Stewart
+    Sylvester solve, LU inverse).  Both carry relative errors ~ cond u."""
+    n = 200
+    A, _ = O.conditioned_matrix(n, cond=cond, seed=31)
+    W, P = mak.left_polar(dev_t(A, dev))
+    dW = O.gaussian(n, n, np.float64, seed=32)
+    dP = O.gaussian(n, n, np.float64, seed=33)
+    dA3 = mak.empty_colmajor_batch(1, n, n, torch.float64, dev)
+    dA3.zero_()
+    _, iters = mak.left_polar_pullback_batched_(dA3, mak.colmajor(W).unsqueeze(0), mak.colmajor(P).unsqueeze(0),
+                                                dev_t(dW, dev).unsqueeze(0), dev_t(dP, dev).unsqueeze(0))
+    ref = O.left_polar_pullback(np.zeros((n, n)), A, (host(W), host(P)), (dW, dP))
+    err = rel(host(dA3)[0], ref)
+    print(f"polar pullback cond {cond:.0e}: {iters} CG steps, rel err {err:.2e}", flush=True)
+    assert err < tol
