@@ -231,7 +231,8 @@ int mak_right_polar_pushforward_z(void* stream, int m, int n, const void* Wh, in
 /* ----------------------------------------------------------- adjoints --- */
 
 /* Device workspace of the reverse rules for `batch` matrices of size m x n:
- * kind 0 = mak_qr_pullback_*, kind 1 = mak_left_polar_pullback_*. */
+ * kind 0 = mak_qr_pullback_*, kind 1 = mak_left_polar_pullback_*,
+ * kind 2 = mak_qr_full_cotangent_*. */
 int mak_pullback_workspace_size(int kind, int dtype, int batch, int m, int n, size_t* bytes);
 
 /* qr_pullback!(dA, A, (Q, R), (dQ, dR)) for `batch` equally sized matrices
@@ -240,16 +241,36 @@ int mak_pullback_workspace_size(int kind, int dtype, int batch, int m, int n, si
  * mak_qr_compact_wy_*, with the gauge signs when `positive` matches the
  * primal's positive = true; R is the primal's R (min(m,n) x n, only its upper
  * triangle is read); p is the rank (qr_rank, src/pullbacks/qr.jl:1-2).
- * dQ (m x min(m,n)) / dR may be NULL (absent cotangent).
+ * dQ (m x min(m,n)) / dR may be NULL (absent cotangent).  gauge_out
+ * (device, `batch` doubles, may be NULL) receives the largest magnitude of the
+ * gauge-dependent cotangent entries, dQ[:, p:min(m,n)] and the upper triangle
+ * of dR[p:, p:] (the caller warns when it exceeds its gauge tolerance, as
+ * src/pullbacks/qr.jl:50-51); it is written asynchronously.
  * Replaces qr_pullback! (src/pullbacks/qr.jl:73-120). */
 int mak_qr_pullback_d(void* stream, int batch, int m, int n, int p, int positive, const double* A, int lda,
                       long long strideA, const double* T, long long strideT, const double* R, int ldr,
                       long long strideR, const double* dQ, int lddq, long long strideDQ, const double* dR, int lddr,
-                      long long strideDR, double* dA, int ldda, long long strideDA, void* work, size_t work_bytes);
+                      long long strideDR, double* dA, int ldda, long long strideDA, void* work, size_t work_bytes,
+                      double* gauge_out);
 int mak_qr_pullback_z(void* stream, int batch, int m, int n, int p, int positive, const void* A, int lda,
                       long long strideA, const void* T, long long strideT, const void* R, int ldr, long long strideR,
                       const void* dQ, int lddq, long long strideDQ, const void* dR, int lddr, long long strideDR,
-                      void* dA, int ldda, long long strideDA, void* work, size_t work_bytes);
+                      void* dA, int ldda, long long strideDA, void* work, size_t work_bytes, double* gauge_out);
+
+/* The cotangent preparation of a qr_full! primal (ΔQ m x m): dQ1 (m x
+ * min(m,n)) = the compact cotangent mak_qr_pullback_* takes and gauge_out
+ * (device, `batch` doubles, may be NULL) the largest magnitude of its
+ * gauge-dependent part.  Rank p = min(m,n): dQ3 = dQ[:, k:m] is split into
+ * its part along Q1, which moves into dQ1 -= Q3 (Q1^H dQ3)^H, and the rest
+ * (gauge); p < min(m,n): dQ1 = dQ[:, :k], gauge over dQ[:, p:m].  Q is the
+ * full Q (m x m).  Replaces check_and_prepare_qr_cotangents
+ * (src/pullbacks/qr.jl:4-53) for full Q. */
+int mak_qr_full_cotangent_d(void* stream, int batch, int m, int n, int p, const double* Q, int ldq, long long strideQ,
+                            const double* dQ, int lddq, long long strideDQ, double* dQ1, int ldd1, long long strideD1,
+                            double* gauge_out, void* work, size_t work_bytes);
+int mak_qr_full_cotangent_z(void* stream, int batch, int m, int n, int p, const void* Q, int ldq, long long strideQ,
+                            const void* dQ, int lddq, long long strideDQ, void* dQ1, int ldd1, long long strideD1,
+                            double* gauge_out, void* work, size_t work_bytes);
 
 /* left_polar_pullback!(dA, A, (W, P), (dW, dP)) for `batch` equally sized
  * matrices (m >= n), accumulating into dA.  The Sylvester equation

@@ -376,6 +376,41 @@ template <typename T> size_t qr_pullback_ws(int batch, int m, int n) {
   b += al256((size_t)batch * k * k * sizeof(T));      // triu(R11)
   return b;
 }
+// out[b] = max(out[b], max |X_ij|) over a rows x cols block of matrix b
+// (upper = 1: only i <= j), by a grid-stride loop per batch member and an
+// integer atomic max on the bit patterns (monotone for non-negative doubles).
+template <typename T>
+__global__ void bmaxabs_kernel(int rows, int cols, const T* X, int64_t ldx, int64_t sx, int upper, double* out) {
+  __shared__ double red[256];
+  const T* x = X + (int64_t)blockIdx.z * sx;
+  const int64_t tot = (int64_t)rows * cols;
+  double v = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (int64_t)gridDim.x * 256) {
+    const int r = (int)(e % rows), c = (int)(e / rows);
+    if (!upper || r <= c) v = fmax(v, sqrt(s_abs2(x[r + (int64_t)c * ldx])));
+  }
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    atomicMax(reinterpret_cast<unsigned long long*>(out + blockIdx.z), (unsigned long long)__double_as_longlong(red[0]));
+}
+__global__ void bzero_doubles_kernel(double* p, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+template <typename T>
+void bmaxabs(int batch, int rows, int cols, const T* X, int64_t ldx, int64_t sx, int upper, double* out,
+             hipStream_t s) {
+  if (batch <= 0 || rows <= 0 || cols <= 0 || X == nullptr) return;
+  const int64_t tot = (int64_t)rows * cols;
+  const unsigned gx = (unsigned)std::min<int64_t>(cdiv64(tot, 256), 64);
+  hipLaunchKernelGGL((bmaxabs_kernel<T>), dim3(gx, 1, batch), dim3(256), 0, s, rows, cols, X, ldx, sx, upper, out);
+}
+
 template <typename T>
 __global__ void bsylv_divide_kernel(int n, T* X, int64_t ldx, int64_t sx, const double* lam, int64_t sl) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -410,7 +445,8 @@ template <typename T> size_t polar_pullback_ws(int batch, int m, int n) {
 template <typename T>
 int qr_pullback(hipStream_t s, int batch, int m, int n, int p, int positive, const T* A, int64_t lda, int64_t sa,
                 const T* Tp, int64_t st, const T* R, int64_t ldr, int64_t sr, const T* dQ, int64_t lddq, int64_t sdq,
-                const T* dR, int64_t lddr, int64_t sdr, T* dA, int64_t ldda, int64_t sda, void* work, size_t wbytes) {
+                const T* dR, int64_t lddr, int64_t sdr, T* dA, int64_t ldda, int64_t sda, void* work, size_t wbytes,
+                double* gauge_out) {
   if (batch < 0) return -2;
   if (m < 0) return -3;
   if (n < 0) return -4;
@@ -423,6 +459,13 @@ int qr_pullback(hipStream_t s, int batch, int m, int n, int p, int positive, con
   if (ldda < std::max(1, m)) return -22;
   if (work == nullptr && batch > 0 && p > 0) return -24;
   if (wbytes < qr_pullback_ws<T>(batch, m, n)) return -25;
+  if (gauge_out && batch > 0) {
+    // the gauge-dependent part of the cotangents (src/pullbacks/qr.jl:27-29,
+    // :34-47): dQ[:, p:k] and the upper triangle of dR[p:k, p:n]
+    hipLaunchKernelGGL(bzero_doubles_kernel, dim3(cdiv(batch, 256)), dim3(256), 0, s, gauge_out, batch);
+    if (dQ) bmaxabs<T>(batch, m, k - p, dQ + (int64_t)p * lddq, lddq, sdq, 0, gauge_out, s);
+    if (dR) bmaxabs<T>(batch, k - p, n - p, dR + p + (int64_t)p * lddr, lddr, sdr, 1, gauge_out, s);
+  }
   if (batch == 0 || p == 0) return 0;
   char* w = reinterpret_cast<char*>(work);
   auto take = [&](size_t bytes) { char* q = w; w += al256(bytes); return reinterpret_cast<T*>(q); };
@@ -608,12 +651,68 @@ int polar_pullback(hipStream_t s, int batch, int m, int n, const T* W, int64_t l
   return done ? 0 : 6;
 }
 
+size_t qr_full_cotangent_ws_bytes(size_t elem, int batch, int m, int n) {
+  const int k = std::min(m, n), r = m - k;
+  return al256((size_t)batch * k * std::max(r, 0) * elem) + al256((size_t)batch * m * std::max(r, 0) * elem) + 256;
+}
+template <typename T> size_t qr_full_cotangent_ws(int batch, int m, int n) {
+  return qr_full_cotangent_ws_bytes(sizeof(T), batch, m, n);
+}
+
+// check_and_prepare_qr_cotangents for a qr_full! cotangent (src/pullbacks/
+// qr.jl:4-53): dQ (m x m, as the full Q) -> dQ1 (m x k, the compact
+// cotangent the pullback takes) and gauge_out[b] (device) = max |.| of the
+// gauge-dependent part: for p = k, dQ3 - Q1 Q1^H dQ3 (dQ3 = dQ[:, k:m]), with
+// dQ1 = dQ[:, :k] - Q3 (Q1^H dQ3)^H; for p < k, dQ[:, p:m], with dQ1 =
+// dQ[:, :k].  Q: the full Q (m x m).
+template <typename T>
+int qr_full_cotangent(hipStream_t s, int batch, int m, int n, int p, const T* Q, int64_t ldq, int64_t sq, const T* dQ,
+                      int64_t lddq, int64_t sdq, T* dQ1, int64_t ldd1, int64_t sd1, double* gauge_out, void* work,
+                      size_t wbytes) {
+  if (batch < 0) return -2;
+  if (m < 0) return -3;
+  if (n < 0) return -4;
+  const int k = std::min(m, n), r = m - k;
+  if (p < 0 || p > k) return -5;
+  if (ldq < std::max(1, m)) return -7;
+  if (lddq < std::max(1, m)) return -10;
+  if (ldd1 < std::max(1, m)) return -13;
+  if (wbytes < qr_full_cotangent_ws<T>(batch, m, n) || (work == nullptr && batch > 0 && r > 0)) return -17;
+  if (batch == 0 || m == 0) return 0;
+  if (Q == nullptr || dQ == nullptr || dQ1 == nullptr) return -6;
+  const int64_t smk = (int64_t)m * k;
+  hipLaunchKernelGGL((bcopy_kernel<T>), egrid(smk, batch), dim3(256), 0, s, m, k, dQ, lddq, sdq, dQ1, ldd1, sd1);
+  if (gauge_out) hipLaunchKernelGGL(bzero_doubles_kernel, dim3(cdiv(batch, 256)), dim3(256), 0, s, gauge_out, batch);
+  if (r == 0) return hipGetLastError() == hipSuccess ? 0 : 1;
+  if (p < k) {
+    if (gauge_out) bmaxabs<T>(batch, m, m - p, dQ + (int64_t)p * lddq, lddq, sdq, 0, gauge_out, s);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+  }
+  char* w = reinterpret_cast<char*>(work);
+  T* X = reinterpret_cast<T*>(w);                                                  // Q1^H dQ3 (k x r)
+  T* T3 = reinterpret_cast<T*>(w + al256((size_t)batch * k * r * sizeof(T)));     // dQ3 - Q1 X (m x r)
+  const int64_t skr = (int64_t)k * r, smr = (int64_t)m * r;
+  const T* dQ3 = dQ + (int64_t)k * lddq;
+  const T* Q3 = Q + (int64_t)k * ldq;
+  gemm<T>(OP_C, OP_N, bga<T>(batch, k, r, m, Q, ldq, sq, dQ3, lddq, sdq, X, k, skr, 1.0, 0.0), s);
+  if (gauge_out) {
+    hipLaunchKernelGGL((bcopy_kernel<T>), egrid(smr, batch), dim3(256), 0, s, m, r, dQ3, lddq, sdq, T3, (int64_t)m,
+                       smr);
+    gemm<T>(OP_N, OP_N, bga<T>(batch, m, r, k, Q, ldq, sq, X, k, skr, T3, m, smr, -1.0, 1.0), s);
+    bmaxabs<T>(batch, m, r, T3, m, smr, 0, gauge_out, s);
+  }
+  gemm<T>(OP_N, OP_C, bga<T>(batch, m, k, r, Q3, ldq, sq, X, k, skr, dQ1, ldd1, sd1, -1.0, 1.0), s);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 #define MAK_INST_PB(T)                                                                                            \
   template size_t qr_pullback_ws<T>(int, int, int);                                                               \
   template size_t polar_pullback_ws<T>(int, int, int);                                                            \
   template int qr_pullback<T>(hipStream_t, int, int, int, int, int, const T*, int64_t, int64_t, const T*, int64_t,\
                               const T*, int64_t, int64_t, const T*, int64_t, int64_t, const T*, int64_t, int64_t, \
-                              T*, int64_t, int64_t, void*, size_t);                                               \
+                              T*, int64_t, int64_t, void*, size_t, double*);                                      \
+  template int qr_full_cotangent<T>(hipStream_t, int, int, int, int, const T*, int64_t, int64_t, const T*,        \
+                                    int64_t, int64_t, T*, int64_t, int64_t, double*, void*, size_t);              \
   template int polar_pullback<T>(hipStream_t, int, int, int, const T*, int64_t, int64_t, const T*, int64_t,       \
                                  int64_t, const T*, int64_t, int64_t, const T*, int64_t, int64_t, T*, int64_t,    \
                                  int64_t, int, double, void*, size_t, int*);
@@ -633,6 +732,8 @@ int mak_pullback_workspace_size(int kind, int dtype, int batch, int m, int n, si
                                            : mak::qr_pullback_ws<mak::zd>(batch, m, n);
   else if (kind == 1) *bytes = dtype == MAK_F64 ? mak::polar_pullback_ws<double>(batch, m, n)
                                                 : mak::polar_pullback_ws<mak::zd>(batch, m, n);
+  else if (kind == 2) *bytes = dtype == MAK_F64 ? mak::qr_full_cotangent_ws<double>(batch, m, n)
+                                                : mak::qr_full_cotangent_ws<mak::zd>(batch, m, n);
   else return -1;
   if (dtype != MAK_F64 && dtype != MAK_C64) return -2;
   return 0;
@@ -643,13 +744,23 @@ int mak_pullback_workspace_size(int kind, int dtype, int batch, int m, int n, si
                                long long sa,                                                                       \
                                const CT* Tp, long long st, const CT* R, int ldr, long long sr, const CT* dQ,       \
                                int lddq, long long sdq, const CT* dR, int lddr, long long sdr, CT* dA, int ldda,   \
-                               long long sda, void* work, size_t wbytes) {                                         \
+                               long long sda, void* work, size_t wbytes, double* gauge_out) {                      \
     return mak::qr_pullback<T>((hipStream_t)s, batch, m, n, p, positive, (const T*)A, lda, sa, (const T*)Tp, st,   \
                                (const T*)R, ldr, sr, (const T*)dQ, lddq, sdq, (const T*)dR, lddr, sdr, (T*)dA,    \
-                               ldda, sda, work, wbytes);                                                           \
+                               ldda, sda, work, wbytes, gauge_out);                                                \
   }
 MAK_QRPB(d, double, double)
 MAK_QRPB(z, mak::zd, void)
+
+#define MAK_QRFC(SUFFIX, T, CT)                                                                                     \
+  int mak_qr_full_cotangent_##SUFFIX(void* s, int batch, int m, int n, int p, const CT* Q, int ldq, long long sq,  \
+                                     const CT* dQ, int lddq, long long sdq, CT* dQ1, int ldd1, long long sd1,      \
+                                     double* gauge_out, void* work, size_t wbytes) {                                \
+    return mak::qr_full_cotangent<T>((hipStream_t)s, batch, m, n, p, (const T*)Q, ldq, sq, (const T*)dQ, lddq,    \
+                                     sdq, (T*)dQ1, ldd1, sd1, gauge_out, work, wbytes);                            \
+  }
+MAK_QRFC(d, double, double)
+MAK_QRFC(z, mak::zd, void)
 
 #define MAK_PPB(SUFFIX, T, CT)                                                                                      \
   int mak_left_polar_pullback_##SUFFIX(void* s, int batch, int m, int n, const CT* W, int ldw, long long sw,        \

@@ -53,9 +53,13 @@ SIGNATURES = {
     "mak_right_polar_pushforward_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
     "mak_pullback_workspace_size": (_i, [_i, _i, _i, _i, _i, C.POINTER(_sz)]),
     "mak_qr_pullback_d": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _i, _ll, _vp, _ll, _vp, _i, _ll, _vp, _i, _ll,
-                               _vp, _i, _ll, _vp, _i, _ll, _vp, _sz]),
+                               _vp, _i, _ll, _vp, _i, _ll, _vp, _sz, _vp]),
     "mak_qr_pullback_z": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _i, _ll, _vp, _ll, _vp, _i, _ll, _vp, _i, _ll,
-                               _vp, _i, _ll, _vp, _i, _ll, _vp, _sz]),
+                               _vp, _i, _ll, _vp, _i, _ll, _vp, _sz, _vp]),
+    "mak_qr_full_cotangent_d": (_i, [_vp, _i, _i, _i, _i, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll, _vp, _vp,
+                                     _sz]),
+    "mak_qr_full_cotangent_z": (_i, [_vp, _i, _i, _i, _i, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll, _vp, _vp,
+                                     _sz]),
     "mak_left_polar_pullback_d": (_i, [_vp, _i, _i, _i, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll,
                                        _vp, _i, _ll, _i, _d, _vp, _sz, _vp]),
     "mak_left_polar_pullback_z": (_i, [_vp, _i, _i, _i, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll,

@@ -407,3 +407,36 @@ def test_polar_pushforwards_match_finite_differences(dt):
         Prp, Whp = O.right_polar_svd(B + h * dB)
         Prm, Whm = O.right_polar_svd(B - h * dB)
         assert rel(dWh, (Whp - Whm) / (2 * h)) < 1e-7 and rel(dPr, (Prp - Prm) / (2 * h)) < 1e-7
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("m,n,r", [(19, 17, 11), (17, 19, 12)])
+def test_qr_pullback_rank_deficient_finite_differences(m, n, r, dt):
+    """Rank-deficient branch (p = rank < min(m, n)) of the QR pullback,
+    pinned by central differences along the rank-preserving curve
+    A(h) = (G + h E) B (I + h F) through A = G B (rank r), whose tangent is
+    E B + A F.  The cotangents see only Q[:, :p] and R[:p, :] (gauge
+    independent: zero on the remaining columns / rows); dR[:p, p:] is nonzero,
+    so the dA[:, p:] += Q1 dR12 and dQ1 -= Q1 dR12 R12^H terms are exercised."""
+    G = O.gaussian(m, r, dt, seed=60)
+    B = O.gaussian(r, n, dt, seed=61)
+    E = O.gaussian(m, r, dt, seed=62)
+    F = O.gaussian(n, n, dt, seed=63)
+    A = G @ B
+    Q, R = O.qr_compact(A)
+    k = min(m, n)
+    p = O.qr_rank(R)
+    assert p == r
+    dQ = np.zeros((m, k), dtype=A.dtype)
+    dQ[:, :p] = O.gaussian(m, p, dt, seed=64)
+    dR = np.zeros((k, n), dtype=A.dtype)
+    dR[:p, :] = np.triu(O.gaussian(p, n, dt, seed=65))
+    dA = O.qr_pullback(np.zeros_like(A), A, (Q, R), (dQ, dR))
+
+    def loss(h):
+        Qh, Rh = O.qr_compact((G + h * E) @ B @ (np.eye(n) + h * F))
+        return np.real(np.vdot(dQ[:, :p], Qh[:, :p]) + np.vdot(dR[:p], Rh[:p]))
+    h = 1e-6
+    fd = (loss(h) - loss(-h)) / (2 * h)
+    ad = np.real(np.vdot(dA, E @ B + A @ F))
+    assert abs(fd - ad) < 1e-6 * max(1.0, abs(ad)), (fd, ad)

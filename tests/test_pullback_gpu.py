@@ -267,3 +267,72 @@ def test_left_polar_pullback_ill_conditioned(cond, tol, dev):
     err = rel(host(dA3)[0], ref)
     print(f"polar pullback cond {cond:.0e}: {iters} CG steps, rel err {err:.2e}", flush=True)
     assert err < tol
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+def test_qr_pullback_full_q_cotangent(dt, dev):
+    """qr_full! cotangents (ΔQ m x m): the ΔQ3 part along Q1 enters ΔQ1
+    (src/pullbacks/qr.jl:19-26); the result matches the oracle's rule on the
+    same full (Q, R).  A ΔQ3 with a component outside Q1's span is gauge
+    dependent and warns; one inside it does not."""
+    m, n = 90, 40
+    A = O.gaussian(m, n, dt, seed=81)
+    At = dev_t(A, dev)
+    Qf, Rf = mak.qr_full(At)
+    Qh, Rh = host(Qf), host(Rf)
+    dR = np.triu(O.gaussian(m, n, dt, seed=83))
+    dR[n:] = 0
+    # gauge-invariant: dQ3 = Q1 X
+    dQ = O.gaussian(m, m, dt, seed=82)
+    dQ[:, n:] = Qh[:, :n] @ O.gaussian(n, m - n, dt, seed=84)
+    dA0 = O.gaussian(m, n, dt, seed=85)
+    dA = dev_t(dA0, dev)
+    import warnings as _w
+    with _w.catch_warnings():
+        _w.simplefilter("error")
+        mak.qr_pullback_(dA, At, (Qf, Rf), (dev_t(dQ, dev), dev_t(dR, dev)))
+    ref = O.qr_pullback(dA0, A, (Qh, Rh), (dQ, dR))
+    assert rel(host(dA), ref) < 1e-10, rel(host(dA), ref)
+    # gauge-dependent: a random dQ3 warns, and the rule still matches the oracle's
+    dQ2 = O.gaussian(m, m, dt, seed=86)
+    dA = dev_t(dA0, dev)
+    with pytest.warns(UserWarning, match="sensitive to gauge"):
+        mak.qr_pullback_(dA, At, (Qf, Rf), (dev_t(dQ2, dev), dev_t(dR, dev)))
+    with pytest.warns(UserWarning, match="sensitive to gauge"):
+        ref2 = O.qr_pullback(dA0, A, (Qh, Rh), (dQ2, dR))
+    assert rel(host(dA), ref2) < 1e-10
+
+
+def test_qr_pullback_gauge_warning_rank_deficient(dev):
+    """Rank-deficient R (p < min(m,n)): cotangent entries on Q[:, p:] or on
+    the upper triangle of R[p:, p:] are gauge dependent and warn
+    (src/pullbacks/qr.jl:27-29,34-47); zero ones do not."""
+    m, n, r = 60, 30, 20
+    A = O.gaussian(m, r, np.float64, seed=91) @ O.gaussian(r, n, np.float64, seed=92)
+    At = dev_t(A, dev)
+    Q, R, tape = mak.qr_compact_rrule(At)
+    assert mak.qr_rank(R) == r
+    dQ = O.gaussian(m, n, np.float64, seed=93)
+    dQ[:, r:] = 0
+    dR = np.triu(O.gaussian(n, n, np.float64, seed=94))
+    dR[r:] = 0
+    import warnings as _w
+    dA = dev_t(np.zeros((m, n)), dev)
+    with _w.catch_warnings():
+        _w.simplefilter("error")
+        mak.qr_pullback_(dA, At, (Q, R), (dev_t(dQ, dev), dev_t(dR, dev)), tape=tape)
+    ref = O.qr_pullback(np.zeros((m, n)), A, (host(Q), host(R)), (dQ, dR))
+    assert rel(host(dA), ref) < 1e-10
+    dQ[:, r + 2] = 1.0
+    with pytest.warns(UserWarning, match="sensitive to gauge"):
+        mak.qr_pullback_(dev_t(np.zeros((m, n)), dev), At, (Q, R), (dev_t(dQ, dev), dev_t(dR, dev)), tape=tape)
+
+
+def test_qr_ranks_batched_one_transfer(dev):
+    """qr_ranks over a stack equals qr_rank per matrix (mixed ranks)."""
+    Rs = []
+    for b, r in enumerate([5, 8, 8, 3]):
+        A = O.gaussian(30, r, np.float64, seed=100 + b) @ O.gaussian(r, 8, np.float64, seed=110 + b)
+        Rs.append(mak.qr_compact(dev_t(A, dev))[1])
+    R3 = mak.colmajor_batch(torch.stack(Rs))
+    assert mak.qr_ranks(R3) == [O.qr_rank(host(R)) for R in Rs] == [5, 8, 8, 3]
