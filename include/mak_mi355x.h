@@ -85,6 +85,18 @@ int mak_qr_compact_wy_d(void* stream, int m, int n, double* A, int lda, double* 
 int mak_qr_compact_wy_z(void* stream, int m, int n, void* A, int lda, void* Q, int ldq, void* R, int ldr, void* Tout,
                         int positive, void* work, size_t work_bytes);
 
+/* qr_compact! without a workspace argument.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as mak_qr_compact_d; the workspace is a static
+ * 64 MiB device array per GPU, held until the process ends (problems whose
+ * mak_qr_workspace_size exceeds 64 MiB return 7; Float64 1024 x 512 needs
+ * about 43 MiB).  The call allocates nothing.  Calls on one GPU share the
+ * array: calls that overlap in time on one GPU (other streams or threads)
+ * race, so the caller serialises them. */
+int mak_qr_compact_noalloc_d(void* stream, int m, int n, double* A, int lda, double* Q, int ldq, double* R, int ldr,
+                             int positive);
+
 /* ------------------------------------------------------ LQ, null spaces --- */
 
 /* Device workspace bytes of the LQ entry points (full = 0 / 1). */

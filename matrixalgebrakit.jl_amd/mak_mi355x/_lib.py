@@ -28,6 +28,7 @@ SIGNATURES = {
     "mak_qr_full_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
     "mak_qr_compact_wy_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
     "mak_qr_compact_wy_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _sz]),
+    "mak_qr_compact_noalloc_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i]),
     "mak_lq_workspace_size": (_i, [_i, _i, _i, _i, C.POINTER(_sz)]),
     "mak_lq_compact_d": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
     "mak_lq_compact_z": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _i, _vp, _sz]),
