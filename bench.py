@@ -358,6 +358,9 @@ def main():
                          "committed counter summary in profiles/, if present)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (the FETCH_SIZE / WRITE_SIZE calibration kernel)")
+    ap.add_argument("--polar-profile", type=int, default=None, metavar="PREC",
+                    help="profiling aid: only left_polar! on the 4096^2 cond-1e6 input at precision PREC "
+                         "(64 / 32 / 16), --warmup untimed + --steps timed calls")
     args = ap.parse_args()
     if args.traffic_bytes is None:
         prof = ROOT / "profiles" / "qr_compact_8192x1024_f64_hbm.json"
@@ -385,6 +388,23 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "launches": args.steps}), flush=True)
+        return
+
+    if args.polar_profile is not None:
+        n = 4096
+        U, _ = mak.qr_compact(gaussian_dev(n, n, torch.float64, 22, dev))
+        V, _ = mak.qr_compact(gaussian_dev(n, n, torch.float64, 23, dev))
+        sig = torch.logspace(0, -6, n, dtype=torch.float64, device=dev)
+        A = mak.colmajor((U * sig[None, :]) @ V.T)
+        W, P = mak.initialize_output_polar(A, False)
+        alg = mak.MI355XPolar(precision=args.polar_profile)
+        ms = timed(lambda i: mak.left_polar_(A, (W, P), alg), args.steps, args.warmup, stream)
+        info = mak.last_polar_info()
+        fl, split = polar_flops(n, info)
+        print(json.dumps({"workload": f"left_polar! 4096^2 cond 1e6 precision {args.polar_profile}",
+                          "ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 3),
+                          "gflop_by_precision": {k: round(v / 1e9, 1) for k, v in split.items()},
+                          "info": info}), flush=True)
         return
 
     m, n = 8192, 1024

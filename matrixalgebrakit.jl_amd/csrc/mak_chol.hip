@@ -2,25 +2,31 @@
 // for FP64 / ComplexF64 Hermitian positive definite matrices.
 //
 // Small and medium n (n <= kCholTileMaxN): ONE persistent launch.  The
-// matrix is cut into 32 x 32 tiles and [G | I] is eliminated right-looking:
-//   D(k)        R_kk, X_kk = chol_inv(G_kk)                  (one wave)
-//   P(k, j)     R_kj = X_kk^H G_kj                           (j > k)
-//   U(k, i, j)  G_ij -= R_ki^H R_kj                          (k < i <= j)
-//   U'(k, j, l) Z_jl -= X_jk R_kl                            (j <= k < l)
-//   P'(k, j)    X_jk = Z_jk X_kk                             (j < k)
-// (X R = I column block by column block: X_jl = -(sum_{k=j}^{l-1} X_jk R_kl) X_ll.)
-// Every task is a 32 x 32 x 32 product on FP64 MFMA except D, whose in-wave
-// elimination carries the identity along (the inverse costs no extra
-// dependency chain).  Workgroups take tasks from an atomic counter in a fixed
-// topological order with a one-step look-ahead (the next diagonal block's D
-// and P tasks come before the bulk of the current step's updates), and wait
-// for the per-tile version counters the task needs:
-//   gst[i][j]  G/R tile (i <= j): number of U updates applied; i + 1 = R final
-//   zst[j][l]  X tile (j <= l): number of U' updates applied; l - j + 1 = final
+// matrix is cut into 32 x 32 tiles and [G | I] is eliminated right-looking,
+// one PAIR of tile columns (64 columns) per step.  Pair K has tiles
+// c0 = 2K, c1 = 2K + 1; its 2 x 2 tile diagonal block is factored by the
+// panel workgroup entirely in LDS:
+//   D(c0); R01 = X00^H G01; G11 -= R01^H R01; D(c1); X01 = -X00 R01 X11
+// (D: one-wave Cholesky with inverse of a 32 x 32 tile).  The other
+// workgroups run 32 x 32 tasks whose products contract over the pair (K = 64
+// on FP64 MFMA):
+//   P2(K, j)      [R_c0j; R_c1j] = X_KK^H [G_c0j; G_c1j]        j >= 2K+2
+//   U2(K, i, j)   G_ij -= R_c0i^H R_c0j + R_c1i^H R_c1j        2K+2 <= i <= j
+//   UQ2(K, j, l)  Z_jl -= X_jc0 R_c0l + X_jc1 R_c1l            j < 2K+2 <= l
+//   PQ2(K, j)     [X_jc0 X_jc1] = [Z_jc0 Z_jc1] X_KK           j < 2K
+// (X R = I column block by column block: X_{j,L} = -(sum_K X_{j,K} R_{K,L}) X_LL.)
+// A step thus costs two hand-offs between workgroups (P2, then U2 of the
+// next pair's tiles) for 64 columns.  Workgroups take tasks from an atomic
+// counter in a fixed topological order (the next pair's inputs first, the
+// bulk last) and wait for the per-tile version counters the task needs:
+//   gst[i][j]  G/R tile (i <= j): pair steps applied; i/2 + 1 = R final
+//   zst[j][l]  X tile (j <= l): UQ2 updates applied; l/2 - j/2 + 1 = final
 // Every dependency of a task precedes it in the order, and a workgroup only
 // waits after it has taken its task, so the waits always end; each wait is
 // still bounded (kSpinLimit polls), and a timeout or a non-positive pivot sets
 // an abort flag that every later wait and task honours, so the grid drains.
+// An odd tile count is padded with one virtual tile: its loads are the
+// identity (diagonal) or zero, its stores are dropped.
 //
 // Larger n: a blocked loop over kCholBlock-column diagonal blocks, each
 // factored by the tile kernel, with MFMA GEMMs for the panel, the trailing
@@ -46,74 +52,57 @@ constexpr int TB = 32;                 // tile size
 constexpr long kSpinLimit = 1L << 22;  // polls per wait before the abort flag is raised
 
 // ---------------------------------------------------------------- task order
-enum CholTaskType { CT_D = 0, CT_P = 1, CT_UQ = 2, CT_PQ = 3, CT_U = 4 };
+enum CholTaskType { CT_P2 = 1, CT_U2 = 2, CT_UQ2 = 3, CT_PQ2 = 4 };
 
 struct CholTask {
-  int type, k, i, j;
+  int type, k, i, j;  // k: the pair step
 };
 
-// Tasks of iteration k, by segment, in the order they are taken:
-//   D(k)  P(k, k+1)                               (the panel workgroup's)
-//   s1  P(k, j)            j = k+2 .. nt-1
-//   s4a U(k-1, k+1, j)     j = k+1 .. nt-1        (row k+1 of step k-1)
-//   --  U(k, k+1, k+1)                            (the panel workgroup's)
-//   s6  U(k, k+1, j)       j = k+2 .. nt-1
-//   s2  U'(k-1, j, k)      j < k
-//   s3  P'(k, j)           j < k
-//   s4b U(k-1, i, j)       k+2 <= i <= j <= nt-1
-//   s5  U'(k-1, j, l)      j < k < l
-// The inputs of the next panel steps (P(k, k+2), row k+1 of step k-1 and
-// U(k, k+1, k+2)) come first; the bulk updates and the inverse follow.
-// panel = 1 leaves out the panel workgroup's tasks (D(k), P(k, k+1) and
-// U(k, k+1, k+1)): the order the other workgroups take theirs in.
-constexpr int kCholSegs = 10;
-__host__ __device__ inline void chol_iter_counts(int nt, int k, int c[kCholSegs], int panel = 0) {
-  const int rest = nt - 2 - k > 0 ? nt - 2 - k : 0;  // columns k+2 .. nt-1
-  c[0] = panel ? 0 : 1;                              // D(k)
-  c[1] = (!panel && k <= nt - 2) ? 1 : 0;            // P(k, k+1)
-  c[2] = rest;                                       // s1
-  c[3] = (k >= 1 && k <= nt - 2) ? nt - 1 - k : 0;   // s4a
-  c[4] = (!panel && k <= nt - 2) ? 1 : 0;            // U(k, k+1, k+1)
-  c[5] = rest;                                       // s6
-  c[6] = k;                                          // s2
-  c[7] = k;                                          // s3
-  c[8] = k >= 1 ? rest * (rest + 1) / 2 : 0;         // s4b
-  c[9] = k * (nt - 1 - k);                           // s5
+// Tasks of iteration K (T tiles, T even), by segment, in queue order:
+//   A  P2(K, j)        j = 2K+2 .. T-1
+//   B  U2(K, i, j)     i = 2K+2, 2K+3, j = i .. T-1   (the next pair's rows)
+//   C  PQ2(K, j)       j = 0 .. 2K-1
+//   E  UQ2(K, j, l)    j = 0 .. 2K+1, l = 2K+2 .. T-1
+//   F  U2(K, i, j)     2K+4 <= i <= j <= T-1          (the bulk)
+constexpr int kCholSegs = 5;
+__host__ __device__ inline void chol_iter_counts(int T, int K, int c[kCholSegs]) {
+  const int r0 = 2 * K + 2;
+  const int rem = T - r0 > 0 ? T - r0 : 0;
+  const int b = rem >= 2 ? rem - 2 : 0;
+  c[0] = rem;
+  c[1] = rem > 0 ? 2 * rem - 1 : 0;
+  c[2] = 2 * K;
+  c[3] = r0 * rem;
+  c[4] = b * (b + 1) / 2;
 }
 
-__host__ __device__ inline int chol_iter_total(int nt, int k, int panel = 0) {
+__host__ __device__ inline int chol_iter_total(int T, int K) {
   int c[kCholSegs];
-  chol_iter_counts(nt, k, c, panel);
-  int t = 0;
-  for (int q = 0; q < kCholSegs; ++q) t += c[q];
-  return t;
+  chol_iter_counts(T, K, c);
+  return c[0] + c[1] + c[2] + c[3] + c[4];
 }
 
-// u-th task of iteration k (0 <= u < chol_iter_total(nt, k, panel)).
-__host__ __device__ inline CholTask chol_task_in_iter(int nt, int k, int u, int panel = 0) {
+// u-th task of iteration K (0 <= u < chol_iter_total(T, K)).
+__host__ __device__ inline CholTask chol_task_in_iter(int T, int K, int u) {
   int c[kCholSegs];
-  chol_iter_counts(nt, k, c, panel);
-  CholTask t{CT_D, k, k, k};
+  chol_iter_counts(T, K, c);
+  const int r0 = 2 * K + 2, rem = T - r0;
+  CholTask t{CT_P2, K, 0, 0};
   int q = 0;
   while (q < kCholSegs - 1 && u >= c[q]) u -= c[q++];
   switch (q) {
-    case 0: break;
-    case 1: t.type = CT_P; t.j = k + 1; break;
-    case 2: t.type = CT_P; t.j = k + 2 + u; break;
-    case 3: t.type = CT_U; t.k = k - 1; t.i = k + 1; t.j = k + 1 + u; break;
-    case 4: t.type = CT_U; t.i = k + 1; t.j = k + 1; break;
-    case 5: t.type = CT_U; t.i = k + 1; t.j = k + 2 + u; break;
-    case 6: t.type = CT_UQ; t.k = k - 1; t.i = u; t.j = k; break;
-    case 7: t.type = CT_PQ; t.i = u; t.j = k; break;
-    case 8: {  // pairs k+2 <= i <= j <= nt-1, row by row
-      int i = k + 2, len = nt - i;
-      while (u >= len) { u -= len; ++i; --len; }
-      t.type = CT_U; t.k = k - 1; t.i = i; t.j = i + u;
+    case 0: t.type = CT_P2; t.j = r0 + u; break;
+    case 1:
+      t.type = CT_U2;
+      if (u < rem) { t.i = r0; t.j = r0 + u; }
+      else { t.i = r0 + 1; t.j = r0 + 1 + (u - rem); }
       break;
-    }
-    default: {
-      const int w = nt - 1 - k;
-      t.type = CT_UQ; t.k = k - 1; t.i = u / w; t.j = k + 1 + u % w;
+    case 2: t.type = CT_PQ2; t.j = u; break;
+    case 3: t.type = CT_UQ2; t.i = u / rem; t.j = r0 + u % rem; break;
+    default: {  // pairs r0+2 <= i <= j <= T-1, row by row
+      int i = r0 + 2, len = T - i;
+      while (u >= len) { u -= len; ++i; --len; }
+      t.type = CT_U2; t.i = i; t.j = i + u;
       break;
     }
   }
@@ -125,14 +114,15 @@ struct CholTileArgs {
   int64_t ldg;
   void* X;        // out: R^{-1} (upper, strict lower part zeroed)
   int64_t ldx;
-  int n, nt, ntasks;
+  int n, nt;      // nt: tiles, rounded up to even
+  int ntasks;
   int* gst;       // nt * nt
   int* zst;       // nt * nt
   int* ctl;       // [0] task counter, [1] abort flag, [2] first non-positive pivot (1-based) or 0
   int* info;      // optional: receives ctl[2] + info_offset on failure (when *info == 0)
   int info_offset;
   long long* trace;  // optional (tests): wall-clock stamps of the panel's iterations, 8 per iteration
-  int iter_off[kCholTileMaxNt + 1];  // first queue task of every iteration
+  int iter_off[kCholTileMaxNt / 2 + 1];  // first queue task of every iteration
 };
 
 // ---------------------------------------------------------------- sync helpers
@@ -349,6 +339,81 @@ template <typename T> __device__ inline void clear_tile(T* M, int64_t ld, int n,
   zero_tile<T>(M, ld, n, ti, tj);
 }
 
+__device__ inline void load_fast4(double (*S0)[33], const double* p0, double (*S1)[33], const double* p1,
+                                  double (*S2)[33], const double* p2, double (*S3)[33], const double* p3) {
+  v2d a, b, c, d, e, f, g, h;
+  asm volatile("global_load_dwordx4 %0, %8, off sc1\n\tglobal_load_dwordx4 %1, %8, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %2, %9, off sc1\n\tglobal_load_dwordx4 %3, %9, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %4, %10, off sc1\n\tglobal_load_dwordx4 %5, %10, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %6, %11, off sc1\n\tglobal_load_dwordx4 %7, %11, off offset:16 sc1\n\t"
+               "s_waitcnt vmcnt(0)"
+               : "=&v"(a), "=&v"(b), "=&v"(c), "=&v"(d), "=&v"(e), "=&v"(f), "=&v"(g), "=&v"(h)
+               : "v"(p0), "v"(p1), "v"(p2), "v"(p3)
+               : "memory");
+  put_chunk(S0, a, b);
+  put_chunk(S1, c, d);
+  put_chunk(S2, e, f);
+  put_chunk(S3, g, h);
+}
+__device__ inline void load_fast5(double (*S0)[33], const double* p0, double (*S1)[33], const double* p1,
+                                  double (*S2)[33], const double* p2, double (*S3)[33], const double* p3,
+                                  double (*S4)[33], const double* p4) {
+  v2d a, b, c, d, e, f, g, h, x, y;
+  asm volatile("global_load_dwordx4 %0, %10, off sc1\n\tglobal_load_dwordx4 %1, %10, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %2, %11, off sc1\n\tglobal_load_dwordx4 %3, %11, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %4, %12, off sc1\n\tglobal_load_dwordx4 %5, %12, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %6, %13, off sc1\n\tglobal_load_dwordx4 %7, %13, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %8, %14, off sc1\n\tglobal_load_dwordx4 %9, %14, off offset:16 sc1\n\t"
+               "s_waitcnt vmcnt(0)"
+               : "=&v"(a), "=&v"(b), "=&v"(c), "=&v"(d), "=&v"(e), "=&v"(f), "=&v"(g), "=&v"(h), "=&v"(x),
+                 "=&v"(y)
+               : "v"(p0), "v"(p1), "v"(p2), "v"(p3), "v"(p4)
+               : "memory");
+  put_chunk(S0, a, b);
+  put_chunk(S1, c, d);
+  put_chunk(S2, e, f);
+  put_chunk(S3, g, h);
+  put_chunk(S4, x, y);
+}
+
+// CNT (1 .. 5) tiles into LDS (S0 .. S4 from s0 .. s4, the first CNT used):
+// one asm block of coherent 16-byte loads when every tile allows the fast
+// path, else element loads.  CNT is a template parameter so that nothing is
+// indexed at run time (no private-memory arrays).
+template <int CNT, typename T>
+__device__ inline void load_n(T (*S0)[33], TileSrc s0, T (*S1)[33], TileSrc s1, T (*S2)[33], TileSrc s2,
+                              T (*S3)[33], TileSrc s3, T (*S4)[33], TileSrc s4, int n) {
+  auto fast_of = [&](const TileSrc& q) {
+    return tile_fast<T>(reinterpret_cast<const T*>(q.M), q.ld, n, q.ti, q.tj);
+  };
+  bool fast = fast_of(s0);
+  if constexpr (CNT > 1) fast = fast && fast_of(s1);
+  if constexpr (CNT > 2) fast = fast && fast_of(s2);
+  if constexpr (CNT > 3) fast = fast && fast_of(s3);
+  if constexpr (CNT > 4) fast = fast && fast_of(s4);
+  if constexpr (!ScalarTraits<T>::is_complex) {
+    if (fast) {
+      auto chunk = [&](const TileSrc& q) {
+        return tile_chunk(reinterpret_cast<const double*>(q.M), q.ld, q.ti, q.tj);
+      };
+      if constexpr (CNT == 1) load_fast1(S0, chunk(s0));
+      else if constexpr (CNT == 2) load_fast2(S0, chunk(s0), S1, chunk(s1));
+      else if constexpr (CNT == 3) load_fast3(S0, chunk(s0), S1, chunk(s1), S2, chunk(s2));
+      else if constexpr (CNT == 4) load_fast4(S0, chunk(s0), S1, chunk(s1), S2, chunk(s2), S3, chunk(s3));
+      else load_fast5(S0, chunk(s0), S1, chunk(s1), S2, chunk(s2), S3, chunk(s3), S4, chunk(s4));
+      return;
+    }
+  }
+  auto slow = [&](T (*S)[33], const TileSrc& q) {
+    load_tile<T>(S, reinterpret_cast<const T*>(q.M), q.ld, n, q.ti, q.tj);
+  };
+  slow(S0, s0);
+  if constexpr (CNT > 1) slow(S1, s1);
+  if constexpr (CNT > 2) slow(S2, s2);
+  if constexpr (CNT > 3) slow(S3, s3);
+  if constexpr (CNT > 4) slow(S4, s4);
+}
+
 // ---------------------------------------------------------------- the kernel
 // Factor one diagonal tile in LDS (S: in G, out R; Xo: out R^{-1}) by wave 0.
 // Returns the bad pivot (0 = fine), uniform over the workgroup.
@@ -370,25 +435,29 @@ __device__ inline int factor_diag(T (*S)[33], T (*Xo)[33], T* rowbuf) {
 }
 
 template <typename T>
-__device__ inline void report_bad(const CholTileArgs& a, int k, int bad) {
+__device__ inline void report_bad(const CholTileArgs& a, int tile, int bad) {
   if (threadIdx.x == 0) {
-    atomicCAS(a.ctl + 2, 0, k * TB + bad);
+    atomicCAS(a.ctl + 2, 0, tile * TB + bad);
     __hip_atomic_store(a.ctl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (a.info != nullptr) atomicCAS(a.info, 0, a.info_offset + k * TB + bad);
+    if (a.info != nullptr) atomicCAS(a.info, 0, a.info_offset + tile * TB + bad);
   }
 }
 
-// The panel workgroup: the diagonal chain.  Iteration k >= 1 takes
-// G_{k-1,k} and G_kk after the updates of steps < k-1 and, with X_{k-1,k-1}
-// still in LDS from iteration k-1, runs
-//   P(k-1, k):    R_{k-1,k} = X_{k-1,k-1}^H G_{k-1,k}      (published at once)
-//   U(k-1, k, k): G_kk -= R_{k-1,k}^H R_{k-1,k}            (in LDS)
-//   D(k):         R_kk, X_kk = chol_inv(G_kk)
-// so a step costs one load of two tiles and two publications, not three
-// task hand-offs between workgroups.
+// Several version counters at once (after the stores of the task completed).
+__device__ inline void publish_n(int cnt, int* const* p, const int* v) {
+  stores_done();
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 0; q < cnt; ++q) __hip_atomic_store(p[q], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The panel workgroup: the 2 x 2 tile diagonal block of every pair, in LDS.
+// Iteration K waits for the block's tiles to carry the K earlier pair steps,
+// then runs D(c0), R01 = X00^H G01, G11 -= R01^H R01, D(c1),
+// X01 = -X00 (R01 X11), and publishes R and X of the block.
 template <typename T>
-__device__ void chol_panel(const CholTileArgs& a, T (*tA)[33], T (*tB)[33], T (*tC)[33], T (*tX)[33],
-                           T* rowbuf) {
+__device__ void chol_panel(const CholTileArgs& a, T (*tA)[33], T (*tB)[33], T (*tC)[33], T (*tD)[33],
+                           T (*tE)[33], T (*tX)[33], T* rowbuf) {
   T* G = reinterpret_cast<T*>(a.G);
   T* X = reinterpret_cast<T*>(a.X);
   const int nt = a.nt;
@@ -399,54 +468,69 @@ __device__ void chol_panel(const CholTileArgs& a, T (*tA)[33], T (*tB)[33], T (*
   auto stamp = [&](int k, int slot) {
     if (tr != nullptr && threadIdx.x == 0) tr[k * 8 + slot] = (long long)__builtin_amdgcn_s_memrealtime();
   };
-  for (int k = 0; k < nt; ++k) {
-    stamp(k, 0);
-    if (k == 0) {
-      if (!inputs_ready(ctl)) return;
-      load_tiles<T>(1, tB, TileSrc{G, a.ldg, 0, 0}, tB, TileSrc{}, tB, TileSrc{}, a.n);
-    } else {
-      wait_ge(&gs[(k - 1) * nt + k], k - 1, ctl);
-      wait_ge(&gs[k * nt + k], k - 1, ctl);
-      if (!inputs_ready(ctl)) return;
-      stamp(k, 1);
-      load_tiles<T>(2, tA, TileSrc{G, a.ldg, k - 1, k}, tB, TileSrc{G, a.ldg, k, k}, tB, TileSrc{}, a.n);
-      __syncthreads();
-      stamp(k, 2);
-      mm32<T>(tC, tX, true, tA, false, 1.0, false);     // R_{k-1,k}
-      put_tile<T>(tC, G, a.ldg, a.n, k - 1, k);
-      clear_tile<T>(G, a.ldg, a.n, k, k - 1);
-      publish(&gs[(k - 1) * nt + k], k);
-      stamp(k, 3);
-      mm32<T>(tB, tC, true, tC, false, -1.0, true);    // G_kk after step k-1
+  for (int K = 0; 2 * K < nt; ++K) {
+    const int c0 = 2 * K, c1 = 2 * K + 1;
+    stamp(K, 0);
+    if (K > 0) {
+      wait_ge(&gs[c0 * nt + c0], K, ctl);
+      wait_ge(&gs[c0 * nt + c1], K, ctl);
+      wait_ge(&gs[c1 * nt + c1], K, ctl);
     }
+    if (!inputs_ready(ctl)) return;
+    stamp(K, 1);
+    load_n<3, T>(tA, TileSrc{G, a.ldg, c0, c0}, tB, TileSrc{G, a.ldg, c0, c1}, tC, TileSrc{G, a.ldg, c1, c1},
+                 tC, TileSrc{}, tC, TileSrc{}, a.n);
     __syncthreads();
-    stamp(k, 4);
-    const int bad = factor_diag<T>(tB, tX, rowbuf);    // tX <- X_kk (X_{k-1,k-1} is no longer needed)
-    if (bad != 0) {
-      report_bad<T>(a, k, bad);
-      return;
+    stamp(K, 2);
+    // h = 0: D(c0) on tA -> R00 (tA), X00 (tX); h = 1: R01 = X00^H G01 (tB),
+    // G11 -= R01^H R01, D(c1) on tC -> R11 (tC), X11 (tD).  One loop body, so
+    // the in-wave factorisation is instantiated once.
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1) {
+        mm32<T>(tB, tX, true, tB, false, 1.0, false);
+        mm32<T>(tC, tB, true, tB, false, -1.0, true);
+      }
+      stamp(K, 3 + h);
+      const int bad = factor_diag<T>(h ? tC : tA, h ? tD : tX, rowbuf);
+      if (bad != 0) {
+        report_bad<T>(a, c0 + h, bad);
+        return;
+      }
     }
-    stamp(k, 5);
-    put_tile<T>(tB, G, a.ldg, a.n, k, k);
-    put_tile<T>(tX, X, a.ldx, a.n, k, k);
-    publish(&zs[k * nt + k], 1, &gs[k * nt + k], k + 1);
-    stamp(k, 6);
+    stamp(K, 5);
+    mm32<T>(tE, tB, false, tD, false, 1.0, false);        // R01 X11
+    mm32<T>(tE, tX, false, tE, false, -1.0, false);       // X01 = -X00 (R01 X11)
+    put_tile<T>(tA, G, a.ldg, a.n, c0, c0);
+    put_tile<T>(tB, G, a.ldg, a.n, c0, c1);
+    put_tile<T>(tC, G, a.ldg, a.n, c1, c1);
+    clear_tile<T>(G, a.ldg, a.n, c1, c0);
+    put_tile<T>(tX, X, a.ldx, a.n, c0, c0);
+    put_tile<T>(tE, X, a.ldx, a.n, c0, c1);
+    put_tile<T>(tD, X, a.ldx, a.n, c1, c1);
+    clear_tile<T>(X, a.ldx, a.n, c1, c0);
+    int* fp[6] = {&zs[c0 * nt + c0], &zs[c0 * nt + c1], &zs[c1 * nt + c1],
+                  &gs[c0 * nt + c0], &gs[c0 * nt + c1], &gs[c1 * nt + c1]};
+    const int fv[6] = {1, 1, 1, K + 1, K + 1, K + 1};
+    publish_n(6, fp, fv);
+    stamp(K, 6);
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
-  __shared__ T tA[TB][33], tB[TB][33], tC[TB][33], tX[TB][33];
+  __shared__ T tA[TB][33], tB[TB][33], tC[TB][33], tD[TB][33], tE[TB][33], tX[TB][33];
   __shared__ __align__(16) T rowbuf[2 * TB];
   __shared__ int task_s;
   if (blockIdx.x == 0) {
-    chol_panel<T>(a, tA, tB, tC, tX, rowbuf);
+    chol_panel<T>(a, tA, tB, tC, tD, tE, tX, rowbuf);
     return;
   }
   T* G = reinterpret_cast<T*>(a.G);
   T* X = reinterpret_cast<T*>(a.X);
   const int nt = a.nt;
   int* ctl = a.ctl;
+  int* gs = a.gst;
+  int* zs = a.zst;
   int iter = 0;
   for (;;) {
     if (threadIdx.x == 0) task_s = atomicAdd(ctl, 1);
@@ -454,60 +538,102 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
     const int t = task_s;
     if (t >= a.ntasks) break;
     while (a.iter_off[iter + 1] <= t) ++iter;  // tasks are taken in increasing order
-    const CholTask tk = chol_task_in_iter(nt, iter, t - a.iter_off[iter], 1);
-    const int k = tk.k, i = tk.i, j = tk.j;
-    int* gs = a.gst;
-    int* zs = a.zst;
+    const CholTask tk = chol_task_in_iter(nt, iter, t - a.iter_off[iter]);
+    const int K = tk.k, c0 = 2 * K, c1 = 2 * K + 1;
     switch (tk.type) {
-      case CT_P: {  // R_kj = X_kk^H G_kj; the mirrored lower tile of G is zeroed
-        wait_ge(&gs[k * nt + k], k + 1, ctl);
-        wait_ge(&gs[k * nt + j], k, ctl);
+      case CT_P2: {  // [R_c0j; R_c1j] = X_KK^H [G_c0j; G_c1j]; the mirrored lower tiles of G are zeroed
+        const int j = tk.j;
+        wait_ge(&gs[c1 * nt + c1], K + 1, ctl);  // the panel's block (R and X published together)
+        wait_ge(&gs[c0 * nt + j], K, ctl);
+        wait_ge(&gs[c1 * nt + j], K, ctl);
         if (!inputs_ready(ctl)) break;
-        load_tiles<T>(2, tA, TileSrc{X, a.ldx, k, k}, tB, TileSrc{G, a.ldg, k, j}, tB, TileSrc{}, a.n);
+        load_n<5, T>(tA, TileSrc{X, a.ldx, c0, c0}, tB, TileSrc{X, a.ldx, c0, c1}, tC, TileSrc{X, a.ldx, c1, c1},
+                     tD, TileSrc{G, a.ldg, c0, j}, tE, TileSrc{G, a.ldg, c1, j}, a.n);
         __syncthreads();
-        mm32<T>(tC, tA, true, tB, false, 1.0, false);
-        put_tile<T>(tC, G, a.ldg, a.n, k, j);
-        clear_tile<T>(G, a.ldg, a.n, j, k);
-        publish(&gs[k * nt + j], k + 1);
+        mm32_2<T>(tE, tB, tD, tC, tE, true, false, 1.0, false);  // R_c1j = X01^H G0 + X11^H G1
+        mm32<T>(tD, tA, true, tD, false, 1.0, false);            // R_c0j = X00^H G0
+        put_tile<T>(tD, G, a.ldg, a.n, c0, j);
+        put_tile<T>(tE, G, a.ldg, a.n, c1, j);
+        clear_tile<T>(G, a.ldg, a.n, j, c0);
+        clear_tile<T>(G, a.ldg, a.n, j, c1);
+        int* fp[2] = {&gs[c0 * nt + j], &gs[c1 * nt + j]};
+        const int fv[2] = {K + 1, K + 1};
+        publish_n(2, fp, fv);
         break;
       }
-      case CT_PQ: {  // X_ik = Z_ik X_kk (i < k = j); the mirrored lower tile of X is zeroed
-        const int kk = j;
-        wait_ge(&gs[kk * nt + kk], kk + 1, ctl);
-        wait_ge(&zs[i * nt + kk], kk - i, ctl);
+      case CT_U2: {  // G_ij -= R_c0i^H R_c0j + R_c1i^H R_c1j
+        const int i = tk.i, j = tk.j;
+        wait_ge(&gs[c0 * nt + i], K + 1, ctl);
+        wait_ge(&gs[c1 * nt + i], K + 1, ctl);
+        wait_ge(&gs[c0 * nt + j], K + 1, ctl);
+        wait_ge(&gs[c1 * nt + j], K + 1, ctl);
+        wait_ge(&gs[i * nt + j], K, ctl);
         if (!inputs_ready(ctl)) break;
-        load_tiles<T>(2, tA, TileSrc{X, a.ldx, i, kk}, tB, TileSrc{X, a.ldx, kk, kk}, tB, TileSrc{}, a.n);
-        __syncthreads();
-        mm32<T>(tC, tA, false, tB, false, 1.0, false);
-        put_tile<T>(tC, X, a.ldx, a.n, i, kk);
-        clear_tile<T>(X, a.ldx, a.n, kk, i);
-        publish(&zs[i * nt + kk], kk - i + 1);
+        if (i == j) {
+          load_n<3, T>(tA, TileSrc{G, a.ldg, c0, i}, tB, TileSrc{G, a.ldg, c1, i}, tE, TileSrc{G, a.ldg, i, j},
+                       tE, TileSrc{}, tE, TileSrc{}, a.n);
+          __syncthreads();
+          mm32_2<T>(tE, tA, tA, tB, tB, true, false, -1.0, true);
+        } else {
+          load_n<5, T>(tA, TileSrc{G, a.ldg, c0, i}, tB, TileSrc{G, a.ldg, c1, i}, tC, TileSrc{G, a.ldg, c0, j},
+                       tD, TileSrc{G, a.ldg, c1, j}, tE, TileSrc{G, a.ldg, i, j}, a.n);
+          __syncthreads();
+          mm32_2<T>(tE, tA, tC, tB, tD, true, false, -1.0, true);
+        }
+        put_tile<T>(tE, G, a.ldg, a.n, i, j);
+        int* fp[1] = {&gs[i * nt + j]};
+        const int fv[1] = {K + 1};
+        publish_n(1, fp, fv);
         break;
       }
-      case CT_U: {  // G_ij -= R_ki^H R_kj
-        wait_ge(&gs[k * nt + i], k + 1, ctl);
-        wait_ge(&gs[k * nt + j], k + 1, ctl);
-        wait_ge(&gs[i * nt + j], k, ctl);
+      case CT_UQ2: {  // Z_jl -= X_jc0 R_c0l + X_jc1 R_c1l (X_c1c0 = 0); the first update overwrites
+        const int jr = tk.i, l = tk.j;
+        const int base = jr / 2;  // the pair of row jr: its first update comes from step K = base
+        const int fin = K - base + 1;  // X_jr,c0 / X_jr,c1 final value
+        const bool first = K == base;
+        const bool has0 = jr <= c0;
+        if (has0) wait_ge(&zs[jr * nt + c0], fin, ctl);
+        wait_ge(&zs[jr * nt + c1], fin, ctl);
+        wait_ge(&gs[c0 * nt + l], K + 1, ctl);
+        wait_ge(&gs[c1 * nt + l], K + 1, ctl);
+        if (!first) wait_ge(&zs[jr * nt + l], K - base, ctl);
         if (!inputs_ready(ctl)) break;
-        load_tiles<T>(3, tA, TileSrc{G, a.ldg, k, i}, tB, TileSrc{G, a.ldg, k, j}, tC, TileSrc{G, a.ldg, i, j}, a.n);
+        {
+          // tB = X_j,c1, tD = R_c1l, [tE = Z_jl], [tA = X_j,c0, tC = R_c0l]
+          const TileSrc xj1{X, a.ldx, jr, c1}, r1l{G, a.ldg, c1, l}, zjl{X, a.ldx, jr, l},
+              xj0{X, a.ldx, jr, c0}, r0l{G, a.ldg, c0, l};
+          if (first && has0) load_n<4, T>(tB, xj1, tD, r1l, tA, xj0, tC, r0l, tC, TileSrc{}, a.n);
+          else if (first) load_n<2, T>(tB, xj1, tD, r1l, tD, TileSrc{}, tD, TileSrc{}, tD, TileSrc{}, a.n);
+          else if (has0) load_n<5, T>(tB, xj1, tD, r1l, tE, zjl, tA, xj0, tC, r0l, a.n);
+          else load_n<3, T>(tB, xj1, tD, r1l, tE, zjl, tE, TileSrc{}, tE, TileSrc{}, a.n);
+        }
         __syncthreads();
-        mm32<T>(tC, tA, true, tB, false, -1.0, true);
-        put_tile<T>(tC, G, a.ldg, a.n, i, j);
-        publish(&gs[i * nt + j], k + 1);
+        mm32_2<T>(tE, has0 ? tA : nullptr, tC, tB, tD, false, false, -1.0, !first);
+        put_tile<T>(tE, X, a.ldx, a.n, jr, l);
+        int* fp[1] = {&zs[jr * nt + l]};
+        const int fv[1] = {K - base + 1};
+        publish_n(1, fp, fv);
         break;
       }
-      case CT_UQ: {  // Z_il -= X_ik R_kl (i <= k < l = j); the first update (k == i) overwrites
-        const int l = j;
-        wait_ge(&zs[i * nt + k], k - i + 1, ctl);
-        wait_ge(&gs[k * nt + l], k + 1, ctl);
-        if (k > i) wait_ge(&zs[i * nt + l], k - i, ctl);
+      case CT_PQ2: {  // [X_jc0 X_jc1] = [Z_jc0 Z_jc1] X_KK (j < c0); the mirrored lower tiles of X are zeroed
+        const int jr = tk.j;
+        const int cnt = K - jr / 2;  // UQ2 updates of Z_jr,c0 and Z_jr,c1
+        wait_ge(&gs[c1 * nt + c1], K + 1, ctl);
+        wait_ge(&zs[jr * nt + c0], cnt, ctl);
+        wait_ge(&zs[jr * nt + c1], cnt, ctl);
         if (!inputs_ready(ctl)) break;
-        load_tiles<T>(k > i ? 3 : 2, tA, TileSrc{X, a.ldx, i, k}, tB, TileSrc{G, a.ldg, k, l}, tC,
-                      TileSrc{X, a.ldx, i, l}, a.n);
+        load_n<5, T>(tA, TileSrc{X, a.ldx, jr, c0}, tB, TileSrc{X, a.ldx, jr, c1}, tC, TileSrc{X, a.ldx, c0, c0},
+                     tD, TileSrc{X, a.ldx, c0, c1}, tE, TileSrc{X, a.ldx, c1, c1}, a.n);
         __syncthreads();
-        mm32<T>(tC, tA, false, tB, false, -1.0, k > i);
-        put_tile<T>(tC, X, a.ldx, a.n, i, l);
-        publish(&zs[i * nt + l], k - i + 1);
+        mm32_2<T>(tB, tA, tD, tB, tE, false, false, 1.0, false);  // X_jc1 = Z_jc0 X01 + Z_jc1 X11
+        mm32<T>(tA, tA, false, tC, false, 1.0, false);             // X_jc0 = Z_jc0 X00
+        put_tile<T>(tA, X, a.ldx, a.n, jr, c0);
+        put_tile<T>(tB, X, a.ldx, a.n, jr, c1);
+        clear_tile<T>(X, a.ldx, a.n, c0, jr);
+        clear_tile<T>(X, a.ldx, a.n, c1, jr);
+        int* fp[2] = {&zs[jr * nt + c0], &zs[jr * nt + c1]};
+        const int fv[2] = {cnt + 1, cnt + 1};
+        publish_n(2, fp, fv);
         break;
       }
       default:
@@ -521,10 +647,16 @@ template <typename T> __global__ void chol_zero_ints_kernel(int* p, int n) {
   for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) p[e] = 0;
 }
 
+// tiles of the pair-step kernel: ceil(n / 32) rounded up to even
+inline int chol_tiles(int n) {
+  const int t = std::max(1, cdiv(n, TB));
+  return t + (t & 1);
+}
+
 }  // namespace
 
 size_t chol_tile_flag_ints(int n) {
-  const int nt = std::max(1, cdiv(std::min(n, kCholTileMaxN), TB));
+  const int nt = chol_tiles(std::min(n, kCholTileMaxN));
   return (size_t)2 * nt * nt + 64;
 }
 
@@ -539,13 +671,14 @@ int chol_tile_factor(T* G, int64_t ldg, int n, T* X, int64_t ldx, int* flags, in
   CholTileArgs a{};
   a.G = G; a.ldg = ldg; a.X = X; a.ldx = ldx;
   a.n = n;
-  a.nt = cdiv(n, TB);
+  a.nt = chol_tiles(n);
   int off = 0;
-  for (int k = 0; k < a.nt; ++k) {
-    a.iter_off[k] = off;
-    off += chol_iter_total(a.nt, k, 1);
+  const int nq = a.nt / 2;
+  for (int K = 0; K < nq; ++K) {
+    a.iter_off[K] = off;
+    off += chol_iter_total(a.nt, K);
   }
-  a.iter_off[a.nt] = off;
+  a.iter_off[nq] = off;
   a.ntasks = off;
   a.gst = flags;
   a.zst = flags + a.nt * a.nt;
@@ -566,62 +699,69 @@ template int chol_tile_factor<double>(double*, int64_t, int, double*, int64_t, i
                                       long long*);
 template int chol_tile_factor<zd>(zd*, int64_t, int, zd*, int64_t, int*, int*, int, hipStream_t, int, long long*);
 
-// Host-side check of the task order (test hook): every task once, every
-// dependency earlier in the order, and the other workgroups' queue equal to
-// the order without the panel workgroup's tasks (D(k), P(k, k+1),
-// U(k, k+1, k+1)), which that workgroup runs in the same relative order.
-// Returns the task count, or -1.
-int chol_tile_order_check(int nt) {
-  if (nt <= 0 || nt > kCholTileMaxNt) return -1;
-  {
-    std::vector<CholTask> full, queue;
-    for (int k = 0; k < nt; ++k) {
-      for (int u = 0; u < chol_iter_total(nt, k); ++u) {
-        const CholTask t = chol_task_in_iter(nt, k, u);
-        const bool panel = t.type == CT_D || (t.type == CT_P && t.j == t.k + 1) ||
-                           (t.type == CT_U && t.i == t.k + 1 && t.j == t.k + 1);
-        if (!panel) full.push_back(t);
-      }
-      for (int u = 0; u < chol_iter_total(nt, k, 1); ++u) queue.push_back(chol_task_in_iter(nt, k, u, 1));
-    }
-    if (full.size() != queue.size()) return -1;
-    for (size_t q = 0; q < full.size(); ++q)
-      if (full[q].type != queue[q].type || full[q].k != queue[q].k || full[q].i != queue[q].i ||
-          full[q].j != queue[q].j)
-        return -1;
-  }
-  std::vector<int> gst(nt * nt, 0), zst(nt * nt, 0);
+// Host-side check of the task order (test hook) for ntiles tiles (rounded up
+// to T, even): every
+// task once, every dependency (the waits of the kernel, with the panel's
+// iteration K run just before the tasks of iteration K) satisfied by tasks
+// earlier in the order, and the final counters as documented.  Returns the
+// task count, or -1.
+int chol_tile_order_check(int ntiles) {
+  if (ntiles <= 0 || ntiles > kCholTileMaxNt) return -1;
+  const int T = ntiles + (ntiles & 1);
+  std::vector<int> gst(T * T, 0), zst(T * T, 0);
+  auto G = [&](int r, int c) -> int& { return gst[r * T + c]; };
+  auto Z = [&](int r, int c) -> int& { return zst[r * T + c]; };
   int total = 0;
-  for (int k = 0; k < nt; ++k) {
-    const int cnt = chol_iter_total(nt, k);
+  for (int K = 0; K < T / 2; ++K) {
+    const int c0 = 2 * K, c1 = 2 * K + 1;
+    // the panel's iteration K
+    if (G(c0, c0) < K || G(c0, c1) < K || G(c1, c1) < K) return -1;
+    G(c0, c0) = G(c0, c1) = G(c1, c1) = K + 1;
+    Z(c0, c0) = Z(c0, c1) = Z(c1, c1) = 1;
+    const int cnt = chol_iter_total(T, K);
     for (int u = 0; u < cnt; ++u) {
-      const CholTask t = chol_task_in_iter(nt, k, u);
-      const int kk = t.k, i = t.i, j = t.j;
-      auto G = [&](int r, int c) -> int& { return gst[r * nt + c]; };
-      auto Z = [&](int r, int c) -> int& { return zst[r * nt + c]; };
+      const CholTask t = chol_task_in_iter(T, K, u);
+      if (t.k != K) return -1;
       switch (t.type) {
-        case CT_D:
-          if (G(kk, kk) != kk) return -1;
-          G(kk, kk) = kk + 1; Z(kk, kk) = 1; break;
-        case CT_P:
-          if (G(kk, kk) != kk + 1 || G(kk, j) != kk || j <= kk) return -1;
-          G(kk, j) = kk + 1; break;
-        case CT_PQ:
-          if (G(j, j) != j + 1 || Z(i, j) != j - i || i >= j) return -1;
-          Z(i, j) = j - i + 1; break;
-        case CT_U:
-          if (G(kk, i) != kk + 1 || G(kk, j) != kk + 1 || G(i, j) != kk || !(kk < i && i <= j)) return -1;
-          G(i, j) = kk + 1; break;
-        case CT_UQ:
-          if (Z(i, kk) != kk - i + 1 || G(kk, j) != kk + 1 || Z(i, j) != kk - i || !(i <= kk && kk < j)) return -1;
-          Z(i, j) = kk - i + 1; break;
+        case CT_P2: {
+          const int j = t.j;
+          if (j < c1 + 1 || j >= T || G(c1, c1) < K + 1 || G(c0, j) < K || G(c1, j) < K) return -1;
+          if (G(c0, j) != K || G(c1, j) != K) return -1;  // written once
+          G(c0, j) = G(c1, j) = K + 1;
+          break;
+        }
+        case CT_U2: {
+          const int i = t.i, j = t.j;
+          if (!(c1 < i && i <= j && j < T)) return -1;
+          if (G(c0, i) < K + 1 || G(c1, i) < K + 1 || G(c0, j) < K + 1 || G(c1, j) < K + 1) return -1;
+          if (G(i, j) != K) return -1;
+          G(i, j) = K + 1;
+          break;
+        }
+        case CT_UQ2: {
+          const int jr = t.i, l = t.j, base = jr / 2, fin = K - base + 1;
+          if (!(jr <= c1 && c1 < l && l < T)) return -1;
+          if (jr <= c0 && Z(jr, c0) < fin) return -1;
+          if (Z(jr, c1) < fin || G(c0, l) < K + 1 || G(c1, l) < K + 1) return -1;
+          if (Z(jr, l) != K - base) return -1;
+          Z(jr, l) = K - base + 1;
+          break;
+        }
+        case CT_PQ2: {
+          const int jr = t.j, c = K - jr / 2;
+          if (!(jr < c0) || G(c1, c1) < K + 1 || Z(jr, c0) != c || Z(jr, c1) != c) return -1;
+          Z(jr, c0) = Z(jr, c1) = c + 1;
+          break;
+        }
+        default:
+          return -1;
       }
       ++total;
     }
   }
-  for (int r = 0; r < nt; ++r)
-    for (int c = r; c < nt; ++c)
-      if (gst[r * nt + c] != r + 1 || zst[r * nt + c] != c - r + 1) return -1;
+  for (int r = 0; r < T; ++r)
+    for (int c = r; c < T; ++c)
+      if (G(r, c) != r / 2 + 1 || Z(r, c) != c / 2 - r / 2 + 1) return -1;
   return total;
 }
 

@@ -49,6 +49,48 @@ __device__ inline void mm32(T (*C)[33], const T (*A)[33], bool cA, const T (*B)[
   __syncthreads();
 }
 
+// C = alpha (op(A0) op(B0) + op(A1) op(B1)) (+ C when acc): two products
+// summed in the MFMA accumulators, one write (after a barrier, so C may alias
+// any operand).  A0 = nullptr drops the first product.
+template <typename T>
+__device__ inline void mm32_2(T (*C)[33], const T (*A0)[33], const T (*B0)[33], const T (*A1)[33],
+                              const T (*B1)[33], bool cA, bool cB, double alpha, bool acc) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = tid >> 6;
+  const int bi = w & 1, bj = w >> 1;
+  v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+  auto product = [&](const T (*A)[33], const T (*B)[33]) {
+#pragma unroll
+    for (int kk = 0; kk < 32; kk += 4) {
+      const int k = kk + (lane >> 4);
+      const int i = bi * 16 + (lane & 15);
+      const int j = bj * 16 + (lane & 15);
+      const T a = cA ? s_conj(A[k][i]) : A[i][k];
+      const T b = cB ? s_conj(B[j][k]) : B[k][j];
+      cr = mfma_f64(s_re(b), s_re(a), cr);
+      if constexpr (ScalarTraits<T>::is_complex) {
+        cr = mfma_f64(-s_im(b), s_im(a), cr);
+        ci = mfma_f64(s_im(b), s_re(a), ci);
+        ci = mfma_f64(s_re(b), s_im(a), ci);
+      }
+    }
+  };
+  if (A0 != nullptr) product(A0, B0);
+  product(A1, B1);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = bi * 16 + (lane & 15);
+    const int col = bj * 16 + mfma_f64_row(lane, r);
+    T v;
+    if constexpr (ScalarTraits<T>::is_complex) v = mkz(alpha * cr[r], alpha * ci[r]);
+    else v = alpha * cr[r];
+    C[row][col] = acc ? s_add(C[row][col], v) : v;
+  }
+  __syncthreads();
+}
+
 // Value of v in lane `src` (uniform over the wave) broadcast to the wave through a scalar register.
 __device__ inline double rdl(double v, int src) {
   const uint64_t u = __builtin_bit_cast(uint64_t, v);

@@ -91,13 +91,21 @@ def test_workspace_queries_accept_empty_shapes():
 @pytest.mark.parametrize("nt", [1, 2, 3, 7, 32, 64])
 def test_chol_tile_task_order(nt):
     """The persistent Cholesky kernel's task order (host code in the library):
-    every task appears once and every dependency precedes it, so a workgroup
-    waiting on a task it took can always be satisfied.  Count = nt D + nt(nt-1)
-    P and P' + 2 sum_k (updates)."""
+    every task appears once and every dependency precedes it (the panel's
+    step K runs before the tasks of step K), so a workgroup waiting on a task
+    it took can always be satisfied.  Tiles T = nt rounded up to even; per
+    pair step K with rem = T - 2K - 2 tiles after the pair: rem P2, the next
+    pair's 2 rem - 1 U2, 2K PQ2, (2K + 2) rem UQ2 and the bulk b(b+1)/2 U2
+    (b = rem - 2)."""
     lib = _lib.load()
     lib.maktest_chol_order_check.restype = C.c_int
     lib.maktest_chol_order_check.argtypes = [C.c_int]
-    expect = nt + nt * (nt - 1) + sum((nt - 1 - k) * (nt - k) // 2 + (k + 1) * (nt - 1 - k) for k in range(nt))
+    T = nt + (nt & 1)
+    expect = 0
+    for k in range(T // 2):
+        rem = T - 2 * k - 2
+        b = max(rem - 2, 0)
+        expect += rem + (2 * rem - 1 if rem > 0 else 0) + 2 * k + (2 * k + 2) * rem + b * (b + 1) // 2
     assert lib.maktest_chol_order_check(nt) == expect
     assert lib.maktest_chol_order_check(0) == -1
     assert lib.maktest_chol_order_check(65) == -1
