@@ -688,8 +688,10 @@ int chol_tile_factor(T* G, int64_t ldg, int n, T* X, int64_t ldx, int* flags, in
   a.trace = trace;
   const int nflags = 2 * a.nt * a.nt + 64;
   hipLaunchKernelGGL((chol_zero_ints_kernel<T>), dim3(cdiv(nflags, 256)), dim3(256), 0, s, flags, nflags);
-  // the panel workgroup plus at least one for the other tasks
-  if (grid <= 0) grid = 256;
+  // the panel workgroup plus at least one for the other tasks; up to n ~ 1024
+  // the chain dominates and fewer polling workgroups contend less for the
+  // coherent loads, beyond it the task throughput needs every CU
+  if (grid <= 0) grid = n <= 1280 ? 128 : 256;
   grid = std::max(2, std::min(grid, a.ntasks + 1));
   hipLaunchKernelGGL((chol_tile_kernel<T>), dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError() == hipSuccess ? 0 : 1;
