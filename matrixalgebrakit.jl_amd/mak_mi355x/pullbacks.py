@@ -156,7 +156,7 @@ def qr_pullback_batched_(dA, tapeA, T, R, dQ, dR, p=None, rank_atol=None, positi
     lib = _lib.load()
     dt = _DT[dA.dtype]
     dev = dA.device
-    gauge = torch.zeros(max(batch, 1), dtype=torch.float64, device=dev)
+    gauge = torch.zeros(max(batch, 1), dtype=torch.float64, device=dev) if (full or p < k) else None
     if full:
         # check_and_prepare_qr_cotangents for the full Q (src/pullbacks/qr.jl:19-26)
         nb = C.c_size_t(0)
@@ -190,9 +190,10 @@ def qr_pullback_batched_(dA, tapeA, T, R, dQ, dR, p=None, rank_atol=None, positi
     fname = f"mak_qr_pullback_{'z' if dt else 'd'}"
     code = getattr(lib, fname)(_stream(dev), batch, m, n, p, int(bool(positive)), _ptr(tapeA), lda, sa, _ptr(T), st,
                                _ptr(R), ldr, sr, _ptr(dQ), lddq, sdq, _ptr(dR), lddr, sdr, _ptr(dA), ldda, sda,
-                               _ptr(ws), nbytes.value, C.c_void_p(gauge.data_ptr()))
+                               _ptr(ws), nbytes.value, C.c_void_p(gauge.data_ptr() if gauge is not None else 0))
     _status(code, fname)
-    if batch:
+    # a full-rank compact cotangent has no gauge-dependent entries: nothing to check
+    if batch and (full or p < k):
         if full:
             # p = k: the gauge part of a full cotangent is the one the preparation measured
             gauge = torch.maximum(gauge, gfull)
