@@ -49,10 +49,12 @@ namespace mak {
 namespace {
 
 constexpr int TB = 32;                 // tile size
+constexpr int kYRows = 128;            // rows of B / Y per right-product task
+constexpr int kCholGridY = 256;        // default workgroups with a right product
 constexpr long kSpinLimit = 1L << 22;  // polls per wait before the abort flag is raised
 
 // ---------------------------------------------------------------- task order
-enum CholTaskType { CT_P2 = 1, CT_U2 = 2, CT_UQ2 = 3, CT_PQ2 = 4 };
+enum CholTaskType { CT_P2 = 1, CT_U2 = 2, CT_UQ2 = 3, CT_PQ2 = 4, CT_Y = 5 };
 
 struct CholTask {
   int type, k, i, j;  // k: the pair step
@@ -64,8 +66,16 @@ struct CholTask {
 //   C  PQ2(K, j)       j = 0 .. 2K-1
 //   E  UQ2(K, j, l)    j = 0 .. 2K+1, l = 2K+2 .. T-1
 //   F  U2(K, i, j)     2K+4 <= i <= j <= T-1          (the bulk)
-constexpr int kCholSegs = 5;
-__host__ __device__ inline void chol_iter_counts(int T, int K, int c[kCholSegs]) {
+//   H  Y(K, r)         r = 0 .. ny-1                  (the right product, below)
+// The optional right product Y = B X (B m x n, real data) is computed in the
+// same launch: task Y(K, r) forms rows [64 r, 64 r + 64) of the pair's 64
+// columns of Y, Y[:, pair K] = B[:, 0 : 64K + 64] X[0 : 64K + 64, pair K],
+// once that column block of X is final (the panel's pair K and the 2K tasks
+// PQ2(K, j); a per-pair counter records them).  X fills column block by
+// column block as the elimination proceeds, so the product runs in the time
+// the workgroups would otherwise spend waiting for the chain.
+constexpr int kCholSegs = 6;
+__host__ __device__ inline void chol_iter_counts(int T, int K, int ny, int c[kCholSegs], int ysplit = 1 << 30) {
   const int r0 = 2 * K + 2;
   const int rem = T - r0 > 0 ? T - r0 : 0;
   const int b = rem >= 2 ? rem - 2 : 0;
@@ -74,18 +84,21 @@ __host__ __device__ inline void chol_iter_counts(int T, int K, int c[kCholSegs])
   c[2] = 2 * K;
   c[3] = r0 * rem;
   c[4] = b * (b + 1) / 2;
+  c[5] = K >= ysplit ? 2 * ny : ny;
 }
 
-__host__ __device__ inline int chol_iter_total(int T, int K) {
+__host__ __device__ inline int chol_iter_total(int T, int K, int ny, int ysplit = 1 << 30) {
   int c[kCholSegs];
-  chol_iter_counts(T, K, c);
-  return c[0] + c[1] + c[2] + c[3] + c[4];
+  chol_iter_counts(T, K, ny, c, ysplit);
+  return c[0] + c[1] + c[2] + c[3] + c[4] + c[5];
 }
 
-// u-th task of iteration K (0 <= u < chol_iter_total(T, K)).
-__host__ __device__ inline CholTask chol_task_in_iter(int T, int K, int u) {
+// u-th task of iteration K (0 <= u < chol_iter_total(T, K, ny, ysplit)).
+// Right-product task j: row block j % ny, K half j / ny (pairs K >= ysplit
+// split their contraction in two halves).
+__host__ __device__ inline CholTask chol_task_in_iter(int T, int K, int ny, int u, int ysplit = 1 << 30) {
   int c[kCholSegs];
-  chol_iter_counts(T, K, c);
+  chol_iter_counts(T, K, ny, c, ysplit);
   const int r0 = 2 * K + 2, rem = T - r0;
   CholTask t{CT_P2, K, 0, 0};
   int q = 0;
@@ -99,6 +112,7 @@ __host__ __device__ inline CholTask chol_task_in_iter(int T, int K, int u) {
       break;
     case 2: t.type = CT_PQ2; t.j = u; break;
     case 3: t.type = CT_UQ2; t.i = u / rem; t.j = r0 + u % rem; break;
+    case 5: t.type = CT_Y; t.j = u; break;
     default: {  // pairs r0+2 <= i <= j <= T-1, row by row
       int i = r0 + 2, len = T - i;
       while (u >= len) { u -= len; ++i; --len; }
@@ -122,6 +136,20 @@ struct CholTileArgs {
   int* info;      // optional: receives ctl[2] + info_offset on failure (when *info == 0)
   int info_offset;
   long long* trace;  // optional (tests): wall-clock stamps of the panel's iterations, 8 per iteration
+                     // (slot 7: the end of the pair's last right-product task)
+  // the right product Y = B X (ny = 0: none)
+  const double* B;
+  int64_t ldb;
+  double* Y;
+  int64_t ldy;
+  double* Xc;     // final tiles of X, written once each (ldxc: a multiple of 16)
+  int64_t ldxc;
+  int m;          // rows of B and Y
+  int ny;         // Y row blocks per pair step: ceil(m / kYRows)
+  int ysplit;     // pairs K >= ysplit run two tasks per row block, one per half
+                  // of the contraction, adding atomically into a zeroed Y
+                  // (two addends on zero: the sum is independent of their order)
+  int* xdone;     // per pair: tasks that have published final tiles of its X column block
   int iter_off[kCholTileMaxNt / 2 + 1];  // first queue task of every iteration
 };
 
@@ -451,6 +479,97 @@ __device__ inline void publish_n(int cnt, int* const* p, const int* v) {
     for (int q = 0; q < cnt; ++q) __hip_atomic_store(p[q], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Y(K, r) of the right product: rows [128 r, 128 r + 128) x the pair's
+// columns [64 K, 64 K + 64) of Y = B X, over k < 64 K + 64 (X is upper
+// triangular).  X is read from Xc, the copy of its final tiles (below): plain
+// loads, so a column block is shared through each XCD's L2 by the pair's
+// tasks.  Four waves, each a 64 x 32 quarter (eight FP64 MFMA accumulators:
+// six LDS reads feed eight MFMAs); the slabs of B and X for the next two
+// 16-deep steps are in flight (in registers) while the current one is used.
+// LDS strides keep every read at two passes: B's slab rows 144 doubles
+// apart, X's slab stored transposed with rows 36 doubles apart.
+constexpr int kYLdA = kYRows + 16, kYLdX = 16 + 20;
+constexpr int kYLdsDoubles = 16 * kYLdA + 64 * kYLdX;
+// half: 0 = the whole contraction, stored; 1 / 2 = its first / second half
+// (split at a multiple of 16), added atomically.
+__device__ inline void y_task(const CholTileArgs& a, int K, int r, int half, double* lds) {
+  constexpr int BM = kYRows, BN = 64, BK = 16;
+  constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+  double (*As)[kYLdA] = reinterpret_cast<double (*)[kYLdA]>(lds);
+  double (*Xs)[kYLdX] = reinterpret_cast<double (*)[kYLdX]>(lds + BK * kYLdA);  // Xs[j][k]
+  const double* B = a.B;
+  const double* Xc = a.Xc;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w & 1, wn = w >> 1;
+  const int m0 = r * BM, n0 = K * 64;
+  const int kall = min(a.n, n0 + 64);
+  const int kmid = (kall / 2 + 15) / 16 * 16;
+  const int kbeg = half == 2 ? kmid : 0;
+  const int kend = half == 1 ? kmid : kall;
+  auto fetch = [&](int k0, double (&pa)[NA], double (&px)[NB]) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int gi = m0 + (tid & (BM - 1)), gk = k0 + (tid / BM) + (256 / BM) * u;
+      pa[u] = (gi < a.m && gk < kend) ? B[gi + (int64_t)gk * a.ldb] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int gk = k0 + (tid & 15), gj = n0 + (tid >> 4) + 16 * u;
+      px[u] = (gj < a.n && gk < kend) ? Xc[gk + (int64_t)gj * a.ldxc] : 0.0;
+    }
+  };
+  v4d acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+  // one 16-deep step: the slabs in (pa, px) to LDS, the loads of step
+  // k0 + 2 BK into the freed registers, the MFMAs of this step
+  auto step = [&](int k0, double (&pa)[NA], double (&px)[NB]) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) As[(tid / BM) + (256 / BM) * u][tid & (BM - 1)] = pa[u];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) Xs[(tid >> 4) + 16 * u][tid & 15] = px[u];
+    __syncthreads();
+    if (k0 + 2 * BK < kend) fetch(k0 + 2 * BK, pa, px);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double af[4], bf[2];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = As[kk + (lane >> 4)][wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[j] = Xs[wn * 32 + j * 16 + (lane & 15)][kk + (lane >> 4)];
+      // operands swapped: lane l holds rows l % 16 of the block
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = mfma_f64(bf[j], af[i], acc[i][j]);
+    }
+    __syncthreads();
+  };
+  double pa0[NA], px0[NB], pa1[NA], px1[NB];
+  if (kbeg < kend) fetch(kbeg, pa0, px0);
+  if (kbeg + BK < kend) fetch(kbeg + BK, pa1, px1);
+  for (int k0 = kbeg; k0 < kend; k0 += 2 * BK) {
+    step(k0, pa0, px0);
+    if (k0 + BK < kend) step(k0 + BK, pa1, px1);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = m0 + wm * 64 + i * 16 + (lane & 15);
+        const int col = n0 + wn * 32 + j * 16 + mfma_f64_row(lane, q);
+        if (row < a.m && col < a.n) {
+          double* y = a.Y + row + (int64_t)col * a.ldy;
+          if (half == 0) *y = acc[i][j][q];
+          else atomicAdd(y, acc[i][j][q]);
+        }
+      }
+}
+
 // The panel workgroup: the 2 x 2 tile diagonal block of every pair, in LDS.
 // Iteration K waits for the block's tiles to carry the K earlier pair steps,
 // then runs D(c0), R01 = X00^H G01, G11 -= R01^H R01, D(c1),
@@ -512,13 +631,32 @@ __device__ void chol_panel(const CholTileArgs& a, T (*tA)[33], T (*tB)[33], T (*
                   &gs[c0 * nt + c0], &gs[c0 * nt + c1], &gs[c1 * nt + c1]};
     const int fv[6] = {1, 1, 1, K + 1, K + 1, K + 1};
     publish_n(6, fp, fv);
+    if constexpr (!ScalarTraits<T>::is_complex) {
+      if (a.ny > 0) {
+        // the pair's final X tiles (and the zero lower one) into Xc, then the count
+        put_tile<T>(tX, a.Xc, a.ldxc, a.n, c0, c0);
+        put_tile<T>(tE, a.Xc, a.ldxc, a.n, c0, c1);
+        put_tile<T>(tD, a.Xc, a.ldxc, a.n, c1, c1);
+        clear_tile<T>(a.Xc, a.ldxc, a.n, c1, c0);
+        publish_n(0, nullptr, nullptr);
+        if (threadIdx.x == 0) atomicAdd(a.xdone + K, 1);
+      }
+    }
     stamp(K, 6);
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
-  __shared__ T tA[TB][33], tB[TB][33], tC[TB][33], tD[TB][33], tE[TB][33], tX[TB][33];
+  // six 32 x 32 tiles (a right-product task reuses them)
+  __shared__ __align__(16) T lds[6][TB][33];
+  static_assert(sizeof(lds) >= kYLdsDoubles * sizeof(double), "right-product slabs fit the tiles");
+  T(*tA)[33] = lds[0];
+  T(*tB)[33] = lds[1];
+  T(*tC)[33] = lds[2];
+  T(*tD)[33] = lds[3];
+  T(*tE)[33] = lds[4];
+  T(*tX)[33] = lds[5];
   __shared__ __align__(16) T rowbuf[2 * TB];
   __shared__ int task_s;
   if (blockIdx.x == 0) {
@@ -538,7 +676,7 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
     const int t = task_s;
     if (t >= a.ntasks) break;
     while (a.iter_off[iter + 1] <= t) ++iter;  // tasks are taken in increasing order
-    const CholTask tk = chol_task_in_iter(nt, iter, t - a.iter_off[iter]);
+    const CholTask tk = chol_task_in_iter(nt, iter, a.ny, t - a.iter_off[iter], a.ysplit);
     const int K = tk.k, c0 = 2 * K, c1 = 2 * K + 1;
     switch (tk.type) {
       case CT_P2: {  // [R_c0j; R_c1j] = X_KK^H [G_c0j; G_c1j]; the mirrored lower tiles of G are zeroed
@@ -634,6 +772,24 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
         int* fp[2] = {&zs[jr * nt + c0], &zs[jr * nt + c1]};
         const int fv[2] = {cnt + 1, cnt + 1};
         publish_n(2, fp, fv);
+        if constexpr (!ScalarTraits<T>::is_complex) {
+          if (a.ny > 0) {
+            put_tile<T>(tA, a.Xc, a.ldxc, a.n, jr, c0);
+            put_tile<T>(tB, a.Xc, a.ldxc, a.n, jr, c1);
+            publish_n(0, nullptr, nullptr);
+            if (threadIdx.x == 0) atomicAdd(a.xdone + K, 1);
+          }
+        }
+        break;
+      }
+      case CT_Y: {  // rows of Y's column block K, once X's column block K is final
+        wait_ge(&a.xdone[K], 2 * K + 1, ctl);
+        if (!inputs_ready(ctl)) break;
+        if constexpr (!ScalarTraits<T>::is_complex)
+          y_task(a, K, tk.j % a.ny, K >= a.ysplit ? 1 + tk.j / a.ny : 0, reinterpret_cast<double*>(&lds[0][0][0]));
+        if (a.trace != nullptr && threadIdx.x == 0)  // the pair's last right-product task to finish
+          atomicMax(reinterpret_cast<unsigned long long*>(a.trace + K * 8 + 7),
+                    (unsigned long long)__builtin_amdgcn_s_memrealtime());
         break;
       }
       default:
@@ -645,6 +801,13 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
 
 template <typename T> __global__ void chol_zero_ints_kernel(int* p, int n) {
   for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) p[e] = 0;
+}
+
+// Y[:, c0 : n] = 0 (m rows)
+__global__ void zero_cols_kernel(double* Y, int64_t ldy, int m, int c0, int n) {
+  const int64_t tot = (int64_t)m * (n - c0);
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (int64_t)gridDim.x * 256)
+    Y[e % m + (c0 + e / m) * ldy] = 0.0;
 }
 
 // tiles of the pair-step kernel: ceil(n / 32) rounded up to even
@@ -665,41 +828,62 @@ size_t chol_tile_flag_ints(int n) {
 // can be captured).  info (device, optional): first non-positive pivot + offset.
 template <typename T>
 int chol_tile_factor(T* G, int64_t ldg, int n, T* X, int64_t ldx, int* flags, int* info, int info_offset,
-                     hipStream_t s, int grid, long long* trace) {
+                     hipStream_t s, int grid, long long* trace, const CholRightProduct* rp) {
   if (n <= 0) return 0;
   if (n > kCholTileMaxN) return -3;
+  if (rp != nullptr && (ScalarTraits<T>::is_complex || rp->m < 0)) return -1;
   CholTileArgs a{};
   a.G = G; a.ldg = ldg; a.X = X; a.ldx = ldx;
   a.n = n;
   a.nt = chol_tiles(n);
+  if (rp != nullptr && rp->m > 0) {
+    if (rp->Xc == nullptr || rp->ldxc < n || rp->ldxc % 16 != 0 || (reinterpret_cast<uintptr_t>(rp->Xc) & 127) != 0)
+      return -1;
+    a.B = rp->B; a.ldb = rp->ldb; a.Y = rp->Y; a.ldy = rp->ldy;
+    a.Xc = rp->Xc; a.ldxc = rp->ldxc;
+    a.m = rp->m;
+    a.ny = cdiv(rp->m, kYRows);
+  }
+  // the pairs whose right-product tasks would outlast the elimination: the
+  // last quarter (at least three) split their tasks in two
+  a.ysplit = 1 << 30;
+  if (a.ny > 0 && a.nt / 2 >= 4) a.ysplit = a.nt / 2 - std::max(3, a.nt / 8);
   int off = 0;
   const int nq = a.nt / 2;
   for (int K = 0; K < nq; ++K) {
     a.iter_off[K] = off;
-    off += chol_iter_total(a.nt, K);
+    off += chol_iter_total(a.nt, K, a.ny, a.ysplit);
   }
   a.iter_off[nq] = off;
   a.ntasks = off;
   a.gst = flags;
   a.zst = flags + a.nt * a.nt;
   a.ctl = flags + 2 * a.nt * a.nt;
+  a.xdone = a.ctl + 32;  // one counter per pair (at most 32 pairs)
   a.info = info;
   a.info_offset = info_offset;
   a.trace = trace;
   const int nflags = 2 * a.nt * a.nt + 64;
   hipLaunchKernelGGL((chol_zero_ints_kernel<T>), dim3(cdiv(nflags, 256)), dim3(256), 0, s, flags, nflags);
+  if (a.ny > 0 && a.ysplit < a.nt / 2) {
+    const int c0 = 64 * a.ysplit;  // < n (every pair has a column)
+    hipLaunchKernelGGL(zero_cols_kernel, dim3(1024), dim3(256), 0, s, a.Y, a.ldy, a.m, c0, n);
+  }
   // the panel workgroup plus at least one for the other tasks; up to n ~ 1024
   // the chain dominates and fewer polling workgroups contend less for the
   // coherent loads, beyond it the task throughput needs every CU
-  if (grid <= 0) grid = n <= 1280 ? 128 : 256;
+  // (with a right product the workgroups that wait for the chain have its
+  // tasks to run: every CU)
+  if (grid <= 0) grid = (n <= 1280 && a.ny == 0) ? 128 : kCholGridY;
   grid = std::max(2, std::min(grid, a.ntasks + 1));
   hipLaunchKernelGGL((chol_tile_kernel<T>), dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 template int chol_tile_factor<double>(double*, int64_t, int, double*, int64_t, int*, int*, int, hipStream_t, int,
-                                      long long*);
-template int chol_tile_factor<zd>(zd*, int64_t, int, zd*, int64_t, int*, int*, int, hipStream_t, int, long long*);
+                                      long long*, const CholRightProduct*);
+template int chol_tile_factor<zd>(zd*, int64_t, int, zd*, int64_t, int*, int*, int, hipStream_t, int, long long*,
+                                  const CholRightProduct*);
 
 // Host-side check of the task order (test hook) for ntiles tiles (rounded up
 // to T, even): every
@@ -707,10 +891,10 @@ template int chol_tile_factor<zd>(zd*, int64_t, int, zd*, int64_t, int*, int*, i
 // iteration K run just before the tasks of iteration K) satisfied by tasks
 // earlier in the order, and the final counters as documented.  Returns the
 // task count, or -1.
-int chol_tile_order_check(int ntiles) {
-  if (ntiles <= 0 || ntiles > kCholTileMaxNt) return -1;
+int chol_tile_order_check(int ntiles, int ny, int ysplit) {
+  if (ntiles <= 0 || ntiles > kCholTileMaxNt || ny < 0) return -1;
   const int T = ntiles + (ntiles & 1);
-  std::vector<int> gst(T * T, 0), zst(T * T, 0);
+  std::vector<int> gst(T * T, 0), zst(T * T, 0), xd(T / 2, 0), yd(T / 2, 0);
   auto G = [&](int r, int c) -> int& { return gst[r * T + c]; };
   auto Z = [&](int r, int c) -> int& { return zst[r * T + c]; };
   int total = 0;
@@ -720,9 +904,10 @@ int chol_tile_order_check(int ntiles) {
     if (G(c0, c0) < K || G(c0, c1) < K || G(c1, c1) < K) return -1;
     G(c0, c0) = G(c0, c1) = G(c1, c1) = K + 1;
     Z(c0, c0) = Z(c0, c1) = Z(c1, c1) = 1;
-    const int cnt = chol_iter_total(T, K);
+    ++xd[K];
+    const int cnt = chol_iter_total(T, K, ny, ysplit);
     for (int u = 0; u < cnt; ++u) {
-      const CholTask t = chol_task_in_iter(T, K, u);
+      const CholTask t = chol_task_in_iter(T, K, ny, u, ysplit);
       if (t.k != K) return -1;
       switch (t.type) {
         case CT_P2: {
@@ -753,6 +938,18 @@ int chol_tile_order_check(int ntiles) {
           const int jr = t.j, c = K - jr / 2;
           if (!(jr < c0) || G(c1, c1) < K + 1 || Z(jr, c0) != c || Z(jr, c1) != c) return -1;
           Z(jr, c0) = Z(jr, c1) = c + 1;
+          ++xd[K];
+          break;
+        }
+        case CT_Y: {
+          // the counter's target: X's column block K final (every tile of
+          // rows 0 .. c1 in columns c0, c1 at its final version)
+          const int halves = K >= ysplit ? 2 : 1;
+          if (t.j < 0 || t.j >= halves * ny || xd[K] != 2 * K + 1) return -1;
+          for (int r = 0; r <= c1; ++r)
+            if ((r <= c0 && Z(r, c0) != c0 / 2 - r / 2 + 1) || Z(r, c1) != c1 / 2 - r / 2 + 1) return -1;
+          if (yd[K] != t.j) return -1;  // every row block (and half) once, in order
+          ++yd[K];
           break;
         }
         default:
@@ -764,6 +961,8 @@ int chol_tile_order_check(int ntiles) {
   for (int r = 0; r < T; ++r)
     for (int c = r; c < T; ++c)
       if (G(r, c) != r / 2 + 1 || Z(r, c) != c / 2 - r / 2 + 1) return -1;
+  for (int K = 0; K < T / 2; ++K)
+    if (yd[K] != (K >= ysplit ? 2 : 1) * ny) return -1;
   return total;
 }
 

@@ -5,7 +5,8 @@
 //
 //   G1 = A^H A                       upper triangle, split-K MFMA product
 //   G1 = R1^H R1, X1 = R1^{-1}       persistent tile Cholesky (mak_chol.hip)
-//   Q1 = A X1                        triangular MFMA product
+//   Q1 = A X1                        triangular MFMA product (real data: in
+//                                    the Cholesky launch, as X1 fills)
 //   G2 = Q1^H Q1 = I + E             upper triangle, split-K MFMA product
 //   W2 = I + U with (I + U)^H (I + U) = I + E, in closed form:
 //        U1 = up(E), U = up(E - U1^H U1), W2^{-1} = I - U + U^2
@@ -36,9 +37,18 @@ inline size_t al256(size_t b) { return ((b + 255) / 256) * 256; }
 
 constexpr int kRedWgs = 256;  // workgroups of the ||E|| reduction
 
+// Largest ||E||_F for which the final Q = Q1 + Q1 D (D = W2^{-1} - I, upper
+// triangular, ||D||_F ~ ||E||_F) takes the product Q1 D from FP32 MFMA with
+// FP32 accumulation (real data): its rounding error is at most about
+// n u_32 ||D||_F <= 1e3 * 6e-8 * 1e-10 = 6e-15 per row even in the worst case
+// of the accumulation (typically sqrt(n) u_32 ||E|| ~ 2e-16), against the
+// FP64 product's own 1e-16.  Gaussian inputs have ||E||_F ~ 1e-13.
+constexpr double kLoCorrMaxE = 1e-10;
+
 struct CQ2Layout {
-  size_t offQ1, offG, offE, offX, offU, offW, offT, offSl, offFlags, offPart, offSt, bytes;
-  int ks;  // split-K slices of the Gram products
+  size_t offQ1, offG, offE, offX, offU, offW, offT, offSl, offFlags, offPart, offSt, offXc, bytes;
+  int ks;    // split-K slices of the Gram products
+  int ldxc;  // leading dimension of the Cholesky's copy of X1 (real data)
 };
 
 // Split-K slices of an upper-triangle n x n Gram over m rows on 128 x 128
@@ -68,10 +78,13 @@ template <typename T> CQ2Layout cq2_layout(int m, int n) {
   L.offU = take(nn);
   L.offW = take(nn);
   L.offT = take(nn);
-  L.offSl = take(nn * L.ks);
+  // the Gram slices; later the packed operands of the FP32 correction
+  L.offSl = take(std::max(nn * L.ks, ScalarTraits<T>::is_complex ? 0 : gemm_prec_workspace_bytes(m, n, n)));
   L.offFlags = take(chol_tile_flag_ints(n) * sizeof(int));
   L.offPart = take(kRedWgs * sizeof(double));
   L.offSt = take(64 * sizeof(int));
+  L.ldxc = (n + 15) / 16 * 16;
+  L.offXc = ScalarTraits<T>::is_complex ? 0 : take((size_t)L.ldxc * n * sizeof(T));
   L.bytes = off;
   return L;
 }
@@ -126,6 +139,22 @@ void gram_upper(const T* X, int64_t ldx, int m, int n, T* G, T* slices, int ks, 
                      0, s, slices, g.ksplit, n, G);
 }
 
+// R (n x n, ld ldr) = sum of the ks slices (n x n each) on and above the
+// diagonal, zero below; only when the call was accepted (st[0] == 1).
+constexpr int kRSlices = 4;
+template <typename T>
+__global__ void __launch_bounds__(256) sum_r_slices_kernel(const T* P, int ks, int n, T* R, int64_t ldr, const int* st) {
+  if (st[0] != 1) return;
+  const int64_t tot = (int64_t)n * n;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (int64_t)gridDim.x * 256) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    T acc = s_zero(T());
+    if (r <= c)
+      for (int z = 0; z < ks; ++z) acc = s_add(acc, P[e + (int64_t)z * tot]);
+    R[r + (int64_t)c * ldr] = acc;
+  }
+}
+
 // ||G - I||_F^2 over the upper triangle (off-diagonal entries counted twice),
 // partial per workgroup.
 template <typename T>
@@ -148,8 +177,11 @@ __global__ void __launch_bounds__(256) eye_dist_partial_kernel(const T* G, int n
 
 // st[0] = 1 (accept) when the Cholesky succeeded (info == 0) and ||E||_F <=
 // 1e-6; st[1] = 1 when the second-order terms are needed (||E||_F > 1e-9);
-// st[4..5] = ||E||_F as a double.
-__global__ void cq2_decide_kernel(const double* part, int np, const int* cinfo, int* st) {
+// of the accepted calls, st[3] = 1 when Q = Q1 + Q1 (W2^{-1} - I) takes its
+// correction from FP32 MFMA (lo_ok: real data, and ||E||_F <= kLoCorrMaxE)
+// and st[6] = 1 otherwise (Q = Q1 W2^{-1} on FP64 MFMA); st[4..5] = ||E||_F
+// as a double.
+__global__ void cq2_decide_kernel(const double* part, int np, const int* cinfo, int lo_ok, int* st) {
   __shared__ double red[4];
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += 256) acc += part[i];
@@ -161,6 +193,8 @@ __global__ void cq2_decide_kernel(const double* part, int np, const int* cinfo, 
     const bool ok = (*cinfo == 0) && (e <= 1e-6);  // false for NaN
     st[0] = ok ? 1 : 0;
     st[1] = (ok && e > 1e-9) ? 1 : 0;
+    st[3] = (ok && lo_ok && e <= kLoCorrMaxE) ? 1 : 0;
+    st[6] = (ok && st[3] == 0) ? 1 : 0;
     double* ed = reinterpret_cast<double*>(st + 4);
     ed[0] = e;
   }
@@ -184,9 +218,10 @@ __global__ void cq2_up_kernel(const T* G, const T* Sub, int n, T* U, const int* 
   U[idx] = u;
 }
 
-// W = I + U and Wi = I - U (+ U2 when st[1]), zero below the diagonal.
+// W = I + U and Wi = I - U (+ U2 when st[1]), zero below the diagonal; D
+// (optional) = Wi - I.
 template <typename T>
-__global__ void cq2_w_kernel(const T* U, const T* U2, const int* st, int n, T* W, T* Wi) {
+__global__ void cq2_w_kernel(const T* U, const T* U2, const int* st, int n, T* W, T* Wi, T* D) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)n * n) return;
   const int r = (int)(idx % n), c = (int)(idx / n);
@@ -196,10 +231,13 @@ __global__ void cq2_w_kernel(const T* U, const T* U2, const int* st, int n, T* W
     w = u;
     wi = s_neg(u);
     if (st[1] == 1) wi = s_add(wi, U2[idx]);
+    if (D != nullptr) D[idx] = wi;
     if (r == c) {
       w = s_add(w, s_one(T()));
       wi = s_add(wi, s_one(T()));
     }
+  } else if (D != nullptr) {
+    D[idx] = s_zero(T());
   }
   W[idx] = w;
   Wi[idx] = wi;
@@ -235,14 +273,21 @@ void cholqr2_enqueue(hipStream_t s, int m, int n, const T* A, int64_t lda, T* Q,
   const int64_t nn = (int64_t)n * n;
   const unsigned eg = (unsigned)cdiv64(nn, 256);
   zero_ints(st, 8, s);  // st[2]: the Cholesky's info
-  // pass 1
+  // pass 1 (real data: Q1 = A X1 inside the Cholesky launch, column block by
+  // column block as X1 becomes final)
   gram_upper<T>(A, lda, m, n, G, Sl, L.ks, s);
-  chol_tile_factor<T>(G, n, n, X, n, flags, st + 2, 0, s);
-  gemm<T>(OP_N, OP_N, gq<T>(m, n, n, A, lda, X, n, Q1, m, 1.0, 0.0, TRI_B_UPPER), s);
+  if constexpr (!ScalarTraits<T>::is_complex) {
+    const CholRightProduct rp{A, lda, Q1, m, m, reinterpret_cast<double*>(ws + L.offXc), L.ldxc};
+    chol_tile_factor<T>(G, n, n, X, n, flags, st + 2, 0, s, 0, nullptr, &rp);
+  } else {
+    chol_tile_factor<T>(G, n, n, X, n, flags, st + 2, 0, s);
+    gemm<T>(OP_N, OP_N, gq<T>(m, n, n, A, lda, X, n, Q1, m, 1.0, 0.0, TRI_B_UPPER), s);
+  }
   // pass 2: G2 = I + E, the decision, and W2, W2^{-1} in closed form
   gram_upper<T>(Q1, m, m, n, E, Sl, L.ks, s);
   hipLaunchKernelGGL((eye_dist_partial_kernel<T>), dim3(kRedWgs), dim3(256), 0, s, E, n, part);
-  hipLaunchKernelGGL(cq2_decide_kernel, dim3(1), dim3(256), 0, s, part, kRedWgs, st + 2, st);
+  hipLaunchKernelGGL(cq2_decide_kernel, dim3(1), dim3(256), 0, s, part, kRedWgs, st + 2,
+                     ScalarTraits<T>::is_complex ? 0 : 1, st);
   hipLaunchKernelGGL((cq2_up_kernel<T>), dim3(eg), dim3(256), 0, s, E, (const T*)nullptr, n, U, (const int*)nullptr);
   // second order (st[1] == 1): U = up(E - U1^H U1), U2 = U U
   {
@@ -252,11 +297,38 @@ void cholqr2_enqueue(hipStream_t s, int m, int n, const T* A, int64_t lda, T* Q,
   }
   hipLaunchKernelGGL((cq2_up_kernel<T>), dim3(eg), dim3(256), 0, s, E, (const T*)Tm, n, U, (const int*)(st + 1));
   gemm<T>(OP_N, OP_N, gq<T>(n, n, n, U, n, U, n, Tm, n, 1.0, 0.0, TRI_A_UPPER | TRI_B_UPPER, st + 1, 1), s);
-  hipLaunchKernelGGL((cq2_w_kernel<T>), dim3(eg), dim3(256), 0, s, U, Tm, st, n, W, X);
+  // (real data: D = W2^{-1} - I into E, which is no longer read)
+  T* D = ScalarTraits<T>::is_complex ? nullptr : E;
+  hipLaunchKernelGGL((cq2_w_kernel<T>), dim3(eg), dim3(256), 0, s, U, Tm, st, n, W, X, D);
   // Q = Q1 W2^{-1};  R = W2 R1 (R1 is upper with a zero strict lower part)
-  gemm<T>(OP_N, OP_N, gq<T>(m, n, n, Q1, m, X, n, Q, ldq, 1.0, 0.0, TRI_B_UPPER, st, 1), s);
-  if (R != nullptr)
-    gemm<T>(OP_N, OP_N, gq<T>(n, n, n, W, n, G, n, R, ldr, 1.0, 0.0, TRI_A_UPPER | TRI_B_UPPER, st, 1), s);
+  if constexpr (ScalarTraits<T>::is_complex) {
+    gemm<T>(OP_N, OP_N, gq<T>(m, n, n, Q1, m, X, n, Q, ldq, 1.0, 0.0, TRI_B_UPPER, st + 6, 1), s);
+  } else {
+    // ||E|| small (st[3]): Q = Q1 + Q1 D with the correction on FP32 MFMA
+    // (the packed operands go where the Gram slices were); else on FP64
+    GemmArgs g = gq<T>(m, n, n, Q1, m, D, n, Q, ldq, 1.0, 1.0, TRI_B_UPPER, st + 3, 1);
+    g.Cin = Q1;
+    g.ldcin = m;
+    gemm_prec<T>(32, OP_N, OP_N, g, Sl, s);
+    gemm<T>(OP_N, OP_N, gq<T>(m, n, n, Q1, m, X, n, Q, ldq, 1.0, 0.0, TRI_B_UPPER, st + 6, 1), s);
+  }
+  if (R != nullptr) {
+    if (!ScalarTraits<T>::is_complex && n >= 512 && L.ks >= kRSlices) {
+      // upper x upper on 32 x 32 tiles with K split kRSlices ways (each
+      // workgroup a short MFMA chain; partials in the Gram slices' space),
+      // then the gated sum, zero below the diagonal
+      GemmArgs g = gq<T>(n, n, n, W, n, G, n, Sl, n, 1.0, 0.0, TRI_A_UPPER | TRI_B_UPPER, st, 1);
+      g.tile = 32;
+      g.kchunk = cdiv(cdiv(n, kRSlices), 16) * 16;
+      g.ksplit = cdiv(n, g.kchunk);
+      g.splitStride = nn;
+      gemm<T>(OP_N, OP_N, g, s);
+      hipLaunchKernelGGL((sum_r_slices_kernel<T>), dim3((unsigned)std::min<int64_t>(cdiv64(nn, 256), 2048)), dim3(256),
+                         0, s, Sl, g.ksplit, n, R, ldr, st);
+    } else {
+      gemm<T>(OP_N, OP_N, gq<T>(n, n, n, W, n, G, n, R, ldr, 1.0, 0.0, TRI_A_UPPER | TRI_B_UPPER, st, 1), s);
+    }
+  }
 }
 
 template <typename T> int* cholqr2_status(char* ws, int m, int n) {

@@ -467,7 +467,7 @@ static void gemm_shape(const GemmArgs& g, hipStream_t s) {
       return;
     }
   }
-  if (!cplx && g.tile != 0 && g.gate == nullptr) {
+  if (!cplx && g.tile != 0) {
     if (g.tile >= 128) launch_gemm_tile<128, 128, 16, 2, 2, OPA, OPB, T>(g, s);
     else if (g.tile >= 64) launch_gemm_tile<64, 64, 16, 2, 2, OPA, OPB, T>(g, s);
     else launch_gemm_tile<32, 32, 16, 2, 2, OPA, OPB, T>(g, s);

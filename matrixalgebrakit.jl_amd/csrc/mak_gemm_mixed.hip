@@ -44,7 +44,8 @@ inline int64_t lo_kpad(int k) { return cdiv64(std::max(k, 0), kLoK) * kLoK; }
 // along k through an LDS transpose.
 template <typename TL>
 __global__ void __launch_bounds__(256) pack_lo_kernel(const double* S, int64_t ld, int kcontig, int R, int K, TL* D,
-                                                      int64_t Kp) {
+                                                      int64_t Kp, const int* gate, int want) {
+  if (gate_closed(gate, want)) return;
   __shared__ float tile[64][65];
   const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -65,10 +66,10 @@ __global__ void __launch_bounds__(256) pack_lo_kernel(const double* S, int64_t l
 
 template <typename TL>
 void pack_lo(const double* S, int64_t ld, int kcontig, int R, int K, TL* D, int64_t Rp, int64_t Kp,
-             hipStream_t s) {
+             hipStream_t s, const int* gate, int want) {
   if (Rp <= 0 || Kp <= 0) return;
   dim3 grid((unsigned)(Kp / 64), (unsigned)(Rp / 64));
-  hipLaunchKernelGGL((pack_lo_kernel<TL>), grid, dim3(256), 0, s, S, ld, kcontig, R, K, D, Kp);
+  hipLaunchKernelGGL((pack_lo_kernel<TL>), grid, dim3(256), 0, s, S, ld, kcontig, R, K, D, Kp, gate, want);
 }
 
 // native 16-byte vector (HIP's uint4 is a struct, which keeps arrays of it in
@@ -83,14 +84,23 @@ struct LoGemmArgs {
   const void* Bp;        // op(B)^T, Np x Kp, k contiguous
   double* C;
   int64_t ldc;
+  const double* Cin;     // the beta term's matrix (C itself by default)
+  int64_t ldcin;
   double alpha, beta;
   int tri;
   int upper;             // only the tiles on or above the diagonal (Hermitian results)
+  const int* gate;       // run only if *gate == gate_want (when non-null)
+  int gate_want;
 };
 
 // Tile order: XCD-contiguous ranges of tiles, except under triangular hints,
 // where the work per tile varies and the round-robin deal over the XCDs keeps
 // them balanced; the index whose K range grows is reversed (longest first).
+// For op(B) upper triangular with an even number of tile columns and at most
+// two workgroups per CU (256 CUs), the first half of the launch takes the
+// column blocks longest first and the second half shortest first, so
+// workgroups b and b + total / 2, which share a CU, carry equal work together.
+constexpr int kLoCus = 256;
 __device__ inline void lo_tile_coords(int& tm, int& tn, int tri) {
   const int gx = gridDim.x, gy = gridDim.y;
   const int total = gx * gy;
@@ -99,7 +109,10 @@ __device__ inline void lo_tile_coords(int& tm, int& tn, int tri) {
   tm = L % gx;
   tn = L / gx;
   if (tri & TRI_A_LOWER) tm = gx - 1 - tm;
-  if (tri & TRI_B_UPPER) tn = gy - 1 - tn;
+  if (tri & TRI_B_UPPER) {
+    if ((gy & 1) == 0 && !(tri & TRI_A_LOWER) && total <= 2 * kLoCus) tn = tn < gy / 2 ? gy - 1 - tn : tn - gy / 2;
+    else tn = gy - 1 - tn;
+  }
 }
 
 template <int PREC>
@@ -109,6 +122,7 @@ __global__ void __launch_bounds__(256, 2) gemm_lo_kernel(LoGemmArgs g) {
   constexpr int TM = 4, TN = 4;              // 16 x 16 blocks per wave (64 x 64)
   __shared__ u32x4 As[kLoRows][9];           // 8 x 16 bytes per row + 16 bytes pad
   __shared__ u32x4 Bs[kLoRows][9];
+  if (gate_closed(g.gate, g.gate_want)) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   int tmi, tni;
@@ -210,7 +224,7 @@ __global__ void __launch_bounds__(256, 2) gemm_lo_kernel(LoGemmArgs g) {
         if (row < g.M && col < g.N && (!g.upper || row <= col)) {
           double* cp = g.C + row + (int64_t)col * g.ldc;
           const double v = alpha * (double)acc[i][j][r];
-          *cp = (beta != 0.0) ? fma(beta, *cp, v) : v;
+          *cp = (beta != 0.0) ? fma(beta, g.Cin[row + (int64_t)col * g.ldcin], v) : v;
         }
       }
 }
@@ -222,18 +236,35 @@ void gemm_lo(int opA, int opB, const GemmArgs& g, void* ws, hipStream_t s) {
   TL* Ap = reinterpret_cast<TL*>(ws);
   TL* Bp = Ap + Mp * Kp;
   // op(A)(i, k): A[i + k lda] (N) or A[k + i lda] (T / C, real data)
-  pack_lo<TL>(reinterpret_cast<const double*>(g.A), g.lda, opA != OP_N, g.M, g.K, Ap, Mp, Kp, s);
+  pack_lo<TL>(reinterpret_cast<const double*>(g.A), g.lda, opA != OP_N, g.M, g.K, Ap, Mp, Kp, s, g.gate,
+              g.gate_want);
   // op(B)^T(j, k) = op(B)(k, j): B[k + j ldb] (N) or B[j + k ldb] (T / C)
-  pack_lo<TL>(reinterpret_cast<const double*>(g.B), g.ldb, opB == OP_N, g.N, g.K, Bp, Np, Kp, s);
+  pack_lo<TL>(reinterpret_cast<const double*>(g.B), g.ldb, opB == OP_N, g.N, g.K, Bp, Np, Kp, s, g.gate,
+              g.gate_want);
   LoGemmArgs a{};
   a.M = g.M; a.N = g.N; a.Kp = Kp;
   a.Ap = Ap; a.Bp = Bp;
   a.C = reinterpret_cast<double*>(g.C); a.ldc = g.ldc;
+  a.Cin = g.Cin != nullptr ? reinterpret_cast<const double*>(g.Cin) : a.C;
+  a.ldcin = g.Cin != nullptr ? g.ldcin : g.ldc;
   a.alpha = g.alpha_re; a.beta = g.beta_re;
   a.tri = g.tri;
   a.upper = g.upperC;
+  a.gate = g.gate;
+  a.gate_want = g.gate_want;
   dim3 grid((unsigned)(Mp / kLoRows), (unsigned)(Np / kLoRows), 1);
   hipLaunchKernelGGL((gemm_lo_kernel<PREC>), grid, dim3(256), 0, s, a);
+}
+
+// Y = X (M x N), when the gate is open.
+__global__ void copy_gated_kernel(const double* X, int64_t ldx, double* Y, int64_t ldy, int M, int N,
+                                  const int* gate, int want) {
+  if (gate_closed(gate, want)) return;
+  const int64_t tot = (int64_t)M * N;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (int64_t)gridDim.x * 256) {
+    const int64_t r = e % M, c = e / M;
+    Y[r + c * ldy] = X[r + c * ldx];
+  }
 }
 
 }  // namespace
@@ -249,7 +280,18 @@ template <typename T> void gemm_prec(int prec, int opA, int opB, GemmArgs g, voi
   if constexpr (ScalarTraits<T>::is_complex) {
     gemm<T>(opA, opB, g, s);
   } else {
-    if (prec == 64 || g.M <= 0 || g.N <= 0) { gemm<T>(opA, opB, g, s); return; }
+    if (g.M <= 0 || g.N <= 0) return;
+    if (prec == 64) {
+      if (g.Cin != nullptr && g.beta_re != 0.0 && g.Cin != g.C) {
+        // the FP64 kernels take the beta term from C: copy Cin there first
+        const int64_t tot = (int64_t)g.M * g.N;
+        hipLaunchKernelGGL(copy_gated_kernel, dim3((unsigned)std::min<int64_t>(cdiv64(tot, 256), 4096)), dim3(256), 0,
+                           s, reinterpret_cast<const double*>(g.Cin), g.ldcin, reinterpret_cast<double*>(g.C), g.ldc,
+                           g.M, g.N, g.gate, g.gate_want);
+      }
+      gemm<T>(opA, opB, g, s);
+      return;
+    }
     if (prec == 16) gemm_lo<16>(opA, opB, g, ws, s);
     else gemm_lo<32>(opA, opB, g, ws, s);
   }

@@ -147,6 +147,10 @@ struct GemmArgs {
   // Tile size (square) forced for the real kernels: 0 = chosen by the
   // dispatcher from the launch size, else 32, 64 or 128.
   int tile;
+  // FP32 / BF16 products only (gemm_prec with prec 32 or 16): when Cin is not
+  // null, C = alpha op(A) op(B) + beta Cin (Cin m x n, ld ldcin; it may be C).
+  const void* Cin;
+  int64_t ldcin;
 };
 
 // C = alpha op(A) op(B) + beta C on FP64 MFMA (real) or 4 real MFMAs (complex).
@@ -179,13 +183,29 @@ template <typename T> void gemm_prec(int prec, int opA, int opB, GemmArgs g, voi
 // null): set to info_offset + j when the leading j x j minor is not positive
 // definite (if still 0).  grid <= 0: default workgroup count.  Returns 0 or a
 // HIP launch error (1); -3 when n is too large.
+//
+// rp (real data only, may be null): also Y = B X (B, Y m x n, column-major),
+// computed in the same launch as X's column blocks become final.  Xc: n
+// columns of scratch, leading dimension ldxc >= n a multiple of 16, 128-byte
+// aligned (a copy of X's final tiles that the product reads through the
+// caches: with whole 128-byte lines per column, no cached line holds a value
+// that is not final yet).
 constexpr int kCholTileMaxNt = 64;
 constexpr int kCholTileMaxN = 32 * kCholTileMaxNt;
+struct CholRightProduct {
+  const double* B;
+  int64_t ldb;
+  double* Y;
+  int64_t ldy;
+  int m;
+  double* Xc;
+  int64_t ldxc;
+};
 size_t chol_tile_flag_ints(int n);
 template <typename T>
 int chol_tile_factor(T* G, int64_t ldg, int n, T* X, int64_t ldx, int* flags, int* info, int info_offset,
-                     hipStream_t s, int grid = 0, long long* trace = nullptr);
-int chol_tile_order_check(int nt);
+                     hipStream_t s, int grid = 0, long long* trace = nullptr, const CholRightProduct* rp = nullptr);
+int chol_tile_order_check(int nt, int ny, int ysplit);
 template <typename T>
 void potrf_upper(T* Z, int64_t ldz, int n, T* Winv, int64_t ldw, T* tmp, int* dinfo, hipStream_t s);
 template <typename T>

@@ -507,7 +507,10 @@ template <typename T> void qr_record(const char* ws, int m, int n, int full, int
   out[1] = h[0] ? h[2] : -1;       // first panel handed to Householder TSQR (-1: none)
   out[2] = st[2];                  // Cholesky info of the whole-matrix path
   out[3] = L.offCq2 != 0;          // the whole-matrix path exists for this shape
-  for (int i = 4; i < 8; ++i) out[i] = st[i];  // ||E||_F of the whole-matrix path (double)
+  out[4] = st[4];                  // ||E||_F of the whole-matrix path (a double in out[4..5])
+  out[5] = st[5];
+  out[6] = st[3];                  // 1: its final product took the FP32 correction
+  out[7] = st[6];                  // 1: its final product ran on FP64
 }
 
 template <typename T>

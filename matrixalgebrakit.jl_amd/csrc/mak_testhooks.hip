@@ -124,9 +124,19 @@ int maktest_chol_tile_trace(void* stream, int n, double* G, int ldg, double* X, 
                             int grid, long long* trace) {
   return mak::chol_tile_factor<double>(G, ldg, n, X, ldx, flags, info, 0, (hipStream_t)stream, grid, trace);
 }
-// Host-only check of the tile kernel's task order for nt tiles per side: the
-// task count when every dependency precedes its task, else -1.
-int maktest_chol_order_check(int nt) { return mak::chol_tile_order_check(nt); }
+// As maktest_chol_tile (real), also Y = B X (B, Y m x n) in the same launch;
+// Xc: ldxc * n doubles of scratch (ldxc >= n a multiple of 16, 128-byte aligned).
+// trace (may be null): as maktest_chol_tile_trace.
+int maktest_chol_tile_product(void* stream, int n, double* G, int ldg, double* X, int ldx, int* flags, int* info,
+                              int grid, int m, const double* B, int ldb, double* Y, int ldy, double* Xc, int ldxc,
+                              long long* trace) {
+  const mak::CholRightProduct rp{B, ldb, Y, ldy, m, Xc, ldxc};
+  return mak::chol_tile_factor<double>(G, ldg, n, X, ldx, flags, info, 0, (hipStream_t)stream, grid, trace, &rp);
+}
+// Host-only check of the tile kernel's task order for nt tiles per side, ny
+// right-product row blocks per pair step and pairs from ysplit on splitting
+// them in two: the task count when every dependency precedes its task, else -1.
+int maktest_chol_order_check(int nt, int ny, int ysplit) { return mak::chol_tile_order_check(nt, ny, ysplit); }
 
 // Blocked factorisation only: A <- compact WY (V below, R above), one 32x32 T
 // per panel written to Tpanels (32 x 32 x npanels); ws from the QR query.

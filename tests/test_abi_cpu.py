@@ -88,27 +88,31 @@ def test_workspace_queries_accept_empty_shapes():
         assert lib.mak_qr_batched_workspace_size(dt, 5, ms, ns, 1, C.byref(b)) == 0 and b.value > 0
 
 
+@pytest.mark.parametrize("ny,ysplit", [(0, 1 << 30), (1, 1 << 30), (128, 1 << 30), (64, 13), (3, 0)])
 @pytest.mark.parametrize("nt", [1, 2, 3, 7, 32, 64])
-def test_chol_tile_task_order(nt):
+def test_chol_tile_task_order(nt, ny, ysplit):
     """The persistent Cholesky kernel's task order (host code in the library):
     every task appears once and every dependency precedes it (the panel's
     step K runs before the tasks of step K), so a workgroup waiting on a task
     it took can always be satisfied.  Tiles T = nt rounded up to even; per
     pair step K with rem = T - 2K - 2 tiles after the pair: rem P2, the next
-    pair's 2 rem - 1 U2, 2K PQ2, (2K + 2) rem UQ2 and the bulk b(b+1)/2 U2
-    (b = rem - 2)."""
+    pair's 2 rem - 1 U2, 2K PQ2, (2K + 2) rem UQ2, the bulk b(b+1)/2 U2
+    (b = rem - 2) and ny right-product tasks (2 ny from pair ysplit on, one
+    per half of the contraction), each after X's column block K is final."""
     lib = _lib.load()
     lib.maktest_chol_order_check.restype = C.c_int
-    lib.maktest_chol_order_check.argtypes = [C.c_int]
+    lib.maktest_chol_order_check.argtypes = [C.c_int, C.c_int, C.c_int]
     T = nt + (nt & 1)
     expect = 0
     for k in range(T // 2):
         rem = T - 2 * k - 2
         b = max(rem - 2, 0)
         expect += rem + (2 * rem - 1 if rem > 0 else 0) + 2 * k + (2 * k + 2) * rem + b * (b + 1) // 2
-    assert lib.maktest_chol_order_check(nt) == expect
-    assert lib.maktest_chol_order_check(0) == -1
-    assert lib.maktest_chol_order_check(65) == -1
+        expect += ny * (2 if k >= ysplit else 1)
+    assert lib.maktest_chol_order_check(nt, ny, ysplit) == expect
+    assert lib.maktest_chol_order_check(0, ny, ysplit) == -1
+    assert lib.maktest_chol_order_check(65, ny, ysplit) == -1
+    assert lib.maktest_chol_order_check(nt, -1, ysplit) == -1
 
 
 def test_qr_q_aliasing_rules():

@@ -162,6 +162,57 @@ def test_chol_tile_any_grid(grid, dt, dev):
     assert fl[2 * nt * nt + 1] == 0
 
 
+@pytest.mark.parametrize("grid", [2, 3, 0])
+@pytest.mark.parametrize("n,m", [(300, 300), (300, 1000), (1000, 4099), (1024, 8192), (37, 70)])
+def test_chol_tile_right_product(n, m, grid, dev):
+    """Y = B X computed inside the Cholesky launch (task by task, as X's
+    column blocks become final) equals B @ X of the launch's own X, for any
+    workgroup count and ragged m, n; the factors are those of the plain
+    launch."""
+    lib = chol_tile_hook()
+    lib.maktest_chol_tile_product.restype = C.c_int
+    lib.maktest_chol_tile_product.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                              C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                              C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+    Bh = O.gaussian(m, n, np.float64, seed=11 + n + m)
+    Zh = Bh.T @ Bh
+    Z = mak.colmajor(torch.from_numpy(np.asfortranarray(Zh)).to(dev))
+    Z0 = Z.clone()
+    X = mak.colmajor(torch.full((n, n), 5.0, dtype=torch.float64, device=dev))
+    X0 = X.clone()
+    B = mak.colmajor(torch.from_numpy(np.asfortranarray(Bh)).to(dev))
+    Y = mak.colmajor(torch.full((m, n), 7.0, dtype=torch.float64, device=dev))
+    ldxc = -(-n // 16) * 16
+    Xc = torch.full((ldxc * n,), 3.0, dtype=torch.float64, device=dev)
+    nfl = lib.maktest_chol_tile_flag_ints(n)
+    flags = torch.full((nfl,), -9, dtype=torch.int32, device=dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    st = mak.api._stream(dev)
+    code = lib.maktest_chol_tile_product(st, n, Z.data_ptr(), Z.stride(1), X.data_ptr(), X.stride(1),
+                                         flags.data_ptr(), info.data_ptr(), grid, m, B.data_ptr(), B.stride(1),
+                                         Y.data_ptr(), Y.stride(1), Xc.data_ptr(), ldxc, None)
+    torch.cuda.synchronize()
+    assert code == 0 and int(info.item()) == 0
+    # the copy holds X (upper part and the diagonal pairs' lower tiles as written)
+    Xcp = Xc.cpu().numpy().reshape(n, ldxc).T[:n, :]
+    assert np.array_equal(np.triu(Xcp), X.cpu().numpy())
+    # misaligned scratch or leading dimension is rejected
+    assert lib.maktest_chol_tile_product(st, n, Z.data_ptr(), Z.stride(1), X.data_ptr(), X.stride(1),
+                                         flags.data_ptr(), info.data_ptr(), grid, m, B.data_ptr(), B.stride(1),
+                                         Y.data_ptr(), Y.stride(1), Xc.data_ptr(), ldxc + 1, None) == -1
+    Xh, Yh = X.cpu().numpy(), Y.cpu().numpy()
+    assert np.linalg.norm(Yh - Bh @ Xh) / np.linalg.norm(Bh @ Xh) < 1e-13
+    # Y = B R^{-1} has orthonormal columns (Z = B^T B, well conditioned)
+    assert np.linalg.norm(Yh.T @ Yh - np.eye(n)) < 1e-11
+    # the factors are the plain launch's
+    flags.fill_(-9)
+    code = lib.maktest_chol_tile(st, 0, n, Z0.data_ptr(), Z0.stride(1), X0.data_ptr(), X0.stride(1),
+                                 flags.data_ptr(), info.data_ptr(), grid)
+    torch.cuda.synchronize()
+    assert code == 0
+    assert np.array_equal(X0.cpu().numpy(), Xh) and np.array_equal(Z0.cpu().numpy(), Z.cpu().numpy())
+
+
 # ------------------------------------------- FP32 / BF16 MFMA products ----
 
 def gemm_prec_hook():

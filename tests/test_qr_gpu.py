@@ -31,7 +31,8 @@ def run_qr_rec(A, dev, full=False, positive=True, wy=False):
     """qr through the C ABI with a workspace kept for the status record
     (maktest_qr_record): returns Q, R and the record
     [whole-matrix CholeskyQR2 taken, first rejected panel column, its
-    Cholesky info, whole-matrix path available, ...]."""
+    Cholesky info, whole-matrix path available, ||E||_F (two ints of a
+    double), FP32 correction taken, FP64 final product taken]."""
     import ctypes as C
     from mak_mi355x import _lib
     At = to_dev(A, dev)
@@ -59,6 +60,11 @@ def run_qr_rec(A, dev, full=False, positive=True, wy=False):
     rec = (C.c_int * 8)()
     assert lib.maktest_qr_record(dt, int(full), m, n, C.c_void_p(ws.data_ptr()), rec) == 0
     return Q.cpu().numpy(), R.cpu().numpy(), list(rec)
+
+
+def rec_norm(rec):
+    """||E||_F of the whole-matrix CholeskyQR2 (record entries 4, 5)."""
+    return float(np.array(rec[4:6], dtype=np.int32).view(np.float64)[0])
 
 
 def run_qr(A, dev, full=False, positive=True, with_r=True):
@@ -225,6 +231,13 @@ def test_qr_whole_matrix_cholqr2_matches_oracle(m, n, dt, dev):
     A = O.gaussian(m, n, dt, seed=7 * m + n)
     Q, R, rec = run_qr_rec(A, dev)
     assert rec[3] == 1 and rec[0] == 1 and rec[2] == 0, rec
+    # one final product; real Gaussian inputs have ||Q1^H Q1 - I|| small
+    # enough for the FP32 correction (complex data: always FP64)
+    assert rec[6] + rec[7] == 1
+    if dt == np.float64:
+        assert (rec[6] == 1) == (rec_norm(rec) <= 1e-10), rec_norm(rec)
+    else:
+        assert rec[7] == 1
     Qo, Ro = O.qr_compact(A, positive=True)
     assert rel(Q, Qo) < 1e-12 and rel(R, Ro) < 1e-12
     assert np.all(np.tril(R, -1) == 0)
@@ -247,6 +260,13 @@ def test_qr_whole_matrix_cholqr2_conditioning(cond, dev):
         assert rec[0] == 1
     if cond >= 1e8:
         assert rec[0] == 0
+    # an accepted call takes exactly one final product, FP32 correction only
+    # for ||E||_F <= 1e-10 (cond 1e5 here: ||E|| ~ 1e-7, the FP64 product)
+    assert rec[6] + rec[7] == rec[0]
+    if rec[0]:
+        assert (rec[6] == 1) == (rec_norm(rec) <= 1e-10), rec_norm(rec)
+    if cond == 1e5:
+        assert rec[7] == 1
     assert rel(Q @ R, A) < 1e-14
     assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 1e-13
     assert np.all(np.diagonal(R) >= 0)
