@@ -374,6 +374,7 @@ struct BatchedQrPlan {
     int rootOff, rootCnt;          // TSQR roots (class 0 node tasks) of tall panels
     int rfOff, rfCnt;              // TSQR root finish tasks
     int exOff, exCnt, exRows;      // TSQR expansion tasks, largest leaf
+    int exNodes;                   // most leaves of a tall panel in the step
     int applyOff, applyCnt;        // geqrf (trans = 1)
   };
   std::vector<Step> steps;
@@ -390,15 +391,23 @@ namespace {
 template <typename T> size_t elem() { return sizeof(T); }
 
 // Task lists for problem set P (pointers into A/Q/R of P) with chunk width nc.
-// Largest panel one node holds, and the TSQR scratch of one tall problem
-// (two leaves + one root): leaf T (2), leaf V1 (2), stacked R (64 x 32),
-// root T, M, U -- in elements.
+// Largest panel one node holds; taller panels are factored by a one-level
+// TSQR tree of leaves of at most leaf_rows<T>() rows each (two for Float64,
+// two to four for ComplexF64, whose taller nodes would not fit a lane's
+// registers) and a root node over their stacked R factors.  The TSQR scratch
+// of one tall problem: leaf T (kMaxLeaves), leaf V1 (kMaxLeaves), stacked R
+// (kMaxLeaves * 32 x 32), root T, M, U -- in elements.
 template <typename T> int node_cap() {
   int c = 0;
   for (int cls = 0; cls < 3; ++cls) c = std::max(c, node_class_rows<T>(cls));
   return c;
 }
-constexpr size_t kTreeElems = 2 * NB * NB + 2 * NB * NB + 2 * NB * NB + 3 * NB * NB;
+template <typename T> int leaf_rows() { return node_cap<T>(); }
+template <typename T> int max_leaves() { return ScalarTraits<T>::is_complex ? 4 : 2; }
+// leaves of a panel of H > node_cap<T>() rows (H <= max_leaves * leaf_rows)
+template <typename T> int leaf_count(int H) { return std::max(2, cdiv(H, leaf_rows<T>())); }
+constexpr int kMaxLeaves = 4;
+constexpr size_t kTreeElems = (size_t)(3 * kMaxLeaves + 3) * NB * NB;
 
 template <typename T>
 void build_tasks(const std::vector<BProb>& P, const std::vector<int>& idx, const std::vector<size_t>& tOff,
@@ -442,16 +451,17 @@ void build_tasks(const std::vector<BProb>& P, const std::vector<int>& idx, const
           node.push_back(t);
         } else {
           T* tb = treeBase + treeOff[i];
-          const int H1 = 2 * nbc;
-          for (int leaf = 0; leaf < 2; ++leaf) {
-            const int r0 = (leaf * H) / 2, r1 = ((leaf + 1) * H) / 2;
+          const int L = leaf_count<T>(H);
+          const int H1 = L * nbc;
+          for (int leaf = 0; leaf < L; ++leaf) {
+            const int r0 = (int)(((int64_t)leaf * H) / L), r1 = (int)(((int64_t)(leaf + 1) * H) / L);
             if (cls_of(r1 - r0) != cls) continue;
             NodeTask t{};
             t.X = Ap + r0;
             t.ldx = p.lda;
-            t.Tout = tb + (size_t)leaf * NB * NB;                 // leaf T
-            t.V1 = tb + (size_t)(2 + leaf) * NB * NB;            // leaf V1
-            t.Rn = tb + (size_t)4 * NB * NB + (size_t)leaf * nbc;  // stacked R (ld H1)
+            t.Tout = tb + (size_t)leaf * NB * NB;                                 // leaf T
+            t.V1 = tb + (size_t)(kMaxLeaves + leaf) * NB * NB;                   // leaf V1
+            t.Rn = tb + (size_t)2 * kMaxLeaves * NB * NB + (size_t)leaf * nbc;  // stacked R (ld H1)
             t.ldr = H1;
             t.H = r1 - r0;
             t.nbc = nbc;
@@ -466,6 +476,7 @@ void build_tasks(const std::vector<BProb>& P, const std::vector<int>& idx, const
     steps[s].rfOff = (int)rf.size();
     steps[s].exOff = (int)ex.size();
     steps[s].exRows = 0;
+    steps[s].exNodes = 0;
     for (int i : idx) {
       const BProb& p = P[i];
       const int k = std::min(p.m, p.n);
@@ -473,16 +484,17 @@ void build_tasks(const std::vector<BProb>& P, const std::vector<int>& idx, const
       const int H = p.m - j0;
       if (H <= cap) continue;
       const int nbc = std::min(NB, k - j0);
-      const int H1 = 2 * nbc;
+      const int L = leaf_count<T>(H);
+      const int H1 = L * nbc;
       T* A = reinterpret_cast<T*>(p.A);
       T* Ap = A + j0 + (int64_t)j0 * p.lda;
       T* tb = treeBase + treeOff[i];
       T* T0 = tb;
-      T* V1 = tb + (size_t)2 * NB * NB;
-      T* X1 = tb + (size_t)4 * NB * NB;
-      T* T1 = tb + (size_t)6 * NB * NB;
-      T* M = tb + (size_t)7 * NB * NB;
-      T* U = tb + (size_t)8 * NB * NB;
+      T* V1 = tb + (size_t)kMaxLeaves * NB * NB;
+      T* X1 = tb + (size_t)2 * kMaxLeaves * NB * NB;
+      T* T1 = tb + (size_t)3 * kMaxLeaves * NB * NB;
+      T* M = T1 + (size_t)NB * NB;
+      T* U = M + (size_t)NB * NB;
       NodeTask t{};
       t.X = X1;
       t.ldx = H1;
@@ -497,11 +509,12 @@ void build_tasks(const std::vector<BProb>& P, const std::vector<int>& idx, const
       r.Apanel = Ap; r.lda = p.lda;
       rf.push_back(r);
       TsqrExpandArgs e{};
-      e.H = H; e.nodes = 2; e.chunk = 0; e.nbc = nbc;
+      e.H = H; e.nodes = L; e.chunk = 0; e.nbc = nbc;
       e.X = Ap; e.ldx = p.lda; e.V1base = V1; e.v1stride = NB * NB; e.v1ld = NB;
       e.Tnodes = T0; e.Mtop = M; e.Xtop = X1; e.ldxtop = H1; e.Umat = U;
       ex.push_back(e);
-      steps[s].exRows = std::max(steps[s].exRows, cdiv(H, 2) + 1);
+      steps[s].exRows = std::max(steps[s].exRows, cdiv(H, L) + 1);
+      steps[s].exNodes = std::max(steps[s].exNodes, L);
     }
     steps[s].rootCnt = (int)node.size() - steps[s].rootOff;
     steps[s].rfCnt = (int)rf.size() - steps[s].rfOff;
@@ -588,7 +601,7 @@ int enqueue(const BatchedQrPlan& pl, const std::vector<BatchedQrPlan::Step>& ste
     if (steps[st].rootCnt) {
       launch_geqr2_node_tasks<T>(dnode + steps[st].rootOff, steps[st].rootCnt, 0, s);
       launch_tsqr_root_finish_tasks<T>(drf + steps[st].rfOff, steps[st].rfCnt, s);
-      launch_tsqr_expand_tasks<T>(dex + steps[st].exOff, steps[st].exCnt, 2, steps[st].exRows, s);
+      launch_tsqr_expand_tasks<T>(dex + steps[st].exOff, steps[st].exCnt, steps[st].exNodes, steps[st].exRows, s);
       launches += 3;
     }
     if (steps[st].applyCnt) {
@@ -657,7 +670,7 @@ Sizes plan_sizes(const std::vector<BProb>& P, const std::vector<int>& batched, c
     }
   }
   z.tblocks = al256(tb * sizeof(T));
-  z.nodeTasks = al256(sizeof(NodeTask) * std::max<size_t>(nt + 2 * tall, 1));
+  z.nodeTasks = al256(sizeof(NodeTask) * std::max<size_t>(nt + (size_t)kMaxLeaves * tall, 1));
   z.tree = al256(sizeof(T) * kTreeElems * std::max<size_t>(ntall, 1));
   z.rfTasks = al256(sizeof(TsqrRootArgs) * std::max<size_t>(tall, 1));
   z.exTasks = al256(sizeof(TsqrExpandArgs) * std::max<size_t>(tall, 1));
@@ -679,7 +692,7 @@ template <typename T>
 void split_problems(const std::vector<BProb>& P, std::vector<int>& batched, std::vector<int>& fallback) {
   batched.clear();
   fallback.clear();
-  const int maxrows = 2 * node_cap<T>();  // two TSQR leaves per panel at most
+  const int maxrows = max_leaves<T>() * leaf_rows<T>();  // the one-level TSQR tree's reach
   for (int i = 0; i < (int)P.size(); ++i) {
     if (std::min(P[i].m, P[i].n) == 0) {
       fallback.push_back(i);  // empty factorisations: the driver's identity / empty-R path

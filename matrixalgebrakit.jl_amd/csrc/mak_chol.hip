@@ -136,7 +136,8 @@ struct CholTileArgs {
   int* info;      // optional: receives ctl[2] + info_offset on failure (when *info == 0)
   int info_offset;
   long long* trace;  // optional (tests): wall-clock stamps of the panel's iterations, 8 per iteration
-                     // (slot 7: the end of the pair's last right-product task)
+                     // (slot 7: the end of the pair's last right-product task), then
+                     // from entry 256 the start and end of every right-product task
   // the right product Y = B X (ny = 0: none)
   const double* B;
   int64_t ldb;
@@ -785,11 +786,17 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
       case CT_Y: {  // rows of Y's column block K, once X's column block K is final
         wait_ge(&a.xdone[K], 2 * K + 1, ctl);
         if (!inputs_ready(ctl)) break;
+        const long long t_start = (long long)__builtin_amdgcn_s_memrealtime();
         if constexpr (!ScalarTraits<T>::is_complex)
           y_task(a, K, tk.j % a.ny, K >= a.ysplit ? 1 + tk.j / a.ny : 0, reinterpret_cast<double*>(&lds[0][0][0]));
-        if (a.trace != nullptr && threadIdx.x == 0)  // the pair's last right-product task to finish
-          atomicMax(reinterpret_cast<unsigned long long*>(a.trace + K * 8 + 7),
-                    (unsigned long long)__builtin_amdgcn_s_memrealtime());
+        if (a.trace != nullptr && threadIdx.x == 0) {
+          const long long t_end = (long long)__builtin_amdgcn_s_memrealtime();
+          // the pair's last right-product task to finish; every task's start
+          // and end after the panel stamps (tests: 8 * 32 + 2 * ntasks entries)
+          atomicMax(reinterpret_cast<unsigned long long*>(a.trace + K * 8 + 7), (unsigned long long)t_end);
+          a.trace[8 * 32 + 2 * t] = t_start;
+          a.trace[8 * 32 + 2 * t + 1] = t_end;
+        }
         break;
       }
       default:

@@ -375,19 +375,31 @@ __device__ __forceinline__ void node2_step(T (&a)[KR][2], Node2Lds<T, NW, 4 * KR
     }
   }
   wave_lds_sync();
-  T x[KR];
+  // With eight waves a lane has half the registers: x is not kept but read
+  // from LDS where it is used (this wave's copy, unchanged until its next
+  // step), so the column data alone stays in registers.
+  constexpr bool kXLds = NW * KR > 64;
+  T x[kXLds ? 1 : KR];
+  auto xk = [&](int k) -> T {
+    if constexpr (kXLds) return S.xs[w][g * KR + k];
+    else return x[k];
+  };
+  if constexpr (!kXLds) {
 #pragma unroll
-  for (int k = 0; k < KR; ++k) x[k] = S.xs[w][g * KR + k];
+    for (int k = 0; k < KR; ++k) x[k] = S.xs[w][g * KR + k];
+  }
 
   // 2. partial products conj(x)^T a over this lane's rows
   T p0a = s_zero(T()), p0b = s_zero(T()), p1a = s_zero(T()), p1b = s_zero(T());
 #pragma unroll
   for (int k = 0; k < KR; k += 2) {
-    p0a = s_cfma(x[k], a[k][0], p0a);
-    p1a = s_cfma(x[k], a[k][1], p1a);
+    const T x0 = xk(k);
+    p0a = s_cfma(x0, a[k][0], p0a);
+    p1a = s_cfma(x0, a[k][1], p1a);
     if (k + 1 < KR) {
-      p0b = s_cfma(x[k + 1], a[k + 1][0], p0b);
-      p1b = s_cfma(x[k + 1], a[k + 1][1], p1b);
+      const T x1 = xk(k + 1);
+      p0b = s_cfma(x1, a[k + 1][0], p0b);
+      p1b = s_cfma(x1, a[k + 1][1], p1b);
     }
   }
   // 3. sum the four row groups: lanes < 32 -> column cl, lanes >= 32 -> cl + 16
@@ -440,14 +452,14 @@ __device__ __forceinline__ void node2_step(T (&a)[KR][2], Node2Lds<T, NW, 4 * KR
   if (w == 0) {
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
-      const T coef = (k < 8) ? sel(d + 4 * k == 0, s_one(T()), s_mul(scale, x[k])) : s_mul(scale, x[k]);
+      const T coef = (k < 8) ? sel(d + 4 * k == 0, s_one(T()), s_mul(scale, xk(k))) : s_mul(scale, xk(k));
       a[k][0] = s_sub(a[k][0], s_mul(coef, g0));
       a[k][1] = s_sub(a[k][1], s_mul(coef, g1));
     }
   } else {
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
-      const T v = s_mul(scale, x[k]);
+      const T v = s_mul(scale, xk(k));
       a[k][0] = s_sub(a[k][0], s_mul(v, g0));
       a[k][1] = s_sub(a[k][1], s_mul(v, g1));
     }
@@ -822,6 +834,7 @@ __device__ __forceinline__ void tsqr_expand_body(const TsqrExpandArgs& args, con
   const int lane = tid & 63;
   const int w = tid >> 6;
   const int nbc = args.nbc;
+  if (node >= args.nodes) return;  // a task-list launch covers the most nodes of any task
   int r0, r1;
   node_rows(node, args.nodes, args.H, args.chunk, r0, r1);
   const int hnode = r1 - r0;
@@ -991,10 +1004,13 @@ void launch_geqr2_node(const Geqr2NodeArgs& a, int leaf_rows_max, hipStream_t s)
   (void)leaf_rows_max;
 }
 
+// Rows of the node size classes of the task-list launches (0: none).  A
+// ComplexF64 node of more than 256 rows does not fit a lane's registers (it
+// spills to scratch), so complex panels beyond 256 rows go to TSQR leaves.
 template <typename T> int node_class_rows(int cls) {
   if (cls == 0) return 256;
-  if (cls == 1) return 512;
-  return ScalarTraits<T>::is_complex ? 0 : 1024;
+  if (ScalarTraits<T>::is_complex) return 0;
+  return cls == 1 ? 512 : 1024;
 }
 
 template <typename T> void launch_geqr2_node_tasks(const NodeTask* tasks, int ntasks, int cls, hipStream_t s) {

@@ -126,7 +126,9 @@ int maktest_chol_tile_trace(void* stream, int n, double* G, int ldg, double* X, 
 }
 // As maktest_chol_tile (real), also Y = B X (B, Y m x n) in the same launch;
 // Xc: ldxc * n doubles of scratch (ldxc >= n a multiple of 16, 128-byte aligned).
-// trace (may be null): as maktest_chol_tile_trace.
+// trace (may be null): as maktest_chol_tile_trace, and from entry 256 the
+// start and end stamps of every right-product task (by queue index: 256 + 2 *
+// the total task count entries).
 int maktest_chol_tile_product(void* stream, int n, double* G, int ldg, double* X, int ldx, int* flags, int* info,
                               int grid, int m, const double* B, int ldb, double* Y, int ldy, double* Xc, int ldxc,
                               long long* trace) {

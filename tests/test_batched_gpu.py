@@ -43,9 +43,10 @@ def test_batched_edge_shapes_match_oracle(dt, dev):
         Qo, Ro = O.qr_compact(A)
         assert rel(Q.cpu().numpy(), Qo) < 1e-12, (m, n, rel(Q.cpu().numpy(), Qo))
         assert rel(R.cpu().numpy(), Ro) < 1e-12, (m, n)
-    # panels taller than one node use a two-leaf TSQR inside the batch; blocks
-    # taller than two nodes (1024 rows ComplexF64, 2048 Float64) and empty
-    # blocks run through the single-matrix driver
+    # panels taller than one node use a one-level TSQR inside the batch (two
+    # leaves of up to 1024 rows Float64, two to four of up to 256 rows
+    # ComplexF64); blocks taller than that (1024 rows ComplexF64, 2048
+    # Float64) and empty blocks run through the single-matrix driver
     tall = 1024 if dt == np.complex128 else 2048
     assert info["fallback_problems"] == sum(1 for m, n in EDGE if m > tall or min(m, n) == 0)
     plan.close()
