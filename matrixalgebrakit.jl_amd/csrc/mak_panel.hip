@@ -375,10 +375,11 @@ __device__ __forceinline__ void node2_step(T (&a)[KR][2], Node2Lds<T, NW, 4 * KR
     }
   }
   wave_lds_sync();
-  // With eight waves a lane has half the registers: x is not kept but read
-  // from LDS where it is used (this wave's copy, unchanged until its next
-  // step), so the column data alone stays in registers.
-  constexpr bool kXLds = NW * KR > 64;
+  // With eight waves (or complex data) a lane has too few registers for a
+  // second copy: x is not kept but read from LDS where it is used (this
+  // wave's copy, unchanged until its next step), so the column data alone
+  // stays in registers.
+  constexpr bool kXLds = NW * KR > 64 || ScalarTraits<T>::is_complex;
   T x[kXLds ? 1 : KR];
   auto xk = [&](int k) -> T {
     if constexpr (kXLds) return S.xs[w][g * KR + k];
@@ -1016,8 +1017,8 @@ template <typename T> int node_class_rows(int cls) {
 template <typename T> void launch_geqr2_node_tasks(const NodeTask* tasks, int ntasks, int cls, hipStream_t s) {
   if (ntasks <= 0) return;
   if constexpr (ScalarTraits<T>::is_complex) {
-    if (cls == 0) hipLaunchKernelGGL((geqr2_node2_tasks_kernel<T, 4, 16>), dim3(ntasks), dim3(256), 0, s, tasks);
-    else hipLaunchKernelGGL((geqr2_node2_tasks_kernel<T, 8, 16>), dim3(ntasks), dim3(512), 0, s, tasks);
+    (void)cls;  // one class: 256 rows
+    hipLaunchKernelGGL((geqr2_node2_tasks_kernel<T, 4, 16>), dim3(ntasks), dim3(256), 0, s, tasks);
   } else {
     if (cls == 0) hipLaunchKernelGGL((geqr2_node2_tasks_kernel<T, 4, 16>), dim3(ntasks), dim3(256), 0, s, tasks);
     else if (cls == 1) hipLaunchKernelGGL((geqr2_node2_tasks_kernel<T, 4, 32>), dim3(ntasks), dim3(256), 0, s, tasks);
