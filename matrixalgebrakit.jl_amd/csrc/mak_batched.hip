@@ -17,10 +17,18 @@
 // grid size come from a small autotuner that times every candidate on a
 // scratch copy of the plan's shapes and caches the winner per shape set.
 //
-// Panels taller than one node (512 rows ComplexF64, 1024 Float64) are factored
-// by a two-leaf TSQR tree with Householder reconstruction (the single-matrix
-// panel kernels, launched over task lists); problems taller than two nodes run
-// through the single-matrix driver inside the same graph.
+// Every panel of at least 64 rows is first factored by CholeskyQR2 with
+// Householder reconstruction in one workgroup (cholqr2_panel_tasks_kernel:
+// four MFMA passes over the panel instead of 32 sequential column steps); Q1
+// goes to the problem's Q columns of that panel, unused until Q is formed, so
+// the panel itself is only read until the conditioning check passes.  The
+// Householder launches below are gated per task on that check and run only
+// for rejected panels (and for panels under 64 rows).
+//
+// Panels taller than one node (256 rows ComplexF64, 1024 Float64) are factored
+// by a one-level TSQR tree with Householder reconstruction (the single-matrix
+// panel kernels, launched over task lists); problems taller than the tree's
+// reach run through the single-matrix driver inside the same graph.
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -364,7 +372,7 @@ struct BatchedQrPlan {
   char* ws = nullptr;
   size_t ws_bytes = 0;
   size_t offProbs = 0, offT = 0, offNode = 0, offApply = 0, offFallback = 0, offArena = 0, arenaBytes = 0;
-  size_t offTree = 0, offRf = 0, offEx = 0;
+  size_t offTree = 0, offRf = 0, offEx = 0, offCq = 0, offSt = 0;
   size_t fallbackBytes = 0;
   std::vector<size_t> tOff;        // per problem: element offset of its T blocks
   std::vector<size_t> treeOff;     // per problem with panels taller than a node: TSQR scratch offset
@@ -375,10 +383,12 @@ struct BatchedQrPlan {
     int rfOff, rfCnt;              // TSQR root finish tasks
     int exOff, exCnt, exRows;      // TSQR expansion tasks, largest leaf
     int exNodes;                   // most leaves of a tall panel in the step
+    int cqOff, cqCnt;              // CholeskyQR2 panel tasks (run first)
     int applyOff, applyCnt;        // geqrf (trans = 1)
   };
   std::vector<Step> steps;
   std::vector<int> qOff, qCnt;     // orgqr per step (trans = 0)
+  std::vector<int> cqIdx;          // status words of the CholeskyQR2 panel tasks (test hook)
   int nc = 64, gridMult = 2, cus = 256;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -409,18 +419,26 @@ template <typename T> int leaf_count(int H) { return std::max(2, cdiv(H, leaf_ro
 constexpr int kMaxLeaves = 4;
 constexpr size_t kTreeElems = (size_t)(3 * kMaxLeaves + 3) * NB * NB;
 
+// Panels of at least this many rows get a CholeskyQR2 task.
+constexpr int kCqMinRows = 64;
+
 template <typename T>
 void build_tasks(const std::vector<BProb>& P, const std::vector<int>& idx, const std::vector<size_t>& tOff,
-                 const std::vector<size_t>& treeOff, const T* Tbase, T* treeBase, int nc, std::vector<NodeTask>& node,
-                 std::vector<TsqrRootArgs>& rf, std::vector<TsqrExpandArgs>& ex, std::vector<ApplyTask>& apply,
+                 const std::vector<size_t>& treeOff, const T* Tbase, T* treeBase, int* stBase, int nc,
+                 std::vector<CholQrTask>& cq, std::vector<NodeTask>& node, std::vector<TsqrRootArgs>& rf,
+                 std::vector<TsqrExpandArgs>& ex, std::vector<ApplyTask>& apply,
                  std::vector<BatchedQrPlan::Step>& steps, std::vector<int>& qOff, std::vector<int>& qCnt) {
   int S = 0;
   for (int i : idx) S = std::max(S, cdiv(std::min(P[i].m, P[i].n), NB));
   steps.assign(S, BatchedQrPlan::Step{});
+  cq.clear();
   node.clear();
   rf.clear();
   ex.clear();
   apply.clear();
+  // the status word of panel s of problem i (one per T block)
+  auto status_of = [&](int i, int s) { return stBase + tOff[i] / ((size_t)NB * NB) + s; };
+  auto gate_of = [&](int i, int s, int H) -> const int* { return H >= kCqMinRows ? status_of(i, s) : nullptr; };
   const int cap = node_cap<T>();
   auto cls_of = [&](int H) {
     for (int cls = 0; cls < 3; ++cls)
@@ -429,6 +447,26 @@ void build_tasks(const std::vector<BProb>& P, const std::vector<int>& idx, const
   };
   for (int s = 0; s < S; ++s) {
     const int j0 = s * NB;
+    // CholeskyQR2 tasks of the step's panels of at least kCqMinRows rows
+    steps[s].cqOff = (int)cq.size();
+    for (int i : idx) {
+      const BProb& p = P[i];
+      const int k = std::min(p.m, p.n);
+      if (j0 >= k) continue;
+      const int H = p.m - j0;
+      if (H < kCqMinRows) continue;
+      CholQrTask t{};
+      t.X = reinterpret_cast<T*>(p.A) + j0 + (int64_t)j0 * p.lda;
+      t.ldx = p.lda;
+      t.Q1 = reinterpret_cast<T*>(p.Q) + j0 + (int64_t)j0 * p.ldq;
+      t.ldq = p.ldq;
+      t.Tout = const_cast<T*>(Tbase) + tOff[i] + (size_t)s * NB * NB;
+      t.status = status_of(i, s);
+      t.H = H;
+      t.nbc = std::min(NB, k - j0);
+      cq.push_back(t);
+    }
+    steps[s].cqCnt = (int)cq.size() - steps[s].cqOff;
     // leaves of tall panels and whole panels, grouped by row class
     for (int cls = 0; cls < 3; ++cls) {
       steps[s].nodeOff[cls] = (int)node.size();
@@ -446,6 +484,7 @@ void build_tasks(const std::vector<BProb>& P, const std::vector<int>& idx, const
           t.X = Ap;
           t.ldx = p.lda;
           t.Tout = const_cast<T*>(Tbase) + tOff[i] + (size_t)s * NB * NB;
+          t.gate = gate_of(i, s, H);
           t.H = H;
           t.nbc = nbc;
           node.push_back(t);
@@ -463,6 +502,7 @@ void build_tasks(const std::vector<BProb>& P, const std::vector<int>& idx, const
             t.V1 = tb + (size_t)(kMaxLeaves + leaf) * NB * NB;                   // leaf V1
             t.Rn = tb + (size_t)2 * kMaxLeaves * NB * NB + (size_t)leaf * nbc;  // stacked R (ld H1)
             t.ldr = H1;
+            t.gate = gate_of(i, s, H);
             t.H = r1 - r0;
             t.nbc = nbc;
             node.push_back(t);
@@ -495,10 +535,12 @@ void build_tasks(const std::vector<BProb>& P, const std::vector<int>& idx, const
       T* T1 = tb + (size_t)3 * kMaxLeaves * NB * NB;
       T* M = T1 + (size_t)NB * NB;
       T* U = M + (size_t)NB * NB;
+      const int* gate = gate_of(i, s, H);
       NodeTask t{};
       t.X = X1;
       t.ldx = H1;
       t.Tout = T1;
+      t.gate = gate;
       t.H = H1;
       t.nbc = nbc;
       node.push_back(t);
@@ -507,11 +549,13 @@ void build_tasks(const std::vector<BProb>& P, const std::vector<int>& idx, const
       r.nchain = 1; r.chainV1[0] = V1; r.chainLd[0] = NB; r.chainT[0] = T0;
       r.Mtop = M; r.Tout = const_cast<T*>(Tbase) + tOff[i] + (size_t)s * NB * NB; r.Uout = U;
       r.Apanel = Ap; r.lda = p.lda;
+      r.gate = gate; r.gate_want = 1;
       rf.push_back(r);
       TsqrExpandArgs e{};
       e.H = H; e.nodes = L; e.chunk = 0; e.nbc = nbc;
       e.X = Ap; e.ldx = p.lda; e.V1base = V1; e.v1stride = NB * NB; e.v1ld = NB;
       e.Tnodes = T0; e.Mtop = M; e.Xtop = X1; e.ldxtop = H1; e.Umat = U;
+      e.gate = gate; e.gate_want = 1;
       ex.push_back(e);
       steps[s].exRows = std::max(steps[s].exRows, cdiv(H, L) + 1);
       steps[s].exNodes = std::max(steps[s].exNodes, L);
@@ -589,10 +633,14 @@ size_t count_apply(const std::vector<BProb>& P, const std::vector<int>& idx, int
 template <typename T>
 int enqueue(const BatchedQrPlan& pl, const std::vector<BatchedQrPlan::Step>& steps, const std::vector<int>& qOff,
             const std::vector<int>& qCnt, const BProb* dprobs, const std::vector<BProb>& P, int nprob_batched,
-            const NodeTask* dnode, const TsqrRootArgs* drf, const TsqrExpandArgs* dex, const ApplyTask* dapply,
-            int nc, int grid, int positive, bool with_fallback, hipStream_t s) {
+            const CholQrTask* dcq, const NodeTask* dnode, const TsqrRootArgs* drf, const TsqrExpandArgs* dex,
+            const ApplyTask* dapply, int nc, int grid, int positive, bool with_fallback, hipStream_t s) {
   int launches = 0;
   for (size_t st = 0; st < steps.size(); ++st) {
+    if (steps[st].cqCnt) {
+      launch_cholqr2_panel_tasks<T>(dcq + steps[st].cqOff, steps[st].cqCnt, s);
+      ++launches;
+    }
     for (int cls = 0; cls < 3; ++cls) {
       if (steps[st].nodeCnt[cls] == 0) continue;
       launch_geqr2_node_tasks<T>(dnode + steps[st].nodeOff[cls], steps[st].nodeCnt[cls], cls, s);
@@ -648,7 +696,7 @@ uint64_t shape_hash(int dtype, const std::vector<BProb>& P, const std::vector<in
 }
 
 struct Sizes {
-  size_t probs, tblocks, nodeTasks, applyTasks, fallback, arena, tree, rfTasks, exTasks;
+  size_t probs, tblocks, nodeTasks, applyTasks, fallback, arena, tree, rfTasks, exTasks, cqTasks, status;
 };
 
 template <typename T>
@@ -671,6 +719,8 @@ Sizes plan_sizes(const std::vector<BProb>& P, const std::vector<int>& batched, c
   }
   z.tblocks = al256(tb * sizeof(T));
   z.nodeTasks = al256(sizeof(NodeTask) * std::max<size_t>(nt + (size_t)kMaxLeaves * tall, 1));
+  z.cqTasks = al256(sizeof(CholQrTask) * std::max<size_t>(nt, 1));
+  z.status = al256(sizeof(int) * std::max<size_t>(nt, 1));
   z.tree = al256(sizeof(T) * kTreeElems * std::max<size_t>(ntall, 1));
   z.rfTasks = al256(sizeof(TsqrRootArgs) * std::max<size_t>(tall, 1));
   z.exTasks = al256(sizeof(TsqrExpandArgs) * std::max<size_t>(tall, 1));
@@ -707,10 +757,12 @@ void split_problems(const std::vector<BProb>& P, std::vector<int>& batched, std:
   });
 }
 
-bool upload_tasks(NodeTask* dnode, const std::vector<NodeTask>& node, TsqrRootArgs* drf,
-                  const std::vector<TsqrRootArgs>& rf, TsqrExpandArgs* dex, const std::vector<TsqrExpandArgs>& ex,
-                  ApplyTask* dapply, const std::vector<ApplyTask>& apply) {
+bool upload_tasks(CholQrTask* dcq, const std::vector<CholQrTask>& cq, NodeTask* dnode,
+                  const std::vector<NodeTask>& node, TsqrRootArgs* drf, const std::vector<TsqrRootArgs>& rf,
+                  TsqrExpandArgs* dex, const std::vector<TsqrExpandArgs>& ex, ApplyTask* dapply,
+                  const std::vector<ApplyTask>& apply) {
   bool ok = true;
+  if (!cq.empty()) ok &= hipMemcpy(dcq, cq.data(), sizeof(CholQrTask) * cq.size(), hipMemcpyHostToDevice) == hipSuccess;
   if (!node.empty()) ok &= hipMemcpy(dnode, node.data(), sizeof(NodeTask) * node.size(), hipMemcpyHostToDevice) == hipSuccess;
   if (!rf.empty()) ok &= hipMemcpy(drf, rf.data(), sizeof(TsqrRootArgs) * rf.size(), hipMemcpyHostToDevice) == hipSuccess;
   if (!ex.empty()) ok &= hipMemcpy(dex, ex.data(), sizeof(TsqrExpandArgs) * ex.size(), hipMemcpyHostToDevice) == hipSuccess;
@@ -734,6 +786,11 @@ int plan_create(BatchedQrPlan* pl, int count, const int* m, const int* n, void* 
     if (R && R[i] && ldr[i] < std::max(1, k)) return -11;
     if ((int64_t)m[i] * n[i] > 0 && A[i] == nullptr) return -6;
     if ((int64_t)m[i] * k > 0 && Q[i] == nullptr) return -8;
+    // Q is written while A is still read (and holds scratch during the factorisation)
+    if (ranges_overlap(reinterpret_cast<const char*>(A[i]), (int64_t)lda[i] * (int64_t)sizeof(T), m[i] * (int)sizeof(T),
+                       n[i], reinterpret_cast<const char*>(Q[i]), (int64_t)ldq[i] * (int64_t)sizeof(T),
+                       m[i] * (int)sizeof(T), k))
+      return -8;
     pl->probs[i] = BProb{A[i], lda[i], Q[i], ldq[i], R ? R[i] : nullptr, R ? ldr[i] : 1, m[i], n[i]};
   }
   split_problems<T>(pl->probs, pl->batched, pl->fallback);
@@ -750,6 +807,8 @@ int plan_create(BatchedQrPlan* pl, int count, const int* m, const int* n, void* 
   pl->offTree = off; off += z.tree;
   pl->offRf = off; off += z.rfTasks;
   pl->offEx = off; off += z.exTasks;
+  pl->offCq = off; off += z.cqTasks;
+  pl->offSt = off; off += z.status;
   if (work == nullptr || work_bytes < off) return -14;
   pl->ws = reinterpret_cast<char*>(work);
   pl->ws_bytes = work_bytes;
@@ -772,6 +831,9 @@ int plan_create(BatchedQrPlan* pl, int count, const int* m, const int* n, void* 
   T* treeBase = reinterpret_cast<T*>(pl->ws + pl->offTree);
   TsqrRootArgs* drf = reinterpret_cast<TsqrRootArgs*>(pl->ws + pl->offRf);
   TsqrExpandArgs* dex = reinterpret_cast<TsqrExpandArgs*>(pl->ws + pl->offEx);
+  CholQrTask* dcq = reinterpret_cast<CholQrTask*>(pl->ws + pl->offCq);
+  int* stBase = reinterpret_cast<int*>(pl->ws + pl->offSt);
+  std::vector<CholQrTask> cq;
   BProb* dprobs = reinterpret_cast<BProb*>(pl->ws + pl->offProbs);
   BProb* dprobs_arena = dprobs + std::max<size_t>(pl->probs.size(), 1);
   NodeTask* dnode = reinterpret_cast<NodeTask*>(pl->ws + pl->offNode);
@@ -823,15 +885,15 @@ int plan_create(BatchedQrPlan* pl, int count, const int* m, const int* n, void* 
     hipEventCreate(&e1);
     double best = 1e30;
     for (auto c : cands) {
-      build_tasks<T>(AP, pl->batched, pl->tOff, pl->treeOff, Tbase, treeBase, c.first, node, rf, ex, apply, steps,
-                     qOff, qCnt);
-      upload_tasks(dnode, node, drf, rf, dex, ex, dapply, apply);
+      build_tasks<T>(AP, pl->batched, pl->tOff, pl->treeOff, Tbase, treeBase, stBase, c.first, cq, node, rf, ex,
+                     apply, steps, qOff, qCnt);
+      upload_tasks(dcq, cq, dnode, node, drf, rf, dex, ex, dapply, apply);
       float ms = 0.f;
       for (int rep = 0; rep < 2; ++rep) {
         hipLaunchKernelGGL((arena_fill_kernel<T>), dim3(1024), dim3(256), 0, ts, ar, (int64_t)(pl->arenaBytes / sizeof(T)));
         hipEventRecord(e0, ts);
-        enqueue<T>(*pl, steps, qOff, qCnt, dprobs_arena, AP, (int)pl->batched.size(), dnode, drf, dex, dapply,
-                   c.first, c.second * pl->cus, pl->positive, false, ts);
+        enqueue<T>(*pl, steps, qOff, qCnt, dprobs_arena, AP, (int)pl->batched.size(), dcq, dnode, drf, dex,
+                   dapply, c.first, c.second * pl->cus, pl->positive, false, ts);
         hipEventRecord(e1, ts);
         hipEventSynchronize(e1);
         hipEventElapsedTime(&ms, e0, e1);
@@ -850,9 +912,11 @@ int plan_create(BatchedQrPlan* pl, int count, const int* m, const int* n, void* 
     g_tune_cache[key] = {pl->nc, pl->gridMult};
   }
   // ---- final task lists on the caller's matrices
-  build_tasks<T>(pl->probs, pl->batched, pl->tOff, pl->treeOff, Tbase, treeBase, pl->nc, node, rf, ex, apply, steps,
-                 qOff, qCnt);
-  if (!upload_tasks(dnode, node, drf, rf, dex, ex, dapply, apply)) return 1;
+  build_tasks<T>(pl->probs, pl->batched, pl->tOff, pl->treeOff, Tbase, treeBase, stBase, pl->nc, cq, node, rf, ex,
+                 apply, steps, qOff, qCnt);
+  if (!upload_tasks(dcq, cq, dnode, node, drf, rf, dex, ex, dapply, apply)) return 1;
+  pl->cqIdx.clear();
+  for (const CholQrTask& t : cq) pl->cqIdx.push_back((int)(t.status - stBase));
   pl->steps = steps;
   pl->qOff = qOff;
   pl->qCnt = qCnt;
@@ -862,8 +926,8 @@ int plan_create(BatchedQrPlan* pl, int count, const int* m, const int* n, void* 
   int st = 0;
   if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) st = 1;
   if (!st) {
-    pl->launches = enqueue<T>(*pl, steps, qOff, qCnt, dprobs, pl->probs, (int)pl->batched.size(), dnode, drf, dex,
-                              dapply, pl->nc, pl->gridMult * pl->cus, pl->positive, true, cs);
+    pl->launches = enqueue<T>(*pl, steps, qOff, qCnt, dprobs, pl->probs, (int)pl->batched.size(), dcq, dnode, drf,
+                              dex, dapply, pl->nc, pl->gridMult * pl->cus, pl->positive, true, cs);
     if (hipStreamEndCapture(cs, &pl->graph) != hipSuccess) st = 1;
   }
   if (!st && hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0) != hipSuccess) st = 1;
@@ -885,7 +949,8 @@ size_t batched_qr_workspace_bytes(int dtype, int count, const int* m, const int*
     split_problems<zd>(P, b, f);
     z = plan_sizes<zd>(P, b, f, tune);
   }
-  return z.probs + z.tblocks + z.nodeTasks + z.applyTasks + z.fallback + z.arena + z.tree + z.rfTasks + z.exTasks;
+  return z.probs + z.tblocks + z.nodeTasks + z.applyTasks + z.fallback + z.arena + z.tree + z.rfTasks + z.exTasks +
+         z.cqTasks + z.status;
 }
 
 }  // namespace mak
@@ -950,6 +1015,25 @@ int mak_qr_batched_plan_info(const mak_qr_batched_plan* plan, int* chunk_cols, i
   if (grid_per_cu) *grid_per_cu = plan->p.gridMult;
   if (launches) *launches = plan->p.launches;
   if (fallback_problems) *fallback_problems = (int)plan->p.fallback.size();
+  return 0;
+}
+
+// Test hook: after an execution (synchronised by the caller), the number of
+// panels the CholeskyQR2 tasks accepted and rejected (the rejected ones were
+// refactored by the gated Householder launches).
+int maktest_batched_panel_status(const mak_qr_batched_plan* plan, int* accepted, int* rejected) {
+  if (plan == nullptr || accepted == nullptr || rejected == nullptr) return -1;
+  const mak::BatchedQrPlan& p = plan->p;
+  *accepted = *rejected = 0;
+  if (p.cqIdx.empty()) return 0;
+  int most = 0;
+  for (int i : p.cqIdx) most = std::max(most, i);
+  std::vector<int> st((size_t)most + 1, 0);
+  if (hipMemcpy(st.data(), p.ws + p.offSt, sizeof(int) * st.size(), hipMemcpyDeviceToHost) != hipSuccess) return 1;
+  for (int i : p.cqIdx) {
+    if (st[i] == 0) ++*accepted;
+    else if (st[i] == 1) ++*rejected;
+  }
   return 0;
 }
 

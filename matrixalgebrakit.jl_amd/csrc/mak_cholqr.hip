@@ -49,12 +49,15 @@ template <> struct CqTile<double> { static constexpr int TR = 128; };
 template <> struct CqTile<zd> { static constexpr int TR = 64; };
 
 // How gram_tiles forms Y from the tile X of each row block:
-//   kPlain   Y = X;
-//   kTimesB  Y = X B (B: 32 x 32 in LDS), Y also written to Yout (H x 32, ld H).
-enum GramMode { kPlain = 0, kTimesB = 1 };
+//   kPlain     Y = X;
+//   kTimesB    Y = X B (B: 32 x 32 in LDS), Y also written to Yout (H x ycols,
+//              ld ldy; by default H x 32, ld H);
+//   kProduct   as kTimesB, without the Gram (only Yout is formed).
+enum GramMode { kPlain = 0, kTimesB = 1, kProduct = 2 };
 
-// G += Y^H Y over the row tiles blockIdx.x, blockIdx.x + gridDim.x, ... of X
-// (H x nbc, columns nbc..31 read as zero), with Y formed as MODE says.
+// G += Y^H Y over the row tiles tfirst, tfirst + tstep, ... of X (H x nbc,
+// columns nbc..31 read as zero; by default tfirst = blockIdx.x, tstep =
+// gridDim.x), with Y formed as MODE says.
 // The next tile's loads are issued before the current tile is used.
 // Wave w accumulates the 16 x 16 block (w % 2, w / 2) of G in (cr, ci).
 // With Xn (not null), the same row tiles of Xn (H x nbcn, ld ldx) are staged
@@ -70,7 +73,8 @@ template <typename T, int MODE, typename Pre = NoPre>
 __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*Xs)[NB + 1], const T (*Bs)[NB + 1],
                                   T* Yout, v4d& cr, v4d& ci, const T* Xn = nullptr, int nbcn = 0,
                                   T (*Xns)[NB + 1] = nullptr, v4d* nr = nullptr, v4d* ni = nullptr,
-                                  Pre pre = Pre()) {
+                                  Pre pre = Pre(), int tfirst = -1, int tstep = 0, int64_t ldy = 0,
+                                  int ycols = NB) {
   constexpr int TR = CqTile<T>::TR;
   constexpr int PER = TR * NB / 256;
   constexpr bool CPLX = ScalarTraits<T>::is_complex;
@@ -100,10 +104,12 @@ __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*X
       }
     }
   };
-  int t = blockIdx.x;
+  const int step = tstep > 0 ? tstep : (int)gridDim.x;
+  const int64_t ldyo = ldy > 0 ? ldy : (int64_t)H;
+  int t = tfirst >= 0 ? tfirst : (int)blockIdx.x;
   if (t < ntile) load(t);
   pre();
-  for (; t < ntile; t += gridDim.x) {
+  for (; t < ntile; t += step) {
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -117,7 +123,7 @@ __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*X
         Xns[e % TR][e / TR] = curn[q];
       }
     }
-    if (t + (int)gridDim.x < ntile) load(t + gridDim.x);
+    if (t + step < ntile) load(t + step);
     __syncthreads();
     if (next) {
 #pragma unroll 4
@@ -134,7 +140,7 @@ __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*X
         }
       }
     }
-    if constexpr (MODE == kTimesB) {
+    if constexpr (MODE == kTimesB || MODE == kProduct) {
       // Y = X B as transposed 16 x 16 blocks (Y^T = B^T X^T): lane l, register
       // r holds Y(i0 + l % 16, j0 + 4 r + l / 16), so stores are row-contiguous.
       // Wave w owns rows [w TR / 4, (w + 1) TR / 4).
@@ -182,12 +188,13 @@ __device__ inline void gram_tiles(const T* X, int64_t ldx, int H, int nbc, T (*X
             T v;
             if constexpr (CPLX) v = mkz(yr[ib][jb][r], yi[ib][jb][r]);
             else v = yr[ib][jb][r];
-            Xs[i][j] = v;
-            if (row < H) Yout[row + (int64_t)j * H] = v;
+            if constexpr (MODE == kTimesB) Xs[i][j] = v;
+            if (row < H && j < ycols) Yout[row + (int64_t)j * ldyo] = v;
           }
       }
       __syncthreads();
     }
+    if constexpr (MODE == kProduct) continue;
 #pragma unroll 4
     for (int kk = 0; kk < TR; kk += 4) {
       const T a = Xs[kk + lk][ti * 16 + lr];  // (Y^H)(i, k) = conj(Y(k, i))
@@ -620,6 +627,174 @@ __global__ void __launch_bounds__(256) cholqr2_finish_kernel(const T* X, int64_t
   }
 }
 
+// Wave w's 16 x 16 block (w % 2, w / 2) of a Gram accumulated by gram_tiles
+// into G (32 x 32 in LDS; identity outside nbc x nbc).  Returns nothing; the
+// caller synchronises.
+template <typename T>
+__device__ inline void acc_to_lds(T (*G)[NB + 1], const v4d& cr, const v4d& ci, int nbc) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ti = w & 1, tj = w >> 1;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = ti * 16 + mfma_f64_row(lane, r), j = tj * 16 + (lane & 15);
+    T v;
+    if constexpr (ScalarTraits<T>::is_complex) v = mkz(cr[r], ci[r]);
+    else v = cr[r];
+    G[i][j] = (i < nbc && j < nbc) ? v : ((i == j) ? s_one(T()) : s_zero(T()));
+  }
+}
+
+// One panel per workgroup from a task list (the panels of a batch of
+// independent problems): the same CholeskyQR2 with Householder
+// reconstruction as cholqr_panel, every pass in one workgroup:
+//   G1 = X^H X, G1 = R1^H R1 (one wave), Q1 = X R1^{-1} to the task's Q1
+//   scratch with G2 = Q1^H Q1 alongside, W2 (closed form or one wave), the
+//   check, R = W2 R1, the LU of the top of Q with signs D, T, M, and
+//   V_rest = Q1_rest M into the panel.
+// X is only read until the check has passed: a rejected panel (status 1) is
+// untouched, and the Householder launches gated on its status factor it.
+template <typename T>
+__global__ void __launch_bounds__(256) cholqr2_panel_tasks_kernel(const CholQrTask* tasks) {
+  constexpr int TR = CqTile<T>::TR;
+  constexpr int XM = TR / NB;
+  constexpr int NPOOL = (XM + 4 > 8) ? XM + 4 : 8;
+  __shared__ T pool[NPOOL][NB][NB + 1];
+  __shared__ T dv[NB];
+  __shared__ int bad_s;
+  __shared__ double dev_s[4];
+  const CholQrTask tk = tasks[blockIdx.x];
+  T* X = reinterpret_cast<T*>(tk.X);
+  T* Q1 = reinterpret_cast<T*>(tk.Q1);
+  const int64_t ldx = tk.ldx, ldq = tk.ldq;
+  const int H = tk.H, nbc = tk.nbc;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  T (*Bs)[NB + 1] = pool[XM];      // R1^{-1}
+  T (*R1)[NB + 1] = pool[XM + 1];  // G1, then R1, then R = W2 R1
+  // pass 1
+  {
+    v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+    gram_tiles<T, kPlain>(X, ldx, H, nbc, pool[0], nullptr, nullptr, cr, ci, nullptr, 0, nullptr, nullptr, nullptr,
+                          NoPre(), 0, 1);
+    acc_to_lds<T>(R1, cr, ci, nbc);
+  }
+  __syncthreads();
+  if (w == 0) {
+    const int b = wave_chol_inv32<T>(R1, Bs);
+    if (lane == 0) bad_s = b;
+  }
+  __syncthreads();
+  if (bad_s != 0) {
+    if (tid == 0) *tk.status = 1;
+    return;
+  }
+  // pass 2: Q1 = X R1^{-1} (nbc columns) and G2 = Q1^H Q1
+  T (*G)[NB + 1] = pool[XM + 2];  // G2, then W2
+  {
+    v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+    gram_tiles<T, kTimesB>(X, ldx, H, nbc, pool[0], Bs, Q1, cr, ci, nullptr, 0, nullptr, nullptr, nullptr, NoPre(),
+                           0, 1, ldq, nbc);
+    acc_to_lds<T>(G, cr, ci, nbc);
+  }
+  __syncthreads();
+  double dev = 0.0;
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e % NB, j = e / NB;
+    dev += s_abs2(s_sub(G[i][j], (i == j) ? s_one(T()) : s_zero(T())));
+  }
+  dev = wave_sum(dev);
+  if (lane == 0) dev_s[w] = dev;
+  __syncthreads();
+  const double devF2 = dev_s[0] + dev_s[1] + dev_s[2] + dev_s[3];
+  // The reconstruction's matrices.  Pool slots 0 .. XM - 1 stage the row
+  // tiles again in the last pass, so they hold only what is dead by then;
+  // M (in Qt) and L (in Ls) outlive it, in R1^{-1}'s slot and the last one.
+  T (*Yl)[NB + 1] = pool[0];                          // W2^{-1}
+  T (*Li)[NB + 1] = pool[1];                          // L^{-1}
+  T (*Ms)[NB + 1] = pool[XM >= 4 ? 2 : XM + 3];       // closed-form scratch, U D^H, then T
+  T (*Ui)[NB + 1] = pool[XM >= 4 ? 3 : XM + 4];       // U^{-1}
+  T (*Qt)[NB + 1] = pool[XM];                         // top of Q1, then of Q; then M
+  T (*Ls)[NB + 1] = pool[NPOOL - 1];                  // L (unit lower), U on and above
+  static_assert(NPOOL - 1 > XM + 2 && (XM >= 4 || XM + 4 < NPOOL - 1), "distinct slots");
+  // G2 = I + E: closed form for ||E||_F <= 1e-6 (as in the whole-panel
+  // kernel), else one wave factors G2
+  const bool near_identity = devF2 <= 1e-12;
+  if (!near_identity && w == 0) {
+    const int b = wave_chol_inv32<T>(G, Yl);
+    if (lane == 0) bad_s = b;
+  }
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e % NB, j = e / NB;
+    Qt[i][j] = (j < nbc) ? Q1[i + (int64_t)j * ldq] : s_zero(T());  // H >= 64: rows 0..31 exist
+  }
+  if (near_identity) {
+    auto up = [&](int i, int j, T v) {
+      return (i < j) ? v : ((i == j) ? s_scale(s_add(v, s_conj(v)), 0.25) : s_zero(T()));
+    };
+    for (int e = tid; e < NB * NB; e += 256) {
+      const int i = e % NB, j = e / NB;
+      Yl[i][j] = up(i, j, s_sub(G[i][j], (i == j) ? s_one(T()) : s_zero(T())));  // U1
+    }
+    __syncthreads();
+    mm32<T>(Ms, Yl, true, Yl, false, 1.0, false);  // U1^H U1
+    __syncthreads();
+    for (int e = tid; e < NB * NB; e += 256) {
+      const int i = e % NB, j = e / NB;
+      const T u = up(i, j, s_sub(s_sub(G[i][j], (i == j) ? s_one(T()) : s_zero(T())), Ms[i][j]));
+      Yl[i][j] = u;                                   // U
+      G[i][j] = (i == j) ? s_add(s_one(T()), u) : u;  // W2 = I + U
+    }
+    __syncthreads();
+    mm32<T>(Ms, Yl, false, Yl, false, 1.0, false);  // U^2
+    __syncthreads();
+    for (int e = tid; e < NB * NB; e += 256) {
+      const int i = e % NB, j = e / NB;
+      const T v = s_add(s_neg(Yl[i][j]), Ms[i][j]);
+      Yl[i][j] = (i == j) ? s_add(s_one(T()), v) : v;  // W2^{-1} = I - U + U^2
+    }
+  }
+  __syncthreads();
+  if ((!near_identity && bad_s != 0) || !(devF2 <= 0.01)) {
+    if (tid == 0) *tk.status = 1;
+    return;
+  }
+  mm32<T>(R1, G, false, R1, false, 1.0, false);   // R = W2 R1
+  mm32<T>(Qt, Qt, false, Yl, false, 1.0, false);  // top of Q = Q1 W2^{-1}
+  for (int e = tid; e < NB * NB; e += 256) {      // identity outside nbc x nbc
+    const int r = e % NB, cc = e / NB;
+    Qt[r][cc] = sel(r < nbc && cc < nbc, Qt[r][cc], sel(r == cc, s_one(T()), s_zero(T())));
+  }
+  lu32_recon<T>(Qt, Ls, dv, Ui, Li);
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int r = e % NB, cc = e / NB;
+    Ms[r][cc] = (r <= cc) ? s_mul(Ls[r][cc], s_conj(dv[cc])) : s_zero(T());
+  }
+  __syncthreads();
+  mm32<T>(Ms, Ms, false, Li, true, -1.0, false);  // T = -(U D^H) L^{-H}
+  mm32<T>(Qt, Yl, false, Ui, false, 1.0, false);  // M = W2^{-1} U^{-1}
+  T* Tout = reinterpret_cast<T*>(tk.Tout);
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int r = e % NB, cc = e / NB;
+    const bool in = r < nbc && cc < nbc;
+    Tout[e] = (in && r <= cc) ? Ms[r][cc] : s_zero(T());
+    // padding outside nbc x nbc keeps M's product with the zero-padded tiles exact
+    if (!in) Qt[r][cc] = s_zero(T());
+  }
+  __syncthreads();
+  // V_rest = Q1_rest M (rows nbc .. H - 1) into the panel, then its top rows:
+  // D R on and above the diagonal, L below (the top rows are read by nothing
+  // after this point)
+  if (H > nbc) {
+    v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+    gram_tiles<T, kProduct>(Q1 + nbc, ldq, H - nbc, nbc, pool[0], Qt, X + nbc, cr, ci, nullptr, 0, nullptr,
+                            nullptr, nullptr, NoPre(), 0, 1, ldx, nbc);
+  }
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int r = e % NB, cc = e / NB;
+    if (r < nbc && cc < nbc) X[r + (int64_t)cc * ldx] = (r <= cc) ? s_mul(dv[r], R1[r][cc]) : Ls[r][cc];
+  }
+  if (tid == 0) *tk.status = 0;
+}
+
 template <typename T> GemmArgs gg(int M, int N, int K, const T* A, int64_t lda, const T* B, int64_t ldb, T* C,
                                   int64_t ldc, const int* gate) {
   GemmArgs g{};
@@ -672,6 +847,12 @@ void cholqr_panel(T* Ap, int64_t lda, int H, int nbc, T* Tpan, T* cw, int* statu
                      carried == 1 ? 1 : 0, ScalarTraits<T>::is_complex ? 1 : 0);
   if (H > nbc) gemm<T>(OP_N, OP_N, gg<T>(H - nbc, nbc, nbc, Q1 + nbc, H, M, NB, Ap + nbc, lda, status), s);
 }
+
+template <typename T> void launch_cholqr2_panel_tasks(const CholQrTask* tasks, int ntasks, hipStream_t s) {
+  if (ntasks > 0) hipLaunchKernelGGL((cholqr2_panel_tasks_kernel<T>), dim3(ntasks), dim3(256), 0, s, tasks);
+}
+template void launch_cholqr2_panel_tasks<double>(const CholQrTask*, int, hipStream_t);
+template void launch_cholqr2_panel_tasks<zd>(const CholQrTask*, int, hipStream_t);
 
 template <typename T> void cholqr_carry_buffers(T* cw, T** Gw, T** partN, T** R1, T** R1inv) {
   *Gw = cw;

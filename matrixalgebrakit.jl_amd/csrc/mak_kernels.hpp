@@ -38,9 +38,24 @@ struct NodeTask {
   void* Rn;         // optional: R (nbc x nbc) of the node, ld ldr (TSQR leaves)
   int64_t ldr;
   void* V1;         // optional: top 32 x 32 of V (unit lower), ld 32
+  const int* gate;  // optional: the task runs only if *gate == 1 (its panel's CholeskyQR2 was rejected)
   int H;
   int nbc;
 };
+
+// One panel of a batch factored by CholeskyQR2 with Householder
+// reconstruction in one workgroup (cholqr2_panel_tasks_kernel).
+struct CholQrTask {
+  void* X;          // H x nbc panel (H >= 64): compact WY form on success, untouched otherwise
+  int64_t ldx;
+  void* Q1;         // H x nbc scratch (Q1 = X R1^{-1}), ld ldq
+  int64_t ldq;
+  void* Tout;       // 32 x 32 T factor (ld 32)
+  int* status;      // out: 0 accepted, 1 rejected (the Householder launches gated on it run)
+  int H;
+  int nbc;
+};
+template <typename T> void launch_cholqr2_panel_tasks(const CholQrTask* tasks, int ntasks, hipStream_t s);
 
 constexpr int kMaxTreeLevels = 8;
 

@@ -152,12 +152,14 @@ __device__ inline void node_step(T (&a)[RPT][NB], NodeLds<T, NB, NT / WAVE>& S, 
   const T alpha = S.rowj[par][OFF];
   const T rji = S.rowj[par][is];
   T tau, beta, scale;
-  if (nu2 == 0.0 && s_im(alpha) == 0.0) {
+  const double bn = sqrt(s_abs2(alpha) + nu2);
+  if ((nu2 == 0.0 && s_im(alpha) == 0.0) || !(bn > 0.0)) {
+    // nothing to annihilate -- or a column whose squares underflow (entries
+    // below ~1e-154): no reflection, the column is left as it is
     tau = s_zero(T());
     beta = alpha;
     scale = s_zero(T());
   } else {
-    const double bn = sqrt(s_abs2(alpha) + nu2);
     const double br = s_re(alpha) >= 0.0 ? -bn : bn;
     beta = s_from_real<T>(br);
     tau = s_scale(s_sub(beta, alpha), 1.0 / br);
@@ -422,12 +424,14 @@ __device__ __forceinline__ void node2_step(T (&a)[KR][2], Node2Lds<T, NW, 4 * KR
   const T r0 = S.rowj[PAR][cl];
   const T r1 = S.rowj[PAR][cl + 16];
   T tau, beta, scale;
-  if (nu2 == 0.0 && s_im(alpha) == 0.0) {
+  const double bn = sqrt(s_abs2(alpha) + nu2);
+  if ((nu2 == 0.0 && s_im(alpha) == 0.0) || !(bn > 0.0)) {
+    // nothing to annihilate -- or a column whose squares underflow (entries
+    // below ~1e-154): no reflection, the column is left as it is
     tau = s_zero(T());
     beta = alpha;
     scale = s_zero(T());
   } else {
-    const double bn = sqrt(s_abs2(alpha) + nu2);
     const double br = s_re(alpha) >= 0.0 ? -bn : bn;
     beta = s_from_real<T>(br);
     tau = s_scale(s_sub(beta, alpha), 1.0 / br);
@@ -616,6 +620,7 @@ template <typename T, int NW, int KR>
 __global__ void __launch_bounds__(NW * WAVE) geqr2_node2_tasks_kernel(const NodeTask* tasks) {
   __shared__ Node2Lds<T, NW, 4 * KR> S;
   const NodeTask t = tasks[blockIdx.x];
+  if (t.gate != nullptr && *t.gate != 1) return;  // its panel was factored by CholeskyQR2
   node2_body<T, NW, KR>(S, reinterpret_cast<T*>(t.X), t.ldx, t.H, t.nbc, reinterpret_cast<T*>(t.Tout),
                         reinterpret_cast<T*>(t.Rn), t.ldr, reinterpret_cast<T*>(t.V1));
 }

@@ -65,6 +65,74 @@ def test_batched_random_shapes_match_oracle(dt, dev):
         assert np.array_equal(Ad.cpu().numpy(), A)   # qr_compact copies its input
 
 
+def panel_status(plan):
+    """(accepted, rejected) CholeskyQR2 panels of the last execution."""
+    import ctypes as C
+    from mak_mi355x import _lib
+    lib = _lib.load()
+    lib.maktest_batched_panel_status.restype = C.c_int
+    lib.maktest_batched_panel_status.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    a, r = C.c_int(-1), C.c_int(-1)
+    assert lib.maktest_batched_panel_status(plan._h, C.byref(a), C.byref(r)) == 0
+    return a.value, r.value
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+def test_batched_cholqr2_panels_taken(dt, dev):
+    """Every panel of at least 64 rows is factored by the CholeskyQR2 panel
+    task for Gaussian blocks (none falls back), and the factors are the
+    oracle's."""
+    shapes = mak.block_shapes(200, seed=9, mmax=1000)
+    host, As = make_blocks(shapes, dt, 300, dev)
+    plan = mak.BatchedQRPlan(As)
+    Qs, Rs = plan.execute()
+    torch.cuda.synchronize()
+    acc, rej = panel_status(plan)
+    panels = sum(1 for m, n in shapes for j0 in range(0, min(m, n), 32) if m - j0 >= 64)
+    assert (acc, rej) == (panels, 0)
+    for i in (0, 50, 120, 199):
+        Qo, Ro = O.qr_compact(host[i])
+        assert rel(Qs[i].cpu().numpy(), Qo) < 1e-12 and rel(Rs[i].cpu().numpy(), Ro) < 1e-12, shapes[i]
+    plan.close()
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+def test_batched_rejected_cholqr2_panels_fall_back(dt, dev):
+    """Panels the CholeskyQR2 check rejects -- an exactly zero column, two
+    equal columns, a column of magnitude 1e-300 (its squares underflow) --
+    are refactored by the gated Householder launches: residual,
+    orthogonality and gauge of Householder QR, finite everywhere.  A column
+    made dependent by an earlier panel (left as rounding noise by the
+    trailing update) is accepted, and the factorisation is still a QR of the
+    block.  Gaussian blocks alongside match the oracle."""
+    Z = O.gaussian(300, 64, dt, seed=31)
+    Z[:, 5] = 0.0
+    D = O.gaussian(256, 40, dt, seed=33)
+    D[:, 1] = D[:, 0]
+    S = O.gaussian(400, 64, dt, seed=32)
+    S[:, 7] *= 1e-300
+    L = O.gaussian(300, 70, dt, seed=21)
+    L[:, 40] = L[:, 3]
+    host = [Z, D, S, L, O.gaussian(500, 200, dt, seed=24), O.gaussian(120, 40, dt, seed=25)]
+    As = [mak.colmajor(torch.from_numpy(np.asfortranarray(h)).to(dev)) for h in host]
+    plan = mak.BatchedQRPlan(As)
+    Qs, Rs = plan.execute()
+    torch.cuda.synchronize()
+    acc, rej = panel_status(plan)
+    assert rej >= 3 and acc > 0
+    for h, Q, R in zip(host, Qs, Rs):
+        Qh, Rh = Q.cpu().numpy(), R.cpu().numpy()
+        assert np.all(np.isfinite(Qh)) and np.all(np.isfinite(Rh))
+        assert rel(Qh @ Rh, h) < 1e-13
+        assert np.linalg.norm(Qh.conj().T @ Qh - np.eye(Qh.shape[1])) < 1e-12
+        d = np.diagonal(Rh)
+        assert np.all(d.real >= 0) and np.all(d.imag == 0)
+    for i in (4, 5):
+        Qo, Ro = O.qr_compact(host[i])
+        assert rel(Qs[i].cpu().numpy(), Qo) < 1e-12 and rel(Rs[i].cpu().numpy(), Ro) < 1e-12
+    plan.close()
+
+
 def test_batched_plan_replay_and_tuning(dev):
     """The captured graph replays on new data in the same buffers; a tuned plan
     gives the same factors."""
