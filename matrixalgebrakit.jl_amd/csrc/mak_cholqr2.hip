@@ -35,7 +35,7 @@ namespace {
 
 inline size_t al256(size_t b) { return ((b + 255) / 256) * 256; }
 
-constexpr int kRedWgs = 256;  // workgroups of the ||E|| reduction
+constexpr int kRedWgs = 1024;  // workgroups of the second pass's sweep (and its ||E|| partials)
 
 // Largest ||E||_F for which the final Q = Q1 + Q1 D (D = W2^{-1} - I, upper
 // triangular, ||D||_F ~ ||E||_F) takes the product Q1 D from FP32 MFMA with
@@ -125,8 +125,8 @@ __global__ void __launch_bounds__(256) sum_upper_slices_kernel(const T* P, int k
   }
 }
 
-template <typename T>
-void gram_upper(const T* X, int64_t ldx, int m, int n, T* G, T* slices, int ks, hipStream_t s) {
+// The split-K slices of the upper Gram X^H X; returns their count.
+template <typename T> int gram_upper_slices(const T* X, int64_t ldx, int m, int n, T* slices, int ks, hipStream_t s) {
   GemmArgs g = gq<T>(n, n, m, X, ldx, X, ldx, slices, n, 1.0, 0.0, 0);
   g.upperC = 1;
   if (!ScalarTraits<T>::is_complex) g.tile = kGramTile;
@@ -134,9 +134,54 @@ void gram_upper(const T* X, int64_t ldx, int m, int n, T* G, T* slices, int ks, 
   g.ksplit = cdiv(m, g.kchunk);
   g.splitStride = (int64_t)n * n;
   gemm<T>(OP_C, OP_N, g, s);
+  return g.ksplit;
+}
+
+template <typename T>
+void gram_upper(const T* X, int64_t ldx, int m, int n, T* G, T* slices, int ks, hipStream_t s) {
+  const int nsl = gram_upper_slices<T>(X, ldx, m, n, slices, ks, s);
   const int64_t tot = (int64_t)n * (n + 1) / 2;
   hipLaunchKernelGGL((sum_upper_slices_kernel<T>), dim3((unsigned)std::min<int64_t>(cdiv64(tot, 256), 2048)), dim3(256),
-                     0, s, slices, g.ksplit, n, G);
+                     0, s, slices, nsl, n, G);
+}
+
+// The second pass's first kernel, in one sweep over the n x n elements:
+// G2 = I + E on and above the diagonal from the ks slices (into E), U1 =
+// up(E) into U (zero below the diagonal: later products read whole diagonal
+// tiles), and this workgroup's part of ||E||_F^2 (off-diagonal entries
+// counted twice) into part.
+template <typename T>
+__global__ void __launch_bounds__(256) cq2_sum_up_kernel(const T* P, int ks, int n, T* E, T* U, double* part) {
+  __shared__ double red[4];
+  const int64_t tot = (int64_t)n * n;
+  double dev = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (int64_t)gridDim.x * 256) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    T u = s_zero(T());
+    if (r <= c) {
+      T acc[4] = {s_zero(T()), s_zero(T()), s_zero(T()), s_zero(T())};
+      int z = 0;
+      for (; z + 4 <= ks; z += 4)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = s_add(acc[q], P[e + (int64_t)(z + q) * tot]);
+      for (; z < ks; ++z) acc[0] = s_add(acc[0], P[e + (int64_t)z * tot]);
+      const T g = s_add(s_add(acc[0], acc[1]), s_add(acc[2], acc[3]));
+      E[e] = g;
+      if (r == c) {
+        const double d = s_re(g) - 1.0;
+        u = s_from_real<T>(0.5 * d);
+        dev += s_abs2(s_sub(g, s_one(T())));
+      } else {
+        u = g;
+        dev += 2.0 * s_abs2(g);
+      }
+    }
+    U[e] = u;
+  }
+  dev = wave_sum(dev);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dev;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 // R (n x n, ld ldr) = sum of the ks slices (n x n each) on and above the
@@ -153,26 +198,6 @@ __global__ void __launch_bounds__(256) sum_r_slices_kernel(const T* P, int ks, i
       for (int z = 0; z < ks; ++z) acc = s_add(acc, P[e + (int64_t)z * tot]);
     R[r + (int64_t)c * ldr] = acc;
   }
-}
-
-// ||G - I||_F^2 over the upper triangle (off-diagonal entries counted twice),
-// partial per workgroup.
-template <typename T>
-__global__ void __launch_bounds__(256) eye_dist_partial_kernel(const T* G, int n, double* part) {
-  __shared__ double red[4];
-  const int64_t tot = (int64_t)n * n;
-  double acc = 0.0;
-  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * 256) {
-    const int r = (int)(idx % n), c = (int)(idx / n);
-    if (r > c) continue;
-    T v = G[idx];
-    if (r == c) v = s_sub(v, s_one(T()));
-    acc += (r < c ? 2.0 : 1.0) * s_abs2(v);
-  }
-  acc = wave_sum(acc);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 // st[0] = 1 (accept) when the Cholesky succeeded (info == 0) and ||E||_F <=
@@ -283,12 +308,12 @@ void cholqr2_enqueue(hipStream_t s, int m, int n, const T* A, int64_t lda, T* Q,
     chol_tile_factor<T>(G, n, n, X, n, flags, st + 2, 0, s);
     gemm<T>(OP_N, OP_N, gq<T>(m, n, n, A, lda, X, n, Q1, m, 1.0, 0.0, TRI_B_UPPER), s);
   }
-  // pass 2: G2 = I + E, the decision, and W2, W2^{-1} in closed form
-  gram_upper<T>(Q1, m, m, n, E, Sl, L.ks, s);
-  hipLaunchKernelGGL((eye_dist_partial_kernel<T>), dim3(kRedWgs), dim3(256), 0, s, E, n, part);
+  // pass 2: G2 = I + E (slices summed with U1 = up(E) and ||E||_F in one
+  // sweep), the decision, and W2, W2^{-1} in closed form
+  const int ks2 = gram_upper_slices<T>(Q1, m, m, n, Sl, L.ks, s);
+  hipLaunchKernelGGL((cq2_sum_up_kernel<T>), dim3(kRedWgs), dim3(256), 0, s, Sl, ks2, n, E, U, part);
   hipLaunchKernelGGL(cq2_decide_kernel, dim3(1), dim3(256), 0, s, part, kRedWgs, st + 2,
                      ScalarTraits<T>::is_complex ? 0 : 1, st);
-  hipLaunchKernelGGL((cq2_up_kernel<T>), dim3(eg), dim3(256), 0, s, E, (const T*)nullptr, n, U, (const int*)nullptr);
   // second order (st[1] == 1): U = up(E - U1^H U1), U2 = U U
   {
     GemmArgs g = gq<T>(n, n, n, U, n, U, n, Tm, n, 1.0, 0.0, TRI_A_LOWER | TRI_B_UPPER, st + 1, 1);
