@@ -10,13 +10,15 @@
 // (D: one-wave Cholesky with inverse of a 32 x 32 tile).  The other
 // workgroups run 32 x 32 tasks whose products contract over the pair (K = 64
 // on FP64 MFMA):
-//   P2(K, j)      [R_c0j; R_c1j] = X_KK^H [G_c0j; G_c1j]        j >= 2K+2
-//   U2(K, i, j)   G_ij -= R_c0i^H R_c0j + R_c1i^H R_c1j        2K+2 <= i <= j
+//   P2(K, j)      [R_c0j; R_c1j] = X_KK^H [G_c0j; G_c1j]        j >= 2K+4
+//   U2(K, i, j)   G_ij -= R_c0i^H R_c0j + R_c1i^H R_c1j        2K+2 <= i <= j, j >= 2K+4
 //   UQ2(K, j, l)  Z_jl -= X_jc0 R_c0l + X_jc1 R_c1l            j < 2K+2 <= l
 //   PQ2(K, j)     [X_jc0 X_jc1] = [Z_jc0 Z_jc1] X_KK           j < 2K
 // (X R = I column block by column block: X_{j,L} = -(sum_K X_{j,K} R_{K,L}) X_LL.)
-// A step thus costs two hand-offs between workgroups (P2, then U2 of the
-// next pair's tiles) for 64 columns.  Workgroups take tasks from an atomic
+// The panel itself forms P2(K, j) for the next pair's columns j = 2K+2, 2K+3
+// and U2(K) of the next pair's diagonal block, which it keeps in LDS: the
+// chain of pair steps runs inside one workgroup, which only waits for the
+// workers' updates of earlier pairs (long done by then).  Workgroups take tasks from an atomic
 // counter in a fixed topological order (the next pair's inputs first, the
 // bulk last) and wait for the per-tile version counters the task needs:
 //   gst[i][j]  G/R tile (i <= j): pair steps applied; i/2 + 1 = R final
@@ -61,8 +63,8 @@ struct CholTask {
 };
 
 // Tasks of iteration K (T tiles, T even), by segment, in queue order:
-//   A  P2(K, j)        j = 2K+2 .. T-1
-//   B  U2(K, i, j)     i = 2K+2, 2K+3, j = i .. T-1   (the next pair's rows)
+//   A  P2(K, j)        j = 2K+4 .. T-1
+//   B  U2(K, i, j)     i = 2K+2, 2K+3, j = 2K+4 .. T-1  (the next pair's rows)
 //   C  PQ2(K, j)       j = 0 .. 2K-1
 //   E  UQ2(K, j, l)    j = 0 .. 2K+1, l = 2K+2 .. T-1
 //   F  U2(K, i, j)     2K+4 <= i <= j <= T-1          (the bulk)
@@ -79,8 +81,8 @@ __host__ __device__ inline void chol_iter_counts(int T, int K, int ny, int c[kCh
   const int r0 = 2 * K + 2;
   const int rem = T - r0 > 0 ? T - r0 : 0;
   const int b = rem >= 2 ? rem - 2 : 0;
-  c[0] = rem;
-  c[1] = rem > 0 ? 2 * rem - 1 : 0;
+  c[0] = b;       // P2 past the panel's look-ahead columns
+  c[1] = 2 * b;   // U2 of the next pair's rows, right of its diagonal block
   c[2] = 2 * K;
   c[3] = r0 * rem;
   c[4] = b * (b + 1) / 2;
@@ -104,11 +106,11 @@ __host__ __device__ inline CholTask chol_task_in_iter(int T, int K, int ny, int 
   int q = 0;
   while (q < kCholSegs - 1 && u >= c[q]) u -= c[q++];
   switch (q) {
-    case 0: t.type = CT_P2; t.j = r0 + u; break;
+    case 0: t.type = CT_P2; t.j = r0 + 2 + u; break;
     case 1:
       t.type = CT_U2;
-      if (u < rem) { t.i = r0; t.j = r0 + u; }
-      else { t.i = r0 + 1; t.j = r0 + 1 + (u - rem); }
+      t.i = u < c[0] ? r0 : r0 + 1;
+      t.j = r0 + 2 + (u < c[0] ? u : u - c[0]);
       break;
     case 2: t.type = CT_PQ2; t.j = u; break;
     case 3: t.type = CT_UQ2; t.i = u / rem; t.j = r0 + u % rem; break;
@@ -135,9 +137,12 @@ struct CholTileArgs {
   int* ctl;       // [0] task counter, [1] abort flag, [2] first non-positive pivot (1-based) or 0
   int* info;      // optional: receives ctl[2] + info_offset on failure (when *info == 0)
   int info_offset;
-  long long* trace;  // optional (tests): wall-clock stamps of the panel's iterations, 8 per iteration
-                     // (slot 7: the end of the pair's last right-product task), then
-                     // from entry 256 the start and end of every right-product task
+  long long* trace;  // optional (tests): wall-clock stamps of the panel's iterations, 8 per iteration:
+                     // 3, 4, 5 before D(c0), before D(c1), after X01; 6 at the end; real
+                     // data: 1, 2, 0 after the look-ahead's waits, loads and R products
+                     // (complex: 0, 1, 2 at the start); 7 the end of the pair's last
+                     // right-product task.  From entry 256 the start and end of every
+                     // right-product task.
   // the right product Y = B X (ny = 0: none)
   const double* B;
   int64_t ldb;
@@ -157,29 +162,13 @@ struct CholTileArgs {
 // ---------------------------------------------------------------- sync helpers
 __device__ inline int ld_flag(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
-// Thread 0 polls until *p >= want (or the abort flag is set, or the spin limit
-// is reached, which raises it).  The caller follows with inputs_ready().
-__device__ inline void wait_ge(const int* p, int want, int* ctl) {
-  if (threadIdx.x == 0) {
-    long spins = 0;
-    while (ld_flag(p) < want) {
-      if (ld_flag(ctl + 1) != 0) break;
-      if (++spins > kSpinLimit) {
-        __hip_atomic_store(ctl + 1, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-  }
-}
-
 // Memory protocol between workgroups.  Tiles that tasks exchange are moved
 // with agent-scope relaxed atomic loads and stores (sc1: coherent across the
 // XCDs' L2 caches location by location), so no whole-cache writeback or
 // invalidate is needed:
 //   producer: every wave waits for its stores to complete (vmcnt(0)), then a
 //             workgroup barrier, then thread 0 stores the version counter;
-//   consumer: thread 0 polls the counter, then a workgroup barrier, then the
+//   consumer: wave 0 polls the counters, then a workgroup barrier, then the
 //             tile loads.
 // A load issued after the counter was seen reads the completed stores.
 __device__ inline bool inputs_ready(int* ctl) {
@@ -187,6 +176,31 @@ __device__ inline bool inputs_ready(int* ctl) {
   if (threadIdx.x == 0) abort_s = ld_flag(ctl + 1);
   __syncthreads();
   return abort_s == 0;
+}
+
+// Thread 0 polls a task's counters in turn until *p[q] >= v[q] for every q
+// (or the abort flag is set, or the spin limit is reached, which raises it),
+// then a workgroup barrier, the consumer's side of the protocol above; false
+// when the launch is aborting.  (Polling all counters at once from the lanes
+// of a wave measured slower: it multiplies the polling traffic of the
+// waiting workgroups.)
+template <int N>
+__device__ inline bool wait_all(int* const (&p)[N], const int (&v)[N], int* ctl) {
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      long spins = 0;
+      while (ld_flag(p[q]) < v[q]) {
+        if (ld_flag(ctl + 1) != 0) break;
+        if (++spins > kSpinLimit) {
+          __hip_atomic_store(ctl + 1, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+  }
+  return inputs_ready(ctl);
 }
 
 __device__ inline void stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // s_waitcnt vmcnt(0)
@@ -443,6 +457,68 @@ __device__ inline void load_n(T (*S0)[33], TileSrc s0, T (*S1)[33], TileSrc s1, 
   if constexpr (CNT > 4) slow(S4, s4);
 }
 
+// Seven tiles into LDS in one batch of coherent loads (the panel's
+// look-ahead); element loads when a tile does not allow the fast path.
+__device__ inline void load_fast7(double (*S0)[33], const double* p0, double (*S1)[33], const double* p1,
+                                  double (*S2)[33], const double* p2, double (*S3)[33], const double* p3,
+                                  double (*S4)[33], const double* p4, double (*S5)[33], const double* p5,
+                                  double (*S6)[33], const double* p6) {
+  v2d a0, b0, a1, b1, a2, b2, a3, b3, a4, b4, a5, b5, a6, b6;
+  asm volatile("global_load_dwordx4 %0, %14, off sc1\n\tglobal_load_dwordx4 %1, %14, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %2, %15, off sc1\n\tglobal_load_dwordx4 %3, %15, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %4, %16, off sc1\n\tglobal_load_dwordx4 %5, %16, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %6, %17, off sc1\n\tglobal_load_dwordx4 %7, %17, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %8, %18, off sc1\n\tglobal_load_dwordx4 %9, %18, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %10, %19, off sc1\n\tglobal_load_dwordx4 %11, %19, off offset:16 sc1\n\t"
+               "global_load_dwordx4 %12, %20, off sc1\n\tglobal_load_dwordx4 %13, %20, off offset:16 sc1\n\t"
+               "s_waitcnt vmcnt(0)"
+               : "=&v"(a0), "=&v"(b0), "=&v"(a1), "=&v"(b1), "=&v"(a2), "=&v"(b2), "=&v"(a3), "=&v"(b3),
+                 "=&v"(a4), "=&v"(b4), "=&v"(a5), "=&v"(b5), "=&v"(a6), "=&v"(b6)
+               : "v"(p0), "v"(p1), "v"(p2), "v"(p3), "v"(p4), "v"(p5), "v"(p6)
+               : "memory");
+  put_chunk(S0, a0, b0);
+  put_chunk(S1, a1, b1);
+  put_chunk(S2, a2, b2);
+  put_chunk(S3, a3, b3);
+  put_chunk(S4, a4, b4);
+  put_chunk(S5, a5, b5);
+  put_chunk(S6, a6, b6);
+}
+template <typename T>
+__device__ inline void load_7(T (*S0)[33], TileSrc s0, T (*S1)[33], TileSrc s1, T (*S2)[33], TileSrc s2,
+                              T (*S3)[33], TileSrc s3, T (*S4)[33], TileSrc s4, T (*S5)[33], TileSrc s5,
+                              T (*S6)[33], TileSrc s6, int n) {
+  auto fast_of = [&](const TileSrc& q) {
+    return tile_fast<T>(reinterpret_cast<const T*>(q.M), q.ld, n, q.ti, q.tj);
+  };
+  if constexpr (!ScalarTraits<T>::is_complex) {
+    if (fast_of(s0) && fast_of(s1) && fast_of(s2) && fast_of(s3) && fast_of(s4) && fast_of(s5) && fast_of(s6)) {
+      auto chunk = [&](const TileSrc& q) {
+        return tile_chunk(reinterpret_cast<const double*>(q.M), q.ld, q.ti, q.tj);
+      };
+      load_fast7(S0, chunk(s0), S1, chunk(s1), S2, chunk(s2), S3, chunk(s3), S4, chunk(s4), S5, chunk(s5), S6,
+                 chunk(s6));
+      __syncthreads();
+      return;
+    }
+  }
+  auto slow = [&](T (*S)[33], const TileSrc& q) {
+    load_tile<T>(S, reinterpret_cast<const T*>(q.M), q.ld, n, q.ti, q.tj);
+  };
+  slow(S0, s0);
+  slow(S1, s1);
+  slow(S2, s2);
+  slow(S3, s3);
+  slow(S4, s4);
+  slow(S5, s5);
+  slow(S6, s6);
+  __syncthreads();
+}
+
+// LDS tiles of the panel workgroup: ten for real data, seven for complex
+// (ten 16-byte tiles would not fit the 160 KB of LDS).
+template <typename T> constexpr int kPanelTiles = ScalarTraits<T>::is_complex ? 7 : 10;
+
 // ---------------------------------------------------------------- the kernel
 // Factor one diagonal tile in LDS (S: in G, out R; Xo: out R^{-1}) by wave 0.
 // Returns the bad pivot (0 = fine), uniform over the workgroup.
@@ -572,12 +648,20 @@ __device__ inline void y_task(const CholTileArgs& a, int K, int r, int half, dou
 }
 
 // The panel workgroup: the 2 x 2 tile diagonal block of every pair, in LDS.
-// Iteration K waits for the block's tiles to carry the K earlier pair steps,
-// then runs D(c0), R01 = X00^H G01, G11 -= R01^H R01, D(c1),
-// X01 = -X00 (R01 X11), and publishes R and X of the block.
+// Iteration K runs D(c0), R01 = X00^H G01, G11 -= R01^H R01, D(c1),
+// X01 = -X00 (R01 X11), and publishes R and X of the block.  Then it looks
+// ahead to the next pair (rows c0, c1 against columns r0 = c0 + 2, r0 + 1):
+// it forms and publishes R_c0,r0 .. R_c1,r0+1 (the work of P2(K, r0 .. r0+1))
+// and applies pair K's update to the next diagonal block, which it keeps in
+// LDS for iteration K + 1 (the work of U2(K) on that block), so the chain
+// hands nothing to the workers and back between two pair steps.  The block
+// it loads must carry the K earlier updates, which the workers apply (U2 of
+// the earlier pairs).  Tiles: tA, tB, tC the block, tX, tE, tD its X, and
+// the look-ahead's R tiles (tF .. tI for real data, where the next block
+// lands in tA .. tC; complex data reuses the block's tiles and tF, and the
+// roles rotate every iteration).
 template <typename T>
-__device__ void chol_panel(const CholTileArgs& a, T (*tA)[33], T (*tB)[33], T (*tC)[33], T (*tD)[33],
-                           T (*tE)[33], T (*tX)[33], T* rowbuf) {
+__device__ void chol_panel(const CholTileArgs& a, T (*lds)[TB][33], T* rowbuf) {
   T* G = reinterpret_cast<T*>(a.G);
   T* X = reinterpret_cast<T*>(a.X);
   const int nt = a.nt;
@@ -588,20 +672,31 @@ __device__ void chol_panel(const CholTileArgs& a, T (*tA)[33], T (*tB)[33], T (*
   auto stamp = [&](int k, int slot) {
     if (tr != nullptr && threadIdx.x == 0) tr[k * 8 + slot] = (long long)__builtin_amdgcn_s_memrealtime();
   };
+  T(*tA)[33] = lds[0];
+  T(*tB)[33] = lds[1];
+  T(*tC)[33] = lds[2];
+  T(*tD)[33] = lds[3];
+  T(*tE)[33] = lds[4];
+  T(*tX)[33] = lds[5];
+  T(*tF)[33] = lds[6];
+  // real data only (ten tiles): the look-ahead's R tiles are tF .. tI
+  constexpr bool kCx = ScalarTraits<T>::is_complex;
+  T(*tG)[33] = lds[kCx ? 6 : 7];
+  T(*tH)[33] = lds[kCx ? 6 : 8];
+  T(*tI)[33] = lds[kCx ? 6 : 9];
   for (int K = 0; 2 * K < nt; ++K) {
-    const int c0 = 2 * K, c1 = 2 * K + 1;
-    stamp(K, 0);
-    if (K > 0) {
-      wait_ge(&gs[c0 * nt + c0], K, ctl);
-      wait_ge(&gs[c0 * nt + c1], K, ctl);
-      wait_ge(&gs[c1 * nt + c1], K, ctl);
+    const int c0 = 2 * K, c1 = 2 * K + 1, r0 = c0 + 2;
+    if (kCx) stamp(K, 0);
+    if (K == 0) {
+      if (!inputs_ready(ctl)) return;
+      load_n<3, T>(tA, TileSrc{G, a.ldg, c0, c0}, tB, TileSrc{G, a.ldg, c0, c1}, tC, TileSrc{G, a.ldg, c1, c1},
+                   tC, TileSrc{}, tC, TileSrc{}, a.n);
+      __syncthreads();
     }
-    if (!inputs_ready(ctl)) return;
-    stamp(K, 1);
-    load_n<3, T>(tA, TileSrc{G, a.ldg, c0, c0}, tB, TileSrc{G, a.ldg, c0, c1}, tC, TileSrc{G, a.ldg, c1, c1},
-                 tC, TileSrc{}, tC, TileSrc{}, a.n);
-    __syncthreads();
-    stamp(K, 2);
+    if (kCx) {
+      stamp(K, 1);
+      stamp(K, 2);
+    }
     // h = 0: D(c0) on tA -> R00 (tA), X00 (tX); h = 1: R01 = X00^H G01 (tB),
     // G11 -= R01^H R01, D(c1) on tC -> R11 (tC), X11 (tD).  One loop body, so
     // the in-wave factorisation is instantiated once.
@@ -620,37 +715,139 @@ __device__ void chol_panel(const CholTileArgs& a, T (*tA)[33], T (*tB)[33], T (*
     stamp(K, 5);
     mm32<T>(tE, tB, false, tD, false, 1.0, false);        // R01 X11
     mm32<T>(tE, tX, false, tE, false, -1.0, false);       // X01 = -X00 (R01 X11)
-    put_tile<T>(tA, G, a.ldg, a.n, c0, c0);
-    put_tile<T>(tB, G, a.ldg, a.n, c0, c1);
-    put_tile<T>(tC, G, a.ldg, a.n, c1, c1);
-    clear_tile<T>(G, a.ldg, a.n, c1, c0);
-    put_tile<T>(tX, X, a.ldx, a.n, c0, c0);
-    put_tile<T>(tE, X, a.ldx, a.n, c0, c1);
-    put_tile<T>(tD, X, a.ldx, a.n, c1, c1);
-    clear_tile<T>(X, a.ldx, a.n, c1, c0);
-    int* fp[6] = {&zs[c0 * nt + c0], &zs[c0 * nt + c1], &zs[c1 * nt + c1],
-                  &gs[c0 * nt + c0], &gs[c0 * nt + c1], &gs[c1 * nt + c1]};
-    const int fv[6] = {1, 1, 1, K + 1, K + 1, K + 1};
-    publish_n(6, fp, fv);
     if constexpr (!ScalarTraits<T>::is_complex) {
+      // Real data, ten tiles: the pair's block first (R, X and the Xc copy,
+      // one drain, its counters), so the workers' tasks of this pair start
+      // at once; then the look-ahead's seven loads in one batch (G of rows
+      // c0, c1 in columns r0, r0 + 1 into tF .. tI, the next block into
+      // tA .. tC), its R tiles, their stores draining behind the update of
+      // the next block, and their counters.
+      put_tile<T>(tA, G, a.ldg, a.n, c0, c0);
+      put_tile<T>(tB, G, a.ldg, a.n, c0, c1);
+      put_tile<T>(tC, G, a.ldg, a.n, c1, c1);
+      clear_tile<T>(G, a.ldg, a.n, c1, c0);
+      put_tile<T>(tX, X, a.ldx, a.n, c0, c0);
+      put_tile<T>(tE, X, a.ldx, a.n, c0, c1);
+      put_tile<T>(tD, X, a.ldx, a.n, c1, c1);
+      clear_tile<T>(X, a.ldx, a.n, c1, c0);
       if (a.ny > 0) {
-        // the pair's final X tiles (and the zero lower one) into Xc, then the count
         put_tile<T>(tX, a.Xc, a.ldxc, a.n, c0, c0);
         put_tile<T>(tE, a.Xc, a.ldxc, a.n, c0, c1);
         put_tile<T>(tD, a.Xc, a.ldxc, a.n, c1, c1);
         clear_tile<T>(a.Xc, a.ldxc, a.n, c1, c0);
-        publish_n(0, nullptr, nullptr);
-        if (threadIdx.x == 0) atomicAdd(a.xdone + K, 1);
+      }
+      {
+        int* fp[6] = {&zs[c0 * nt + c0], &zs[c0 * nt + c1], &zs[c1 * nt + c1],
+                      &gs[c0 * nt + c0], &gs[c0 * nt + c1], &gs[c1 * nt + c1]};
+        const int fv[6] = {1, 1, 1, K + 1, K + 1, K + 1};
+        publish_n(6, fp, fv);
+        if (a.ny > 0 && threadIdx.x == 0) atomicAdd(a.xdone + K, 1);
+      }
+      if (r0 < nt) {
+        {
+          int* const wp[7] = {&gs[c0 * nt + r0], &gs[c1 * nt + r0], &gs[c0 * nt + r0 + 1], &gs[c1 * nt + r0 + 1],
+                              &gs[r0 * nt + r0], &gs[r0 * nt + r0 + 1], &gs[(r0 + 1) * nt + r0 + 1]};
+          const int wv[7] = {K, K, K, K, K, K, K};
+          if (!wait_all<7>(wp, wv, ctl)) return;
+        }
+        stamp(K, 1);
+        load_7(tF, TileSrc{G, a.ldg, c0, r0}, tG, TileSrc{G, a.ldg, c1, r0}, tH, TileSrc{G, a.ldg, c0, r0 + 1},
+               tI, TileSrc{G, a.ldg, c1, r0 + 1}, tA, TileSrc{G, a.ldg, r0, r0}, tB, TileSrc{G, a.ldg, r0, r0 + 1},
+               tC, TileSrc{G, a.ldg, r0 + 1, r0 + 1}, a.n);
+        stamp(K, 2);
+        mm32_2<T>(tG, tE, tF, tD, tG, true, false, 1.0, false);  // R_c1,r0 = X01^H G_c0,r0 + X11^H G_c1,r0
+        mm32<T>(tF, tX, true, tF, false, 1.0, false);            // R_c0,r0 = X00^H G_c0,r0
+        mm32_2<T>(tI, tE, tH, tD, tI, true, false, 1.0, false);  // R_c1,r0+1
+        mm32<T>(tH, tX, true, tH, false, 1.0, false);            // R_c0,r0+1
+        stamp(K, 0);
+        put_tile<T>(tF, G, a.ldg, a.n, c0, r0);
+        put_tile<T>(tG, G, a.ldg, a.n, c1, r0);
+        put_tile<T>(tH, G, a.ldg, a.n, c0, r0 + 1);
+        put_tile<T>(tI, G, a.ldg, a.n, c1, r0 + 1);
+        clear_tile<T>(G, a.ldg, a.n, r0, c0);
+        clear_tile<T>(G, a.ldg, a.n, r0, c1);
+        clear_tile<T>(G, a.ldg, a.n, r0 + 1, c0);
+        clear_tile<T>(G, a.ldg, a.n, r0 + 1, c1);
+        mm32_2<T>(tA, tF, tF, tG, tG, true, false, -1.0, true);  // G_r0,r0 -= R_.,r0^H R_.,r0
+        mm32_2<T>(tB, tF, tH, tG, tI, true, false, -1.0, true);  // G_r0,r0+1 -= R_.,r0^H R_.,r0+1
+        mm32_2<T>(tC, tH, tH, tI, tI, true, false, -1.0, true);  // G_r0+1,r0+1 -= R_.,r0+1^H R_.,r0+1
+        int* rp[4] = {&gs[c0 * nt + r0], &gs[c1 * nt + r0], &gs[c0 * nt + r0 + 1], &gs[c1 * nt + r0 + 1]};
+        const int rv[4] = {K + 1, K + 1, K + 1, K + 1};
+        publish_n(4, rp, rv);
+      }
+      stamp(K, 6);
+    } else {
+      // Complex data, seven tiles: publish the block, form and publish the
+      // look-ahead's R tiles (tA, tB, tC, tF), then load the next block into
+      // the X tiles and update it; the roles rotate.
+      put_tile<T>(tA, G, a.ldg, a.n, c0, c0);
+      put_tile<T>(tB, G, a.ldg, a.n, c0, c1);
+      put_tile<T>(tC, G, a.ldg, a.n, c1, c1);
+      clear_tile<T>(G, a.ldg, a.n, c1, c0);
+      put_tile<T>(tX, X, a.ldx, a.n, c0, c0);
+      put_tile<T>(tE, X, a.ldx, a.n, c0, c1);
+      put_tile<T>(tD, X, a.ldx, a.n, c1, c1);
+      clear_tile<T>(X, a.ldx, a.n, c1, c0);
+      int* fp[6] = {&zs[c0 * nt + c0], &zs[c0 * nt + c1], &zs[c1 * nt + c1],
+                    &gs[c0 * nt + c0], &gs[c0 * nt + c1], &gs[c1 * nt + c1]};
+      const int fv[6] = {1, 1, 1, K + 1, K + 1, K + 1};
+      publish_n(6, fp, fv);
+      stamp(K, 6);
+      if (r0 < nt) {
+        {
+          int* const wp[4] = {&gs[c0 * nt + r0], &gs[c1 * nt + r0], &gs[c0 * nt + r0 + 1], &gs[c1 * nt + r0 + 1]};
+          const int wv[4] = {K, K, K, K};
+          if (!wait_all<4>(wp, wv, ctl)) return;
+        }
+        load_n<4, T>(tA, TileSrc{G, a.ldg, c0, r0}, tB, TileSrc{G, a.ldg, c1, r0}, tC,
+                     TileSrc{G, a.ldg, c0, r0 + 1}, tF, TileSrc{G, a.ldg, c1, r0 + 1}, tF, TileSrc{}, a.n);
+        __syncthreads();
+        mm32_2<T>(tB, tE, tA, tD, tB, true, false, 1.0, false);  // R_c1,r0 = X01^H G_c0,r0 + X11^H G_c1,r0
+        mm32<T>(tA, tX, true, tA, false, 1.0, false);            // R_c0,r0 = X00^H G_c0,r0
+        mm32_2<T>(tF, tE, tC, tD, tF, true, false, 1.0, false);  // R_c1,r0+1
+        mm32<T>(tC, tX, true, tC, false, 1.0, false);            // R_c0,r0+1
+        put_tile<T>(tA, G, a.ldg, a.n, c0, r0);
+        put_tile<T>(tB, G, a.ldg, a.n, c1, r0);
+        put_tile<T>(tC, G, a.ldg, a.n, c0, r0 + 1);
+        put_tile<T>(tF, G, a.ldg, a.n, c1, r0 + 1);
+        clear_tile<T>(G, a.ldg, a.n, r0, c0);
+        clear_tile<T>(G, a.ldg, a.n, r0, c1);
+        clear_tile<T>(G, a.ldg, a.n, r0 + 1, c0);
+        clear_tile<T>(G, a.ldg, a.n, r0 + 1, c1);
+        int* rp[4] = {&gs[c0 * nt + r0], &gs[c1 * nt + r0], &gs[c0 * nt + r0 + 1], &gs[c1 * nt + r0 + 1]};
+        const int rv[4] = {K + 1, K + 1, K + 1, K + 1};
+        publish_n(4, rp, rv);
+        // the next block (K updates) into the X tiles, then pair K's update
+        {
+          int* const wp[3] = {&gs[r0 * nt + r0], &gs[r0 * nt + r0 + 1], &gs[(r0 + 1) * nt + r0 + 1]};
+          const int wv[3] = {K, K, K};
+          if (!wait_all<3>(wp, wv, ctl)) return;
+        }
+        load_n<3, T>(tX, TileSrc{G, a.ldg, r0, r0}, tE, TileSrc{G, a.ldg, r0, r0 + 1}, tD,
+                     TileSrc{G, a.ldg, r0 + 1, r0 + 1}, tD, TileSrc{}, tD, TileSrc{}, a.n);
+        __syncthreads();
+        mm32_2<T>(tX, tA, tA, tB, tB, true, false, -1.0, true);  // G_r0,r0 -= R_.,r0^H R_.,r0
+        mm32_2<T>(tE, tA, tC, tB, tF, true, false, -1.0, true);  // G_r0,r0+1 -= R_.,r0^H R_.,r0+1
+        mm32_2<T>(tD, tC, tC, tF, tF, true, false, -1.0, true);  // G_r0+1,r0+1 -= R_.,r0+1^H R_.,r0+1
+        T(*oA)[33] = tA;
+        T(*oB)[33] = tB;
+        T(*oC)[33] = tC;
+        tA = tX;
+        tB = tE;
+        tC = tD;
+        tX = oA;
+        tE = oB;
+        tD = oC;
       }
     }
-    stamp(K, 6);
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
-  // six 32 x 32 tiles (a right-product task reuses them)
-  __shared__ __align__(16) T lds[6][TB][33];
+  // the panel's 32 x 32 tiles (ten for real data, seven for complex), six
+  // for the worker tasks (a right-product task reuses them)
+  __shared__ __align__(16) T lds[kPanelTiles<T>][TB][33];
   static_assert(sizeof(lds) >= kYLdsDoubles * sizeof(double), "right-product slabs fit the tiles");
   T(*tA)[33] = lds[0];
   T(*tB)[33] = lds[1];
@@ -661,7 +858,7 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
   __shared__ __align__(16) T rowbuf[2 * TB];
   __shared__ int task_s;
   if (blockIdx.x == 0) {
-    chol_panel<T>(a, tA, tB, tC, tD, tE, tX, rowbuf);
+    chol_panel<T>(a, lds, rowbuf);
     return;
   }
   T* G = reinterpret_cast<T*>(a.G);
@@ -682,10 +879,12 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
     switch (tk.type) {
       case CT_P2: {  // [R_c0j; R_c1j] = X_KK^H [G_c0j; G_c1j]; the mirrored lower tiles of G are zeroed
         const int j = tk.j;
-        wait_ge(&gs[c1 * nt + c1], K + 1, ctl);  // the panel's block (R and X published together)
-        wait_ge(&gs[c0 * nt + j], K, ctl);
-        wait_ge(&gs[c1 * nt + j], K, ctl);
-        if (!inputs_ready(ctl)) break;
+        {
+          // the panel's block (R and X published together), G_c0j and G_c1j with K updates
+          int* const wp[3] = {&gs[c1 * nt + c1], &gs[c0 * nt + j], &gs[c1 * nt + j]};
+          const int wv[3] = {K + 1, K, K};
+          if (!wait_all<3>(wp, wv, ctl)) break;
+        }
         load_n<5, T>(tA, TileSrc{X, a.ldx, c0, c0}, tB, TileSrc{X, a.ldx, c0, c1}, tC, TileSrc{X, a.ldx, c1, c1},
                      tD, TileSrc{G, a.ldg, c0, j}, tE, TileSrc{G, a.ldg, c1, j}, a.n);
         __syncthreads();
@@ -702,12 +901,12 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
       }
       case CT_U2: {  // G_ij -= R_c0i^H R_c0j + R_c1i^H R_c1j
         const int i = tk.i, j = tk.j;
-        wait_ge(&gs[c0 * nt + i], K + 1, ctl);
-        wait_ge(&gs[c1 * nt + i], K + 1, ctl);
-        wait_ge(&gs[c0 * nt + j], K + 1, ctl);
-        wait_ge(&gs[c1 * nt + j], K + 1, ctl);
-        wait_ge(&gs[i * nt + j], K, ctl);
-        if (!inputs_ready(ctl)) break;
+        {
+          int* const wp[5] = {&gs[c0 * nt + i], &gs[c1 * nt + i], &gs[c0 * nt + j], &gs[c1 * nt + j],
+                              &gs[i * nt + j]};
+          const int wv[5] = {K + 1, K + 1, K + 1, K + 1, K};
+          if (!wait_all<5>(wp, wv, ctl)) break;
+        }
         if (i == j) {
           load_n<3, T>(tA, TileSrc{G, a.ldg, c0, i}, tB, TileSrc{G, a.ldg, c1, i}, tE, TileSrc{G, a.ldg, i, j},
                        tE, TileSrc{}, tE, TileSrc{}, a.n);
@@ -731,12 +930,13 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
         const int fin = K - base + 1;  // X_jr,c0 / X_jr,c1 final value
         const bool first = K == base;
         const bool has0 = jr <= c0;
-        if (has0) wait_ge(&zs[jr * nt + c0], fin, ctl);
-        wait_ge(&zs[jr * nt + c1], fin, ctl);
-        wait_ge(&gs[c0 * nt + l], K + 1, ctl);
-        wait_ge(&gs[c1 * nt + l], K + 1, ctl);
-        if (!first) wait_ge(&zs[jr * nt + l], K - base, ctl);
-        if (!inputs_ready(ctl)) break;
+        {
+          // (counters start at 0: a target of 0 is always met)
+          int* const wp[5] = {&zs[jr * nt + c0], &zs[jr * nt + c1], &gs[c0 * nt + l], &gs[c1 * nt + l],
+                              &zs[jr * nt + l]};
+          const int wv[5] = {has0 ? fin : 0, fin, K + 1, K + 1, first ? 0 : K - base};
+          if (!wait_all<5>(wp, wv, ctl)) break;
+        }
         {
           // tB = X_j,c1, tD = R_c1l, [tE = Z_jl], [tA = X_j,c0, tC = R_c0l]
           const TileSrc xj1{X, a.ldx, jr, c1}, r1l{G, a.ldg, c1, l}, zjl{X, a.ldx, jr, l},
@@ -757,10 +957,11 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
       case CT_PQ2: {  // [X_jc0 X_jc1] = [Z_jc0 Z_jc1] X_KK (j < c0); the mirrored lower tiles of X are zeroed
         const int jr = tk.j;
         const int cnt = K - jr / 2;  // UQ2 updates of Z_jr,c0 and Z_jr,c1
-        wait_ge(&gs[c1 * nt + c1], K + 1, ctl);
-        wait_ge(&zs[jr * nt + c0], cnt, ctl);
-        wait_ge(&zs[jr * nt + c1], cnt, ctl);
-        if (!inputs_ready(ctl)) break;
+        {
+          int* const wp[3] = {&gs[c1 * nt + c1], &zs[jr * nt + c0], &zs[jr * nt + c1]};
+          const int wv[3] = {K + 1, cnt, cnt};
+          if (!wait_all<3>(wp, wv, ctl)) break;
+        }
         load_n<5, T>(tA, TileSrc{X, a.ldx, jr, c0}, tB, TileSrc{X, a.ldx, jr, c1}, tC, TileSrc{X, a.ldx, c0, c0},
                      tD, TileSrc{X, a.ldx, c0, c1}, tE, TileSrc{X, a.ldx, c1, c1}, a.n);
         __syncthreads();
@@ -784,8 +985,11 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
         break;
       }
       case CT_Y: {  // rows of Y's column block K, once X's column block K is final
-        wait_ge(&a.xdone[K], 2 * K + 1, ctl);
-        if (!inputs_ready(ctl)) break;
+        {
+          int* const wp[1] = {&a.xdone[K]};
+          const int wv[1] = {2 * K + 1};
+          if (!wait_all<1>(wp, wv, ctl)) break;
+        }
         const long long t_start = (long long)__builtin_amdgcn_s_memrealtime();
         if constexpr (!ScalarTraits<T>::is_complex)
           y_task(a, K, tk.j % a.ny, K >= a.ysplit ? 1 + tk.j / a.ny : 0, reinterpret_cast<double*>(&lds[0][0][0]));
@@ -907,11 +1111,23 @@ int chol_tile_order_check(int ntiles, int ny, int ysplit) {
   int total = 0;
   for (int K = 0; K < T / 2; ++K) {
     const int c0 = 2 * K, c1 = 2 * K + 1;
-    // the panel's iteration K
-    if (G(c0, c0) < K || G(c0, c1) < K || G(c1, c1) < K) return -1;
+    // the panel's iteration K (its block carries K updates: K = 0, or the
+    // look-ahead of iteration K - 1 checked below)
+    if (G(c0, c0) != K || G(c0, c1) != K || G(c1, c1) != K) return -1;
     G(c0, c0) = G(c0, c1) = G(c1, c1) = K + 1;
     Z(c0, c0) = Z(c0, c1) = Z(c1, c1) = 1;
     ++xd[K];
+    const int r0 = c0 + 2;
+    if (r0 < T) {
+      // the look-ahead waits only for queue tasks of earlier iterations
+      for (int r = c0; r <= c1; ++r)
+        for (int c = r0; c <= r0 + 1; ++c) {
+          if (G(r, c) != K) return -1;
+          G(r, c) = K + 1;
+        }
+      if (G(r0, r0) != K || G(r0, r0 + 1) != K || G(r0 + 1, r0 + 1) != K) return -1;
+      G(r0, r0) = G(r0, r0 + 1) = G(r0 + 1, r0 + 1) = K + 1;
+    }
     const int cnt = chol_iter_total(T, K, ny, ysplit);
     for (int u = 0; u < cnt; ++u) {
       const CholTask t = chol_task_in_iter(T, K, ny, u, ysplit);

@@ -93,12 +93,15 @@ def test_workspace_queries_accept_empty_shapes():
 def test_chol_tile_task_order(nt, ny, ysplit):
     """The persistent Cholesky kernel's task order (host code in the library):
     every task appears once and every dependency precedes it (the panel's
-    step K runs before the tasks of step K), so a workgroup waiting on a task
-    it took can always be satisfied.  Tiles T = nt rounded up to even; per
-    pair step K with rem = T - 2K - 2 tiles after the pair: rem P2, the next
-    pair's 2 rem - 1 U2, 2K PQ2, (2K + 2) rem UQ2, the bulk b(b+1)/2 U2
-    (b = rem - 2) and ny right-product tasks (2 ny from pair ysplit on, one
-    per half of the contraction), each after X's column block K is final."""
+    step K, with its look-ahead to the next pair, runs before the tasks of
+    step K and waits only for tasks of earlier steps), so a workgroup waiting
+    on a task it took can always be satisfied.  Tiles T = nt rounded up to
+    even; per pair step K with rem = T - 2K - 2 tiles after the pair and
+    b = max(rem - 2, 0): b P2 (the panel forms the next pair's two), the next
+    pair's 2b U2 right of its diagonal block (the panel updates that block),
+    2K PQ2, (2K + 2) rem UQ2, the bulk b(b+1)/2 U2 and ny right-product tasks
+    (2 ny from pair ysplit on, one per half of the contraction), each after
+    X's column block K is final."""
     lib = _lib.load()
     lib.maktest_chol_order_check.restype = C.c_int
     lib.maktest_chol_order_check.argtypes = [C.c_int, C.c_int, C.c_int]
@@ -107,7 +110,7 @@ def test_chol_tile_task_order(nt, ny, ysplit):
     for k in range(T // 2):
         rem = T - 2 * k - 2
         b = max(rem - 2, 0)
-        expect += rem + (2 * rem - 1 if rem > 0 else 0) + 2 * k + (2 * k + 2) * rem + b * (b + 1) // 2
+        expect += b + 2 * b + 2 * k + (2 * k + 2) * rem + b * (b + 1) // 2
         expect += ny * (2 if k >= ysplit else 1)
     assert lib.maktest_chol_order_check(nt, ny, ysplit) == expect
     assert lib.maktest_chol_order_check(0, ny, ysplit) == -1
