@@ -419,71 +419,4 @@ __device__ __forceinline__ int wave_chol_aug32(T (*S)[33], T (*Y)[33], T* row) {
   return bad;
 }
 
-// Cholesky factorisation with inverse of a 32 x 32 Hermitian positive definite
-// tile by a whole 256-thread workgroup (every thread must call it):
-//   S (LDS, 32 x 33): upper triangle of G on entry (strict lower part not
-//   read); on exit R (upper, zero below) with G = R^H R.
-//   Y (LDS, 32 x 33): on exit X = R^{-1} (upper, zero below).
-//   rows: LDS scratch of 4 * 32 elements (two double-buffered row pairs).
-// Thread (c, q) = (t % 32, t / 32) keeps rows q, q + 8, q + 16, q + 24 of
-// column c of G and of M = R^{-H} (M starts as I) in registers.  Step k: the
-// owners of row k publish its raw values (G'(k, :), M'(k, :)) in LDS, one
-// workgroup barrier, then every thread updates its rows r > k,
-//   G'(r, c) -= conj(G'(k, r)) G'(k, c) / d,  M'(r, c) -= conj(G'(k, r)) M'(k, c) / d,
-// with d = G'(k, k), and the owners scale row k by 1/sqrt(d).  The row
-// buffers alternate between steps, so one barrier per column suffices.
-// Returns 0, or k + 1 for the first pivot k that is not positive.
-template <typename T>
-__device__ __forceinline__ int wg_chol_aug32(T (*S)[33], T (*Y)[33], T* rows) {
-  const int t = threadIdx.x;
-  const int c = t & 31, q = t >> 5;
-  T g[4], mm[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int r = q + 8 * i;
-    g[i] = (r <= c) ? S[r][c] : s_zero(T());
-    mm[i] = (r == c) ? s_one(T()) : s_zero(T());
-  }
-  int bad = 0;
-#pragma unroll
-  for (int k = 0; k < 32; ++k) {
-    T* rg = rows + (k & 1) * 64;
-    T* rm = rg + 32;
-    const int qk = k & 7, ik = k >> 3;
-    if (q == qk) {
-      rg[c] = g[ik];
-      rm[c] = mm[ik];
-    }
-    __syncthreads();
-    const double d = s_re(rg[k]);
-    if (bad == 0 && !(d > 0.0)) bad = k + 1;
-    const double dd = (d > 0.0) ? d : 1.0;
-    const double inv = recip_nr(dd);
-    const T a = s_scale(rg[c], inv), b = s_scale(rm[c], inv);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int r = q + 8 * i;
-      if (r > k) {
-        const T cf = s_conj(rg[r]);
-        g[i] = s_sub(g[i], s_mul(cf, a));
-        mm[i] = s_sub(mm[i], s_mul(cf, b));
-      }
-    }
-    if (q == qk) {
-      const double rs = rsqrt_nr(dd);
-      g[ik] = s_scale(rg[c], rs);
-      mm[ik] = s_scale(rm[c], rs);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int r = q + 8 * i;
-    S[r][c] = (r <= c) ? g[i] : s_zero(T());
-    Y[c][r] = (r >= c) ? s_conj(mm[i]) : s_zero(T());  // X = M^H
-  }
-  __syncthreads();
-  return bad;
-}
-
 }  // namespace mak
