@@ -44,6 +44,12 @@ constexpr int kRedWgs = 1024;  // workgroups of the second pass's sweep (and its
 // of the accumulation (typically sqrt(n) u_32 ||E|| ~ 2e-16), against the
 // FP64 product's own 1e-16.  Gaussian inputs have ||E||_F ~ 1e-13.
 constexpr double kLoCorrMaxE = 1e-10;
+// ... and on BF16 MFMA (FP32 accumulation) when ||E||_F <= kBf16CorrMaxE: the
+// correction Q1 D has ||D||_F ~ ||E||_F / 2, and rounding both operands to
+// BF16 (relative 2^-8) errs by at most 2^-8 ||Q1||_F ||D||_F ~ 2e-3 sqrt(n)
+// ||E||_F <= 1e-16 sqrt(n): no more than Q's own FP64 rounding (u sqrt(n)),
+// and in practice (errors of random sign) some thirty times less.
+constexpr double kBf16CorrMaxE = 5e-14;
 
 struct CQ2Layout {
   size_t offQ1, offG, offE, offX, offU, offW, offT, offSl, offFlags, offPart, offSt, offXc, bytes;
@@ -202,10 +208,11 @@ __global__ void __launch_bounds__(256) sum_r_slices_kernel(const T* P, int ks, i
 
 // st[0] = 1 (accept) when the Cholesky succeeded (info == 0) and ||E||_F <=
 // 1e-6; st[1] = 1 when the second-order terms are needed (||E||_F > 1e-9);
-// of the accepted calls, st[3] = 1 when Q = Q1 + Q1 (W2^{-1} - I) takes its
-// correction from FP32 MFMA (lo_ok: real data, and ||E||_F <= kLoCorrMaxE)
-// and st[6] = 1 otherwise (Q = Q1 W2^{-1} on FP64 MFMA); st[4..5] = ||E||_F
-// as a double.
+// of the accepted calls, exactly one of st[7], st[3], st[6] is 1: Q = Q1 +
+// Q1 (W2^{-1} - I) takes its correction from BF16 MFMA (st[7]: lo_ok, i.e.
+// real data, and ||E||_F <= kBf16CorrMaxE) or from FP32 MFMA (st[3]: lo_ok
+// and ||E||_F <= kLoCorrMaxE), else Q = Q1 W2^{-1} runs on FP64 MFMA (st[6]);
+// st[4..5] = ||E||_F as a double.
 __global__ void cq2_decide_kernel(const double* part, int np, const int* cinfo, int lo_ok, int* st) {
   __shared__ double red[4];
   double acc = 0.0;
@@ -218,8 +225,9 @@ __global__ void cq2_decide_kernel(const double* part, int np, const int* cinfo, 
     const bool ok = (*cinfo == 0) && (e <= 1e-6);  // false for NaN
     st[0] = ok ? 1 : 0;
     st[1] = (ok && e > 1e-9) ? 1 : 0;
-    st[3] = (ok && lo_ok && e <= kLoCorrMaxE) ? 1 : 0;
-    st[6] = (ok && st[3] == 0) ? 1 : 0;
+    st[7] = (ok && lo_ok && e <= kBf16CorrMaxE) ? 1 : 0;
+    st[3] = (ok && lo_ok && st[7] == 0 && e <= kLoCorrMaxE) ? 1 : 0;
+    st[6] = (ok && st[3] == 0 && st[7] == 0) ? 1 : 0;
     double* ed = reinterpret_cast<double*>(st + 4);
     ed[0] = e;
   }
@@ -329,11 +337,14 @@ void cholqr2_enqueue(hipStream_t s, int m, int n, const T* A, int64_t lda, T* Q,
   if constexpr (ScalarTraits<T>::is_complex) {
     gemm<T>(OP_N, OP_N, gq<T>(m, n, n, Q1, m, X, n, Q, ldq, 1.0, 0.0, TRI_B_UPPER, st + 6, 1), s);
   } else {
-    // ||E|| small (st[3]): Q = Q1 + Q1 D with the correction on FP32 MFMA
-    // (the packed operands go where the Gram slices were); else on FP64
-    GemmArgs g = gq<T>(m, n, n, Q1, m, D, n, Q, ldq, 1.0, 1.0, TRI_B_UPPER, st + 3, 1);
+    // ||E|| small: Q = Q1 + Q1 D with the correction on BF16 (st[7]) or FP32
+    // (st[3]) MFMA (the packed operands go where the Gram slices were); else
+    // on FP64
+    GemmArgs g = gq<T>(m, n, n, Q1, m, D, n, Q, ldq, 1.0, 1.0, TRI_B_UPPER, st + 7, 1);
     g.Cin = Q1;
     g.ldcin = m;
+    gemm_prec<T>(16, OP_N, OP_N, g, Sl, s);
+    g.gate = st + 3;
     gemm_prec<T>(32, OP_N, OP_N, g, Sl, s);
     gemm<T>(OP_N, OP_N, gq<T>(m, n, n, Q1, m, X, n, Q, ldq, 1.0, 0.0, TRI_B_UPPER, st + 6, 1), s);
   }

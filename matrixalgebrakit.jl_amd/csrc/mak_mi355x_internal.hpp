@@ -28,7 +28,7 @@ void orgqr_blocked(const T* A, int64_t lda, int m, int n, T* Q, int64_t ldq, int
 template <typename T>
 int qr_driver(hipStream_t s, int full, int m, int n, T* A, int64_t lda, T* Q, int64_t ldq, T* R, int64_t ldr,
               int positive, void* work, size_t work_bytes, T* Tout);
-template <typename T> void qr_record(const char* ws, int m, int n, int full, int out[8]);
+template <typename T> void qr_record(const char* ws, int m, int n, int full, int out[9]);
 
 // CholeskyQR2 panel with Householder reconstruction (mak_cholqr.hip): factors
 // the H x nbc panel Ap (columns j0.. of the matrix) in place into compact WY

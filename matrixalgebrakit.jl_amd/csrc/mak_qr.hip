@@ -495,7 +495,7 @@ void orgqr_blocked(const T* A, int64_t lda, int m, int n, T* Q, int64_t ldq, int
 // Status words of the last call that used this workspace (read by the test
 // suite through maktest_qr_record): the CholeskyQR2 panel flags (flag, ticket,
 // first rejected column) and the whole-matrix CholeskyQR2 status.
-template <typename T> void qr_record(const char* ws, int m, int n, int full, int out[8]) {
+template <typename T> void qr_record(const char* ws, int m, int n, int full, int out[9]) {
   const int minmn = std::min(m, n);
   QrLayout<T> L = qr_layout<T>(m, n, full ? m : minmn);
   int h[3] = {0, 0, 0};
@@ -511,6 +511,7 @@ template <typename T> void qr_record(const char* ws, int m, int n, int full, int
   out[5] = st[5];
   out[6] = st[3];                  // 1: its final product took the FP32 correction
   out[7] = st[6];                  // 1: its final product ran on FP64
+  out[8] = st[7];                  // 1: its final product took the BF16 correction
 }
 
 template <typename T>

@@ -60,7 +60,9 @@ int maktest_gemm_prec(void* stream, int prec, int opA, int opB, int M, int N, in
 // out[0] 1 when the whole-matrix CholeskyQR2 result was taken, out[1] the
 // first 32-column panel handed to Householder TSQR (-1: none), out[2] the
 // whole-matrix path's Cholesky info, out[3] 1 when that path exists for the
-// shape, out[4..5] its ||Q1^H Q1 - I||_F (a double).
+// shape, out[4..5] its ||Q1^H Q1 - I||_F (a double), out[6 / 7 / 8] 1 when
+// its final product took the FP32 correction / ran on FP64 / took the BF16
+// correction (nine ints).
 int maktest_qr_record(int dtype, int full, int m, int n, const void* work, int* out) {
   if (dtype == MAK_F64) mak::qr_record<double>((const char*)work, m, n, full, out);
   else mak::qr_record<mak::zd>((const char*)work, m, n, full, out);

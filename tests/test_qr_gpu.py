@@ -32,7 +32,8 @@ def run_qr_rec(A, dev, full=False, positive=True, wy=False):
     (maktest_qr_record): returns Q, R and the record
     [whole-matrix CholeskyQR2 taken, first rejected panel column, its
     Cholesky info, whole-matrix path available, ||E||_F (two ints of a
-    double), FP32 correction taken, FP64 final product taken]."""
+    double), FP32 correction taken, FP64 final product taken, BF16
+    correction taken]."""
     import ctypes as C
     from mak_mi355x import _lib
     At = to_dev(A, dev)
@@ -57,7 +58,7 @@ def run_qr_rec(A, dev, full=False, positive=True, wy=False):
                                                                             mak.api._ptr(ws), nb.value)
     torch.cuda.synchronize()
     assert code == 0
-    rec = (C.c_int * 8)()
+    rec = (C.c_int * 9)()
     assert lib.maktest_qr_record(dt, int(full), m, n, C.c_void_p(ws.data_ptr()), rec) == 0
     return Q.cpu().numpy(), R.cpu().numpy(), list(rec)
 
@@ -232,10 +233,13 @@ def test_qr_whole_matrix_cholqr2_matches_oracle(m, n, dt, dev):
     Q, R, rec = run_qr_rec(A, dev)
     assert rec[3] == 1 and rec[0] == 1 and rec[2] == 0, rec
     # one final product; real Gaussian inputs have ||Q1^H Q1 - I|| small
-    # enough for the FP32 correction (complex data: always FP64)
-    assert rec[6] + rec[7] == 1
+    # enough for the BF16 (<= 5e-14) or FP32 (<= 1e-10) correction (complex
+    # data: always FP64)
+    assert rec[6] + rec[7] + rec[8] == 1
     if dt == np.float64:
-        assert (rec[6] == 1) == (rec_norm(rec) <= 1e-10), rec_norm(rec)
+        e = rec_norm(rec)
+        assert (rec[8] == 1) == (e <= 5e-14), e
+        assert (rec[6] == 1) == (5e-14 < e <= 1e-10), e
     else:
         assert rec[7] == 1
     Qo, Ro = O.qr_compact(A, positive=True)
@@ -260,11 +264,14 @@ def test_qr_whole_matrix_cholqr2_conditioning(cond, dev):
         assert rec[0] == 1
     if cond >= 1e8:
         assert rec[0] == 0
-    # an accepted call takes exactly one final product, FP32 correction only
-    # for ||E||_F <= 1e-10 (cond 1e5 here: ||E|| ~ 1e-7, the FP64 product)
-    assert rec[6] + rec[7] == rec[0]
+    # an accepted call takes exactly one final product, the BF16 correction
+    # for ||E||_F <= 5e-14, FP32 up to 1e-10 (cond 1e5 here: ||E|| ~ 1e-7, the
+    # FP64 product)
+    assert rec[6] + rec[7] + rec[8] == rec[0]
     if rec[0]:
-        assert (rec[6] == 1) == (rec_norm(rec) <= 1e-10), rec_norm(rec)
+        e = rec_norm(rec)
+        assert (rec[8] == 1) == (e <= 5e-14), e
+        assert (rec[6] == 1) == (5e-14 < e <= 1e-10), e
     if cond == 1e5:
         assert rec[7] == 1
     assert rel(Q @ R, A) < 1e-14
