@@ -39,7 +39,9 @@ constexpr int kRedWgs = 1024;  // workgroups of the second pass's sweep (and its
 
 // Largest ||E||_F for which the final Q = Q1 + Q1 D (D = W2^{-1} - I, upper
 // triangular, ||D||_F ~ ||E||_F) takes the product Q1 D from FP32 MFMA with
-// FP32 accumulation (real data): its rounding error is at most about
+// FP32 accumulation (complex data through its real embedding, where each
+// entry of the product is a sum of 2n real terms): its rounding error is at
+// most about
 // n u_32 ||D||_F <= 1e3 * 6e-8 * 1e-10 = 6e-15 per row even in the worst case
 // of the accumulation (typically sqrt(n) u_32 ||E|| ~ 2e-16), against the
 // FP64 product's own 1e-16.  Gaussian inputs have ||E||_F ~ 1e-13.
@@ -84,8 +86,9 @@ template <typename T> CQ2Layout cq2_layout(int m, int n) {
   L.offU = take(nn);
   L.offW = take(nn);
   L.offT = take(nn);
-  // the Gram slices; later the packed operands of the FP32 correction
-  L.offSl = take(std::max(nn * L.ks, ScalarTraits<T>::is_complex ? 0 : gemm_prec_workspace_bytes(m, n, n)));
+  // the Gram slices; later the packed operands of the BF16 / FP32 correction
+  L.offSl = take(std::max(nn * L.ks, ScalarTraits<T>::is_complex ? gemm_lo_complex_workspace_bytes(m, n, n)
+                                                                    : gemm_prec_workspace_bytes(m, n, n)));
   L.offFlags = take(chol_tile_flag_ints(n) * sizeof(int));
   L.offPart = take(kRedWgs * sizeof(double));
   L.offSt = take(64 * sizeof(int));
@@ -209,9 +212,9 @@ __global__ void __launch_bounds__(256) sum_r_slices_kernel(const T* P, int ks, i
 // st[0] = 1 (accept) when the Cholesky succeeded (info == 0) and ||E||_F <=
 // 1e-6; st[1] = 1 when the second-order terms are needed (||E||_F > 1e-9);
 // of the accepted calls, exactly one of st[7], st[3], st[6] is 1: Q = Q1 +
-// Q1 (W2^{-1} - I) takes its correction from BF16 MFMA (st[7]: lo_ok, i.e.
-// real data, and ||E||_F <= kBf16CorrMaxE) or from FP32 MFMA (st[3]: lo_ok
-// and ||E||_F <= kLoCorrMaxE), else Q = Q1 W2^{-1} runs on FP64 MFMA (st[6]);
+// Q1 (W2^{-1} - I) takes its correction from BF16 MFMA (st[7]: lo_ok and
+// ||E||_F <= kBf16CorrMaxE) or from FP32 MFMA (st[3]: lo_ok and ||E||_F <=
+// kLoCorrMaxE), else Q = Q1 W2^{-1} runs on FP64 MFMA (st[6]);
 // st[4..5] = ||E||_F as a double.
 __global__ void cq2_decide_kernel(const double* part, int np, const int* cinfo, int lo_ok, int* st) {
   __shared__ double red[4];
@@ -320,8 +323,7 @@ void cholqr2_enqueue(hipStream_t s, int m, int n, const T* A, int64_t lda, T* Q,
   // sweep), the decision, and W2, W2^{-1} in closed form
   const int ks2 = gram_upper_slices<T>(Q1, m, m, n, Sl, L.ks, s);
   hipLaunchKernelGGL((cq2_sum_up_kernel<T>), dim3(kRedWgs), dim3(256), 0, s, Sl, ks2, n, E, U, part);
-  hipLaunchKernelGGL(cq2_decide_kernel, dim3(1), dim3(256), 0, s, part, kRedWgs, st + 2,
-                     ScalarTraits<T>::is_complex ? 0 : 1, st);
+  hipLaunchKernelGGL(cq2_decide_kernel, dim3(1), dim3(256), 0, s, part, kRedWgs, st + 2, 1, st);
   // second order (st[1] == 1): U = up(E - U1^H U1), U2 = U U
   {
     GemmArgs g = gq<T>(n, n, n, U, n, U, n, Tm, n, 1.0, 0.0, TRI_A_LOWER | TRI_B_UPPER, st + 1, 1);
@@ -330,22 +332,26 @@ void cholqr2_enqueue(hipStream_t s, int m, int n, const T* A, int64_t lda, T* Q,
   }
   hipLaunchKernelGGL((cq2_up_kernel<T>), dim3(eg), dim3(256), 0, s, E, (const T*)Tm, n, U, (const int*)(st + 1));
   gemm<T>(OP_N, OP_N, gq<T>(n, n, n, U, n, U, n, Tm, n, 1.0, 0.0, TRI_A_UPPER | TRI_B_UPPER, st + 1, 1), s);
-  // (real data: D = W2^{-1} - I into E, which is no longer read)
-  T* D = ScalarTraits<T>::is_complex ? nullptr : E;
+  // D = W2^{-1} - I into E, which is no longer read
+  T* D = E;
   hipLaunchKernelGGL((cq2_w_kernel<T>), dim3(eg), dim3(256), 0, s, U, Tm, st, n, W, X, D);
-  // Q = Q1 W2^{-1};  R = W2 R1 (R1 is upper with a zero strict lower part)
-  if constexpr (ScalarTraits<T>::is_complex) {
-    gemm<T>(OP_N, OP_N, gq<T>(m, n, n, Q1, m, X, n, Q, ldq, 1.0, 0.0, TRI_B_UPPER, st + 6, 1), s);
-  } else {
-    // ||E|| small: Q = Q1 + Q1 D with the correction on BF16 (st[7]) or FP32
-    // (st[3]) MFMA (the packed operands go where the Gram slices were); else
-    // on FP64
+  // Q = Q1 W2^{-1};  R = W2 R1 (R1 is upper with a zero strict lower part).
+  // ||E|| small: Q = Q1 + Q1 D with the correction on BF16 (st[7]) or FP32
+  // (st[3]) MFMA (complex data through its real embedding; the packed
+  // operands go where the Gram slices were); else Q1 W2^{-1} on FP64
+  {
     GemmArgs g = gq<T>(m, n, n, Q1, m, D, n, Q, ldq, 1.0, 1.0, TRI_B_UPPER, st + 7, 1);
     g.Cin = Q1;
     g.ldcin = m;
-    gemm_prec<T>(16, OP_N, OP_N, g, Sl, s);
-    g.gate = st + 3;
-    gemm_prec<T>(32, OP_N, OP_N, g, Sl, s);
+    if constexpr (ScalarTraits<T>::is_complex) {
+      gemm_lo_complex(16, g, Sl, s);
+      g.gate = st + 3;
+      gemm_lo_complex(32, g, Sl, s);
+    } else {
+      gemm_prec<T>(16, OP_N, OP_N, g, Sl, s);
+      g.gate = st + 3;
+      gemm_prec<T>(32, OP_N, OP_N, g, Sl, s);
+    }
     gemm<T>(OP_N, OP_N, gq<T>(m, n, n, Q1, m, X, n, Q, ldq, 1.0, 0.0, TRI_B_UPPER, st + 6, 1), s);
   }
   if (R != nullptr) {

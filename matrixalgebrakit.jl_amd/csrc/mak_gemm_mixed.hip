@@ -72,6 +72,61 @@ void pack_lo(const double* S, int64_t ld, int kcontig, int R, int K, TL* D, int6
   hipLaunchKernelGGL((pack_lo_kernel<TL>), grid, dim3(256), 0, s, S, ld, kcontig, R, K, D, Kp, gate, want);
 }
 
+// Complex data through its real embedding, with the real and imaginary parts
+// interleaved along k (k' = 2 k + part): for C = A B (A M x K, B K x N
+// complex), C_re and C_im side by side (M x 2N) = [A packed: A(i, k).re and
+// .im at k' = 2k, 2k + 1] x [B embedded: 2K x 2N, (2k + p, j) = the entry
+// of [[B.re, B.im], [-B.im, B.re]] in block row p, column j].  A k-clip for
+// upper triangular B stays a prefix of k' (k <= j).
+//
+// A (M x K complex, ld in complex entries) -> D (Rp x Kp, k' contiguous):
+// one 64-row x 32-column block per workgroup, read down the columns (16-byte
+// steps) and written along k' through LDS.
+template <typename TL>
+__global__ void __launch_bounds__(256) pack_cplx_a_kernel(const double* S, int64_t ld, int R, int K, TL* D,
+                                                          int64_t Kp, const int* gate, int want) {
+  if (gate_closed(gate, want)) return;
+  __shared__ float tre[32][65], tim[32][65];
+  const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int q = ty; q < 32; q += 4) {
+    const int r = r0 + tx, k = k0 + q;
+    float re = 0.f, im = 0.f;
+    if (r < R && k < K) {
+      const double* e = S + 2 * (r + (int64_t)k * ld);
+      re = (float)e[0];
+      im = (float)e[1];
+    }
+    tre[q][tx] = re;
+    tim[q][tx] = im;
+  }
+  __syncthreads();
+  for (int q = ty; q < 64; q += 4) {
+    const float v = (tx & 1) ? tim[tx >> 1][q] : tre[tx >> 1][q];
+    D[(int64_t)(r0 + q) * Kp + 2 * k0 + tx] = (TL)v;
+  }
+}
+// B (K x N complex) -> D (Np x Kp): row j < 2N, column k' < 2K holds the
+// embedded entry above (zero beyond); reads along k are contiguous.
+template <typename TL>
+__global__ void __launch_bounds__(256) pack_cplx_b_kernel(const double* S, int64_t ld, int K, int N, TL* D,
+                                                          int64_t Kp, const int* gate, int want) {
+  if (gate_closed(gate, want)) return;
+  const int j0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int kp = k0 + tx, k = kp >> 1, pk = kp & 1;
+  for (int q = ty; q < 64; q += 4) {
+    const int j = j0 + q;
+    float v = 0.f;
+    if (j < 2 * N && k < K) {
+      const int pj = j >= N ? 1 : 0, c = j - pj * N;
+      const double x = S[2 * (k + (int64_t)c * ld) + (pk == pj ? 0 : 1)];
+      v = (float)((pk == 1 && pj == 0) ? -x : x);
+    }
+    D[(int64_t)j * Kp + kp] = (TL)v;
+  }
+}
+
 // native 16-byte vector (HIP's uint4 is a struct, which keeps arrays of it in
 // scratch memory)
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -91,6 +146,9 @@ struct LoGemmArgs {
   int upper;             // only the tiles on or above the diagonal (Hermitian results)
   const int* gate;       // run only if *gate == gate_want (when non-null)
   int gate_want;
+  int cplx_n;            // > 0: complex C (N = 2 cplx_n real columns, C_re then C_im) from the
+                         // real embedding; C and Cin are then complex arrays (ld in complex entries)
+  int cplx_upperB;       // complex B upper triangular: k' clipped to 2 (max column of the tile + 1)
 };
 
 // Tile order: XCD-contiguous ranges of tiles, except under triangular hints,
@@ -136,6 +194,12 @@ __global__ void __launch_bounds__(256, 2) gemm_lo_kernel(LoGemmArgs g) {
   if (g.tri & TRI_B_UPPER) kend = std::min<int64_t>(kend, n0 + kLoRows);
   if (g.tri & TRI_A_UPPER) kbeg = std::max<int64_t>(kbeg, m0);
   if (g.tri & TRI_B_LOWER) kbeg = std::max<int64_t>(kbeg, n0);
+  if (g.cplx_upperB) {
+    // the tile's complex columns: c = j (C_re half) or j - cplx_n (C_im half)
+    const int n = g.cplx_n, jlast = n0 + kLoRows - 1;
+    const int cmax = (jlast < n) ? jlast : (n0 >= n ? std::min(jlast - n, n - 1) : n - 1);
+    kend = std::min<int64_t>(kend, 2 * (int64_t)(cmax + 1));
+  }
   kbeg = kbeg / BK * BK;
   kend = std::min<int64_t>(g.Kp, (kend + BK - 1) / BK * BK);
 
@@ -222,9 +286,14 @@ __global__ void __launch_bounds__(256, 2) gemm_lo_kernel(LoGemmArgs g) {
       for (int r = 0; r < 4; ++r) {
         const int row = rowb + i * 16, col = colb + j * 16 + mfma_f32_row(lane, r);
         if (row < g.M && col < g.N && (!g.upper || row <= col)) {
-          double* cp = g.C + row + (int64_t)col * g.ldc;
+          int64_t co = row + (int64_t)col * g.ldc, ci = row + (int64_t)col * g.ldcin;
+          if (g.cplx_n > 0) {
+            const int part = col >= g.cplx_n ? 1 : 0, c = col - part * g.cplx_n;
+            co = 2 * (row + (int64_t)c * g.ldc) + part;
+            ci = 2 * (row + (int64_t)c * g.ldcin) + part;
+          }
           const double v = alpha * (double)acc[i][j][r];
-          *cp = (beta != 0.0) ? fma(beta, g.Cin[row + (int64_t)col * g.ldcin], v) : v;
+          g.C[co] = (beta != 0.0) ? fma(beta, g.Cin[ci], v) : v;
         }
       }
 }
@@ -252,6 +321,35 @@ void gemm_lo(int opA, int opB, const GemmArgs& g, void* ws, hipStream_t s) {
   a.upper = g.upperC;
   a.gate = g.gate;
   a.gate_want = g.gate_want;
+  dim3 grid((unsigned)(Mp / kLoRows), (unsigned)(Np / kLoRows), 1);
+  hipLaunchKernelGGL((gemm_lo_kernel<PREC>), grid, dim3(256), 0, s, a);
+}
+
+// Complex C = alpha A B + beta Cin (real alpha, beta; op N, N) on FP32 / BF16
+// MFMA through the real embedding; TRI_B_UPPER clips k.
+template <int PREC>
+void gemm_lo_cplx(const GemmArgs& g, void* ws, hipStream_t s) {
+  using TL = typename LoType<PREC>::T;
+  const int64_t Mp = lo_rows(g.M), Np = lo_rows(2 * g.N), Kp = lo_kpad(2 * g.K);
+  TL* Ap = reinterpret_cast<TL*>(ws);
+  TL* Bp = Ap + Mp * Kp;
+  hipLaunchKernelGGL((pack_cplx_a_kernel<TL>), dim3((unsigned)(Kp / 64), (unsigned)(Mp / 64)), dim3(256), 0, s,
+                     reinterpret_cast<const double*>(g.A), g.lda, g.M, g.K, Ap, Kp, g.gate, g.gate_want);
+  hipLaunchKernelGGL((pack_cplx_b_kernel<TL>), dim3((unsigned)(Kp / 64), (unsigned)(Np / 64)), dim3(256), 0, s,
+                     reinterpret_cast<const double*>(g.B), g.ldb, g.K, g.N, Bp, Kp, g.gate, g.gate_want);
+  LoGemmArgs a{};
+  a.M = g.M; a.N = 2 * g.N; a.Kp = Kp;
+  a.Ap = Ap; a.Bp = Bp;
+  a.C = reinterpret_cast<double*>(g.C); a.ldc = g.ldc;
+  a.Cin = g.Cin != nullptr ? reinterpret_cast<const double*>(g.Cin) : a.C;
+  a.ldcin = g.Cin != nullptr ? g.ldcin : g.ldc;
+  a.alpha = g.alpha_re; a.beta = g.beta_re;
+  a.tri = 0;
+  a.upper = 0;
+  a.gate = g.gate;
+  a.gate_want = g.gate_want;
+  a.cplx_n = g.N;
+  a.cplx_upperB = (g.tri & TRI_B_UPPER) ? 1 : 0;
   dim3 grid((unsigned)(Mp / kLoRows), (unsigned)(Np / kLoRows), 1);
   hipLaunchKernelGGL((gemm_lo_kernel<PREC>), grid, dim3(256), 0, s, a);
 }
@@ -298,5 +396,17 @@ template <typename T> void gemm_prec(int prec, int opA, int opB, GemmArgs g, voi
 }
 template void gemm_prec<double>(int, int, int, GemmArgs, void*, hipStream_t);
 template void gemm_prec<zd>(int, int, int, GemmArgs, void*, hipStream_t);
+
+size_t gemm_lo_complex_workspace_bytes(int M, int N, int K) { return gemm_prec_workspace_bytes(M, 2 * N, 2 * K); }
+
+int gemm_lo_complex(int prec, GemmArgs g, void* ws, hipStream_t s) {
+  if ((prec != 32 && prec != 16) || g.alpha_im != 0.0 || g.beta_im != 0.0 || g.batch > 1 || g.ksplit > 1 ||
+      (g.tri & ~TRI_B_UPPER) != 0 || g.upperC || g.unitLowerA)
+    return -1;
+  if (g.M <= 0 || g.N <= 0) return 0;
+  if (prec == 16) gemm_lo_cplx<16>(g, ws, s);
+  else gemm_lo_cplx<32>(g, ws, s);
+  return 0;
+}
 
 }  // namespace mak

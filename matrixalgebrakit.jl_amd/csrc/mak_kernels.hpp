@@ -190,6 +190,15 @@ void wy_reduce(const void* Wp, int ksplit, int64_t slice, const void* Tm, int tr
 size_t gemm_prec_workspace_bytes(int M, int N, int K);
 template <typename T> void gemm_prec(int prec, int opA, int opB, GemmArgs g, void* ws, hipStream_t s);
 
+// Complex C = alpha A B + beta Cin (A M x K, B K x N, both op N; real alpha,
+// beta; Cin as in gemm_prec; TRI_B_UPPER the only hint) on FP32 (prec 32) or
+// BF16 (prec 16) MFMA with FP32 accumulation, through the real embedding
+// [A.re A.im] [[B.re B.im], [-B.im B.re]].  ws:
+// gemm_lo_complex_workspace_bytes(M, N, K) bytes.  Returns -1 for arguments
+// outside that form (nothing enqueued), else 0.
+size_t gemm_lo_complex_workspace_bytes(int M, int N, int K);
+int gemm_lo_complex(int prec, GemmArgs g, void* ws, hipStream_t s);
+
 // ---------------------------------------------------------------- dense blocks
 // Cholesky with inverse by the persistent tile kernel (mak_chol.hip):
 // G (n x n, upper triangle read) -> R (upper, strict lower part zeroed) with

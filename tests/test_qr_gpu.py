@@ -232,16 +232,13 @@ def test_qr_whole_matrix_cholqr2_matches_oracle(m, n, dt, dev):
     A = O.gaussian(m, n, dt, seed=7 * m + n)
     Q, R, rec = run_qr_rec(A, dev)
     assert rec[3] == 1 and rec[0] == 1 and rec[2] == 0, rec
-    # one final product; real Gaussian inputs have ||Q1^H Q1 - I|| small
-    # enough for the BF16 (<= 5e-14) or FP32 (<= 1e-10) correction (complex
-    # data: always FP64)
+    # one final product; Gaussian inputs have ||Q1^H Q1 - I|| small enough
+    # for the BF16 (<= 5e-14) or FP32 (<= 1e-10) correction (complex data
+    # through its real embedding)
     assert rec[6] + rec[7] + rec[8] == 1
-    if dt == np.float64:
-        e = rec_norm(rec)
-        assert (rec[8] == 1) == (e <= 5e-14), e
-        assert (rec[6] == 1) == (5e-14 < e <= 1e-10), e
-    else:
-        assert rec[7] == 1
+    e = rec_norm(rec)
+    assert (rec[8] == 1) == (e <= 5e-14), e
+    assert (rec[6] == 1) == (5e-14 < e <= 1e-10), e
     Qo, Ro = O.qr_compact(A, positive=True)
     assert rel(Q, Qo) < 1e-12 and rel(R, Ro) < 1e-12
     assert np.all(np.tril(R, -1) == 0)
