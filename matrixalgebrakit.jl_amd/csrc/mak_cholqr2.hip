@@ -194,11 +194,12 @@ __global__ void __launch_bounds__(256) cq2_sum_up_kernel(const T* P, int ks, int
 }
 
 // R (n x n, ld ldr) = sum of the ks slices (n x n each) on and above the
-// diagonal, zero below; only when the call was accepted (st[0] == 1).
+// diagonal, zero below; only when *gate == 1.
 constexpr int kRSlices = 4;
 template <typename T>
-__global__ void __launch_bounds__(256) sum_r_slices_kernel(const T* P, int ks, int n, T* R, int64_t ldr, const int* st) {
-  if (st[0] != 1) return;
+__global__ void __launch_bounds__(256) sum_r_slices_kernel(const T* P, int ks, int n, T* R, int64_t ldr,
+                                                           const int* gate) {
+  if (*gate != 1) return;
   const int64_t tot = (int64_t)n * n;
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (int64_t)gridDim.x * 256) {
     const int r = (int)(e % n), c = (int)(e / n);
@@ -355,20 +356,35 @@ void cholqr2_enqueue(hipStream_t s, int m, int n, const T* A, int64_t lda, T* Q,
     gemm<T>(OP_N, OP_N, gq<T>(m, n, n, Q1, m, X, n, Q, ldq, 1.0, 0.0, TRI_B_UPPER, st + 6, 1), s);
   }
   if (R != nullptr) {
+    // complex data: R = W2 R1 = R1 + U R1 with the correction on BF16 (st[7])
+    // or FP32 (st[3]) MFMA, else (st[6]) on FP64.  U R1 (upper x upper) is
+    // exactly upper triangular and R1's strict lower part is zero, so R's is
+    // too.  (Real data: the split-K FP64 product below is as fast at n ~ 1e3
+    // as the packing and launches of the low-precision one.)
+    const int* r64 = st;
+    if constexpr (ScalarTraits<T>::is_complex) {
+      GemmArgs g = gq<T>(n, n, n, U, n, G, n, R, ldr, 1.0, 1.0, TRI_B_UPPER, st + 7, 1);
+      g.Cin = G;
+      g.ldcin = n;
+      gemm_lo_complex(16, g, Sl, s);
+      g.gate = st + 3;
+      gemm_lo_complex(32, g, Sl, s);
+      r64 = st + 6;
+    }
     if (!ScalarTraits<T>::is_complex && n >= 512 && L.ks >= kRSlices) {
       // upper x upper on 32 x 32 tiles with K split kRSlices ways (each
       // workgroup a short MFMA chain; partials in the Gram slices' space),
       // then the gated sum, zero below the diagonal
-      GemmArgs g = gq<T>(n, n, n, W, n, G, n, Sl, n, 1.0, 0.0, TRI_A_UPPER | TRI_B_UPPER, st, 1);
+      GemmArgs g = gq<T>(n, n, n, W, n, G, n, Sl, n, 1.0, 0.0, TRI_A_UPPER | TRI_B_UPPER, r64, 1);
       g.tile = 32;
       g.kchunk = cdiv(cdiv(n, kRSlices), 16) * 16;
       g.ksplit = cdiv(n, g.kchunk);
       g.splitStride = nn;
       gemm<T>(OP_N, OP_N, g, s);
       hipLaunchKernelGGL((sum_r_slices_kernel<T>), dim3((unsigned)std::min<int64_t>(cdiv64(nn, 256), 2048)), dim3(256),
-                         0, s, Sl, g.ksplit, n, R, ldr, st);
+                         0, s, Sl, g.ksplit, n, R, ldr, r64);
     } else {
-      gemm<T>(OP_N, OP_N, gq<T>(n, n, n, W, n, G, n, R, ldr, 1.0, 0.0, TRI_A_UPPER | TRI_B_UPPER, st, 1), s);
+      gemm<T>(OP_N, OP_N, gq<T>(n, n, n, W, n, G, n, R, ldr, 1.0, 0.0, TRI_A_UPPER | TRI_B_UPPER, r64, 1), s);
     }
   }
 }
