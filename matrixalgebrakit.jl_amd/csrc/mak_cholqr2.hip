@@ -341,18 +341,10 @@ void cholqr2_enqueue(hipStream_t s, int m, int n, const T* A, int64_t lda, T* Q,
   // (st[3]) MFMA (complex data through its real embedding; the packed
   // operands go where the Gram slices were); else Q1 W2^{-1} on FP64
   {
-    GemmArgs g = gq<T>(m, n, n, Q1, m, D, n, Q, ldq, 1.0, 1.0, TRI_B_UPPER, st + 7, 1);
+    GemmArgs g = gq<T>(m, n, n, Q1, m, D, n, Q, ldq, 1.0, 1.0, TRI_B_UPPER);
     g.Cin = Q1;
     g.ldcin = m;
-    if constexpr (ScalarTraits<T>::is_complex) {
-      gemm_lo_complex(16, g, Sl, s);
-      g.gate = st + 3;
-      gemm_lo_complex(32, g, Sl, s);
-    } else {
-      gemm_prec<T>(16, OP_N, OP_N, g, Sl, s);
-      g.gate = st + 3;
-      gemm_prec<T>(32, OP_N, OP_N, g, Sl, s);
-    }
+    gemm_lo_reduced(g, OP_N, OP_N, st + 7, st + 3, Sl, s, ScalarTraits<T>::is_complex);
     gemm<T>(OP_N, OP_N, gq<T>(m, n, n, Q1, m, X, n, Q, ldq, 1.0, 0.0, TRI_B_UPPER, st + 6, 1), s);
   }
   if (R != nullptr) {
@@ -363,12 +355,10 @@ void cholqr2_enqueue(hipStream_t s, int m, int n, const T* A, int64_t lda, T* Q,
     // as the packing and launches of the low-precision one.)
     const int* r64 = st;
     if constexpr (ScalarTraits<T>::is_complex) {
-      GemmArgs g = gq<T>(n, n, n, U, n, G, n, R, ldr, 1.0, 1.0, TRI_B_UPPER, st + 7, 1);
+      GemmArgs g = gq<T>(n, n, n, U, n, G, n, R, ldr, 1.0, 1.0, TRI_B_UPPER);
       g.Cin = G;
       g.ldcin = n;
-      gemm_lo_complex(16, g, Sl, s);
-      g.gate = st + 3;
-      gemm_lo_complex(32, g, Sl, s);
+      gemm_lo_reduced(g, OP_N, OP_N, st + 7, st + 3, Sl, s, true);
       r64 = st + 6;
     }
     if (!ScalarTraits<T>::is_complex && n >= 512 && L.ks >= kRSlices) {

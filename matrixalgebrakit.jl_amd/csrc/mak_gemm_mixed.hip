@@ -43,10 +43,8 @@ inline int64_t lo_kpad(int k) { return cdiv64(std::max(k, 0), kLoK) * kLoK; }
 // block per workgroup, read along the source's contiguous index and written
 // along k through an LDS transpose.
 template <typename TL>
-__global__ void __launch_bounds__(256) pack_lo_kernel(const double* S, int64_t ld, int kcontig, int R, int K, TL* D,
-                                                      int64_t Kp, const int* gate, int want) {
-  if (gate_closed(gate, want)) return;
-  __shared__ float tile[64][65];
+__device__ inline void pack_lo_body(const double* S, int64_t ld, int kcontig, int R, int K, TL* D, int64_t Kp,
+                                    float (*tile)[65]) {
   const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   if (kcontig) {
@@ -62,6 +60,32 @@ __global__ void __launch_bounds__(256) pack_lo_kernel(const double* S, int64_t l
   }
   __syncthreads();
   for (int q = ty; q < 64; q += 4) D[(int64_t)(r0 + q) * Kp + k0 + tx] = (TL)tile[q][tx];
+}
+template <typename TL>
+__global__ void __launch_bounds__(256) pack_lo_kernel(const double* S, int64_t ld, int kcontig, int R, int K, TL* D,
+                                                      int64_t Kp, const int* gate, int want) {
+  __shared__ float tile[64][65];
+  if (gate_closed(gate, want)) return;
+  pack_lo_body<TL>(S, ld, kcontig, R, K, D, Kp, tile);
+}
+// Both operands of a product in one launch (blockIdx.z: 0 = A, R = M rows;
+// 1 = B, R = N rows), into BF16 when *g16 == 1, else FP32 when *g32 == 1,
+// else nothing; A at (TL*) base, B at (TL*) base + boff.
+__global__ void __launch_bounds__(256) pack_lo_dual_kernel(const double* A, int64_t lda, int acontig, int M,
+                                                           const double* B, int64_t ldb, int bcontig, int N, int K,
+                                                           void* base, int64_t boff, int64_t Kp, int ablocks,
+                                                           int bblocks, const int* g16, const int* g32) {
+  __shared__ float tile[64][65];
+  const bool h = *g16 == 1;
+  if (!h && *g32 != 1) return;
+  const bool isB = blockIdx.z == 1;
+  if ((int)blockIdx.y >= (isB ? bblocks : ablocks)) return;
+  const double* S = isB ? B : A;
+  const int64_t ld = isB ? ldb : lda;
+  const int kc = isB ? bcontig : acontig, R = isB ? N : M;
+  const int64_t off = isB ? boff : 0;
+  if (h) pack_lo_body<__bf16>(S, ld, kc, R, K, reinterpret_cast<__bf16*>(base) + off, Kp, tile);
+  else pack_lo_body<float>(S, ld, kc, R, K, reinterpret_cast<float*>(base) + off, Kp, tile);
 }
 
 template <typename TL>
@@ -83,10 +107,8 @@ void pack_lo(const double* S, int64_t ld, int kcontig, int R, int K, TL* D, int6
 // one 64-row x 32-column block per workgroup, read down the columns (16-byte
 // steps) and written along k' through LDS.
 template <typename TL>
-__global__ void __launch_bounds__(256) pack_cplx_a_kernel(const double* S, int64_t ld, int R, int K, TL* D,
-                                                          int64_t Kp, const int* gate, int want) {
-  if (gate_closed(gate, want)) return;
-  __shared__ float tre[32][65], tim[32][65];
+__device__ inline void pack_cplx_a_body(const double* S, int64_t ld, int R, int K, TL* D, int64_t Kp,
+                                        float (*tre)[65], float (*tim)[65]) {
   const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 32;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   for (int q = ty; q < 32; q += 4) {
@@ -109,9 +131,7 @@ __global__ void __launch_bounds__(256) pack_cplx_a_kernel(const double* S, int64
 // B (K x N complex) -> D (Np x Kp): row j < 2N, column k' < 2K holds the
 // embedded entry above (zero beyond); reads along k are contiguous.
 template <typename TL>
-__global__ void __launch_bounds__(256) pack_cplx_b_kernel(const double* S, int64_t ld, int K, int N, TL* D,
-                                                          int64_t Kp, const int* gate, int want) {
-  if (gate_closed(gate, want)) return;
+__device__ inline void pack_cplx_b_body(const double* S, int64_t ld, int K, int N, TL* D, int64_t Kp) {
   const int j0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int kp = k0 + tx, k = kp >> 1, pk = kp & 1;
@@ -126,6 +146,26 @@ __global__ void __launch_bounds__(256) pack_cplx_b_kernel(const double* S, int64
     D[(int64_t)j * Kp + kp] = (TL)v;
   }
 }
+// Both complex packs in one launch (blockIdx.z: 0 = A, 1 = B), into BF16 when
+// *g16 == 1, else FP32 when *g32 == 1, else nothing; A at (TL*) base, B at
+// (TL*) base + boff.
+__global__ void __launch_bounds__(256) pack_cplx_dual_kernel(const double* A, int64_t lda, int M, const double* B,
+                                                             int64_t ldb, int K, int N, void* base, int64_t boff,
+                                                             int64_t Kp, int ablocks, int bblocks, const int* g16,
+                                                             const int* g32) {
+  __shared__ float tre[32][65], tim[32][65];
+  const bool h = *g16 == 1;
+  if (!h && *g32 != 1) return;
+  if (blockIdx.z == 0) {
+    if ((int)blockIdx.y >= ablocks) return;
+    if (h) pack_cplx_a_body<__bf16>(A, lda, M, K, reinterpret_cast<__bf16*>(base), Kp, tre, tim);
+    else pack_cplx_a_body<float>(A, lda, M, K, reinterpret_cast<float*>(base), Kp, tre, tim);
+  } else {
+    if ((int)blockIdx.y >= bblocks) return;
+    if (h) pack_cplx_b_body<__bf16>(B, ldb, K, N, reinterpret_cast<__bf16*>(base) + boff, Kp);
+    else pack_cplx_b_body<float>(B, ldb, K, N, reinterpret_cast<float*>(base) + boff, Kp);
+  }
+}
 
 // native 16-byte vector (HIP's uint4 is a struct, which keeps arrays of it in
 // scratch memory)
@@ -136,7 +176,8 @@ struct LoGemmArgs {
   int M, N;
   int64_t Kp;            // packed k extent (multiple of 64)
   const void* Ap;        // op(A), Mp x Kp, k contiguous
-  const void* Bp;        // op(B)^T, Np x Kp, k contiguous
+  const void* Bp;        // op(B)^T, Np x Kp, k contiguous (null: at Ap + bOff elements)
+  int64_t bOff;
   double* C;
   int64_t ldc;
   const double* Cin;     // the beta term's matrix (C itself by default)
@@ -173,14 +214,13 @@ __device__ inline void lo_tile_coords(int& tm, int& tn, int tri) {
   }
 }
 
+// As, Bs: kLoRows x 9 16-byte vectors each (8 per row + 16 bytes pad)
 template <int PREC>
-__global__ void __launch_bounds__(256, 2) gemm_lo_kernel(LoGemmArgs g) {
+__device__ inline void gemm_lo_body(const LoGemmArgs& g, u32x4 (*As)[9], u32x4 (*Bs)[9]) {
   using TL = typename LoType<PREC>::T;
   constexpr int BK = 128 / (int)sizeof(TL);  // k values per 128-byte slab
   constexpr int TM = 4, TN = 4;              // 16 x 16 blocks per wave (64 x 64)
-  __shared__ u32x4 As[kLoRows][9];           // 8 x 16 bytes per row + 16 bytes pad
-  __shared__ u32x4 Bs[kLoRows][9];
-  if (gate_closed(g.gate, g.gate_want)) return;
+  const void* gBp = g.Bp != nullptr ? g.Bp : reinterpret_cast<const TL*>(g.Ap) + g.bOff;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   int tmi, tni;
@@ -206,7 +246,7 @@ __global__ void __launch_bounds__(256, 2) gemm_lo_kernel(LoGemmArgs g) {
   // loads: thread t moves 16-byte vector (t % 8) of rows t / 8 + 32 u
   const int lr = tid >> 3, lv = tid & 7;
   const char* Ab = reinterpret_cast<const char*>(g.Ap) + ((int64_t)(m0 + lr) * g.Kp) * (int64_t)sizeof(TL) + lv * 16;
-  const char* Bb = reinterpret_cast<const char*>(g.Bp) + ((int64_t)(n0 + lr) * g.Kp) * (int64_t)sizeof(TL) + lv * 16;
+  const char* Bb = reinterpret_cast<const char*>(gBp) + ((int64_t)(n0 + lr) * g.Kp) * (int64_t)sizeof(TL) + lv * 16;
   const int64_t rstride = 32 * g.Kp * (int64_t)sizeof(TL);
   u32x4 pa[4], pb[4];
   auto fetch = [&](int64_t k0) {
@@ -299,6 +339,22 @@ __global__ void __launch_bounds__(256, 2) gemm_lo_kernel(LoGemmArgs g) {
 }
 
 template <int PREC>
+__global__ void __launch_bounds__(256, 2) gemm_lo_kernel(LoGemmArgs g) {
+  __shared__ u32x4 As[kLoRows][9];
+  __shared__ u32x4 Bs[kLoRows][9];
+  if (gate_closed(g.gate, g.gate_want)) return;
+  gemm_lo_body<PREC>(g, As, Bs);
+}
+// BF16 when *g16 == 1, else FP32 when *g32 == 1 (operands packed by the dual
+// packs in that precision), else nothing.
+__global__ void __launch_bounds__(256, 2) gemm_lo_dual_kernel(LoGemmArgs g, const int* g16, const int* g32) {
+  __shared__ u32x4 As[kLoRows][9];
+  __shared__ u32x4 Bs[kLoRows][9];
+  if (*g16 == 1) gemm_lo_body<16>(g, As, Bs);
+  else if (*g32 == 1) gemm_lo_body<32>(g, As, Bs);
+}
+
+template <int PREC>
 void gemm_lo(int opA, int opB, const GemmArgs& g, void* ws, hipStream_t s) {
   using TL = typename LoType<PREC>::T;
   const int64_t Mp = lo_rows(g.M), Np = lo_rows(g.N), Kp = lo_kpad(g.K);
@@ -325,33 +381,50 @@ void gemm_lo(int opA, int opB, const GemmArgs& g, void* ws, hipStream_t s) {
   hipLaunchKernelGGL((gemm_lo_kernel<PREC>), grid, dim3(256), 0, s, a);
 }
 
-// Complex C = alpha A B + beta Cin (real alpha, beta; op N, N) on FP32 / BF16
-// MFMA through the real embedding; TRI_B_UPPER clips k.
-template <int PREC>
-void gemm_lo_cplx(const GemmArgs& g, void* ws, hipStream_t s) {
-  using TL = typename LoType<PREC>::T;
-  const int64_t Mp = lo_rows(g.M), Np = lo_rows(2 * g.N), Kp = lo_kpad(2 * g.K);
-  TL* Ap = reinterpret_cast<TL*>(ws);
-  TL* Bp = Ap + Mp * Kp;
-  hipLaunchKernelGGL((pack_cplx_a_kernel<TL>), dim3((unsigned)(Kp / 64), (unsigned)(Mp / 64)), dim3(256), 0, s,
-                     reinterpret_cast<const double*>(g.A), g.lda, g.M, g.K, Ap, Kp, g.gate, g.gate_want);
-  hipLaunchKernelGGL((pack_cplx_b_kernel<TL>), dim3((unsigned)(Kp / 64), (unsigned)(Np / 64)), dim3(256), 0, s,
-                     reinterpret_cast<const double*>(g.B), g.ldb, g.K, g.N, Bp, Kp, g.gate, g.gate_want);
+// The BF16 and FP32 forms of gemm_lo (and of the complex product through the
+// real embedding) in one set of launches:
+// BF16 when *g16 == 1, else FP32 when *g32 == 1, else nothing (for a choice
+// made on the device, so the form not taken costs no launches).  ws holds the
+// FP32 size of the packed operands.
+void gemm_lo_dual(int opA, int opB, const GemmArgs& g, const int* g16, const int* g32, void* ws, hipStream_t s) {
+  const int64_t Mp = lo_rows(g.M), Np = lo_rows(g.N), Kp = lo_kpad(g.K);
+  if (Mp <= 0 || Np <= 0 || Kp <= 0) return;
+  const int ab = (int)(Mp / 64), bb = (int)(Np / 64);
+  // op(A)(i, k): A[i + k lda] (N) or A[k + i lda] (T / C, real data);
+  // op(B)^T(j, k) = op(B)(k, j): B[k + j ldb] (N) or B[j + k ldb] (T / C)
+  hipLaunchKernelGGL(pack_lo_dual_kernel, dim3((unsigned)(Kp / 64), (unsigned)std::max(ab, bb), 2), dim3(256), 0, s,
+                     reinterpret_cast<const double*>(g.A), g.lda, opA != OP_N ? 1 : 0, g.M,
+                     reinterpret_cast<const double*>(g.B), g.ldb, opB == OP_N ? 1 : 0, g.N, g.K, ws, Mp * Kp, Kp,
+                     ab, bb, g16, g32);
   LoGemmArgs a{};
-  a.M = g.M; a.N = 2 * g.N; a.Kp = Kp;
-  a.Ap = Ap; a.Bp = Bp;
+  a.M = g.M; a.N = g.N; a.Kp = Kp;
+  a.Ap = ws; a.Bp = nullptr; a.bOff = Mp * Kp;
   a.C = reinterpret_cast<double*>(g.C); a.ldc = g.ldc;
   a.Cin = g.Cin != nullptr ? reinterpret_cast<const double*>(g.Cin) : a.C;
   a.ldcin = g.Cin != nullptr ? g.ldcin : g.ldc;
   a.alpha = g.alpha_re; a.beta = g.beta_re;
-  a.tri = 0;
-  a.upper = 0;
-  a.gate = g.gate;
-  a.gate_want = g.gate_want;
+  a.tri = g.tri;
+  a.upper = g.upperC;
+  hipLaunchKernelGGL(gemm_lo_dual_kernel, dim3((unsigned)(Mp / kLoRows), (unsigned)(Np / kLoRows), 1), dim3(256), 0,
+                     s, a, g16, g32);
+}
+void gemm_lo_cplx_dual(const GemmArgs& g, const int* g16, const int* g32, void* ws, hipStream_t s) {
+  const int64_t Mp = lo_rows(g.M), Np = lo_rows(2 * g.N), Kp = lo_kpad(2 * g.K);
+  const int ab = (int)(Mp / 64), bb = (int)(Np / 64);
+  hipLaunchKernelGGL(pack_cplx_dual_kernel, dim3((unsigned)(Kp / 64), (unsigned)std::max(ab, bb), 2), dim3(256), 0,
+                     s, reinterpret_cast<const double*>(g.A), g.lda, g.M, reinterpret_cast<const double*>(g.B), g.ldb,
+                     g.K, g.N, ws, Mp * Kp, Kp, ab, bb, g16, g32);
+  LoGemmArgs a{};
+  a.M = g.M; a.N = 2 * g.N; a.Kp = Kp;
+  a.Ap = ws; a.Bp = nullptr; a.bOff = Mp * Kp;
+  a.C = reinterpret_cast<double*>(g.C); a.ldc = g.ldc;
+  a.Cin = g.Cin != nullptr ? reinterpret_cast<const double*>(g.Cin) : a.C;
+  a.ldcin = g.Cin != nullptr ? g.ldcin : g.ldc;
+  a.alpha = g.alpha_re; a.beta = g.beta_re;
   a.cplx_n = g.N;
   a.cplx_upperB = (g.tri & TRI_B_UPPER) ? 1 : 0;
-  dim3 grid((unsigned)(Mp / kLoRows), (unsigned)(Np / kLoRows), 1);
-  hipLaunchKernelGGL((gemm_lo_kernel<PREC>), grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL(gemm_lo_dual_kernel, dim3((unsigned)(Mp / kLoRows), (unsigned)(Np / kLoRows), 1), dim3(256), 0,
+                     s, a, g16, g32);
 }
 
 // Y = X (M x N), when the gate is open.
@@ -399,13 +472,18 @@ template void gemm_prec<zd>(int, int, int, GemmArgs, void*, hipStream_t);
 
 size_t gemm_lo_complex_workspace_bytes(int M, int N, int K) { return gemm_prec_workspace_bytes(M, 2 * N, 2 * K); }
 
-int gemm_lo_complex(int prec, GemmArgs g, void* ws, hipStream_t s) {
-  if ((prec != 32 && prec != 16) || g.alpha_im != 0.0 || g.beta_im != 0.0 || g.batch > 1 || g.ksplit > 1 ||
-      (g.tri & ~TRI_B_UPPER) != 0 || g.upperC || g.unitLowerA)
-    return -1;
+int gemm_lo_reduced(const GemmArgs& g, int opA, int opB, const int* g16, const int* g32, void* ws, hipStream_t s,
+                    bool cplx) {
   if (g.M <= 0 || g.N <= 0) return 0;
-  if (prec == 16) gemm_lo_cplx<16>(g, ws, s);
-  else gemm_lo_cplx<32>(g, ws, s);
+  if (cplx) {
+    if (opA != OP_N || opB != OP_N || g.alpha_im != 0.0 || g.beta_im != 0.0 || g.batch > 1 || g.ksplit > 1 ||
+        (g.tri & ~TRI_B_UPPER) != 0 || g.upperC || g.unitLowerA)
+      return -1;
+    gemm_lo_cplx_dual(g, g16, g32, ws, s);
+  } else {
+    if (g.batch > 1 || g.ksplit > 1 || g.unitLowerA) return -1;
+    gemm_lo_dual(opA, opB, g, g16, g32, ws, s);
+  }
   return 0;
 }
 

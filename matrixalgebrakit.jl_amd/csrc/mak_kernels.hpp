@@ -190,14 +190,19 @@ void wy_reduce(const void* Wp, int ksplit, int64_t slice, const void* Tm, int tr
 size_t gemm_prec_workspace_bytes(int M, int N, int K);
 template <typename T> void gemm_prec(int prec, int opA, int opB, GemmArgs g, void* ws, hipStream_t s);
 
-// Complex C = alpha A B + beta Cin (A M x K, B K x N, both op N; real alpha,
-// beta; Cin as in gemm_prec; TRI_B_UPPER the only hint) on FP32 (prec 32) or
-// BF16 (prec 16) MFMA with FP32 accumulation, through the real embedding
-// [A.re A.im] [[B.re B.im], [-B.im B.re]].  ws:
-// gemm_lo_complex_workspace_bytes(M, N, K) bytes.  Returns -1 for arguments
-// outside that form (nothing enqueued), else 0.
+// C = alpha op(A) op(B) + beta Cin (Cin as in gemm_prec) on BF16 MFMA when
+// *g16 == 1, else FP32 MFMA when *g32 == 1, else nothing (g.gate is
+// ignored), FP32 accumulation: the precision is chosen on the device, and
+// both forms take the same two launches (the packs, the product).  Real data
+// (cplx false; any ops; ws: gemm_prec_workspace_bytes(M, N, K)) or complex
+// data (cplx true; A M x K and B K x N both op N, real alpha and beta,
+// TRI_B_UPPER the only hint; through the real embedding [A.re A.im]
+// [[B.re B.im], [-B.im B.re]] with the parts interleaved along k; ws:
+// gemm_lo_complex_workspace_bytes(M, N, K)).  Returns -1 for arguments
+// outside those forms (nothing enqueued), else 0.
 size_t gemm_lo_complex_workspace_bytes(int M, int N, int K);
-int gemm_lo_complex(int prec, GemmArgs g, void* ws, hipStream_t s);
+int gemm_lo_reduced(const GemmArgs& g, int opA, int opB, const int* g16, const int* g32, void* ws, hipStream_t s,
+                    bool cplx);
 
 // ---------------------------------------------------------------- dense blocks
 // Cholesky with inverse by the persistent tile kernel (mak_chol.hip):
