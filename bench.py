@@ -455,7 +455,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "qr_compact! 8192x1024 Float64 Gaussian, positive=true", "m": m, "n": n,
-                   "parallelism": "single-gpu"},
+                   "parallelism": "single-gpu",
+                   "precision": "FP64 factors (CholeskyQR2 on FP64 MFMA); the final correction term Q1 D "
+                                "(||D|| ~ 1e-14) on BF16 MFMA with FP32 accumulation, its error bound below "
+                                "FP64 rounding; checks holds the FP64 residual and orthogonality"},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
                      "traffic": args.traffic_bytes,
