@@ -246,16 +246,19 @@ def test_qr_whole_matrix_cholqr2_matches_oracle(m, n, dt, dev):
     assert np.all(d.imag == 0) and np.all(d.real > 0)
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
 @pytest.mark.parametrize("cond", [1e3, 1e5, 1e8, 1e13])
-def test_qr_whole_matrix_cholqr2_conditioning(cond, dev):
+def test_qr_whole_matrix_cholqr2_conditioning(cond, dt, dev):
     """Graded singular values 1 .. 1/cond in random directions: the closed-form
     second pass is accepted while ||Q1^H Q1 - I|| <= 1e-6 (kappa up to about
     1e5) and the input goes on to the Householder path beyond; either way
-    the backward error and orthogonality are those of Householder QR."""
+    the backward error and orthogonality are those of Householder QR.  The
+    final product's tier (BF16 / FP32 correction, or FP64) follows ||E||_F
+    for real and complex data alike."""
     m, n = 3000, 256
-    U, _ = np.linalg.qr(O.gaussian(m, n, np.float64, seed=31))
-    V, _ = np.linalg.qr(O.gaussian(n, n, np.float64, seed=32))
-    A = (U * np.logspace(0, -np.log10(cond), n)[None, :]) @ V.T
+    U, _ = np.linalg.qr(O.gaussian(m, n, dt, seed=31))
+    V, _ = np.linalg.qr(O.gaussian(n, n, dt, seed=32))
+    A = (U * np.logspace(0, -np.log10(cond), n)[None, :]) @ V.conj().T
     Q, R, rec = run_qr_rec(A, dev)
     if cond <= 1e3:
         assert rec[0] == 1
@@ -270,10 +273,12 @@ def test_qr_whole_matrix_cholqr2_conditioning(cond, dev):
         assert (rec[8] == 1) == (e <= 5e-14), e
         assert (rec[6] == 1) == (5e-14 < e <= 1e-10), e
     if cond == 1e5:
-        assert rec[7] == 1
+        # accepted for real data (complex data sits at the 1e-6 threshold)
+        assert rec[7] == 1 if dt == np.float64 else rec[7] == rec[0]
     assert rel(Q @ R, A) < 1e-14
-    assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 1e-13
-    assert np.all(np.diagonal(R) >= 0)
+    assert np.linalg.norm(Q.conj().T @ Q - np.eye(n)) < 1e-13
+    d = np.diagonal(R)
+    assert np.all(d.imag == 0) and np.all(d.real >= 0)
     if cond <= 1e5:
         Qo, Ro = O.qr_compact(A, positive=True)
         assert rel(R, Ro) < 1e-15 * cond * 100
