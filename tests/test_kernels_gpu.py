@@ -111,12 +111,14 @@ def test_potrf_with_inverse(n, dt, dev):
     assert np.linalg.norm(Y @ W - np.eye(n)) / np.sqrt(n) < 1e-12
 
 
-@pytest.mark.parametrize("n,bad", [(150, 100), (2300, 2090)])
-def test_potrf_reports_first_nonpositive_minor(n, bad, dev):
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+@pytest.mark.parametrize("n,bad", [(150, 100), (150, 64), (70, 33), (2300, 2090)])
+def test_potrf_reports_first_nonpositive_minor(n, bad, dt, dev):
     """info = j + 1 for the first minor that is singular (tile kernel alone and
-    inside the blocked loop)."""
-    G = O.gaussian(n, n, np.float64, seed=2)
-    Zh = G.T @ G
+    inside the blocked loop; in the first or second tile of a pair step, where
+    the panel stops before or after its look-ahead)."""
+    G = O.gaussian(n, n, dt, seed=2)
+    Zh = G.conj().T @ G
     Zh[bad:, :] = 0.0
     Zh[:, bad:] = 0.0          # leading bad x bad is PD, minor bad + 1 is singular
     _, _, info = run_potrf(Zh, dev)
