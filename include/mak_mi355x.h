@@ -60,6 +60,10 @@ int mak_qr_workspace_size(int dtype, int full, int m, int n, size_t* bytes);
  * A (m x n) is destroyed (it holds the compact WY factors afterwards),
  * Q is m x min(m,n), R is min(m,n) x n or NULL (R not computed).
  * positive != 0 gives R a real non-negative diagonal (gauge fix).
+ * The factors are FP64-accurate; on the whole-matrix CholeskyQR2 path the
+ * final correction term Q1 (W2^{-1} - I), of norm ~ ||Q1^H Q1 - I||, runs on
+ * BF16 (that norm <= 5e-14) or FP32 (<= 1e-10) MFMA with an error bound below
+ * Q's own FP64 rounding, else on FP64.
  * Replaces qr_compact!(A, QR, ::Householder) -> qr_householder!
  * (src/implementations/qr.jl:110-113, :132-188). */
 int mak_qr_compact_d(void* stream, int m, int n, double* A, int lda, double* Q, int ldq, double* R, int ldr,
