@@ -467,11 +467,15 @@ static void gemm_shape(const GemmArgs& g, hipStream_t s) {
       return;
     }
   }
-  if (!cplx && g.tile != 0) {
-    if (g.tile >= 128) launch_gemm_tile<128, 128, 16, 2, 2, OPA, OPB, T>(g, s);
-    else if (g.tile >= 64) launch_gemm_tile<64, 64, 16, 2, 2, OPA, OPB, T>(g, s);
-    else launch_gemm_tile<32, 32, 16, 2, 2, OPA, OPB, T>(g, s);
-    return;
+  // complex products never take 128x128 tiles (their 256 accumulators per
+  // lane do not fit beside the operands), so that kernel is not instantiated
+  if constexpr (!cplx) {
+    if (g.tile != 0) {
+      if (g.tile >= 128) launch_gemm_tile<128, 128, 16, 2, 2, OPA, OPB, T>(g, s);
+      else if (g.tile >= 64) launch_gemm_tile<64, 64, 16, 2, 2, OPA, OPB, T>(g, s);
+      else launch_gemm_tile<32, 32, 16, 2, 2, OPA, OPB, T>(g, s);
+      return;
+    }
   }
   if (g.M <= 32) {
     if constexpr (!cplx && OPA == OP_T && OPB == OP_N) {
@@ -489,7 +493,7 @@ static void gemm_shape(const GemmArgs& g, hipStream_t s) {
     // short-K updates stream C: smaller tiles keep more loads in flight
     launch_gemm_tile<64, 64, 16, 2, 2, OPA, OPB, T>(g, s);
   } else if (!cplx && wgs(128, 128) >= 192) {
-    launch_gemm_tile<128, 128, 16, 2, 2, OPA, OPB, T>(g, s);
+    if constexpr (!cplx) launch_gemm_tile<128, 128, 16, 2, 2, OPA, OPB, T>(g, s);
   } else if (wgs(64, 64) >= 128) {
     launch_gemm_tile<64, 64, 16, 2, 2, OPA, OPB, T>(g, s);
   } else {

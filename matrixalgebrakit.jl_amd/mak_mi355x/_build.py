@@ -26,7 +26,14 @@ ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 anywhere in device code.
-DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# -amdgpu-mfma-vgpr-form: MFMA accumulators live in VGPRs.  Left to itself the
+# register allocator puts them in AGPRs and moves them back and forth
+# (v_accvgpr_read/write) inside the MFMA loops of the complex GEMM, the tile
+# Cholesky, the WY apply and the CholeskyQR panels; those copies hold FP64
+# MFMA at about half its issue rate.  With the flag those loops carry no copies
+# and no kernel needs more registers.
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                "-mllvm", "-amdgpu-mfma-vgpr-form"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
           "-Wno-unused-result", f"-I{CSRC}", f"-I{INCLUDE}"]
 
