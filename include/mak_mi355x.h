@@ -22,8 +22,10 @@
  *       mak_*_pullback_*   once (the rank / convergence counters);
  *       mak_qr_batched_execute is fully asynchronous (one graph launch).
  *   - Calls are reentrant: all state lives in the caller's workspace (the
- *     batched plans own their graph and stream); concurrent calls need
- *     distinct workspaces.
+ *     batched plans own their graph and stream; the polar's blocked Cholesky,
+ *     n > 2048, forks onto one look-ahead stream per device, created with two
+ *     events at its first use and joined back before the call returns);
+ *     concurrent calls need distinct workspaces.
  *   - `work` is a caller-owned device workspace of at least the size returned
  *     by the matching *_workspace_size query (the library allocates nothing;
  *     this mirrors LAPACK's lwork query, src/yalapack.jl:180-197).

@@ -5,6 +5,7 @@
 //   * Frobenius norms (deterministic two-pass reductions);
 //   * elementwise helpers (identity fill, axpby, Hermitian part).
 #include <algorithm>
+#include <mutex>
 
 #include "mak_common.hpp"
 #include "mak_dense32.hpp"
@@ -78,6 +79,48 @@ __global__ void __launch_bounds__(256) potrf_finish_kernel(T* Z, int64_t ldz, in
 // Diagonal block width of the blocked loop (n > kCholTileMaxN).
 constexpr int kCholBlock = 1024;
 
+// The blocked loop's look-ahead stream, one per device (default priority,
+// created on first use, kept until the process ends), and the two events
+// that fork work onto it and join it back.  A call holds the device's mutex
+// while it enqueues (the events are reused call after call); the work itself
+// runs on the caller's stream and this one.  nullptr: no side stream (the
+// loop then runs on the caller's stream alone).
+struct PotrfSide {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  std::mutex mu;
+};
+constexpr int kPotrfMaxDevices = 64;
+PotrfSide g_potrf_side[kPotrfMaxDevices];
+std::once_flag g_potrf_side_once[kPotrfMaxDevices];
+
+static PotrfSide* potrf_side(hipStream_t s) {
+  int dev = -1;
+  if (s != nullptr) {
+    if (hipStreamGetDevice(s, &dev) != hipSuccess) return nullptr;
+  } else if (hipGetDevice(&dev) != hipSuccess) {
+    return nullptr;
+  }
+  if (dev < 0 || dev >= kPotrfMaxDevices) return nullptr;
+  PotrfSide* ps = &g_potrf_side[dev];
+  std::call_once(g_potrf_side_once[dev], [dev, ps] {
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return;
+    if (cur != dev && hipSetDevice(dev) != hipSuccess) return;
+    hipStream_t t = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipStreamCreateWithFlags(&t, hipStreamNonBlocking) == hipSuccess &&
+        hipEventCreateWithFlags(&a, hipEventDisableTiming) == hipSuccess &&
+        hipEventCreateWithFlags(&b, hipEventDisableTiming) == hipSuccess) {
+      ps->s = t;
+      ps->fork = a;
+      ps->join = b;
+    }
+    if (cur != dev) (void)hipSetDevice(cur);
+  });
+  return ps->s != nullptr ? ps : nullptr;
+}
+
 size_t potrf_tmp_elems(int n) { return (size_t)64 * std::max(n, 1) + (size_t)n * n / 4; }
 
 // Z (n x n, upper part) -> W (upper, in place, strict lower zeroed) with
@@ -108,12 +151,41 @@ void potrf_upper(T* Z, int64_t ldz, int n, T* Winv, int64_t ldw, T* tmp, int* di
   }
   hipLaunchKernelGGL((zero_lower_kernel<T>), dim3((unsigned)cdiv64((int64_t)n * n, 256)), dim3(256), 0, s, Winv, ldw,
                      n);
+  // Look-ahead: once block k's panel has updated the next diagonal block, the
+  // tile kernel's chain for block k + 1 (latency-bound, on half the CUs) runs
+  // beside the rest of block k's trailing update and its inverse update on
+  // the side stream; the caller's stream waits for them before block k + 1's
+  // panel (which reads the updated columns) and before the finish.
+  PotrfSide* side = potrf_side(s);
+  std::unique_lock<std::mutex> lock;
+  if (side != nullptr) lock = std::unique_lock<std::mutex>(side->mu);
+  const hipStream_t s2 = side != nullptr ? side->s : s;
+  bool pending = false;  // side work enqueued since the last join
+  auto join = [&] {
+    if (pending) {
+      hipEventRecord(side->join, s2);
+      hipStreamWaitEvent(s, side->join, 0);
+      pending = false;
+    }
+  };
+  // X_0:k,k = -(X_0:k,0:k W_0:k,k) Xkk on stream t (W_0:k,k^H in the lower blocks Z_k,0:k)
+  auto inverse_update = [&](int k0, int kb, hipStream_t t) {
+    const T* Xkk = Winv + k0 + (int64_t)k0 * ldw;
+    const T* Zk0 = Z + k0;
+    GemmArgs g = gargs<T>(k0, kb, k0, Winv, ldw, Zk0, ldz, tmp, k0, 1.0, 0.0);
+    g.tri = TRI_A_UPPER;
+    gemm<T>(OP_N, OP_C, g, t);
+    g = gargs<T>(k0, kb, kb, tmp, k0, Xkk, ldw, Winv + (int64_t)k0 * ldw, ldw, -1.0, 0.0);
+    g.tri = TRI_B_UPPER;
+    gemm<T>(OP_N, OP_N, g, t);
+  };
   for (int k0 = 0; k0 < n; k0 += kCholBlock) {
     const int kb = std::min(kCholBlock, n - k0);
     const int rest = n - k0 - kb;
     T* Zkk = Z + k0 + (int64_t)k0 * ldz;
     T* Xkk = Winv + k0 + (int64_t)k0 * ldw;
     chol_tile_factor<T>(Zkk, ldz, kb, Xkk, ldw, flags, dinfo, k0, s);
+    join();  // block k - 1's trailing columns and inverse
     if (rest > 0) {
       T* Zkr = Z + k0 + (int64_t)(k0 + kb) * ldz;
       T* Zrk = Z + (k0 + kb) + (int64_t)k0 * ldz;
@@ -122,22 +194,33 @@ void potrf_upper(T* Z, int64_t ldz, int n, T* Winv, int64_t ldw, T* tmp, int* di
       GemmArgs g = gargs<T>(rest, kb, kb, Zkr, ldz, Xkk, ldw, Zrk, ldz, 1.0, 0.0);
       g.tri = TRI_B_UPPER;
       gemm<T>(OP_C, OP_N, g, s);
-      // Z_r,r -= Z_r,k Z_r,k^H (upper triangle)
-      g = gargs<T>(rest, rest, kb, Zrk, ldz, Zrk, ldz, Zrr, ldz, -1.0, 1.0);
+      // Z_r,r -= Z_r,k Z_r,k^H (upper triangle): the next diagonal block here,
+      // the columns beyond it on the side stream
+      const int nb1 = std::min(kb, rest), rest2 = rest - nb1;
+      g = gargs<T>(nb1, nb1, kb, Zrk, ldz, Zrk, ldz, Zrr, ldz, -1.0, 1.0);
       g.upperC = 1;
       gemm<T>(OP_N, OP_C, g, s);
-    }
-    if (k0 > 0) {
-      // tmp (k0 x kb) = X_0:k,0:k W_0:k,k = X_0:k,0:k Z_k,0:k^H ; X_0:k,k = -tmp Xkk
-      const T* Zk0 = Z + k0;  // rows k0.., columns 0..k0-1: W_0:k,k^H
-      GemmArgs g = gargs<T>(k0, kb, k0, Winv, ldw, Zk0, ldz, tmp, k0, 1.0, 0.0);
-      g.tri = TRI_A_UPPER;
-      gemm<T>(OP_N, OP_C, g, s);
-      g = gargs<T>(k0, kb, kb, tmp, k0, Xkk, ldw, Winv + (int64_t)k0 * ldw, ldw, -1.0, 0.0);
-      g.tri = TRI_B_UPPER;
-      gemm<T>(OP_N, OP_N, g, s);
+      if (side != nullptr) {
+        hipEventRecord(side->fork, s);
+        hipStreamWaitEvent(s2, side->fork, 0);
+        pending = true;
+      }
+      if (rest2 > 0) {
+        // rows of the next block, columns beyond it
+        g = gargs<T>(nb1, rest2, kb, Zrk, ldz, Zrk + nb1, ldz, Zrr + (int64_t)nb1 * ldz, ldz, -1.0, 1.0);
+        gemm<T>(OP_N, OP_C, g, s2);
+        // the square beyond it (upper triangle)
+        g = gargs<T>(rest2, rest2, kb, Zrk + nb1, ldz, Zrk + nb1, ldz, Zrr + nb1 + (int64_t)nb1 * ldz, ldz, -1.0,
+                     1.0);
+        g.upperC = 1;
+        gemm<T>(OP_N, OP_C, g, s2);
+      }
+      if (k0 > 0) inverse_update(k0, kb, s2);
+    } else if (k0 > 0) {
+      inverse_update(k0, kb, s);
     }
   }
+  join();
   const int nb32 = cdiv(n, 32);
   hipLaunchKernelGGL((potrf_finish_kernel<T>), dim3(nb32, nb32), dim3(256), 0, s, Z, ldz, n, kCholBlock);
 }
