@@ -79,6 +79,7 @@ template <typename T> struct QrLayout {
   size_t wpElems = 0, w2Elems = 0;
   // 128-column block reflectors of Q's formation (kGroup panels):
   size_t offVc = 0, offS = 0, offTg = 0, offWg = 0;
+  size_t offTs = 0;   // T of every pair of groups (kSGB x kSGB each)
   size_t offQs = 0;   // m x min(m, n) scratch: Q of the in-place mode (Q === A)
   size_t offCq2 = 0;  // whole-matrix CholeskyQR2 (qr_compact!, m >= n), 0 when absent
   size_t bytes = 0;
@@ -86,21 +87,25 @@ template <typename T> struct QrLayout {
 
 inline size_t align256(size_t b) { return ((b + 255) / 256) * 256; }
 
-// Q is formed by block reflectors of kGroup panels (kGB columns).
+// Q is formed by block reflectors of kGroup panels (kGB columns); on large
+// problems pairs of groups are merged into one reflector of kSGB columns
+// (half the passes over Q, and K = 256 for its update).
 constexpr int kGroup = 4;
 constexpr int kGB = kGroup * NB;
+constexpr int kSGB = 2 * kGB;
+constexpr int64_t kSuperGroupMinQ = int64_t(1) << 23;  // m * qcols from which pairs are merged
 
-// Split-K slices (kGB x N x ksplit elements) of the K = m products with kGB
-// rows, maximised over the N <= ncmax that occur (ksplit * N peaks at a
+// Split-K slices (rows x N x ksplit elements) of the K = m products with
+// `rows` rows, maximised over the N <= ncmax that occur (ksplit * N peaks at a
 // column count below ncmax, so every N is tried).
-size_t wp_block_elems(int m, int ncmax) {
+size_t wp_block_elems(int m, int ncmax, int rows) {
   size_t most = 0;
   for (int N = 1; N <= std::max(ncmax, kGB); ++N) {
     int ks, kc;
     splitk_plan(N, m, 128, ks, kc);
     most = std::max(most, (size_t)ks * N);
   }
-  return (size_t)kGB * most;
+  return (size_t)rows * most;
 }
 
 template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
@@ -127,11 +132,11 @@ template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
   L.wpElems = (size_t)NB * ncmax * std::max(ks, 1);
   // ksplit grows as N shrinks: bound ksplit * N over every N <= ncmax
   L.wpElems = std::max(L.wpElems, (size_t)NB * (size_t)(kSplitTargetWgs * 64 + ncmax));
-  if (minmn > NB) L.wpElems = std::max(L.wpElems, wp_block_elems(m, ncmax));
+  if (minmn > NB) L.wpElems = std::max(L.wpElems, wp_block_elems(m, ncmax, minmn > kGB ? kSGB : kGB));
   L.offWp = off;
   off += align256(L.wpElems * sizeof(T));
   L.w2Elems = (size_t)NB * ncmax;
-  if (minmn > NB) L.w2Elems = std::max(L.w2Elems, (size_t)kGB * ncmax);
+  if (minmn > NB) L.w2Elems = std::max(L.w2Elems, (size_t)(minmn > kGB ? kSGB : kGB) * ncmax);
   L.offW2 = off;
   off += align256(L.w2Elems * sizeof(T));
   if (minmn > NB) {
@@ -145,7 +150,11 @@ template <typename T> QrLayout<T> qr_layout(int m, int n, int qcols) {
     L.offTg = off;
     off += align256(ngr * kGB * kGB * sizeof(T));
     L.offWg = off;
-    off += align256((size_t)kGB * ncmax * sizeof(T));
+    off += align256((size_t)(minmn > kGB ? kSGB : kGB) * ncmax * sizeof(T));
+    if (minmn > kGB) {
+      L.offTs = off;
+      off += align256((size_t)cdiv((int)ngr, 2) * kSGB * kSGB * sizeof(T));
+    }
   }
   L.offQs = off;
   off += align256((size_t)m * minmn * sizeof(T));
@@ -478,7 +487,48 @@ void orgqr_blocked(const T* A, int64_t lda, int m, int n, T* Q, int64_t ldq, int
     }
     wy_merge_groups<T>(Sg, Tpanels, npan, minmn, Tg, s);
   }
-  for (int g = ngr - 1; g >= 0; --g) {
+  // Pairs of groups (2G, 2G + 1) as one reflector of kg1 + kg2 columns:
+  //   (I - V1 T1 V1^H)(I - V2 T2 V2^H) = I - [V1 V2] [T1  -T1 (V1^H V2) T2; 0  T2] [V1 V2]^H
+  // (V1^H V2 over the rows where V2 is nonzero, split-K; the two small
+  // products through the W2 scratch).
+  const int nsg = cdiv(ngr, 2);
+  const bool pairs = L.offTs != 0 && ngr >= 2 && (int64_t)m * qcols >= kSuperGroupMinQ;
+  T* Ts = L.offTs != 0 ? reinterpret_cast<T*>(ws + L.offTs) : nullptr;
+  auto group_cols = [&](int g) { return std::min(minmn, (g + 1) * kGB) - g * kGB; };
+  if (pairs) {
+    T* W2 = reinterpret_cast<T*>(ws + L.offW2);
+    for (int G = 0; 2 * G + 1 < ngr; ++G) {
+      const int g1 = 2 * G, g2 = g1 + 1;
+      const int kg1 = kGB, kg2 = group_cols(g2), kgs = kg1 + kg2;
+      const int j1 = g1 * kGB, j2 = g2 * kGB;
+      T* Tsg = Ts + (size_t)G * kSGB * kSGB;  // ld kgs
+      T* S12 = W2;                            // kg1 x kg2
+      T* X12 = W2 + (size_t)kGB * kGB;        // kg1 x kg2
+      const T* T1 = Tg + (size_t)g1 * kGB * kGB;  // ld kg1
+      const T* T2 = Tg + (size_t)g2 * kGB * kGB;  // ld kg2
+      vh_times<T>(Vc + j2 + (int64_t)j1 * m, m, m - j2, kg1, Vc + j2 + (int64_t)j2 * m, m, kg2, S12, ws, L, s);
+      {
+        GemmArgs g{};
+        g.M = kg1; g.N = kg2; g.K = kg1;
+        g.A = T1; g.lda = kg1; g.B = S12; g.ldb = kg1; g.C = X12; g.ldc = kg1;
+        g.alpha_re = 1.0; g.beta_re = 0.0;
+        g.ksplit = 1; g.kchunk = kg1; g.batch = 1;
+        gemm<T>(OP_N, OP_N, g, s);
+      }
+      {
+        GemmArgs g{};
+        g.M = kg1; g.N = kg2; g.K = kg2;
+        g.A = X12; g.lda = kg1; g.B = T2; g.ldb = kg2; g.C = Tsg + (int64_t)kg1 * kgs; g.ldc = kgs;
+        g.alpha_re = -1.0; g.beta_re = 0.0;
+        g.ksplit = 1; g.kchunk = kg2; g.batch = 1;
+        gemm<T>(OP_N, OP_N, g, s);
+      }
+      copy_mat<T>(kg1, kg1, T1, kg1, Tsg, kgs, s);
+      copy_mat<T>(kg2, kg2, T2, kg2, Tsg + kg1 + (int64_t)kg1 * kgs, kgs, s);
+      fill_zero<T>(Tsg + kg1, kgs, kg2, kg1, s);
+    }
+  }
+  auto apply_group = [&](int g) {
     const int p0 = g * kGroup, np = std::min(kGroup, npan - p0);
     const int j0 = p0 * NB;
     const int kg = std::min(minmn, (p0 + np) * NB) - j0;
@@ -489,6 +539,17 @@ void orgqr_blocked(const T* A, int64_t lda, int m, int n, T* Q, int64_t ldq, int
     else
       apply_block_wy<T>(Vc + j0 + (int64_t)j0 * m, m, m - j0, kg, Tg + (size_t)g * kGB * kGB, 0, C, ldq,
                         qcols - j0, ws, L, s);
+  };
+  for (int G = nsg - 1; G >= 0; --G) {
+    const int g1 = 2 * G, g2 = g1 + 1;
+    if (pairs && g2 < ngr) {
+      const int j0 = g1 * kGB, kgs = kGB + group_cols(g2);
+      apply_block_wy<T>(Vc + j0 + (int64_t)j0 * m, m, m - j0, kgs, Ts + (size_t)G * kSGB * kSGB, 0,
+                        Q + j0 + (int64_t)j0 * ldq, ldq, qcols - j0, ws, L, s);
+    } else {
+      if (g2 < ngr) apply_group(g2);
+      apply_group(g1);
+    }
   }
 }
 

@@ -334,6 +334,28 @@ def test_qr_inplace_q(m, n, dt, dev):
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
+@pytest.mark.parametrize("m,n", [(4000, 200), (3000, 290)])
+def test_qr_full_paired_block_reflectors(m, n, dt, dev):
+    """qr_full! on shapes whose Q formation merges pairs of 128-column block
+    reflectors into one (m * m >= 2^23): 200 columns pair a full group with
+    a 72-column one, 290 columns leave the third group (34 columns) unpaired.
+    The first n columns of Q and R equal the oracle's compact factors, Q is
+    unitary and A = Q R."""
+    A = O.gaussian(m, n, dt, seed=5000 + m + n)
+    At = to_dev(A, dev)
+    Q, R = mak.initialize_output_qr(At, True)
+    mak.qr_full_(At.clone(), (Q, R))
+    torch.cuda.synchronize()
+    Qo, Ro = O.qr_compact(A)
+    Rn = R.cpu().numpy()
+    assert np.all(Rn[n:, :] == 0)
+    assert rel(Rn[:n, :], Ro) < 1e-12 and rel(Q[:, :n].cpu().numpy(), Qo) < 1e-12
+    eye = torch.eye(m, dtype=At.dtype, device=dev)
+    assert (torch.linalg.norm(Q.mH @ Q - eye) / np.sqrt(m)).item() < 1e-13
+    assert (torch.linalg.norm(At - Q @ R) / torch.linalg.norm(At)).item() < 1e-13
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.complex128], ids=["f64", "c64"])
 def test_qr_full_8192x1024(dt, dev):
     """qr_full! on the 8192 x 1024 configuration (Q 8192 x 8192): R and the
     first 1024 columns of Q equal the oracle's unique positive-gauge compact
