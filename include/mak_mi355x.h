@@ -9,9 +9,11 @@
  *   - Complex matrices are ComplexF64, interleaved (re, im): the layout of
  *     Julia's ComplexF64 and numpy's complex128.  They are passed as void*.
  *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
- *     Work is enqueued on that stream only; no call synchronises the device
- *     or other streams.  Some calls wait for their own stream to read a small
- *     status word back to the host:
+ *     Work is enqueued on that stream (the polar's blocked Cholesky, n > 2048,
+ *     also forks work onto a per-device look-ahead stream and makes the
+ *     caller's stream wait for it with an event before it returns); no call
+ *     synchronises the device or other streams.  Some calls wait for their
+ *     own stream to read a small status word back to the host:
  *       mak_qr_*           once per call (the CholeskyQR2 acceptance check),
  *                          twice when the whole-matrix CholeskyQR2 result is
  *                          rejected; under stream capture never (the gated
