@@ -51,7 +51,9 @@ namespace mak {
 namespace {
 
 constexpr int TB = 32;                 // tile size
-constexpr int kYRows = 128;            // rows of B / Y per right-product task
+constexpr int kYRows = 128;            // rows of B / Y per right-product task (real data)
+constexpr int kYRowsZ = 64;            // complex data (the same accumulator count)
+template <typename T> constexpr int kYRowsOf = ScalarTraits<T>::is_complex ? kYRowsZ : kYRows;
 constexpr int kCholGridY = 256;        // default workgroups with a right product
 constexpr long kSpinLimit = 1L << 22;  // polls per wait before the abort flag is raised
 
@@ -143,15 +145,15 @@ struct CholTileArgs {
                      // (complex: 0, 1, 2 at the start); 7 the end of the pair's last
                      // right-product task.  From entry 256 the start and end of every
                      // right-product task.
-  // the right product Y = B X (ny = 0: none)
-  const double* B;
+  // the right product Y = B X (ny = 0: none), elements of the factor's type
+  const void* B;
   int64_t ldb;
-  double* Y;
+  void* Y;
   int64_t ldy;
-  double* Xc;     // final tiles of X, written once each (ldxc: a multiple of 16)
+  void* Xc;       // final tiles of X, written once each (ldxc: a multiple of 16)
   int64_t ldxc;
   int m;          // rows of B and Y
-  int ny;         // Y row blocks per pair step: ceil(m / kYRows)
+  int ny;         // Y row blocks per pair step: ceil(m / kYRowsOf<T>)
   int ysplit;     // pairs K >= ysplit run two tasks per row block, one per half
                   // of the contraction, adding atomically into a zeroed Y
                   // (two addends on zero: the sum is independent of their order)
@@ -556,26 +558,33 @@ __device__ inline void publish_n(int cnt, int* const* p, const int* v) {
     for (int q = 0; q < cnt; ++q) __hip_atomic_store(p[q], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Y(K, r) of the right product: rows [128 r, 128 r + 128) x the pair's
-// columns [64 K, 64 K + 64) of Y = B X, over k < 64 K + 64 (X is upper
-// triangular).  X is read from Xc, the copy of its final tiles (below): plain
-// loads, so a column block is shared through each XCD's L2 by the pair's
-// tasks.  Four waves, each a 64 x 32 quarter (eight FP64 MFMA accumulators:
-// six LDS reads feed eight MFMAs); the slabs of B and X for the next two
-// 16-deep steps are in flight (in registers) while the current one is used.
-// LDS strides keep every read at two passes: B's slab rows 144 doubles
-// apart, X's slab stored transposed with rows 36 doubles apart.
-constexpr int kYLdA = kYRows + 16, kYLdX = 16 + 20;
-constexpr int kYLdsDoubles = 16 * kYLdA + 64 * kYLdX;
+// Y(K, r) of the right product: rows [BM r, BM r + BM) x the pair's columns
+// [64 K, 64 K + 64) of Y = B X, over k < 64 K + 64 (X is upper triangular);
+// BM = 128 rows for real data, 64 for complex.  X is read from Xc, the copy
+// of its final tiles (below): plain loads, so a column block is shared
+// through each XCD's L2 by the pair's tasks.  Four waves, each a BM/2 x 32
+// quarter: eight FP64 MFMA accumulators (real: six LDS reads feed eight
+// MFMAs; complex: four real and imaginary accumulator pairs, four MFMAs per
+// complex multiply-add); the slabs of B and X for the next two 16-deep steps
+// are in flight (in registers) while the current one is used.  LDS strides
+// keep every read at two passes (real: B's slab rows 144 doubles apart, X's
+// slab stored transposed with rows 36 doubles apart; complex: 80 and 17
+// elements, 16-byte lanes).
+template <typename T> constexpr int kYLdA = kYRowsOf<T> + 16;
+template <typename T> constexpr int kYLdX = ScalarTraits<T>::is_complex ? 17 : 16 + 20;
+template <typename T> constexpr int kYLdsElems = 16 * kYLdA<T> + 64 * kYLdX<T>;
 // half: 0 = the whole contraction, stored; 1 / 2 = its first / second half
 // (split at a multiple of 16), added atomically.
-__device__ inline void y_task(const CholTileArgs& a, int K, int r, int half, double* lds) {
-  constexpr int BM = kYRows, BN = 64, BK = 16;
+template <typename T>
+__device__ inline void y_task(const CholTileArgs& a, int K, int r, int half, T* lds) {
+  constexpr bool CX = ScalarTraits<T>::is_complex;
+  constexpr int BM = kYRowsOf<T>, BN = 64, BK = 16, LDA = kYLdA<T>, LDX = kYLdX<T>;
   constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
-  double (*As)[kYLdA] = reinterpret_cast<double (*)[kYLdA]>(lds);
-  double (*Xs)[kYLdX] = reinterpret_cast<double (*)[kYLdX]>(lds + BK * kYLdA);  // Xs[j][k]
-  const double* B = a.B;
-  const double* Xc = a.Xc;
+  constexpr int WM = BM / 2, MI = WM / 16;  // rows per wave, 16-row blocks per wave
+  T (*As)[LDA] = reinterpret_cast<T (*)[LDA]>(lds);
+  T (*Xs)[LDX] = reinterpret_cast<T (*)[LDX]>(lds + BK * LDA);  // Xs[j][k]
+  const T* B = static_cast<const T*>(a.B);
+  const T* Xc = static_cast<const T*>(a.Xc);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int m0 = r * BM, n0 = K * 64;
@@ -583,26 +592,29 @@ __device__ inline void y_task(const CholTileArgs& a, int K, int r, int half, dou
   const int kmid = (kall / 2 + 15) / 16 * 16;
   const int kbeg = half == 2 ? kmid : 0;
   const int kend = half == 1 ? kmid : kall;
-  auto fetch = [&](int k0, double (&pa)[NA], double (&px)[NB]) {
+  auto fetch = [&](int k0, T (&pa)[NA], T (&px)[NB]) {
 #pragma unroll
     for (int u = 0; u < NA; ++u) {
       const int gi = m0 + (tid & (BM - 1)), gk = k0 + (tid / BM) + (256 / BM) * u;
-      pa[u] = (gi < a.m && gk < kend) ? B[gi + (int64_t)gk * a.ldb] : 0.0;
+      pa[u] = (gi < a.m && gk < kend) ? B[gi + (int64_t)gk * a.ldb] : s_zero(T());
     }
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
       const int gk = k0 + (tid & 15), gj = n0 + (tid >> 4) + 16 * u;
-      px[u] = (gj < a.n && gk < kend) ? Xc[gk + (int64_t)gj * a.ldxc] : 0.0;
+      px[u] = (gj < a.n && gk < kend) ? Xc[gk + (int64_t)gj * a.ldxc] : s_zero(T());
     }
   };
-  v4d acc[4][2];
+  v4d acc[MI][2], acci[CX ? MI : 1][2];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < MI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 2; ++j) {
+      acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+      if constexpr (CX) acci[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
   // one 16-deep step: the slabs in (pa, px) to LDS, the loads of step
   // k0 + 2 BK into the freed registers, the MFMAs of this step
-  auto step = [&](int k0, double (&pa)[NA], double (&px)[NB]) {
+  auto step = [&](int k0, T (&pa)[NA], T (&px)[NB]) {
 #pragma unroll
     for (int u = 0; u < NA; ++u) As[(tid / BM) + (256 / BM) * u][tid & (BM - 1)] = pa[u];
 #pragma unroll
@@ -611,38 +623,57 @@ __device__ inline void y_task(const CholTileArgs& a, int K, int r, int half, dou
     if (k0 + 2 * BK < kend) fetch(k0 + 2 * BK, pa, px);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      double af[4], bf[2];
+      T af[MI], bf[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = As[kk + (lane >> 4)][wm * 64 + i * 16 + (lane & 15)];
+      for (int i = 0; i < MI; ++i) af[i] = As[kk + (lane >> 4)][wm * WM + i * 16 + (lane & 15)];
 #pragma unroll
       for (int j = 0; j < 2; ++j) bf[j] = Xs[wn * 32 + j * 16 + (lane & 15)][kk + (lane >> 4)];
       // operands swapped: lane l holds rows l % 16 of the block
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < MI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = mfma_f64(bf[j], af[i], acc[i][j]);
+        for (int j = 0; j < 2; ++j) {
+          if constexpr (CX) {
+            acc[i][j] = mfma_f64(bf[j].x, af[i].x, acc[i][j]);
+            acc[i][j] = mfma_f64(-bf[j].y, af[i].y, acc[i][j]);
+            acci[i][j] = mfma_f64(bf[j].y, af[i].x, acci[i][j]);
+            acci[i][j] = mfma_f64(bf[j].x, af[i].y, acci[i][j]);
+          } else {
+            acc[i][j] = mfma_f64(bf[j], af[i], acc[i][j]);
+          }
+        }
     }
     __syncthreads();
   };
-  double pa0[NA], px0[NB], pa1[NA], px1[NB];
+  T pa0[NA], px0[NB], pa1[NA], px1[NB];
   if (kbeg < kend) fetch(kbeg, pa0, px0);
   if (kbeg + BK < kend) fetch(kbeg + BK, pa1, px1);
   for (int k0 = kbeg; k0 < kend; k0 += 2 * BK) {
     step(k0, pa0, px0);
     if (k0 + BK < kend) step(k0 + BK, pa1, px1);
   }
+  T* const Yt = static_cast<T*>(a.Y);
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < MI; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int row = m0 + wm * 64 + i * 16 + (lane & 15);
+        const int row = m0 + wm * WM + i * 16 + (lane & 15);
         const int col = n0 + wn * 32 + j * 16 + mfma_f64_row(lane, q);
         if (row < a.m && col < a.n) {
-          double* y = a.Y + row + (int64_t)col * a.ldy;
-          if (half == 0) *y = acc[i][j][q];
-          else atomicAdd(y, acc[i][j][q]);
+          T* y = Yt + row + (int64_t)col * a.ldy;
+          if constexpr (CX) {
+            if (half == 0) {
+              *y = mkz(acc[i][j][q], acci[i][j][q]);
+            } else {
+              atomicAdd(reinterpret_cast<double*>(y), acc[i][j][q]);
+              atomicAdd(reinterpret_cast<double*>(y) + 1, acci[i][j][q]);
+            }
+          } else {
+            if (half == 0) *y = acc[i][j][q];
+            else atomicAdd(y, acc[i][j][q]);
+          }
         }
       }
 }
@@ -731,10 +762,11 @@ __device__ void chol_panel(const CholTileArgs& a, T (*lds)[TB][33], T* rowbuf) {
       put_tile<T>(tD, X, a.ldx, a.n, c1, c1);
       clear_tile<T>(X, a.ldx, a.n, c1, c0);
       if (a.ny > 0) {
-        put_tile<T>(tX, a.Xc, a.ldxc, a.n, c0, c0);
-        put_tile<T>(tE, a.Xc, a.ldxc, a.n, c0, c1);
-        put_tile<T>(tD, a.Xc, a.ldxc, a.n, c1, c1);
-        clear_tile<T>(a.Xc, a.ldxc, a.n, c1, c0);
+        T* Xc = static_cast<T*>(a.Xc);
+        put_tile<T>(tX, Xc, a.ldxc, a.n, c0, c0);
+        put_tile<T>(tE, Xc, a.ldxc, a.n, c0, c1);
+        put_tile<T>(tD, Xc, a.ldxc, a.n, c1, c1);
+        clear_tile<T>(Xc, a.ldxc, a.n, c1, c0);
       }
       {
         int* fp[6] = {&zs[c0 * nt + c0], &zs[c0 * nt + c1], &zs[c1 * nt + c1],
@@ -788,10 +820,18 @@ __device__ void chol_panel(const CholTileArgs& a, T (*lds)[TB][33], T* rowbuf) {
       put_tile<T>(tE, X, a.ldx, a.n, c0, c1);
       put_tile<T>(tD, X, a.ldx, a.n, c1, c1);
       clear_tile<T>(X, a.ldx, a.n, c1, c0);
+      if (a.ny > 0) {
+        T* Xc = static_cast<T*>(a.Xc);
+        put_tile<T>(tX, Xc, a.ldxc, a.n, c0, c0);
+        put_tile<T>(tE, Xc, a.ldxc, a.n, c0, c1);
+        put_tile<T>(tD, Xc, a.ldxc, a.n, c1, c1);
+        clear_tile<T>(Xc, a.ldxc, a.n, c1, c0);
+      }
       int* fp[6] = {&zs[c0 * nt + c0], &zs[c0 * nt + c1], &zs[c1 * nt + c1],
                     &gs[c0 * nt + c0], &gs[c0 * nt + c1], &gs[c1 * nt + c1]};
       const int fv[6] = {1, 1, 1, K + 1, K + 1, K + 1};
       publish_n(6, fp, fv);
+      if (a.ny > 0 && threadIdx.x == 0) atomicAdd(a.xdone + K, 1);
       stamp(K, 6);
       if (r0 < nt) {
         {
@@ -848,7 +888,7 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
   // the panel's 32 x 32 tiles (ten for real data, seven for complex), six
   // for the worker tasks (a right-product task reuses them)
   __shared__ __align__(16) T lds[kPanelTiles<T>][TB][33];
-  static_assert(sizeof(lds) >= kYLdsDoubles * sizeof(double), "right-product slabs fit the tiles");
+  static_assert(sizeof(lds) >= kYLdsElems<T> * sizeof(T), "right-product slabs fit the tiles");
   T(*tA)[33] = lds[0];
   T(*tB)[33] = lds[1];
   T(*tC)[33] = lds[2];
@@ -974,13 +1014,11 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
         int* fp[2] = {&zs[jr * nt + c0], &zs[jr * nt + c1]};
         const int fv[2] = {cnt + 1, cnt + 1};
         publish_n(2, fp, fv);
-        if constexpr (!ScalarTraits<T>::is_complex) {
-          if (a.ny > 0) {
-            put_tile<T>(tA, a.Xc, a.ldxc, a.n, jr, c0);
-            put_tile<T>(tB, a.Xc, a.ldxc, a.n, jr, c1);
-            publish_n(0, nullptr, nullptr);
-            if (threadIdx.x == 0) atomicAdd(a.xdone + K, 1);
-          }
+        if (a.ny > 0) {
+          put_tile<T>(tA, static_cast<T*>(a.Xc), a.ldxc, a.n, jr, c0);
+          put_tile<T>(tB, static_cast<T*>(a.Xc), a.ldxc, a.n, jr, c1);
+          publish_n(0, nullptr, nullptr);
+          if (threadIdx.x == 0) atomicAdd(a.xdone + K, 1);
         }
         break;
       }
@@ -991,8 +1029,7 @@ __global__ void __launch_bounds__(256) chol_tile_kernel(CholTileArgs a) {
           if (!wait_all<1>(wp, wv, ctl)) break;
         }
         const long long t_start = (long long)__builtin_amdgcn_s_memrealtime();
-        if constexpr (!ScalarTraits<T>::is_complex)
-          y_task(a, K, tk.j % a.ny, K >= a.ysplit ? 1 + tk.j / a.ny : 0, reinterpret_cast<double*>(&lds[0][0][0]));
+        y_task<T>(a, K, tk.j % a.ny, K >= a.ysplit ? 1 + tk.j / a.ny : 0, &lds[0][0][0]);
         if (a.trace != nullptr && threadIdx.x == 0) {
           const long long t_end = (long long)__builtin_amdgcn_s_memrealtime();
           // the pair's last right-product task to finish; every task's start
@@ -1042,7 +1079,7 @@ int chol_tile_factor(T* G, int64_t ldg, int n, T* X, int64_t ldx, int* flags, in
                      hipStream_t s, int grid, long long* trace, const CholRightProduct* rp) {
   if (n <= 0) return 0;
   if (n > kCholTileMaxN) return -3;
-  if (rp != nullptr && (ScalarTraits<T>::is_complex || rp->m < 0)) return -1;
+  if (rp != nullptr && rp->m < 0) return -1;
   CholTileArgs a{};
   a.G = G; a.ldg = ldg; a.X = X; a.ldx = ldx;
   a.n = n;
@@ -1053,7 +1090,7 @@ int chol_tile_factor(T* G, int64_t ldg, int n, T* X, int64_t ldx, int* flags, in
     a.B = rp->B; a.ldb = rp->ldb; a.Y = rp->Y; a.ldy = rp->ldy;
     a.Xc = rp->Xc; a.ldxc = rp->ldxc;
     a.m = rp->m;
-    a.ny = cdiv(rp->m, kYRows);
+    a.ny = cdiv(rp->m, kYRowsOf<T>);
   }
   // the pairs whose right-product tasks would outlast the elimination: the
   // last quarter (at least three) split their tasks in two
@@ -1078,7 +1115,10 @@ int chol_tile_factor(T* G, int64_t ldg, int n, T* X, int64_t ldx, int* flags, in
   hipLaunchKernelGGL((chol_zero_ints_kernel<T>), dim3(cdiv(nflags, 256)), dim3(256), 0, s, flags, nflags);
   if (a.ny > 0 && a.ysplit < a.nt / 2) {
     const int c0 = 64 * a.ysplit;  // < n (every pair has a column)
-    hipLaunchKernelGGL(zero_cols_kernel, dim3(1024), dim3(256), 0, s, a.Y, a.ldy, a.m, c0, n);
+    // complex columns as real ones of twice the length
+    constexpr int w = sizeof(T) / sizeof(double);
+    hipLaunchKernelGGL(zero_cols_kernel, dim3(1024), dim3(256), 0, s, static_cast<double*>(a.Y), w * a.ldy, w * a.m,
+                       c0, n);
   }
   // the panel workgroup plus at least one for the other tasks; up to n ~ 1024
   // the chain dominates and fewer polling workgroups contend less for the

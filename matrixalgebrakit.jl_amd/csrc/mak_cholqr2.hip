@@ -93,7 +93,7 @@ template <typename T> CQ2Layout cq2_layout(int m, int n) {
   L.offPart = take(kRedWgs * sizeof(double));
   L.offSt = take(64 * sizeof(int));
   L.ldxc = (n + 15) / 16 * 16;
-  L.offXc = ScalarTraits<T>::is_complex ? 0 : take((size_t)L.ldxc * n * sizeof(T));
+  L.offXc = take((size_t)L.ldxc * n * sizeof(T));
   L.bytes = off;
   return L;
 }
@@ -310,15 +310,12 @@ void cholqr2_enqueue(hipStream_t s, int m, int n, const T* A, int64_t lda, T* Q,
   const int64_t nn = (int64_t)n * n;
   const unsigned eg = (unsigned)cdiv64(nn, 256);
   zero_ints(st, 8, s);  // st[2]: the Cholesky's info
-  // pass 1 (real data: Q1 = A X1 inside the Cholesky launch, column block by
-  // column block as X1 becomes final)
+  // pass 1: Q1 = A X1 inside the Cholesky launch, column block by column
+  // block as X1 becomes final
   gram_upper<T>(A, lda, m, n, G, Sl, L.ks, s);
-  if constexpr (!ScalarTraits<T>::is_complex) {
-    const CholRightProduct rp{A, lda, Q1, m, m, reinterpret_cast<double*>(ws + L.offXc), L.ldxc};
+  {
+    const CholRightProduct rp{A, lda, Q1, m, m, ws + L.offXc, L.ldxc};
     chol_tile_factor<T>(G, n, n, X, n, flags, st + 2, 0, s, 0, nullptr, &rp);
-  } else {
-    chol_tile_factor<T>(G, n, n, X, n, flags, st + 2, 0, s);
-    gemm<T>(OP_N, OP_N, gq<T>(m, n, n, A, lda, X, n, Q1, m, 1.0, 0.0, TRI_B_UPPER), s);
   }
   // pass 2: G2 = I + E (slices summed with U1 = up(E) and ||E||_F in one
   // sweep), the decision, and W2, W2^{-1} in closed form

@@ -213,21 +213,21 @@ int gemm_lo_reduced(const GemmArgs& g, int opA, int opB, const int* g16, const i
 // definite (if still 0).  grid <= 0: default workgroup count.  Returns 0 or a
 // HIP launch error (1); -3 when n is too large.
 //
-// rp (real data only, may be null): also Y = B X (B, Y m x n, column-major),
+// rp (may be null): also Y = B X (B, Y m x n, column-major, elements of T),
 // computed in the same launch as X's column blocks become final.  Xc: n
-// columns of scratch, leading dimension ldxc >= n a multiple of 16, 128-byte
-// aligned (a copy of X's final tiles that the product reads through the
-// caches: with whole 128-byte lines per column, no cached line holds a value
-// that is not final yet).
+// columns of scratch (elements of T), leading dimension ldxc >= n a multiple
+// of 16, 128-byte aligned (a copy of X's final tiles that the product reads
+// through the caches: with whole 128-byte lines per column, no cached line
+// holds a value that is not final yet).
 constexpr int kCholTileMaxNt = 64;
 constexpr int kCholTileMaxN = 32 * kCholTileMaxNt;
 struct CholRightProduct {
-  const double* B;
+  const void* B;
   int64_t ldb;
-  double* Y;
+  void* Y;
   int64_t ldy;
   int m;
-  double* Xc;
+  void* Xc;
   int64_t ldxc;
 };
 size_t chol_tile_flag_ints(int n);
