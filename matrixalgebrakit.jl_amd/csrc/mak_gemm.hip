@@ -340,13 +340,19 @@ __global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
   const zd* B = reinterpret_cast<const zd*>(g.B) + (int64_t)bat * g.strideB;
   zd* C = reinterpret_cast<zd*>(g.C) + (int64_t)bat * g.strideC + (int64_t)ks * g.splitStride;
 
-  v4d cr[TM][TN], ci[TM][TN];
+  // Three real products per complex multiply-add (Gauss):
+  //   p1 = sum ar br,  p2 = sum ai bi,  p3 = sum (ar + ai)(br + bi),
+  //   re = p1 - p2,  im = p3 - p1 - p2
+  // (three MFMAs instead of four; the error stays within a small multiple
+  // of the usual componentwise bound sum |a| |b|).
+  v4d p1[TM][TN], p2[TM][TN], p3[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-      cr[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
-      ci[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+      p1[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+      p2[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+      p3[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
     }
 
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
@@ -371,26 +377,26 @@ __global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      double ar[TM], ai[TM], nai[TM], br[TN], bi[TN];
+      double ar[TM], ai[TM], as[TM], br[TN], bi[TN], bs[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
         ar[i] = Ar[kk + (lane >> 4)][wm * WTM + i * 16 + (lane & 15)];
         ai[i] = Ai[kk + (lane >> 4)][wm * WTM + i * 16 + (lane & 15)];
-        nai[i] = -ai[i];
+        as[i] = ar[i] + ai[i];
       }
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
         br[j] = Br[kk + (lane >> 4)][wn * WTN + j * 16 + (lane & 15)];
         bi[j] = Bi[kk + (lane >> 4)][wn * WTN + j * 16 + (lane & 15)];
+        bs[j] = br[j] + bi[j];
       }
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          cr[i][j] = mfma_f64(br[j], ar[i], cr[i][j]);
-          cr[i][j] = mfma_f64(bi[j], nai[i], cr[i][j]);
-          ci[i][j] = mfma_f64(bi[j], ar[i], ci[i][j]);
-          ci[i][j] = mfma_f64(br[j], ai[i], ci[i][j]);
+          p1[i][j] = mfma_f64(br[j], ar[i], p1[i][j]);
+          p2[i][j] = mfma_f64(bi[j], ai[i], p2[i][j]);
+          p3[i][j] = mfma_f64(bs[j], as[i], p3[i][j]);
         }
     }
     __syncthreads();
@@ -406,7 +412,8 @@ __global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
         const int row = m0 + wm * WTM + i * 16 + (lane & 15);
         const int col = n0 + wn * WTN + j * 16 + mfma_f64_row(lane, r);
         if (row < g.M && col < g.N) {
-          zd v = s_mul(alpha, mkz(cr[i][j][r], ci[i][j][r]));
+          const double re = p1[i][j][r] - p2[i][j][r], im = p3[i][j][r] - p1[i][j][r] - p2[i][j][r];
+          zd v = s_mul(alpha, mkz(re, im));
           zd* cp = C + row + (int64_t)col * g.ldc;
           if (use_beta) v = s_fma(beta, *cp, v);
           *cp = v;
