@@ -355,26 +355,52 @@ __global__ void __launch_bounds__(256) gemm_z64_kernel(GemmArgs g) {
       p3[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
     }
 
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    for (int e = tid; e < BM * BK; e += 256) {
+  // the next slab is loaded into registers while the current one is used
+  constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+  static_assert(NA * 256 == BM * BK && NB * 256 == BN * BK, "whole slabs per thread");
+  zd pa[NA], pb[NB];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int e = tid + 256 * u;
       int i, k;
       if (OPA == OP_N) { i = e % BM; k = e / BM; }
       else { k = e % BK; i = e / BK; }
       const int gi = m0 + i, gk = k0 + k;
-      zd v = (gi < g.M && gk < kend) ? load_a<OPA>(A, g.lda, gi, gk, g.unitLowerA) : mkz(0.0, 0.0);
-      Ar[k][i] = v.x;
-      Ai[k][i] = v.y;
+      pa[u] = (gi < g.M && gk < kend) ? load_a<OPA>(A, g.lda, gi, gk, g.unitLowerA) : mkz(0.0, 0.0);
     }
-    for (int e = tid; e < BN * BK; e += 256) {
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int e = tid + 256 * u;
       int j, k;
       if (OPB == OP_N) { k = e % BK; j = e / BK; }
       else { j = e % BN; k = e / BN; }
       const int gj = n0 + j, gk = k0 + k;
-      zd v = (gj < g.N && gk < kend) ? load_b<OPB>(B, g.ldb, gk, gj) : mkz(0.0, 0.0);
-      Br[k][j] = v.x;
-      Bi[k][j] = v.y;
+      pb[u] = (gj < g.N && gk < kend) ? load_b<OPB>(B, g.ldb, gk, gj) : mkz(0.0, 0.0);
+    }
+  };
+  if (kbeg < kend) fetch(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int e = tid + 256 * u;
+      int i, k;
+      if (OPA == OP_N) { i = e % BM; k = e / BM; }
+      else { k = e % BK; i = e / BK; }
+      Ar[k][i] = pa[u].x;
+      Ai[k][i] = pa[u].y;
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int e = tid + 256 * u;
+      int j, k;
+      if (OPB == OP_N) { k = e % BK; j = e / BK; }
+      else { j = e % BN; k = e / BN; }
+      Br[k][j] = pb[u].x;
+      Bi[k][j] = pb[u].y;
     }
     __syncthreads();
+    if (k0 + BK < kend) fetch(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       double ar[TM], ai[TM], as[TM], br[TN], bi[TN], bs[TN];
