@@ -85,15 +85,18 @@ template <typename T> __device__ inline T vload_unit(const T* V, int64_t ldv, in
   return V[k + (int64_t)i * ldv];
 }
 
-// 16x16x4 complex product step on FP64 MFMA: acc += a b (conj(a) b when CA)
+// 16x16x4 complex product step on FP64 MFMA, acc += a b (conj(a) b when CA),
+// in three real products (Gauss): p1 += ar br, p2 += ai bi,
+// p3 += (ar + ai)(br + bi); the sum is re = p1 - p2, im = p3 - p1 - p2.
 template <bool CA>
-__device__ inline void zmfma(v4d& cr, v4d& ci, zd a, zd b) {
+__device__ inline void zmfma3(v4d& p1, v4d& p2, v4d& p3, zd a, zd b) {
   const double ai = CA ? -a.y : a.y;
-  cr = mfma_f64(a.x, b.x, cr);
-  cr = mfma_f64(-ai, b.y, cr);
-  ci = mfma_f64(a.x, b.y, ci);
-  ci = mfma_f64(ai, b.x, ci);
+  p1 = mfma_f64(a.x, b.x, p1);
+  p2 = mfma_f64(ai, b.y, p2);
+  p3 = mfma_f64(a.x + ai, b.x + b.y, p3);
 }
+__device__ inline double z3_re(const v4d& p1, const v4d& p2, int r) { return p1[r] - p2[r]; }
+__device__ inline double z3_im(const v4d& p1, const v4d& p2, const v4d& p3, int r) { return p3[r] - p1[r] - p2[r]; }
 
 template <typename T, int NC, int TRANS>
 __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* tasks, int ntasks) {
@@ -117,11 +120,12 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
 
     // ---- phase 1: W = V^H C.  Tiles (2 x NC/16), NC/32 per wave: id = w + 4u
     constexpr int TPW1 = NC / 32;
-    v4d acc_r[TPW1], acc_i[TPW1];
+    v4d acc_r[TPW1], acc_i[TPW1], acc_s[CPLX ? TPW1 : 1];
 #pragma unroll
     for (int u = 0; u < TPW1; ++u) {
       acc_r[u] = v4d{0.0, 0.0, 0.0, 0.0};
       acc_i[u] = v4d{0.0, 0.0, 0.0, 0.0};
+      if constexpr (CPLX) acc_s[u] = v4d{0.0, 0.0, 0.0, 0.0};
     }
     // the next KB-row block of V and C is loaded into registers while the
     // current one is used
@@ -163,7 +167,7 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
         for (int kk = 0; kk < KB; kk += 4) {
           const T a = Vs[kk + lk][ti * 16 + lr];   // V^H(i, k) = conj(V(k, i))
           const T b = Cs[kk + lk][tj * 16 + lr];
-          if constexpr (CPLX) zmfma<true>(acc_r[u], acc_i[u], a, b);
+          if constexpr (CPLX) zmfma3<true>(acc_r[u], acc_i[u], acc_s[u], a, b);
           else acc_r[u] = mfma_f64(a, b, acc_r[u]);
         }
       }
@@ -176,7 +180,7 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = ti * 16 + mfma_f64_row(lane, r), j = tj * 16 + lr;
-        if constexpr (CPLX) Ws[i][j] = mkz(acc_r[u][r], acc_i[u][r]);
+        if constexpr (CPLX) Ws[i][j] = mkz(z3_re(acc_r[u], acc_i[u], r), z3_im(acc_r[u], acc_i[u], acc_s[u], r));
         else Ws[i][j] = acc_r[u][r];
       }
     }
@@ -190,6 +194,7 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
     for (int u = 0; u < TPW1; ++u) {
       acc_r[u] = v4d{0.0, 0.0, 0.0, 0.0};
       acc_i[u] = v4d{0.0, 0.0, 0.0, 0.0};
+      if constexpr (CPLX) acc_s[u] = v4d{0.0, 0.0, 0.0, 0.0};
       const int id = w + 4 * u;
       const int ti = id & 1, tj = id >> 1;
 #pragma unroll
@@ -198,8 +203,8 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
         const T a = TRANS ? Ts[c][i] : Ts[i][c];   // conjugated below when TRANS
         const T b = Ws[c][tj * 16 + lr];
         if constexpr (CPLX) {
-          if (TRANS) zmfma<true>(acc_r[u], acc_i[u], a, b);
-          else zmfma<false>(acc_r[u], acc_i[u], a, b);
+          if (TRANS) zmfma3<true>(acc_r[u], acc_i[u], acc_s[u], a, b);
+          else zmfma3<false>(acc_r[u], acc_i[u], acc_s[u], a, b);
         } else {
           acc_r[u] = mfma_f64(a, b, acc_r[u]);
         }
@@ -213,7 +218,7 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = ti * 16 + mfma_f64_row(lane, r), j = tj * 16 + lr;
-        if constexpr (CPLX) Ws[i][j] = mkz(acc_r[u][r], acc_i[u][r]);
+        if constexpr (CPLX) Ws[i][j] = mkz(z3_re(acc_r[u], acc_i[u], r), z3_im(acc_r[u], acc_i[u], acc_s[u], r));
         else Ws[i][j] = acc_r[u][r];
       }
     }
@@ -260,13 +265,13 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
       for (int u = 0; u < TPW3; ++u) {
         const int id = w + 4 * u;
         const int ti = id % (KB / 16), tj = id / (KB / 16);
-        v4d dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0};
+        v4d dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0}, ds = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int kk = 0; kk < NB; kk += 4) {
           // swapped operands: D' = W2^T V^T, lane owns row ti*16 + lr of C
           const T a = Ws[kk + lk][tj * 16 + lr];  // (W2^T)(j, c)
           const T b = Vs[ti * 16 + lr][kk + lk];  // (V^T)(c, i)
-          if constexpr (CPLX) zmfma<false>(dr, di, a, b);
+          if constexpr (CPLX) zmfma3<false>(dr, di, ds, a, b);
           else dr = mfma_f64(a, b, dr);
         }
         const int row = k0 + ti * 16 + lr;
@@ -275,7 +280,7 @@ __global__ void __launch_bounds__(256) apply_wy_tasks_kernel(const ApplyTask* ta
           const int col = tj * 16 + mfma_f64_row(lane, r);
           if (row < rows && col < ncols) {
             T* cp = C + row + (int64_t)col * t.ldc;
-            if constexpr (CPLX) *cp = s_sub(cv[u][r], mkz(dr[r], di[r]));
+            if constexpr (CPLX) *cp = s_sub(cv[u][r], mkz(z3_re(dr, di, r), z3_im(dr, di, ds, r)));
             else *cp = cv[u][r] - dr[r];
           }
         }
