@@ -5,8 +5,8 @@
 //
 //   G1 = A^H A                       upper triangle, split-K MFMA product
 //   G1 = R1^H R1, X1 = R1^{-1}       persistent tile Cholesky (mak_chol.hip)
-//   Q1 = A X1                        triangular MFMA product (real data: in
-//                                    the Cholesky launch, as X1 fills)
+//   Q1 = A X1                        triangular MFMA product tasks inside the
+//                                    Cholesky launch, as X1 fills
 //   G2 = Q1^H Q1 = I + E             upper triangle, split-K MFMA product
 //   W2 = I + U with (I + U)^H (I + U) = I + E, in closed form:
 //        U1 = up(E), U = up(E - U1^H U1), W2^{-1} = I - U + U^2
@@ -56,7 +56,7 @@ constexpr double kBf16CorrMaxE = 5e-14;
 struct CQ2Layout {
   size_t offQ1, offG, offE, offX, offU, offW, offT, offSl, offFlags, offPart, offSt, offXc, bytes;
   int ks;    // split-K slices of the Gram products
-  int ldxc;  // leading dimension of the Cholesky's copy of X1 (real data)
+  int ldxc;  // leading dimension of the Cholesky's copy of X1
 };
 
 // Split-K slices of an upper-triangle n x n Gram over m rows on 128 x 128
