@@ -99,7 +99,7 @@ int mak_qr_compact_wy_z(void* stream, int m, int n, void* A, int lda, void* Q, i
  * arguments and results as mak_qr_compact_d; the workspace is a static
  * 64 MiB device array per GPU, held until the process ends (problems whose
  * mak_qr_workspace_size exceeds 64 MiB return 7; Float64 1024 x 512 needs
- * about 43 MiB).  The call allocates nothing.  Calls on one GPU share the
+ * about 47 MiB).  The call allocates nothing.  Calls on one GPU share the
  * array: calls that overlap in time on one GPU (other streams or threads)
  * race, so the caller serialises them. */
 int mak_qr_compact_noalloc_d(void* stream, int m, int n, double* A, int lda, double* Q, int ldq, double* R, int ldr,

@@ -35,6 +35,19 @@ def test_workspace_query_is_host_only_and_validates():
     assert lib.mak_qr_workspace_size(0, 0, 10, 10, None) == -5
 
 
+def test_noalloc_documented_case_fits_the_static_scratch():
+    """The allocation-free qr_compact! entry runs on a 64 MiB static device
+    array: the case the header quotes (Float64 1024 x 512, about 47 MiB) and
+    the GPU test's second, smaller call fit it, the 8192 x 1024 headline does
+    not (that call returns 7)."""
+    lib = _lib.load()
+    cap = 64 << 20
+    for (m, n), lo, hi in (((1024, 512), 40 << 20, cap), ((300, 70), 0, cap), ((8192, 1024), cap, None)):
+        nb = C.c_size_t(0)
+        assert lib.mak_qr_workspace_size(0, 0, m, n, C.byref(nb)) == 0
+        assert nb.value >= lo and (hi is None or nb.value <= hi), (m, n, nb.value)
+
+
 def test_entry_points_reject_bad_arguments_before_touching_the_device():
     lib = _lib.load()
     null = C.c_void_p(0)
